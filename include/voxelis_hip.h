@@ -1,0 +1,151 @@
+/*
+ * voxelis_hip.h -- C ABI of the MI355X-native batched SVO-DAG edit path.
+ *
+ * Drop-in boundary for the reference's VoxTree / VoxInterner / Batch / VoxOps
+ * API (WildPixelGames/voxelis, Rust).  Plain pointers and sizes only; every
+ * entry point returns a vx_status (0 = VX_OK, negative = error).  Reference
+ * panics map to error codes and leave the affected tree unchanged.
+ *
+ * Memory: "device" pointers are HIP device allocations (HBM); "host" pointers
+ * are ordinary CPU memory.  One interner owns one HIP stream; entry points are
+ * asynchronous on that stream unless documented otherwise, and writers must be
+ * serialized per interner (the reference's &mut VoxInterner).
+ */
+#ifndef VOXELIS_HIP_H
+#define VOXELIS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    VX_OK = 0,
+    VX_E_INVALID_ARG = -1,
+    VX_E_OOM = -2,         /* interner/macros.rs:37-38 panic "Out of memory" */
+    VX_E_DEPTH = -3,       /* core/max_depth.rs:77-83; batch apply with depth < 2 */
+    VX_E_RANGE = -4,       /* spatial/voxtree.rs:146-148 coordinate assert */
+    VX_E_LEAF_BRICK = -5,  /* voxtree.rs:829-834 + block_id.rs:273-276: types() on a leaf */
+    VX_E_SAME_ROOT = -6,   /* voxtree.rs:311 assert_ne!(new_root, root) */
+    VX_E_HIP = -7,         /* a HIP runtime call failed */
+    VX_E_BUDGET = -8,      /* interner/mod.rs:63-68 budget too small / too large */
+    VX_E_UNSUPPORTED = -9, /* voxel width not built into this library */
+    VX_E_TABLE_FULL = -10, /* pattern table exhausted (load factor too high) */
+    VX_E_POISONED = -11    /* a previous device-side failure left the interner unusable */
+} vx_status;
+
+typedef struct vx_interner vx_interner;
+
+/* Dense per-chunk edit record: the buffers of core/batch.rs:39-45 Batch<T>.
+ * masks: 2*P bytes, (set_mask, clear_mask) per 2x2x2 brick, P = 8^(depth-1);
+ * values: 8*P voxels of the interner's value size, slot i = x | y<<1 | z<<2.
+ * Brick p covers Morton path p<<3 (utils/common.rs:24-55). */
+typedef struct {
+    const uint8_t* masks;  /* device */
+    const void* values;    /* device */
+    uint64_t fill_value;   /* Batch::to_fill (batch.rs:106-111) when has_fill */
+    uint8_t depth;         /* MaxDepth of the target tree */
+    uint8_t has_fill;
+    uint8_t has_patches;   /* Batch::has_patches (batch.rs:127-132) */
+    uint8_t reserved;
+} vx_batch_view;
+
+/* ---- interner: interner/mod.rs ---- */
+
+/* VoxInterner::<T>::with_memory_budget (interner/mod.rs:45-155).
+ * value_size is sizeof(T): 1 (u8/i8) or 2 (u16/i16).  capacity =
+ * budget_bytes / (78 + value_size), exactly the reference's node_size.
+ * load_factor (0 = default 0.5) sizes the branch pattern table. */
+int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_factor,
+                       vx_interner** out);
+int vx_interner_destroy(vx_interner* h);
+/* VoxInterner::capacity (mod.rs:1021-1024) */
+int vx_interner_capacity(const vx_interner* h, uint64_t* capacity);
+/* leaf_patterns().len(), branch_patterns().len() (mod.rs:1031-1037; the EMPTY
+ * branch pattern counts) and live node slots.  Synchronizes the stream. */
+int vx_interner_counts(vx_interner* h, uint64_t* leaf_patterns, uint64_t* branch_patterns,
+                       uint64_t* alive_nodes);
+/* VoxInterner::get_ref (mod.rs:222-232).  Synchronizes. */
+int vx_interner_get_ref(vx_interner* h, uint64_t block_id, uint32_t* ref_count);
+/* VoxInterner::get_value (mod.rs:167-177).  Synchronizes. */
+int vx_interner_get_value(vx_interner* h, uint64_t block_id, uint64_t* value);
+/* VoxInterner::get_children (mod.rs:180-191).  Synchronizes. */
+int vx_interner_get_children(vx_interner* h, uint64_t block_id, uint64_t children[8]);
+/* Counters (hits, misses, collapses, hazards ...); see VX_STAT_* below.  Synchronizes. */
+int vx_interner_stats(vx_interner* h, uint64_t* out, uint32_t n);
+/* Use the caller's HIP stream (hipStream_t) instead of the interner's own. */
+int vx_interner_set_stream(vx_interner* h, void* hip_stream);
+/* Wait for all work on the interner's stream; returns the first device error. */
+int vx_interner_sync(vx_interner* h);
+
+enum {
+    VX_STAT_BRANCH_HITS = 0,
+    VX_STAT_BRANCH_MISSES,
+    VX_STAT_BRANCH_DUPS,       /* same key twice in one launch: resolved to one node */
+    VX_STAT_LEAF_MISSES,
+    VX_STAT_COLLAPSES,
+    VX_STAT_RECYCLED,
+    VX_STAT_H2_UNCOLLAPSED,
+    VX_STAT_H3_LEAKS,
+    VX_STAT_H4_UNDERCOUNT,
+    VX_STAT_H5_COLLAPSE_CLONED,
+    VX_STAT_CANDIDATES,        /* branch keys probed */
+    VX_STAT_PROBES,            /* table slots inspected */
+    VX_STAT_COUNT
+};
+
+/* ---- batched apply: VoxTree::apply_batch (spatial/voxtree.rs:303-328) ---- */
+
+/* Apply batches[i] to tree i for i in [0, n), all in one launch sequence.
+ * roots: device, n BlockIds, read as the current roots and overwritten with
+ * the new ones.  status: device, n int32: 1 = changed (apply_batch returned
+ * true), 0 = unchanged (returned false), < 0 = vx_status of the reference's
+ * panic for that tree (its root is left as it was).  Asynchronous. */
+int vx_trees_apply_batches(vx_interner* h, size_t n, uint64_t* roots,
+                           const vx_batch_view* batches, int32_t* status);
+
+/* Single-tree drop-in for VoxTree::apply_batch with the Batch buffers in host
+ * memory (batch.masks / batch.values are host pointers here).  root is a host
+ * BlockId, updated in place; *changed receives apply_batch's bool.  Synchronous. */
+int vx_tree_apply_batch_host(vx_interner* h, uint64_t* root, const vx_batch_view* batch,
+                             int* changed);
+
+/* ---- batched reads: VoxOpsRead::get (voxtree.rs:145-159, utils/common.rs:122-156) ---- */
+
+/* For q in [0, nq): tree chunk_of_query[q] (< n_trees; roots/depths indexed by
+ * it), voxel xyz[3q..3q+3).  out_values receives the value or 0 (None);
+ * out_present (nullable) receives 1 for Some.  Coordinates outside
+ * [0, 2^depth) or a tree index >= n_trees set out_present = 0xFF and make the
+ * next sync report VX_E_RANGE (the reference panics).  All pointers device.
+ * Asynchronous. */
+int vx_trees_get(vx_interner* h, size_t n_trees, const uint64_t* roots, const uint8_t* depths,
+                 size_t nq, const uint32_t* chunk_of_query, const int32_t* xyz, void* out_values,
+                 uint8_t* out_present);
+
+/* ---- whole-tree ops: VoxOpsBulkWrite (voxtree.rs:263-292) ---- */
+
+/* fill(value) for value != 0, clear() for value == 0, on trees[i] (device
+ * roots).  Asynchronous. */
+int vx_trees_fill(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* values);
+
+/* ---- verification: canonical structural hash (no reference counterpart) ---- */
+
+/* 128-bit Merkle hash of each tree's DAG, independent of node ids: two trees
+ * with the same shape, values and LOD values hash equal in any interner.
+ * out: device, 2*n uint64.  Asynchronous. */
+int vx_trees_canonical_hash(vx_interner* h, size_t n, const uint64_t* roots, uint64_t* out);
+
+/* ---- device memory helpers for callers without a HIP allocator ---- */
+int vx_device_alloc(size_t bytes, void** ptr);
+int vx_device_free(void* ptr);
+int vx_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int vx_memcpy_d2h(void* dst, const void* src, size_t bytes);
+
+const char* vx_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,227 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact reads, equal leaf/branch pattern counts, equal canonical DAG hash
+per tree, equal reference counts where the reference's tests assert them.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_helpers import assert_same, full_reads_gpu, oracle_apply_dense
+
+pytestmark = pytest.mark.gpu
+
+MB = 1024 * 1024
+
+
+@pytest.fixture(scope="module")
+def vx():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("needs a GPU")
+    import voxelis_amd
+    from voxelis_amd import api, workloads
+    return api, workloads
+
+
+def test_config1_plain(vx):
+    api, W = vx
+    it = api.VoxInterner(1, 256 * MB)
+    dense = W.plain_checker(5)
+    masks, values = W.dense_to_batch(dense, 5)
+    import torch
+    f = api.Forest(it, [5])
+    m = api.to_device(masks)
+    v = api.to_device(values)
+    st = f.apply([m], [v])
+    assert st[0] == 1
+    assert it.counts() == (2, 6)
+    root = f.roots_host()[0]
+    assert it.get_ref(root) == 1
+    assert (root >> 47) & 0xFFFF == 0x00FF  # types 0x00, mask 0xFF (Appendix C root shape)
+    reads = full_reads_gpu(f, 0, 5)
+    assert np.array_equal(reads, O.all_coords(5).sum(axis=1) % 2 + 1)
+    it_o = O.Interner(1, 256 * MB)
+    t_o = O.Tree(5)
+    oracle_apply_dense(it_o, t_o, masks, values)
+    assert_same(it, f, it_o, [t_o], what="config1")
+    # every non-root node of the chain holds 8 refs, the two leaves 4 each
+    node = root
+    for _ in range(4):
+        ch = it.get_children(node)
+        assert len(set(ch)) == 1
+        node = ch[0]
+        assert it.get_ref(node) == 8
+    leaves = it.get_children(node)
+    assert sorted(it.get_value(c) for c in leaves) == [1, 1, 1, 1, 2, 2, 2, 2]
+    assert {it.get_ref(c) for c in leaves} == {4}
+
+
+def _run_tree_case(api, depth, steps, value_size=1):
+    """steps: list of ('batch', fill, xyz, values) | ('fill', v); same on both sides."""
+    it = api.VoxInterner(value_size, MB)
+    it_o = O.Interner(value_size, MB)
+    t = api.VoxTree(depth)
+    t_o = O.Tree(depth)
+    for st in steps:
+        if st[0] == "batch":
+            _, fill, xyz, vals = st
+            b = t.create_batch(it)
+            b_o = t_o.create_batch()
+            if fill is not None:
+                b.just_fill(fill)
+                b_o.just_fill(fill)
+            if len(xyz):
+                b.set_many(xyz, vals)
+                b_o.set_many(xyz, np.asarray(vals, dtype=np.uint64))
+            err_o = err_g = None
+            try:
+                ch_o = t_o.apply_batch(it_o, b_o)
+            except O.OraclePanic as e:
+                err_o = e.kind
+            try:
+                ch_g = t.apply_batch(it, b)
+            except api.VxError as e:
+                err_g = e.kind
+            if err_o is not None:
+                return it, t, it_o, t_o, (err_o, err_g)
+            assert err_g is None and ch_o == ch_g
+        else:
+            t.fill(it, st[1])
+            t_o.fill(it_o, st[1])
+    return it, t, it_o, t_o, None
+
+
+def _check_tree(api, it, t, it_o, t_o, refs=True):
+    f = api.Forest(it, [t.max_depth])
+    f.set_roots([t.root])
+    assert_same(it, f, it_o, [t_o])
+    if refs and t.root != 0:
+        assert it.get_ref(t.root) == it_o.get_ref(t_o.root)
+
+
+def _grid(d):
+    n = 1 << d
+    y, z, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    return x.ravel(), y.ravel(), z.ravel()
+
+
+def _xyz(d, sel=None):
+    xyz = O.all_coords(d)
+    return xyz if sel is None else xyz[sel]
+
+
+REFERENCE_BATCH_CASES = {
+    # voxtree.rs test name -> (depth, steps builder)
+    "batch_double_apply": lambda: (5, [("batch", None, np.array([[0, 0, 0]]), [3])] * 2),
+    "batch_expand_shared_leaf": lambda: (5, [("batch", 1, np.array([[0, 0, 0]]), [2])]),
+    "batch_checkerboard": lambda: (5, [("batch", None, _xyz(5), np.where(_xyz(5).sum(1) % 2 == 0, 2, 1))]),
+    "batch_solid_fill_one_by_one": lambda: (5, [("batch", None, _xyz(5), np.full(32768, 3))]),
+    "batch_solid_fill_half": lambda: (5, [("batch", None, _xyz(5, _xyz(5)[:, 1] < 16),
+                                            np.full(16384, v)) for v in range(1, 6)]),
+    "batch_solid_fill_fill_op": lambda: (5, [("batch", 3, np.zeros((0, 3)), [])]),
+    "batch_sparse_fill": lambda: (5, [("batch", None, _xyz(5, (_xyz(5) % 4 == 0).all(1)), np.full(512, 3))]),
+    "batch_gradient_fill": lambda: (5, [("batch", None, _xyz(5), _xyz(5)[:, 0] % 256)]),
+    "batch_hollow_cube": lambda: (5, [("batch", None, _xyz(5, ((_xyz(5) == 0) | (_xyz(5) == 31)).any(1)),
+                                       np.full(int(((_xyz(5) == 0) | (_xyz(5) == 31)).any(1).sum()), 3))]),
+    "batch_diagonal": lambda: (5, [("batch", None, np.array([[i, i, i] for i in range(32)]), np.full(32, 3))]),
+    "batch_random_noise": lambda: (5, [("batch", None, np.random.default_rng(2134).integers(0, 32, (1000, 3)),
+                                        np.random.default_rng(2135).integers(1, 256, 1000))]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(REFERENCE_BATCH_CASES))
+def test_reference_batch_cases(vx, name):
+    api, _ = vx
+    depth, steps = REFERENCE_BATCH_CASES[name]()
+    it, t, it_o, t_o, err = _run_tree_case(api, depth, steps)
+    assert err is None
+    _check_tree(api, it, t, it_o, t_o)
+
+
+def test_double_apply_returns_false(vx):
+    api, _ = vx
+    it = api.VoxInterner(1, MB)
+    t = api.VoxTree(5)
+    b = t.create_batch(it)
+    b.set(0, 0, 0, 3)
+    assert t.apply_batch(it, b)
+    assert not t.apply_batch(it, b)
+
+
+def test_errors_map_reference_panics(vx):
+    api, _ = vx
+    # H6: applying a fill-only batch that reproduces the root (voxtree.rs:311)
+    it, t, it_o, t_o, err = _run_tree_case(api, 3, [("batch", 4, np.zeros((0, 3)), [])] * 2)
+    assert err == ("SAME_ROOT", "SAME_ROOT")
+    # the reference's Phase 0 took a reference before panicking; the HIP path
+    # leaves the tree untouched, so only shape and counts are compared
+    _check_tree(api, it, t, it_o, t_o, refs=False)
+    # H1: partial edit of a collapsed brick leaf (block_id.rs:273-276)
+    xyz = O.all_coords(1)
+    steps = [("batch", None, np.concatenate([xyz, [[7, 7, 7]]]), np.concatenate([np.full(8, 5), [1]])),
+             ("batch", None, np.array([[0, 0, 0]]), [6])]
+    it, t, it_o, t_o, err = _run_tree_case(api, 3, steps)
+    assert err == ("LEAF_TYPES", "LEAF_BRICK")
+    _check_tree(api, it, t, it_o, t_o)  # the tree is left as it was
+    # H9: batch on a depth-1 tree
+    it, t, it_o, t_o, err = _run_tree_case(api, 1, [("batch", None, np.array([[0, 0, 0]]), [1])])
+    assert err == ("DEPTH", "DEPTH")
+
+
+def test_fill_and_clear(vx):
+    api, _ = vx
+    steps = [("fill", 7), ("batch", None, np.array([[1, 2, 3]]), [9]), ("fill", 7), ("fill", 0),
+             ("batch", None, _xyz(3), np.arange(512) % 7 + 1), ("fill", 0)]
+    it, t, it_o, t_o, err = _run_tree_case(api, 3, steps)
+    assert err is None
+    _check_tree(api, it, t, it_o, t_o)
+    assert it.counts() == (0, 1)
+
+
+def test_set_sum_mixed_depths_one_launch(vx):
+    api, W = vx
+    import torch
+    depths = [2, 5, 3, 6, 4, 5, 2]
+    it = api.VoxInterner(2, 64 * MB)
+    it_o = O.Interner(2, 64 * MB)
+    f = api.Forest(it, depths)
+    ms, vs, trees_o = [], [], []
+    for d in depths:
+        n = 1 << d
+        y, z, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+        dense = (x + y + z + 1).astype(np.uint16)
+        m, v = W.dense_to_batch(dense, d)
+        ms.append(api.to_device(m))
+        vs.append(api.to_device(v))
+        t_o = O.Tree(d)
+        oracle_apply_dense(it_o, t_o, m, v)
+        trees_o.append(t_o)
+    st = f.apply(ms, vs)
+    assert (st == 1).all()
+    assert it.counts() == (190, 177 + 1)  # the deepest depth's closed form (D = 6)
+    assert_same(it, f, it_o, trees_o, full_reads=False)
+
+
+def test_terrain_small_grid_and_alternation(vx):
+    api, W = vx
+    import torch
+    grid = 4
+    mA, vA, eA, _ = W.terrain_batches(grid=grid, seed=11)
+    mB, vB, eB, _ = W.terrain_batches(grid=grid, seed=11, below=2, surface=1)
+    it = api.VoxInterner(1, 64 * MB)
+    it_o = O.Interner(1, 64 * MB)
+    n = grid * grid
+    f = api.Forest(it, [5] * n)
+    trees_o = [O.Tree(5) for _ in range(n)]
+    tA = ([api.to_device(mA[i]) for i in range(n)], [api.to_device(vA[i]) for i in range(n)])
+    tB = ([api.to_device(mB[i]) for i in range(n)], [api.to_device(vB[i]) for i in range(n)])
+    for step in range(4):
+        (m, v), (mh, vh) = (tA, (mA, vA)) if step % 2 == 0 else (tB, (mB, vB))
+        st = f.apply(m, v)
+        assert (st == 1).all(), st
+        for i in range(n):
+            assert oracle_apply_dense(it_o, trees_o[i], mh[i], vh[i])[0]
+        assert_same(it, f, it_o, trees_o, full_reads=(step < 2), what=f"step {step}")
+    s = it.stats()
+    assert s["h4_undercount"] == 0 and s["h5_collapse_cloned"] == 0

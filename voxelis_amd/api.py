@@ -1,0 +1,311 @@
+"""Python mirror of the reference API on the HIP path.
+
+VoxInterner  <- interner/mod.rs VoxInterner<T>   (device node columns + pattern tables)
+VoxTree      <- spatial/voxtree.rs VoxTree<T>     (max depth, root id, dirty flag)
+Batch        <- core/batch.rs Batch<T>            (dense per-brick edit buffers, host)
+Forest       -- many trees of one interner with device-resident roots: the batched
+                form of apply_batch / get / fill (one launch sequence per call)
+
+Every compute call goes through libvoxelis_hip.so; device memory is held in torch
+tensors (plumbing only).
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import morton
+from ._lib import STAT_NAMES, BatchView, VxError, check, lib
+
+EMPTY = 0
+INVALID = (1 << 64) - 1
+_DTYPES = {1: (np.uint8, torch.uint8), 2: (np.uint16, torch.uint16)}
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        raise RuntimeError("the HIP path needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+_TORCH_SAFE = {np.dtype(np.uint8): np.uint8, np.dtype(np.uint16): np.int16,
+               np.dtype(np.uint32): np.int32, np.dtype(np.uint64): np.int64}
+
+
+def to_device(arr, device=None):
+    """Upload a numpy array as raw bytes-compatible torch storage (uint16 goes as int16)."""
+    arr = np.ascontiguousarray(arr)
+    view = arr.view(_TORCH_SAFE.get(arr.dtype, arr.dtype))
+    return torch.from_numpy(view).to(device or _dev())
+
+
+def _i64(u):
+    """uint64 BlockId -> int64 bit pattern for torch storage."""
+    return u - (1 << 64) if u >= (1 << 63) else u
+
+
+def _u64(i):
+    return i + (1 << 64) if i < 0 else i
+
+
+class VoxInterner:
+    """VoxInterner::<T>::with_memory_budget (interner/mod.rs:45-155) on the device."""
+
+    def __init__(self, value_size, budget_bytes, load_factor=0.5):
+        self.value_size = value_size
+        self.np_dtype, self.torch_dtype = _DTYPES[value_size]
+        self.device = _dev()
+        h = C.c_void_p()
+        check(lib().vx_interner_create(value_size, budget_bytes, load_factor, C.byref(h)),
+              "vx_interner_create")
+        self.h = h
+        # order library work with torch's work on the current stream
+        check(lib().vx_interner_set_stream(self.h, C.c_void_p(torch.cuda.current_stream().cuda_stream)),
+              "vx_interner_set_stream")
+
+    @classmethod
+    def with_memory_budget(cls, budget_bytes, value_size=1, load_factor=0.5):
+        return cls(value_size, budget_bytes, load_factor)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vx_interner_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def capacity(self):
+        out = C.c_uint64(0)
+        check(lib().vx_interner_capacity(self.h, C.byref(out)))
+        return out.value
+
+    def counts(self):
+        """(leaf_patterns().len(), branch_patterns().len()) incl. the EMPTY pattern."""
+        a, b, c = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        check(lib().vx_interner_counts(self.h, C.byref(a), C.byref(b), C.byref(c)), "counts")
+        return a.value, b.value
+
+    def alive_nodes(self):
+        a, b, c = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        check(lib().vx_interner_counts(self.h, C.byref(a), C.byref(b), C.byref(c)), "counts")
+        return c.value
+
+    def get_ref(self, block_id):
+        out = C.c_uint32(0)
+        check(lib().vx_interner_get_ref(self.h, block_id, C.byref(out)), "get_ref")
+        return out.value
+
+    def get_value(self, block_id):
+        out = C.c_uint64(0)
+        check(lib().vx_interner_get_value(self.h, block_id, C.byref(out)), "get_value")
+        return out.value
+
+    def get_children(self, block_id):
+        out = (C.c_uint64 * 8)()
+        check(lib().vx_interner_get_children(self.h, block_id, out), "get_children")
+        return list(out)
+
+    def stats(self):
+        out = (C.c_uint64 * len(STAT_NAMES))()
+        check(lib().vx_interner_stats(self.h, out, len(STAT_NAMES)), "stats")
+        return dict(zip(STAT_NAMES, list(out)))
+
+    def sync(self):
+        check(lib().vx_interner_sync(self.h), "sync")
+
+
+class Batch:
+    """core/batch.rs Batch<T>: (set, clear) masks and 8 values per 2x2x2 brick."""
+
+    def __init__(self, max_depth, np_dtype=np.uint8):
+        self.max_depth = max_depth
+        self.masks, self.values = morton.dense_batch_arrays(max_depth, np_dtype)
+        self.to_fill = None
+        self.has_patches = False
+
+    def just_set(self, x, y, z, v):
+        self.set_many(np.array([[x, y, z]], dtype=np.int32), np.array([v]))
+        return True
+
+    set = just_set
+
+    def set_many(self, xyz, values):
+        xyz = np.asarray(xyz, dtype=np.int32).reshape(-1, 3)
+        if len(xyz) == 0:
+            return
+        morton.apply_edits_to_dense(self.masks, self.values, xyz, np.asarray(values))
+        self.has_patches = True
+
+    def just_clear(self):
+        self.masks[:] = 0
+        self.values[:] = 0
+        self.to_fill = None
+        self.has_patches = False
+
+    def just_fill(self, v):
+        self.just_clear()
+        self.to_fill = int(v)
+
+    fill = just_fill
+
+    def size(self):
+        return int(np.count_nonzero((self.masks[:, 0] != 0) | (self.masks[:, 1] != 0)))
+
+    def view_host(self):
+        v = BatchView()
+        v.masks = self.masks.ctypes.data if self.has_patches else None
+        v.values = self.values.ctypes.data if self.has_patches else None
+        v.fill_value = self.to_fill or 0
+        v.depth = self.max_depth
+        v.has_fill = 1 if self.to_fill is not None else 0
+        v.has_patches = 1 if self.has_patches else 0
+        return v
+
+
+class VoxTree:
+    """spatial/voxtree.rs VoxTree<T>: the root id lives on the host, nodes in HBM."""
+
+    def __init__(self, max_depth):
+        self.max_depth = max_depth
+        self.root = EMPTY
+        self.dirty = False
+
+    def create_batch(self, interner):
+        return Batch(self.max_depth, interner.np_dtype)
+
+    def voxels_per_axis(self):
+        return 1 << self.max_depth
+
+    def is_empty(self):
+        return self.root == EMPTY
+
+    def is_leaf(self):
+        return (self.root >> 63) == 1
+
+    def apply_batch(self, interner, batch):
+        """VoxTree::apply_batch (voxtree.rs:303-328); reference panics raise VxError."""
+        r = C.c_uint64(self.root)
+        changed = C.c_int(0)
+        v = batch.view_host()
+        check(lib().vx_tree_apply_batch_host(interner.h, C.byref(r), C.byref(v), C.byref(changed)),
+              "apply_batch")
+        self.root = r.value
+        if changed.value:
+            self.dirty = True
+        return bool(changed.value)
+
+    def get_many(self, interner, xyz):
+        f = Forest(interner, [self.max_depth])
+        f.set_roots([self.root])
+        vals, pres = f.get(np.zeros(len(xyz), dtype=np.uint32), xyz)
+        return vals, pres
+
+    def get(self, interner, x, y, z):
+        vals, pres = self.get_many(interner, np.array([[x, y, z]], dtype=np.int32))
+        if pres[0] == 0xFF:
+            raise VxError(-4, "get")
+        return int(vals[0]) if pres[0] else None
+
+    def fill(self, interner, v):
+        f = Forest(interner, [self.max_depth])
+        f.set_roots([self.root])
+        f.fill(np.array([v], dtype=np.uint64))
+        was = self.root
+        self.root = f.roots_host()[0]
+        if v != 0 or was != EMPTY:
+            self.dirty = True
+
+    def clear(self, interner):
+        self.fill(interner, 0)
+
+    def canonical_hash(self, interner):
+        f = Forest(interner, [self.max_depth])
+        f.set_roots([self.root])
+        return f.canonical_hashes()[0]
+
+
+class Forest:
+    """Many trees sharing one interner, roots resident in HBM (the batched form)."""
+
+    def __init__(self, interner, depths):
+        self.it = interner
+        self.n = len(depths)
+        self.depths_host = np.asarray(depths, dtype=np.uint8)
+        self.depths = torch.from_numpy(self.depths_host.copy()).to(interner.device)
+        self.roots = torch.zeros(self.n, dtype=torch.int64, device=interner.device)
+        self.status = torch.zeros(self.n, dtype=torch.int32, device=interner.device)
+
+    def set_roots(self, roots):
+        self.roots.copy_(torch.tensor([_i64(int(r)) for r in roots], dtype=torch.int64))
+
+    def roots_host(self):
+        return [_u64(int(r)) for r in self.roots.cpu().tolist()]
+
+    def make_views(self, masks, values, fills=None, has_patches=None):
+        """Device batch views over per-tree tensors (masks [P,2] u8, values [P,8])."""
+        views = (BatchView * self.n)()
+        for i in range(self.n):
+            v = views[i]
+            hp = True if has_patches is None else bool(has_patches[i])
+            v.masks = masks[i].data_ptr() if (hp and masks[i] is not None) else None
+            v.values = values[i].data_ptr() if (hp and values[i] is not None) else None
+            f = None if fills is None else fills[i]
+            v.fill_value = 0 if f is None else int(f)
+            v.has_fill = 0 if f is None else 1
+            v.has_patches = 1 if hp else 0
+            v.depth = int(self.depths_host[i])
+        return views
+
+    def apply_views(self, views):
+        check(lib().vx_trees_apply_batches(self.it.h, self.n, self.roots.data_ptr(), views,
+                                           self.status.data_ptr()), "apply_batches")
+
+    def apply(self, masks, values, fills=None, has_patches=None, sync=True):
+        views = self.make_views(masks, values, fills, has_patches)
+        self._views = views
+        self._keep = (masks, values)
+        self.apply_views(views)
+        if sync:
+            self.it.sync()
+            return self.status.cpu().numpy()
+        return None
+
+    def get(self, chunk_of, xyz, sync=True):
+        nq = len(chunk_of)
+        dev = self.it.device
+        chunk_t = torch.from_numpy(np.ascontiguousarray(chunk_of, dtype=np.uint32).view(np.int32)).to(dev)
+        xyz_t = torch.from_numpy(np.ascontiguousarray(xyz, dtype=np.int32).reshape(-1)).to(dev)
+        out = torch.zeros(nq * self.it.value_size, dtype=torch.uint8, device=dev)
+        pres = torch.zeros(nq, dtype=torch.uint8, device=dev)
+        check(lib().vx_trees_get(self.it.h, self.n, self.roots.data_ptr(), self.depths.data_ptr(), nq,
+                                 chunk_t.data_ptr(), xyz_t.data_ptr(), out.data_ptr(),
+                                 pres.data_ptr()), "get")
+        torch.cuda.current_stream().synchronize()
+        rc = lib().vx_interner_sync(self.it.h)
+        p = pres.cpu().numpy()
+        if rc != 0 and not (rc == -4 and (p == 0xFF).any()):
+            check(rc, "get")
+        return out.cpu().numpy().view(self.it.np_dtype).astype(np.uint64), p
+
+    def get_device(self, chunk_t, xyz_t, out, pres):
+        check(lib().vx_trees_get(self.it.h, self.n, self.roots.data_ptr(), self.depths.data_ptr(),
+                                 chunk_t.numel(), chunk_t.data_ptr(), xyz_t.data_ptr(),
+                                 out.data_ptr(), pres.data_ptr() if pres is not None else None),
+              "get")
+
+    def fill(self, values):
+        vals = torch.from_numpy(np.asarray(values, dtype=np.uint64).view(np.int64)).to(self.it.device)
+        check(lib().vx_trees_fill(self.it.h, self.n, self.roots.data_ptr(), vals.data_ptr()), "fill")
+        self.it.sync()
+
+    def canonical_hashes(self):
+        out = torch.zeros(2 * self.n, dtype=torch.int64, device=self.it.device)
+        check(lib().vx_trees_canonical_hash(self.it.h, self.n, self.roots.data_ptr(), out.data_ptr()),
+              "canonical_hash")
+        self.it.sync()
+        o = out.cpu().numpy().view(np.uint64).reshape(-1, 2)
+        return [(int(a), int(b)) for a, b in o]

@@ -1,0 +1,626 @@
+// vx_api.hip -- host side of the C ABI declared in include/voxelis_hip.h.
+//
+// One vx_interner = the device node columns + pattern tables + a HIP stream + a
+// grow-only workspace for launch sequences.  All work is enqueued on the
+// interner's stream; synchronizing calls are documented in the header.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/voxelis_hip.h"
+#include "vx_device.h"
+#include "vx_kernels.hip"
+
+using namespace vx;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint64_t kMaxGrid = 8192;
+
+inline unsigned grid_for(uint64_t work) {
+    uint64_t g = (work + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    if (g > kMaxGrid) g = kMaxGrid;
+    return (unsigned)g;
+}
+
+template <typename P>
+bool dmalloc(P** p, size_t bytes) {
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes ? bytes : 1) != hipSuccess) return false;
+    *p = (P*)q;
+    return true;
+}
+
+struct Grow {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t bytes) {
+        if (bytes <= cap) return true;
+        // queued kernels may still use the old buffer
+        if (ptr) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(ptr);
+        }
+        ptr = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 4096);
+        if (hipMalloc(&ptr, want) != hipSuccess) return false;
+        cap = want;
+        return true;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+struct LevelBufs {
+    Grow res, key, hash, id, tslot, slot;
+};
+
+}  // namespace
+
+struct vx_interner {
+    uint32_t vsize = 0;
+    uint64_t capacity = 0;
+    uint64_t tslots = 0;
+    uint64_t* children = nullptr;
+    uint16_t* tms = nullptr;
+    void* values = nullptr;
+    uint32_t* refs = nullptr;
+    uint16_t* gens = nullptr;
+    uint32_t* free_stack = nullptr;
+    uint64_t* tab = nullptr;
+    uint32_t* leaf_map = nullptr;
+    uint8_t* leaf_need = nullptr;
+    uint32_t* new_leaves = nullptr;
+    Ctl* ctl = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int poisoned = 0;
+    // workspace
+    Grow chunks, slot_off, ptr_tables, gc_front[2], stage, scratch;
+    LevelBufs lv[kLevels + 1];
+    // pinned staging ring: a slot is reused only after its copies completed
+    static constexpr int kRing = 4;
+    void* h_pinned_ring[kRing] = {nullptr};
+    size_t h_pinned_cap[kRing] = {0};
+    hipEvent_t ring_ev[kRing] = {nullptr};
+    bool ring_used[kRing] = {false};
+    int ring_next = 0;
+    int ring_cur = 0;
+    void* h_pinned = nullptr;  // the slot of the current call
+};
+
+namespace {
+
+template <typename T>
+Pool<T> pool_of(vx_interner* h) {
+    Pool<T> P;
+    P.children = h->children;
+    P.tms = h->tms;
+    P.values = (T*)h->values;
+    P.refs = h->refs;
+    P.gens = h->gens;
+    P.free_stack = h->free_stack;
+    P.tab = h->tab;
+    P.tmask = h->tslots - 1;
+    P.leaf_map = h->leaf_map;
+    P.leaf_need = h->leaf_need;
+    P.new_leaves = h->new_leaves;
+    P.ctl = h->ctl;
+    P.capacity = (uint32_t)h->capacity;
+    return P;
+}
+
+// Take the next staging slot of the ring, waiting for its previous copies.
+bool pinned_ensure(vx_interner* h, size_t bytes) {
+    int i = h->ring_next;
+    h->ring_next = (i + 1) % vx_interner::kRing;
+    if (!h->ring_ev[i] && hipEventCreateWithFlags(&h->ring_ev[i], hipEventDisableTiming) != hipSuccess)
+        return false;
+    if (h->ring_used[i] && hipEventSynchronize(h->ring_ev[i]) != hipSuccess) return false;
+    h->ring_used[i] = false;
+    if (bytes > h->h_pinned_cap[i]) {
+        if (h->h_pinned_ring[i]) (void)hipHostFree(h->h_pinned_ring[i]);
+        h->h_pinned_ring[i] = nullptr;
+        h->h_pinned_cap[i] = 0;
+        size_t want = std::max<size_t>(bytes, 1 << 16);
+        if (hipHostMalloc(&h->h_pinned_ring[i], want, hipHostMallocDefault) != hipSuccess) return false;
+        h->h_pinned_cap[i] = want;
+    }
+    h->h_pinned = h->h_pinned_ring[i];
+    h->ring_cur = i;
+    return true;
+}
+
+// Mark the current staging slot busy until the stream passes this point.
+void pinned_release(vx_interner* h) {
+    int i = h->ring_cur;
+    if (hipEventRecord(h->ring_ev[i], h->stream) == hipSuccess) h->ring_used[i] = true;
+}
+
+int err_to_status(uint32_t err) {
+    if (err & kErrOom) return VX_E_OOM;
+    if (err & kErrTableFull) return VX_E_TABLE_FULL;
+    if (err & kErrRange) return VX_E_RANGE;
+    if (err & kErrPoison) return VX_E_POISONED;
+    return VX_OK;
+}
+
+#define HIPCHK(x)                        \
+    do {                                 \
+        if ((x) != hipSuccess) return VX_E_HIP; \
+    } while (0)
+
+uint64_t slots_at(uint32_t d, uint32_t h) { return 1ull << (3 * (d - h)); }
+
+template <typename T>
+int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
+               int32_t* status) {
+    hipStream_t s = h->stream;
+    Pool<T> P = pool_of<T>(h);
+    // ---- plan: trees with patches grouped by depth (others first), dense slot
+    // spaces per height
+    auto status_of = [&](const vx_batch_view& b) -> int32_t {
+        if (b.depth > kMaxDepth) return VX_E_DEPTH;
+        // voxtree.rs:792/913: max_depth - 1 underflows for a patched batch of depth < 2
+        if (b.has_patches && b.depth < 2) return VX_E_DEPTH;
+        if (b.has_patches && (!b.masks || !b.values)) return VX_E_INVALID_ARG;
+        return 0;
+    };
+    auto sort_key = [&](uint32_t i) -> uint32_t {
+        const vx_batch_view& b = batches[i];
+        return (b.has_patches && status_of(b) == 0) ? b.depth : 0u;
+    };
+    std::vector<uint32_t> perm(n);
+    for (size_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](uint32_t a, uint32_t b) { return sort_key(a) < sort_key(b); });
+    size_t chunk_bytes = n * sizeof(ChunkDev);
+    size_t off_count = (size_t)(kLevels + 1) * n;
+    size_t stage_bytes = chunk_bytes + off_count * sizeof(uint64_t) + 2 * (kLevels + 1) * sizeof(void*);
+    if (!pinned_ensure(h, stage_bytes)) return VX_E_HIP;
+    ChunkDev* hc = (ChunkDev*)h->h_pinned;
+    uint64_t* hoff = (uint64_t*)((char*)h->h_pinned + chunk_bytes);
+    uint32_t dmax = 0;
+    struct Run { uint32_t start, count, depth; };
+    std::vector<Run> runs;
+    for (size_t c = 0; c < n; c++) {
+        const vx_batch_view& b = batches[perm[c]];
+        ChunkDev& C = hc[c];
+        std::memset(&C, 0, sizeof(C));
+        C.masks = b.masks;
+        C.values = b.values;
+        C.fill = b.fill_value & (sizeof(T) == 1 ? 0xFFull : 0xFFFFull);
+        C.orig = perm[c];
+        C.depth = b.depth;
+        C.has_fill = b.has_fill ? 1 : 0;
+        C.status = status_of(b);
+        C.has_patches = (b.has_patches && C.status == 0) ? 1 : 0;
+        if (!C.has_patches) continue;
+        uint32_t d = b.depth;
+        dmax = std::max<uint32_t>(dmax, d);
+        if (!runs.empty() && runs.back().depth == d && runs.back().start + runs.back().count == c)
+            runs.back().count++;
+        else
+            runs.push_back({(uint32_t)c, 1, d});
+    }
+    for (size_t i = 0; i < off_count; i++) hoff[i] = ~0ull;
+    LevelInfo L[kLevels + 1];
+    std::memset(L, 0, sizeof(L));
+    std::vector<Run> lruns[kLevels + 1];
+    for (uint32_t hh = 1; hh <= dmax; hh++) {
+        uint64_t base = 0;
+        for (const Run& r : runs) {
+            if (r.depth < hh) continue;
+            lruns[hh].push_back(r);
+        }
+        L[hh].h = hh;
+        L[hh].ngroups = (uint32_t)lruns[hh].size();
+        for (size_t g = 0; g < lruns[hh].size(); g++) {
+            const Run& r = lruns[hh][g];
+            L[hh].slot_base[g] = base;
+            L[hh].chunk_base[g] = r.start;
+            L[hh].depth[g] = r.depth;
+            uint64_t per = slots_at(r.depth, hh);
+            for (uint32_t k = 0; k < r.count; k++) hoff[(size_t)hh * n + r.start + k] = base + k * per;
+            base += per * r.count;
+        }
+        L[hh].slot_base[L[hh].ngroups] = base;
+        L[hh].nslots = base;
+    }
+    // ---- workspace
+    if (!h->chunks.ensure(chunk_bytes)) return VX_E_HIP;
+    if (!h->slot_off.ensure(off_count * sizeof(uint64_t))) return VX_E_HIP;
+    uint64_t* res_ptrs_h[kLevels + 1] = {nullptr};
+    uint64_t* cid_ptrs_h[kLevels + 1] = {nullptr};
+    Cands K[kLevels + 1];
+    std::memset(K, 0, sizeof(K));
+    for (uint32_t hh = 1; hh <= dmax; hh++) {
+        uint64_t ns = std::max<uint64_t>(L[hh].nslots, 1);
+        LevelBufs& B = h->lv[hh];
+        if (!B.res.ensure(ns * 8) || !B.key.ensure(ns * 72) || !B.hash.ensure(ns * 8) ||
+            !B.id.ensure(ns * 8) || !B.tslot.ensure(ns * 8) || !B.slot.ensure(ns * 8))
+            return VX_E_HIP;
+        K[hh].key = (uint64_t*)B.key.ptr;
+        K[hh].hash = (uint64_t*)B.hash.ptr;
+        K[hh].id = (uint64_t*)B.id.ptr;
+        K[hh].tslot = (uint64_t*)B.tslot.ptr;
+        K[hh].slot = (uint64_t*)B.slot.ptr;
+        res_ptrs_h[hh] = (uint64_t*)B.res.ptr;
+        cid_ptrs_h[hh] = (uint64_t*)B.id.ptr;
+    }
+    void** hptrs = (void**)((char*)h->h_pinned + chunk_bytes + off_count * sizeof(uint64_t));
+    for (int i = 0; i <= kLevels; i++) {
+        hptrs[i] = res_ptrs_h[i];
+        hptrs[kLevels + 1 + i] = cid_ptrs_h[i];
+    }
+    if (!h->ptr_tables.ensure(2 * (kLevels + 1) * sizeof(void*))) return VX_E_HIP;
+    size_t gcap = (size_t)h->capacity + n + 16;
+    if (!h->gc_front[0].ensure(gcap * 4) || !h->gc_front[1].ensure(gcap * 4)) return VX_E_HIP;
+    ChunkDev* dchunks = (ChunkDev*)h->chunks.ptr;
+    uint64_t* doff = (uint64_t*)h->slot_off.ptr;
+    uint64_t* const* dres = (uint64_t* const*)h->ptr_tables.ptr;
+    uint64_t* const* dcid = dres + (kLevels + 1);
+    HIPCHK(hipMemcpyAsync(dchunks, hc, chunk_bytes, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(doff, hoff, off_count * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(h->ptr_tables.ptr, hptrs, 2 * (kLevels + 1) * sizeof(void*),
+                          hipMemcpyHostToDevice, s));
+    pinned_release(h);
+    uint32_t nvalues = 1u << (8 * sizeof(T));
+    uint32_t* ncand = &h->ctl->ncand[0];
+    // ---- launch sequence
+    k_begin<<<1, 1, 0, s>>>(h->ctl);
+    k_prepare<T><<<grid_for(n), kBlock, 0, s>>>(P, dchunks, (uint32_t)n, roots);
+    if (dmax >= 2) {
+        k_brick_scan<T><<<grid_for(L[1].nslots), kBlock, 0, s>>>(P, dchunks, L[1], res_ptrs_h[1]);
+    }
+    k_leaf_alloc<T><<<1, 1024, 0, s>>>(P, dchunks, (uint32_t)n, nvalues);
+    if (dmax >= 2) {
+        k_brick_build<T><<<grid_for(L[1].nslots), kBlock, 0, s>>>(P, dchunks, L[1], res_ptrs_h[1],
+                                                                  K[1], ncand + 1);
+        k_probe<T><<<grid_for(L[1].nslots), kBlock, 0, s>>>(P, K[1], ncand + 1, res_ptrs_h[1],
+                                                            L[1].nslots);
+        k_commit<T><<<grid_for(L[1].nslots), kBlock, 0, s>>>(P, K[1], ncand + 1, L[1].nslots);
+        for (uint32_t hh = 2; hh <= dmax; hh++) {
+            k_level_build<T><<<grid_for(L[hh].nslots), kBlock, 0, s>>>(
+                P, dchunks, L[hh], doff + (size_t)(hh - 1) * n, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1],
+                res_ptrs_h[hh], K[hh], ncand + hh);
+            k_probe<T><<<grid_for(L[hh].nslots), kBlock, 0, s>>>(P, K[hh], ncand + hh, res_ptrs_h[hh],
+                                                                 L[hh].nslots);
+            k_commit<T><<<grid_for(L[hh].nslots), kBlock, 0, s>>>(P, K[hh], ncand + hh, L[hh].nslots);
+        }
+    }
+    uint32_t* gf0 = (uint32_t*)h->gc_front[0].ptr;
+    uint32_t* gf1 = (uint32_t*)h->gc_front[1].ptr;
+    k_finalize<T><<<grid_for(n), kBlock, 0, s>>>(P, dchunks, (uint32_t)n, doff, (uint32_t)n, dres, dcid,
+                                                 roots, status, gf0);
+    k_end_alloc<<<1, 1, 0, s>>>(h->ctl);
+    k_leaf_cleanup<T><<<grid_for(nvalues), kBlock, 0, s>>>(P);
+    // dec_ref_recursive of the replaced roots: depth <= 7 gives at most 8 rounds
+    uint64_t gbound = std::min<uint64_t>(gcap, h->capacity + n);
+    for (uint32_t r = 0; r < (uint32_t)kLevels + 1; r++) {
+        uint32_t* fr = (r & 1) ? gf1 : gf0;
+        uint32_t* nx = (r & 1) ? gf0 : gf1;
+        uint64_t bound = r == 0 ? n : gbound;
+        k_gc_round<T><<<grid_for(bound), kBlock, 0, s>>>(P, r, fr, nx, r == 0, bound, gcap);
+    }
+    if (hipGetLastError() != hipSuccess) return VX_E_HIP;
+    return VX_OK;
+}
+
+template <typename T>
+int fill_impl(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* vals) {
+    hipStream_t s = h->stream;
+    Pool<T> P = pool_of<T>(h);
+    size_t gcap = (size_t)h->capacity + n + 16;
+    if (!h->gc_front[0].ensure(gcap * 4) || !h->gc_front[1].ensure(gcap * 4)) return VX_E_HIP;
+    uint32_t* gf0 = (uint32_t*)h->gc_front[0].ptr;
+    uint32_t* gf1 = (uint32_t*)h->gc_front[1].ptr;
+    uint32_t nvalues = 1u << (8 * sizeof(T));
+    k_begin<<<1, 1, 0, s>>>(h->ctl);
+    k_fill_prepare<T><<<grid_for(n), kBlock, 0, s>>>(P, (uint32_t)n, roots, vals, gf0);
+    uint64_t gbound = std::min<uint64_t>(gcap, h->capacity + n);
+    for (uint32_t r = 0; r < (uint32_t)kLevels + 1; r++) {
+        uint32_t* fr = (r & 1) ? gf1 : gf0;
+        uint32_t* nx = (r & 1) ? gf0 : gf1;
+        uint64_t bound = r == 0 ? n : gbound;
+        k_gc_round<T><<<grid_for(bound), kBlock, 0, s>>>(P, r, fr, nx, r == 0, bound, gcap);
+    }
+    k_begin<<<1, 1, 0, s>>>(h->ctl);
+    k_leaf_alloc_only<T><<<1, 1024, 0, s>>>(P, nvalues);
+    k_fill_finish<T><<<grid_for(n), kBlock, 0, s>>>(P, (uint32_t)n, roots, vals);
+    k_end_alloc<<<1, 1, 0, s>>>(h->ctl);
+    if (hipGetLastError() != hipSuccess) return VX_E_HIP;
+    return VX_OK;
+}
+
+template <typename T>
+int get_impl(vx_interner* h, size_t nq, size_t ntrees, const uint64_t* roots,
+             const uint8_t* depths, const uint32_t* chunk_of, const int32_t* xyz, void* out,
+             uint8_t* present) {
+    Pool<T> P = pool_of<T>(h);
+    k_get<T><<<grid_for(nq), kBlock, 0, h->stream>>>(P, nq, ntrees, roots, depths, chunk_of, xyz,
+                                                     (T*)out, present);
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
+template <typename T>
+int canon_impl(vx_interner* h, size_t n, const uint64_t* roots, uint64_t* out) {
+    Pool<T> P = pool_of<T>(h);
+    k_canon<T><<<grid_for(n), 64, 0, h->stream>>>(P, (uint32_t)n, roots, out);
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
+int sync_read_ctl(vx_interner* h, Ctl* out) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(out, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    return VX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vx_status_string(int st) {
+    switch (st) {
+        case VX_OK: return "ok";
+        case VX_E_INVALID_ARG: return "invalid argument";
+        case VX_E_OOM: return "out of memory (interner capacity)";
+        case VX_E_DEPTH: return "max depth out of range";
+        case VX_E_RANGE: return "coordinate out of range";
+        case VX_E_LEAF_BRICK: return "partial edit of a collapsed brick leaf";
+        case VX_E_SAME_ROOT: return "apply produced the current root";
+        case VX_E_HIP: return "HIP runtime error";
+        case VX_E_BUDGET: return "memory budget invalid";
+        case VX_E_UNSUPPORTED: return "unsupported voxel width";
+        case VX_E_TABLE_FULL: return "pattern table full";
+        case VX_E_POISONED: return "interner poisoned by an earlier device failure";
+        default: return "unknown";
+    }
+}
+
+int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_factor,
+                       vx_interner** out) {
+    if (!out) return VX_E_INVALID_ARG;
+    *out = nullptr;
+    if (value_size != 1 && value_size != 2) return VX_E_UNSUPPORTED;
+    uint64_t node_size = 4 + 2 + 64 + value_size + 8;  // interner/mod.rs:158-164
+    uint64_t cap = budget_bytes / node_size;
+    if (cap == 0 || cap > 0xFFFFFFFEull) return VX_E_BUDGET;
+    if (!(load_factor > 0.0 && load_factor < 1.0)) load_factor = 0.5;
+    vx_interner* h = new (std::nothrow) vx_interner();
+    if (!h) return VX_E_HIP;
+    h->vsize = value_size;
+    h->capacity = cap;
+    uint64_t want = (uint64_t)((double)cap / load_factor) + 1;
+    uint64_t ts = 1024;
+    while (ts < want) ts <<= 1;
+    h->tslots = ts;
+    uint64_t nv = 1ull << (8 * value_size);
+    bool ok = dmalloc(&h->children, cap * 64) && dmalloc(&h->tms, cap * 2) &&
+              dmalloc(&h->values, cap * value_size) && dmalloc(&h->refs, cap * 4) &&
+              dmalloc(&h->gens, cap * 2) && dmalloc(&h->free_stack, cap * 4) &&
+              dmalloc(&h->tab, ts * 8) && dmalloc(&h->leaf_map, nv * 4) &&
+              dmalloc(&h->leaf_need, nv) && dmalloc(&h->new_leaves, nv * 4) &&
+              dmalloc(&h->ctl, sizeof(Ctl));
+    if (!ok || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        vx_interner_destroy(h);
+        return VX_E_HIP;
+    }
+    h->own_stream = true;
+    // PoolAllocatorLite::new zeroes every column (pool_allocator_lite.rs:29-81);
+    // the EMPTY branch is index 0 with everything zero (interner/mod.rs:83-101).
+    bool z = hipMemset(h->children, 0, cap * 64) == hipSuccess &&
+             hipMemset(h->tms, 0, cap * 2) == hipSuccess &&
+             hipMemset(h->values, 0, cap * value_size) == hipSuccess &&
+             hipMemset(h->refs, 0, cap * 4) == hipSuccess && hipMemset(h->gens, 0, cap * 2) == hipSuccess &&
+             hipMemset(h->tab, 0, ts * 8) == hipSuccess && hipMemset(h->leaf_map, 0, nv * 4) == hipSuccess &&
+             hipMemset(h->leaf_need, 0, nv) == hipSuccess;
+    Ctl c;
+    std::memset(&c, 0, sizeof(c));
+    c.next_index = 1;
+    c.next0 = 1;
+    z = z && hipMemcpy(h->ctl, &c, sizeof(c), hipMemcpyHostToDevice) == hipSuccess;
+    z = z && hipDeviceSynchronize() == hipSuccess;
+    if (!z) {
+        vx_interner_destroy(h);
+        return VX_E_HIP;
+    }
+    *out = h;
+    return VX_OK;
+}
+
+int vx_interner_destroy(vx_interner* h) {
+    if (!h) return VX_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void* ptrs[] = {h->children, h->tms, h->values, h->refs, h->gens, h->free_stack,
+                    h->tab, h->leaf_map, h->leaf_need, h->new_leaves, h->ctl};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    h->chunks.release();
+    h->slot_off.release();
+    h->ptr_tables.release();
+    h->gc_front[0].release();
+    h->gc_front[1].release();
+    h->stage.release();
+    h->scratch.release();
+    for (auto& b : h->lv) {
+        b.res.release();
+        b.key.release();
+        b.hash.release();
+        b.id.release();
+        b.tslot.release();
+        b.slot.release();
+    }
+    for (int i = 0; i < vx_interner::kRing; i++) {
+        if (h->h_pinned_ring[i]) (void)hipHostFree(h->h_pinned_ring[i]);
+        if (h->ring_ev[i]) (void)hipEventDestroy(h->ring_ev[i]);
+    }
+    if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return VX_OK;
+}
+
+int vx_interner_capacity(const vx_interner* h, uint64_t* capacity) {
+    if (!h || !capacity) return VX_E_INVALID_ARG;
+    *capacity = h->capacity;
+    return VX_OK;
+}
+
+int vx_interner_set_stream(vx_interner* h, void* stream) {
+    if (!h) return VX_E_INVALID_ARG;
+    (void)hipStreamSynchronize(h->stream);
+    if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+    h->own_stream = false;
+    h->stream = (hipStream_t)stream;
+    return VX_OK;
+}
+
+int vx_interner_sync(vx_interner* h) {
+    if (!h) return VX_E_INVALID_ARG;
+    Ctl c;
+    int rc = sync_read_ctl(h, &c);
+    if (rc) return rc;
+    if (c.err & kErrPoison) h->poisoned = 1;
+    return err_to_status(c.err);
+}
+
+int vx_interner_counts(vx_interner* h, uint64_t* leaves, uint64_t* branches, uint64_t* alive) {
+    if (!h) return VX_E_INVALID_ARG;
+    Ctl c;
+    int rc = sync_read_ctl(h, &c);
+    if (rc) return rc;
+    if (leaves) *leaves = c.live_leaves;
+    if (branches) *branches = c.live_branches + 1;  // + the EMPTY pattern (mod.rs:97)
+    if (alive) *alive = (uint64_t)c.next_index - c.free_top;
+    return VX_OK;
+}
+
+int vx_interner_stats(vx_interner* h, uint64_t* out, uint32_t n) {
+    if (!h || !out) return VX_E_INVALID_ARG;
+    Ctl c;
+    int rc = sync_read_ctl(h, &c);
+    if (rc) return rc;
+    static const int map[VX_STAT_COUNT] = {ST_HIT, ST_MISS, ST_DUP, ST_LEAF_MISS, ST_COLLAPSE,
+                                           ST_RECYCLED, ST_H2, ST_H3, ST_H4, ST_H5, ST_CAND,
+                                           ST_PROBES};
+    for (uint32_t i = 0; i < n && i < (uint32_t)VX_STAT_COUNT; i++) out[i] = c.st[map[i]];
+    return VX_OK;
+}
+
+int vx_interner_get_ref(vx_interner* h, uint64_t id, uint32_t* ref) {
+    if (!h || !ref || bid_index(id) >= h->capacity) return VX_E_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(ref, h->refs + bid_index(id), 4, hipMemcpyDeviceToHost));
+    return VX_OK;
+}
+
+int vx_interner_get_value(vx_interner* h, uint64_t id, uint64_t* value) {
+    if (!h || !value || bid_index(id) >= h->capacity) return VX_E_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *value = 0;
+    HIPCHK(hipMemcpy(value, (char*)h->values + (size_t)bid_index(id) * h->vsize, h->vsize,
+                     hipMemcpyDeviceToHost));
+    return VX_OK;
+}
+
+int vx_interner_get_children(vx_interner* h, uint64_t id, uint64_t children[8]) {
+    if (!h || !children || bid_index(id) >= h->capacity) return VX_E_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(children, h->children + 8ull * bid_index(id), 64, hipMemcpyDeviceToHost));
+    return VX_OK;
+}
+
+int vx_trees_apply_batches(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
+                           int32_t* status) {
+    if (!h || (n && (!roots || !batches || !status))) return VX_E_INVALID_ARG;
+    if (h->poisoned) return VX_E_POISONED;
+    if (n == 0) return VX_OK;
+    if (n > 0x7FFFFFFFull) return VX_E_INVALID_ARG;
+    return h->vsize == 1 ? apply_impl<uint8_t>(h, n, roots, batches, status)
+                         : apply_impl<uint16_t>(h, n, roots, batches, status);
+}
+
+int vx_tree_apply_batch_host(vx_interner* h, uint64_t* root, const vx_batch_view* batch,
+                             int* changed) {
+    if (!h || !root || !batch || !changed) return VX_E_INVALID_ARG;
+    uint32_t d = batch->depth;
+    if (d > kMaxDepth) return VX_E_DEPTH;
+    uint64_t P = d > 0 ? (1ull << (3 * (d - 1))) : 1ull;
+    size_t mbytes = (batch->has_patches && batch->masks) ? 2 * P : 0;
+    size_t vbytes = (batch->has_patches && batch->values) ? 8 * P * h->vsize : 0;
+    size_t moff = 64, voff = 64 + ((mbytes + 255) / 256) * 256;
+    if (!h->stage.ensure(voff + vbytes + 64)) return VX_E_HIP;
+    char* base = (char*)h->stage.ptr;
+    uint64_t* droot = (uint64_t*)base;
+    int32_t* dstatus = (int32_t*)(base + 8);
+    uint8_t* dm = (uint8_t*)(base + moff);
+    void* dv = base + voff;
+    HIPCHK(hipMemcpyAsync(droot, root, 8, hipMemcpyHostToDevice, h->stream));
+    if (mbytes) HIPCHK(hipMemcpyAsync(dm, batch->masks, mbytes, hipMemcpyHostToDevice, h->stream));
+    if (vbytes) HIPCHK(hipMemcpyAsync(dv, batch->values, vbytes, hipMemcpyHostToDevice, h->stream));
+    vx_batch_view v = *batch;
+    v.masks = mbytes ? dm : nullptr;
+    v.values = vbytes ? dv : nullptr;
+    int rc = vx_trees_apply_batches(h, 1, droot, &v, dstatus);
+    if (rc) return rc;
+    int32_t st = 0;
+    HIPCHK(hipMemcpyAsync(root, droot, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&st, dstatus, 4, hipMemcpyDeviceToHost, h->stream));
+    rc = vx_interner_sync(h);
+    if (rc) return rc;
+    if (st < 0) {
+        *changed = 0;
+        return st;
+    }
+    *changed = st;
+    return VX_OK;
+}
+
+int vx_trees_get(vx_interner* h, size_t ntrees, const uint64_t* roots, const uint8_t* depths,
+                 size_t nq, const uint32_t* chunk_of, const int32_t* xyz, void* out,
+                 uint8_t* present) {
+    if (!h || (nq && (!roots || !depths || !chunk_of || !xyz || !out))) return VX_E_INVALID_ARG;
+    if (nq == 0) return VX_OK;
+    return h->vsize == 1
+               ? get_impl<uint8_t>(h, nq, ntrees, roots, depths, chunk_of, xyz, out, present)
+               : get_impl<uint16_t>(h, nq, ntrees, roots, depths, chunk_of, xyz, out, present);
+}
+
+int vx_trees_fill(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* values) {
+    if (!h || (n && (!roots || !values))) return VX_E_INVALID_ARG;
+    if (h->poisoned) return VX_E_POISONED;
+    if (n == 0) return VX_OK;
+    return h->vsize == 1 ? fill_impl<uint8_t>(h, n, roots, values)
+                         : fill_impl<uint16_t>(h, n, roots, values);
+}
+
+int vx_trees_canonical_hash(vx_interner* h, size_t n, const uint64_t* roots, uint64_t* out) {
+    if (!h || (n && (!roots || !out))) return VX_E_INVALID_ARG;
+    if (n == 0) return VX_OK;
+    return h->vsize == 1 ? canon_impl<uint8_t>(h, n, roots, out)
+                         : canon_impl<uint16_t>(h, n, roots, out);
+}
+
+int vx_device_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return VX_E_INVALID_ARG;
+    return hipMalloc(ptr, bytes ? bytes : 1) == hipSuccess ? VX_OK : VX_E_HIP;
+}
+int vx_device_free(void* ptr) { return hipFree(ptr) == hipSuccess ? VX_OK : VX_E_HIP; }
+int vx_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? VX_OK : VX_E_HIP;
+}
+int vx_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+// vx_device.h -- device-side types and helpers of the MI355X SVO-DAG edit path.
+//
+// Node ids keep the reference's 64-bit BlockId layout (core/block_id.rs:18-30):
+//   bit 63 leaf | 62..55 types | 54..47 mask | 46..32 generation | 31..0 index.
+// The interner is a set of node columns in HBM indexed by the id's index field,
+// a branch pattern table (open addressing, 8-byte slots, atomicCAS insertion) and
+// a direct-mapped leaf table indexed by the voxel value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vx {
+
+constexpr uint64_t kEmpty = 0ull;          // BlockId::EMPTY (block_id.rs:123)
+constexpr uint64_t kInvalid = ~0ull;       // BlockId::INVALID (block_id.rs:120)
+constexpr uint32_t kMaxGeneration = 0x7FFEu;  // BlockId::MAX_GENERATION (block_id.rs:130)
+constexpr int kMaxDepth = 7;               // 6 is the reference's limit; 7 is the extension
+constexpr int kLevels = kMaxDepth + 1;
+
+// A level result that still points at the owner candidate of a key inserted in
+// the same launch: generation field 0x7FFF (never a valid generation), payload =
+// candidate index in that level's candidate list.
+constexpr uint64_t kPendingHi = 0x7FFFull;
+__host__ __device__ inline bool res_is_pending(uint64_t r) { return (r >> 32) == kPendingHi; }
+__host__ __device__ inline uint64_t res_pending(uint32_t cand) { return (kPendingHi << 32) | cand; }
+// Phase-1 marker for "the fill leaf of this chunk" before its id is known.
+constexpr uint64_t kFillMark = (1ull << 63) | (0x7FFFull << 32);
+
+// Table slots: 0 = empty, 1 = tombstone, else tag(31) | pending(1) | payload(32).
+constexpr uint64_t kSlotEmpty = 0ull;
+constexpr uint64_t kSlotTomb = 1ull;
+
+__host__ __device__ inline bool bid_is_leaf(uint64_t id) { return (id >> 63) != 0; }
+__host__ __device__ inline uint32_t bid_index(uint64_t id) { return (uint32_t)id; }
+__host__ __device__ inline uint32_t bid_gen(uint64_t id) { return (uint32_t)(id >> 32) & 0x7FFFu; }
+__host__ __device__ inline uint32_t bid_tm(uint64_t id) { return (uint32_t)(id >> 47) & 0xFFFFu; }
+__host__ __device__ inline uint32_t bid_types(uint64_t id) { return (uint32_t)(id >> 55) & 0xFFu; }
+__host__ __device__ inline uint32_t bid_mask(uint64_t id) { return (uint32_t)(id >> 47) & 0xFFu; }
+__host__ __device__ inline uint64_t make_leaf(uint32_t idx, uint32_t gen) {
+    return (1ull << 63) | ((uint64_t)gen << 32) | idx;
+}
+// tm = (types << 8) | mask lands on bits 62..47
+__host__ __device__ inline uint64_t make_branch(uint32_t idx, uint32_t gen, uint32_t tm) {
+    return ((uint64_t)tm << 47) | ((uint64_t)gen << 32) | idx;
+}
+
+// splitmix64 finalizer (Steele, Lea & Flood 2014; constants from Vigna's splitmix64.c)
+__host__ __device__ inline uint64_t mix64(uint64_t x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+// Hash of a branch key (children ids, types, mask).  Identity is always decided by
+// a full-key compare; the hash only places the key in the table.
+__host__ __device__ inline uint64_t branch_hash(const uint64_t ch[8], uint32_t tm) {
+    uint64_t a = mix64(ch[0] ^ 0x9E3779B97F4A7C15ull) + mix64(ch[1] ^ 0xC2B2AE3D27D4EB4Full);
+    uint64_t b = mix64(ch[2] ^ 0x165667B19E3779F9ull) + mix64(ch[3] ^ 0xD6E8FEB86659FD93ull);
+    uint64_t c = mix64(ch[4] ^ 0xFF51AFD7ED558CCDull) + mix64(ch[5] ^ 0xC4CEB9FE1A85EC53ull);
+    uint64_t d = mix64(ch[6] ^ 0x9FB21C651E98DF25ull) + mix64(ch[7] ^ 0x2545F4914F6CDD1Dull);
+    return mix64((a ^ (b << 1)) + (c ^ (d >> 1)) + 0x5851F42D4C957F2Dull * (tm + 1));
+}
+
+__host__ __device__ inline uint32_t slot_tag(uint64_t h) {
+    uint32_t t = (uint32_t)(h >> 33);
+    return t ? t : 1u;
+}
+__host__ __device__ inline uint64_t slot_make(uint32_t tag, bool pending, uint32_t payload) {
+    return ((uint64_t)tag << 33) | ((uint64_t)(pending ? 1 : 0) << 32) | payload;
+}
+__host__ __device__ inline uint32_t slot_tag_of(uint64_t s) { return (uint32_t)(s >> 33); }
+__host__ __device__ inline bool slot_pending(uint64_t s) { return ((s >> 32) & 1ull) != 0; }
+__host__ __device__ inline uint32_t slot_payload(uint64_t s) { return (uint32_t)s; }
+
+// Device control block: allocator cursors, live counters, errors, statistics.
+enum : uint32_t { kErrOom = 1u, kErrTableFull = 2u, kErrRange = 4u, kErrPoison = 8u };
+
+struct Ctl {
+    uint32_t free_top;    // entries on the free stack
+    uint32_t next_index;  // bump cursor (interner/macros.rs:16-34)
+    uint32_t free_top0;   // snapshot at the start of a launch sequence
+    uint32_t next0;
+    uint32_t alloc_count;  // allocations in the current sequence
+    uint32_t err;
+    uint32_t n_new_leaves;
+    uint32_t pad0;
+    uint64_t live_leaves;
+    uint64_t live_branches;
+    uint64_t tombstones;
+    uint64_t pad1;
+    uint32_t ncand[kLevels + 1];
+    uint32_t gc_count[kLevels + 2];
+    uint64_t st[16];
+};
+
+// Per-tree record of one apply launch sequence (trees sorted by depth on the host).
+struct ChunkDev {
+    const uint8_t* masks;
+    const void* values;
+    uint64_t fill;
+    uint64_t root_in;
+    uint64_t init;  // the node Phase 1/2 descend from (voxtree.rs:742-754)
+    uint32_t orig;  // index in the caller's arrays
+    int32_t status;
+    uint8_t depth;
+    uint8_t has_fill;
+    uint8_t has_patches;
+    uint8_t pad[5];
+};
+
+// Slot space of one level: groups of trees of equal depth, each tree owning
+// 8^(depth-h) consecutive slots at height h (h = 1: bricks, h = depth: root).
+struct LevelInfo {
+    uint64_t slot_base[kLevels + 1];
+    uint32_t chunk_base[kLevels];
+    uint32_t depth[kLevels];
+    uint32_t ngroups;
+    uint32_t h;
+    uint64_t nslots;
+};
+
+template <typename T>
+struct Pool {
+    uint64_t* children;  // capacity * 8
+    uint16_t* tms;       // (types << 8) | mask of branch nodes, 0 for leaves
+    T* values;
+    uint32_t* refs;
+    uint16_t* gens;
+    uint32_t* free_stack;
+    uint64_t* tab;
+    uint64_t tmask;
+    uint32_t* leaf_map;  // value -> node index (0 = none)
+    uint8_t* leaf_need;
+    uint32_t* new_leaves;
+    Ctl* ctl;
+    uint32_t capacity;
+};
+
+// Candidate list of one level (compacted, order of discovery).
+struct Cands {
+    uint64_t* key;    // 9 words per candidate: children[8], tm
+    uint64_t* hash;
+    uint64_t* id;     // owner: the new node's id
+    uint64_t* tslot;  // owner: its table slot; ~0 otherwise
+    uint64_t* slot;   // the level slot this candidate resolves
+};
+
+}  // namespace vx

@@ -1,0 +1,1019 @@
+// vx_kernels.hip -- gfx950 kernels of the batched SVO-DAG edit path.
+//
+// apply_batch (spatial/voxtree.rs:724-1118) is rebuilt as a height-synchronous
+// sweep over dense per-level slot arrays: every tree contributes 8^(D-h) slots at
+// height h, a slot holds the node that replaces the old one at that position
+// (INVALID = untouched), and each height is built from the 8 contiguous slots of
+// the height below.  Interning (interner/mod.rs:627-829) is a two-kernel step per
+// height: build writes candidate keys, probe inserts/looks them up in an HBM
+// open-addressing table with atomicCAS; keys of the same launch resolve to one
+// owner without any waiting, and commit writes the owners' nodes.
+//
+// Reference counting follows the reference's net effect: a new node adds one
+// reference to each non-empty child slot, a tree owns one reference on its root,
+// and the reference's counting quirks are applied as explicit deltas (see
+// apply_drift_* below).  Frees run as a level-parallel worklist after the sweep
+// (dec_ref_recursive, interner/mod.rs:419-534).
+#include <hip/hip_runtime.h>
+
+#include "vx_device.h"
+
+namespace vx {
+
+#define VX_LOOP(i, n)                                                                        \
+    for (uint64_t i##_b = (uint64_t)blockIdx.x * blockDim.x; i##_b < (uint64_t)(n);          \
+         i##_b += (uint64_t)gridDim.x * blockDim.x)                                          \
+        for (uint64_t i = i##_b + threadIdx.x, i##_k = 0; i##_k == 0; i##_k = 1)
+
+enum Stat {
+    ST_HIT = 0, ST_MISS, ST_DUP, ST_LEAF_MISS, ST_COLLAPSE, ST_RECYCLED, ST_H2, ST_H3, ST_H4, ST_H5,
+    ST_CAND, ST_PROBES
+};
+
+__device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
+    uint64_t b = __ballot(pred);
+    if (b == 0) return 0;
+    uint32_t lane = __lane_id();
+    int leader = __ffsll((unsigned long long)b) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(b));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+}
+
+__device__ inline void wave_count(uint64_t* st, bool pred) {
+    uint64_t b = __ballot(pred);
+    if (b == 0) return;
+    int leader = __ffsll((unsigned long long)b) - 1;
+    if ((int)__lane_id() == leader) atomicAdd((unsigned long long*)st, (unsigned long long)__popcll(b));
+}
+
+template <typename T>
+__device__ inline void load8(const T* v, uint32_t out[8]) {
+    if constexpr (sizeof(T) == 1) {
+        uint64_t w = *(const uint64_t*)v;
+#pragma unroll
+        for (int i = 0; i < 8; i++) out[i] = (uint32_t)(w >> (8 * i)) & 0xFFu;
+    } else if constexpr (sizeof(T) == 2) {
+        uint4 w = *(const uint4*)v;
+        uint32_t a[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            out[2 * i] = a[i] & 0xFFFFu;
+            out[2 * i + 1] = a[i] >> 16;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++) out[i] = (uint32_t)v[i];
+    }
+}
+
+// core/voxel.rs:96-141 calc_average: mode; ties keep the first seen unless the
+// leader is the default value and the challenger is not.
+__device__ inline uint32_t calc_average(const uint32_t v[8]) {
+    uint32_t vals[8], cnt[8];
+    uint32_t uniq = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        uint32_t c = v[k];
+        uint32_t i = 0;
+        bool found = false;
+        for (; i < uniq; i++) {
+            if (vals[i] == c) {
+                cnt[i]++;
+                found = true;
+                break;
+            }
+        }
+        if (!found) {
+            vals[uniq] = c;
+            cnt[uniq] = 1;
+            uniq++;
+        }
+    }
+    uint32_t mi = 0, mc = cnt[0];
+    for (uint32_t i = 1; i < uniq; i++) {
+        if (cnt[i] > mc || (cnt[i] == mc && vals[mi] == 0 && vals[i] != 0)) {
+            mc = cnt[i];
+            mi = i;
+        }
+    }
+    return vals[mi];
+}
+
+template <typename T>
+__device__ inline uint64_t leaf_id_of(const Pool<T>& P, uint32_t v) {
+    uint32_t idx = P.leaf_map[v];
+    return make_leaf(idx, P.gens[idx]);
+}
+
+template <typename T>
+__device__ inline uint32_t alloc_node(const Pool<T>& P, bool* ok) {
+    uint32_t k = atomicAdd(&P.ctl->alloc_count, 1u);
+    uint32_t ft0 = P.ctl->free_top0;
+    if (k < ft0) {
+        *ok = true;
+        return P.free_stack[ft0 - 1 - k];
+    }
+    uint64_t b = (uint64_t)P.ctl->next0 + (uint64_t)(k - ft0);
+    if (b >= P.capacity) {
+        atomicOr(&P.ctl->err, kErrOom | kErrPoison);
+        *ok = false;
+        return 0;
+    }
+    *ok = true;
+    return (uint32_t)b;
+}
+
+__device__ inline uint32_t group_of(const LevelInfo& L, uint64_t slot) {
+    uint32_t g = 0;
+    while (g + 1 < L.ngroups && slot >= L.slot_base[g + 1]) g++;
+    return g;
+}
+
+// ------------------------------------------------------------------ launch setup
+
+__global__ void k_begin(Ctl* ctl) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->free_top0 = ctl->free_top;
+        ctl->next0 = ctl->next_index;
+        ctl->alloc_count = 0;
+        ctl->n_new_leaves = 0;
+        for (int i = 0; i <= kLevels; i++) ctl->ncand[i] = 0;
+        for (int i = 0; i < kLevels + 2; i++) ctl->gc_count[i] = 0;
+    }
+}
+
+__global__ void k_end_alloc(Ctl* ctl) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t n = ctl->alloc_count;
+        uint32_t used_free = n < ctl->free_top0 ? n : ctl->free_top0;
+        ctl->free_top = ctl->free_top0 - used_free;
+        uint64_t nx = (uint64_t)ctl->next0 + (n - used_free);
+        ctl->next_index = (uint32_t)(nx > 0xFFFFFFFFull ? 0xFFFFFFFFull : nx);
+        ctl->free_top0 = ctl->free_top;
+        ctl->next0 = ctl->next_index;
+        ctl->alloc_count = 0;
+    }
+}
+
+// Read roots, mark the fill leaf as needed (Phase 0, voxtree.rs:742-754).
+template <typename T>
+__global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_t* roots) {
+    VX_LOOP(c, n) {
+        if (c < n) {
+            ChunkDev& C = chunks[c];
+            C.root_in = roots[C.orig];
+            C.init = C.root_in;
+            if (C.status >= 0 && C.has_fill) {
+                uint32_t v = (uint32_t)C.fill;
+                if (P.leaf_map[v] == 0) P.leaf_need[v] = 1;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Phase 1 scan
+
+// Per brick: descend the old tree (voxtree.rs:792-822), detect the leaf-brick
+// panic (H1), and mark the voxel values whose leaves get_or_create_leaf will need.
+// Leaves the Phase-1 base node (E or split leaf) in the brick's slot.
+template <typename T>
+__global__ void k_brick_scan(Pool<T> P, const ChunkDev* chunks, LevelInfo L, uint64_t* res1) {
+    VX_LOOP(slot, L.nslots) {
+        if (slot < L.nslots) {
+            uint32_t g = group_of(L, slot);
+            uint64_t local = slot - L.slot_base[g];
+            uint32_t D = L.depth[g];
+            uint32_t sh = 3 * (D - 1);
+            uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
+            uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
+            const ChunkDev& C = chunks[c];
+            uint64_t out = kInvalid;
+            uint32_t m = 0;
+            if (C.status >= 0 && C.has_patches) m = C.masks[2ull * p];
+            if (m != 0) {
+                uint32_t v[8];
+                load8<T>((const T*)C.values + 8ull * p, v);
+                bool all_same = m == 0xFFu;
+#pragma unroll
+                for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
+                if (all_same) {
+                    if (P.leaf_map[v[0]] == 0) P.leaf_need[v[0]] = 1;
+                    out = kEmpty;
+                } else {
+                    uint64_t E = kEmpty, Ls = kEmpty;
+                    bool fill_split = false;
+                    if (C.has_fill) {
+                        fill_split = true;
+                    } else {
+                        uint64_t cur = C.root_in;
+                        for (uint32_t cd = 0; cd + 1 < D; cd++) {
+                            if (!bid_is_leaf(cur)) {
+                                uint32_t digit = (p >> (3 * (D - 2 - cd))) & 7u;
+                                if (bid_index(cur) >= P.capacity) {
+                                    atomicOr(&P.ctl->err, kErrPoison);
+                                    cur = kEmpty;
+                                    break;
+                                }
+                                cur = P.children[8ull * bid_index(cur) + digit];
+                            } else {
+                                Ls = cur;
+                                cur = kEmpty;
+                            }
+                            if (cur == kEmpty) break;
+                        }
+                        E = cur;
+                    }
+                    if (E != kEmpty && bid_is_leaf(E)) {
+                        // BlockId::types() on a leaf panics (block_id.rs:273-276)
+                        ((ChunkDev*)chunks)[c].status = -5;
+                    } else {
+                        uint32_t ev[8];
+                        uint32_t have = 0;
+                        if (E != kEmpty) {
+                            for (int i = 0; i < 8; i++) {
+                                uint64_t ch = P.children[8ull * bid_index(E) + i];
+                                if (ch != kEmpty) {
+                                    have |= 1u << i;
+                                    ev[i] = (uint32_t)P.values[bid_index(ch)];
+                                }
+                            }
+                        } else if (fill_split || Ls != kEmpty) {
+                            uint32_t lv = fill_split ? (uint32_t)C.fill : (uint32_t)P.values[bid_index(Ls)];
+                            have = 0xFFu;
+                            for (int i = 0; i < 8; i++) ev[i] = lv;
+                        }
+                        for (int i = 0; i < 8; i++) {
+                            if (!(m & (1u << i))) continue;
+                            if ((have & (1u << i)) && ev[i] == v[i]) continue;
+                            if (P.leaf_map[v[i]] == 0) P.leaf_need[v[i]] = 1;
+                        }
+                        out = fill_split ? kFillMark : (E != kEmpty ? E : Ls);
+                    }
+                }
+            }
+            res1[slot] = out;
+        }
+    }
+}
+
+// One block: create the leaves marked as needed (interner/mod.rs:669-708), then
+// fix each tree's Phase-1 start node.  The fill leaf of a batch that also has
+// patches keeps its reference (H3, voxtree.rs:742-754).
+template <typename T>
+__global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t nvalues) {
+    __shared__ uint32_t created;
+    if (threadIdx.x == 0) created = 0;
+    __syncthreads();
+    for (uint32_t v = threadIdx.x; v < nvalues; v += blockDim.x) {
+        if (P.leaf_need[v]) {
+            P.leaf_need[v] = 0;
+            if (P.leaf_map[v] == 0) {
+                bool ok;
+                uint32_t idx = alloc_node(P, &ok);
+                if (ok) {
+                    P.values[idx] = (T)v;
+                    P.tms[idx] = 0;
+                    P.refs[idx] = 0;
+                    P.leaf_map[v] = idx;
+                    uint32_t k = atomicAdd(&P.ctl->n_new_leaves, 1u);
+                    P.new_leaves[k] = idx;
+                    atomicAdd(&created, 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)created);
+        atomicAdd((unsigned long long*)&P.ctl->st[ST_LEAF_MISS], (unsigned long long)created);
+    }
+    for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
+        ChunkDev& C = chunks[c];
+        if (C.status < 0 || !C.has_fill) continue;
+        uint32_t idx = P.leaf_map[(uint32_t)C.fill];
+        if (idx == 0) continue;  // allocation failed (OOM flagged)
+        C.init = make_leaf(idx, P.gens[idx]);
+        if (C.has_patches) {
+            atomicAdd(&P.refs[idx], 1u);
+            atomicAdd((unsigned long long*)&P.ctl->st[ST_H3], 1ull);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Phase 1 build
+
+template <typename T>
+__device__ inline void emit_candidate(const Cands& K, uint32_t* counter, bool pred,
+                                      const uint64_t ch[8], uint32_t tm, uint64_t slot,
+                                      uint64_t* st) {
+    uint32_t k = wave_append(counter, pred);
+    if (pred) {
+        uint64_t* dst = K.key + 9ull * k;
+#pragma unroll
+        for (int i = 0; i < 8; i++) dst[i] = ch[i];
+        dst[8] = tm;
+        K.hash[k] = branch_hash(ch, tm);
+        K.slot[k] = slot;
+    }
+    wave_count(st, pred);
+}
+
+// Per brick (voxtree.rs:824-896): the all-same brick becomes a leaf; otherwise
+// the brick's children are built from the old node, a split leaf, or nothing, and
+// the changed slots get leaves.  Branch keys go to the level-1 candidate list.
+template <typename T>
+__global__ void k_brick_build(Pool<T> P, const ChunkDev* chunks, LevelInfo L, uint64_t* res1,
+                              Cands K, uint32_t* ncand) {
+    VX_LOOP(slot, L.nslots) {
+        bool valid = slot < L.nslots;
+        bool cand = false;
+        uint64_t ch[8];
+        uint32_t tm = 0;
+        if (valid) {
+            uint64_t base = res1[slot];
+            if (base != kInvalid) {
+                uint32_t g = group_of(L, slot);
+                uint64_t local = slot - L.slot_base[g];
+                uint32_t D = L.depth[g];
+                uint32_t sh = 3 * (D - 1);
+                uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
+                uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
+                const ChunkDev& C = chunks[c];
+                uint64_t out = kInvalid;
+                if (C.status >= 0) {
+                    uint32_t m = C.masks[2ull * p];
+                    uint32_t v[8];
+                    load8<T>((const T*)C.values + 8ull * p, v);
+                    bool all_same = m == 0xFFu;
+#pragma unroll
+                    for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
+                    if (all_same) {
+                        out = leaf_id_of(P, v[0]);
+                        atomicAdd((unsigned long long*)&P.ctl->st[ST_COLLAPSE], 1ull);
+                    } else {
+                        if (base == kFillMark) base = C.init;
+                        bool split = bid_is_leaf(base);
+                        uint32_t types = 0, mask = 0;
+                        if (split) {
+                            for (int i = 0; i < 8; i++) ch[i] = base;
+                            types = mask = 0xFFu;
+                        } else if (base != kEmpty) {
+                            for (int i = 0; i < 8; i++) ch[i] = P.children[8ull * bid_index(base) + i];
+                            types = bid_types(base);
+                            mask = bid_mask(base);
+                        } else {
+                            for (int i = 0; i < 8; i++) ch[i] = kEmpty;
+                        }
+                        uint32_t modified = 0;
+                        for (int i = 0; i < 8; i++) {
+                            if (!(m & (1u << i))) continue;
+                            if (ch[i] != kEmpty && (uint32_t)P.values[bid_index(ch[i])] == v[i]) continue;
+                            ch[i] = leaf_id_of(P, v[i]);
+                            types |= 1u << i;
+                            mask |= 1u << i;
+                            modified |= 1u << i;
+                        }
+                        uint32_t pcm = (uint32_t)__popc(m), pcmod = (uint32_t)__popc(modified);
+                        if (modified == 0) {
+                            // no change: not pushed; the split leaf keeps 8-pc(s) refs (H3)
+                            if (split) {
+                                atomicAdd(&P.refs[bid_index(base)], 8u - pcm);
+                                atomicAdd((unsigned long long*)&P.ctl->st[ST_H3], 1ull);
+                            }
+                        } else {
+                            if (split && pcmod != pcm) {
+                                // slots equal to the split leaf were kept without a ref (H4)
+                                atomicAdd(&P.refs[bid_index(base)], pcmod - pcm);
+                                atomicAdd((unsigned long long*)&P.ctl->st[ST_H4], 1ull);
+                            }
+                            bool uniform = types == 0xFFu;
+                            for (int i = 1; i < 8; i++) uniform = uniform && ch[i] == ch[0];
+                            if (uniform) atomicAdd((unsigned long long*)&P.ctl->st[ST_H2], 1ull);
+                            tm = (types << 8) | mask;
+                            cand = true;
+                            out = res_pending(0xFFFFFFFFu);  // replaced by the probe
+                        }
+                    }
+                }
+                res1[slot] = out;
+            }
+        }
+        emit_candidate<T>(K, ncand, cand, ch, tm, slot, &P.ctl->st[ST_CAND]);
+    }
+}
+
+// ------------------------------------------------------------------ interning
+
+template <typename T>
+__device__ inline bool key_matches_node(const Pool<T>& P, uint32_t idx, const uint64_t key[9]) {
+    if ((uint32_t)P.tms[idx] != (uint32_t)key[8]) return false;
+    const uint64_t* c = P.children + 8ull * idx;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        if (c[i] != key[i]) return false;
+    return true;
+}
+
+__device__ inline bool key_matches_cand(const Cands& K, uint32_t k2, const uint64_t key[9]) {
+    const uint64_t* c = K.key + 9ull * k2;
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+        if (c[i] != key[i]) return false;
+    return true;
+}
+
+// get_or_create_branch (interner/mod.rs:716-829) for one launch's candidates:
+// committed keys resolve to the existing node, the first inserter of a new key
+// owns it (allocates its index), later equal keys point at the owner.
+template <typename T>
+__global__ void k_probe(Pool<T> P, Cands K, const uint32_t* ncand, uint64_t* res, uint64_t bound) {
+    uint32_t n = *ncand;
+    VX_LOOP(k, bound) {
+        bool valid = k < n;
+        int kind = -1;  // 0 hit, 1 dup, 2 miss
+        uint32_t probes = 0;
+        if (valid) {
+            uint64_t key[9];
+#pragma unroll
+            for (int i = 0; i < 9; i++) key[i] = K.key[9ull * k + i];
+            uint64_t h = K.hash[k];
+            uint32_t tag = slot_tag(h);
+            uint32_t tm = (uint32_t)key[8];
+            uint64_t s = h & P.tmask;
+            uint64_t first_free = ~0ull;
+            uint64_t result = 0, tslot = ~0ull;
+            for (uint64_t guard = 0; guard <= 4 * (P.tmask + 1); guard++) {
+                uint64_t e = P.tab[s];
+                probes++;
+                bool resolved = false;
+                if (e == kSlotEmpty) {
+                    uint64_t target = first_free != ~0ull ? first_free : s;
+                    uint64_t expected = target == s ? kSlotEmpty : kSlotTomb;
+                    uint64_t desired = slot_make(tag, true, (uint32_t)k);
+                    uint64_t seen = expected;
+                    bool won = __hip_atomic_compare_exchange_strong(
+                        &P.tab[target], &seen, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                        __HIP_MEMORY_SCOPE_AGENT);
+                    if (won) {
+                        bool ok;
+                        uint32_t idx = alloc_node(P, &ok);
+                        if (ok) {
+                            result = make_branch(idx, P.gens[idx], tm);
+                            tslot = target;
+                        } else {
+                            result = kInvalid;
+                        }
+                        kind = 2;
+                        break;
+                    }
+                    e = seen;
+                    s = target;
+                    if (e != kSlotEmpty && e != kSlotTomb && slot_tag_of(e) == tag) {
+                        if (slot_pending(e)) {
+                            if (key_matches_cand(K, slot_payload(e), key)) {
+                                result = res_pending(slot_payload(e));
+                                kind = 1;
+                                resolved = true;
+                            }
+                        } else if (key_matches_node(P, slot_payload(e), key)) {
+                            uint32_t idx = slot_payload(e);
+                            result = make_branch(idx, P.gens[idx], tm);
+                            kind = 0;
+                            resolved = true;
+                        }
+                    }
+                    if (resolved) break;
+                    s = h & P.tmask;
+                    first_free = ~0ull;
+                    continue;
+                }
+                if (e == kSlotTomb) {
+                    if (first_free == ~0ull) first_free = s;
+                } else if (slot_tag_of(e) == tag) {
+                    if (slot_pending(e)) {
+                        if (key_matches_cand(K, slot_payload(e), key)) {
+                            result = res_pending(slot_payload(e));
+                            kind = 1;
+                            break;
+                        }
+                    } else if (key_matches_node(P, slot_payload(e), key)) {
+                        uint32_t idx = slot_payload(e);
+                        result = make_branch(idx, P.gens[idx], tm);
+                        kind = 0;
+                        break;
+                    }
+                }
+                s = (s + 1) & P.tmask;
+            }
+            if (kind < 0) {
+                atomicOr(&P.ctl->err, kErrTableFull | kErrPoison);
+                result = kInvalid;
+            }
+            K.id[k] = result;
+            K.tslot[k] = tslot;
+            res[K.slot[k]] = result;
+        }
+        wave_count(&P.ctl->st[ST_HIT], kind == 0);
+        wave_count(&P.ctl->st[ST_DUP], kind == 1);
+        wave_count(&P.ctl->st[ST_MISS], kind == 2);
+        if (valid) atomicAdd((unsigned long long*)&P.ctl->st[ST_PROBES], (unsigned long long)probes);
+    }
+}
+
+// Owners write their node (children, LOD value, types/mask), publish the table
+// slot, and take one reference on each non-empty child.
+template <typename T>
+__global__ void k_commit(Pool<T> P, Cands K, const uint32_t* ncand, uint64_t bound) {
+    uint32_t n = *ncand;
+    VX_LOOP(k, bound) {
+        bool own = false;
+        if (k < n) {
+            uint64_t ts = K.tslot[k];
+            uint64_t id = K.id[k];
+            if (ts != ~0ull && id != kInvalid) {
+                own = true;
+                uint32_t idx = bid_index(id);
+                uint64_t key[9];
+#pragma unroll
+                for (int i = 0; i < 9; i++) key[i] = K.key[9ull * k + i];
+                uint32_t vals[8];
+                uint64_t* dst = P.children + 8ull * idx;
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    dst[i] = key[i];
+                    vals[i] = (uint32_t)P.values[bid_index(key[i])];
+                }
+                P.tms[idx] = (uint16_t)key[8];
+                P.values[idx] = (T)calc_average(vals);
+                P.tab[ts] = slot_make(slot_tag(K.hash[k]), false, idx);
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    if (key[i] != kEmpty) atomicAdd(&P.refs[bid_index(key[i])], 1u);
+            }
+        }
+        wave_count(&P.ctl->live_branches, own);
+    }
+}
+
+// ------------------------------------------------------------------ Phase 2
+
+__device__ inline uint64_t resolve(uint64_t r, const uint64_t* cand_id_prev) {
+    return res_is_pending(r) ? cand_id_prev[(uint32_t)r] : r;
+}
+
+// One node at height h >= 2 (voxtree.rs:913-1106): gather the touched children
+// from height h-1, clone untouched ones from the old node at this position (or
+// fill them with the split leaf), collapse 8 equal leaves, else emit a key.
+template <typename T>
+__global__ void k_level_build(Pool<T> P, const ChunkDev* chunks, LevelInfo L,
+                              const uint64_t* slot_off_prev, const uint64_t* res_prev,
+                              const uint64_t* cand_id_prev, uint64_t* res, Cands K,
+                              uint32_t* ncand) {
+    VX_LOOP(slot, L.nslots) {
+        bool valid = slot < L.nslots;
+        bool cand = false;
+        uint64_t ch[8];
+        uint32_t tm = 0;
+        if (valid) {
+            uint32_t g = group_of(L, slot);
+            uint64_t local = slot - L.slot_base[g];
+            uint32_t D = L.depth[g];
+            uint32_t sh = 3 * (D - L.h);
+            uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
+            uint32_t q = (uint32_t)(local & ((1ull << sh) - 1));
+            const ChunkDev& C = chunks[c];
+            uint64_t out = kInvalid;
+            if (C.status >= 0 && C.has_patches) {
+                const uint64_t* src = res_prev + slot_off_prev[c] + 8ull * q;
+                uint32_t types = 0, mask = 0;
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    uint64_t r = src[i];
+                    if (r != kInvalid) {
+                        r = resolve(r, cand_id_prev);
+                        ch[i] = r;
+                        mask |= 1u << i;
+                        types |= (bid_is_leaf(r) ? 1u : 0u) << i;
+                    } else {
+                        ch[i] = kEmpty;
+                    }
+                }
+                if (mask != 0) {
+                    // equivalent node (voxtree.rs:930-952)
+                    uint32_t ell = D - L.h;
+                    uint64_t eq = C.init, lq = kEmpty;
+                    for (uint32_t cd = 0; cd < ell; cd++) {
+                        if (bid_is_leaf(eq)) {
+                            lq = eq;
+                            eq = kEmpty;
+                            break;
+                        }
+                        uint32_t digit = (q >> (3 * (ell - 1 - cd))) & 7u;
+                        if (bid_index(eq) >= P.capacity) {
+                            atomicOr(&P.ctl->err, kErrPoison);
+                            eq = kEmpty;
+                            break;
+                        }
+                        eq = P.children[8ull * bid_index(eq) + digit];
+                        if (eq == kEmpty) break;
+                    }
+                    if (bid_is_leaf(eq)) {
+                        lq = eq;
+                        eq = kEmpty;
+                    }
+                    uint32_t cloned = bid_mask(eq) & ~mask & 0xFFu;
+                    if (mask != 0xFFu) {
+                        if (cloned) {
+                            for (int i = 0; i < 8; i++) {
+                                if (!(cloned & (1u << i))) continue;
+                                ch[i] = P.children[8ull * bid_index(eq) + i];
+                                types |= (bid_is_leaf(ch[i]) ? 1u : 0u) << i;
+                                mask |= 1u << i;
+                            }
+                        } else if (lq != kEmpty) {
+                            uint32_t inv = ~mask & 0xFFu;
+                            types |= inv;
+                            mask |= inv;
+                            for (int i = 0; i < 8; i++)
+                                if (inv & (1u << i)) ch[i] = lq;
+                        }
+                    }
+                    bool all_same = types == 0xFFu;
+                    for (int i = 1; i < 8; i++) all_same = all_same && ch[i] == ch[0];
+                    if (all_same) {
+                        out = ch[0];
+                        atomicAdd((unsigned long long*)&P.ctl->st[ST_COLLAPSE], 1ull);
+                        if (cloned) {
+                            // dec_ref_by(children[0], min(pc(cloned),7)) instead of
+                            // 7 - pc(cloned) (voxtree.rs:1066-1072, H5)
+                            uint32_t pc = (uint32_t)__popc(cloned);
+                            atomicAdd(&P.refs[bid_index(ch[0])], 7u - 2u * pc);
+                            atomicAdd((unsigned long long*)&P.ctl->st[ST_H5], 1ull);
+                        }
+                    } else {
+                        tm = (types << 8) | mask;
+                        cand = true;
+                        out = res_pending(0xFFFFFFFFu);
+                    }
+                }
+            }
+            res[slot] = out;
+        }
+        emit_candidate<T>(K, ncand, cand, ch, tm, slot, &P.ctl->st[ST_CAND]);
+    }
+}
+
+// ------------------------------------------------------------------ finalize + GC
+
+// voxtree.rs:303-328: adopt the new root, take the tree's reference on it, and
+// queue the old root for dec_ref_recursive.
+template <typename T>
+__global__ void k_finalize(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_t* slot_off_all,
+                           uint32_t n_stride, uint64_t* const* res_levels,
+                           uint64_t* const* cand_id_levels, uint64_t* roots, int32_t* status,
+                           uint32_t* gc_front) {
+    VX_LOOP(c, n) {
+        bool push = false;
+        uint64_t old = kEmpty;
+        if (c < n) {
+            ChunkDev& C = chunks[c];
+            int32_t st = C.status;
+            if (st >= 0) {
+                uint64_t nw;
+                if (!C.has_patches) {
+                    nw = C.init;
+                } else {
+                    uint32_t D = C.depth;
+                    uint64_t r = res_levels[D][slot_off_all[(uint64_t)D * n_stride + c]];
+                    nw = (r != kInvalid && res_is_pending(r)) ? cand_id_levels[D][(uint32_t)r] : r;
+                }
+                old = C.root_in;
+                if (nw == kInvalid) {
+                    st = 0;
+                } else if (old != kEmpty && nw == old) {
+                    st = -6;  // assert_ne!(new_root_id, self.root_id) panics
+                } else {
+                    if (nw != kEmpty) atomicAdd(&P.refs[bid_index(nw)], 1u);
+                    push = old != kEmpty;
+                    roots[C.orig] = nw;
+                    st = 1;
+                }
+            }
+            C.status = st;
+            status[C.orig] = st;
+        }
+        uint32_t k = wave_append(&P.ctl->gc_count[0], push);
+        if (push) gc_front[k] = bid_index(old);
+    }
+}
+
+// Leaves created in this sequence that nothing references (their trees hit an
+// error) are released again so the interner is left as before for those trees.
+template <typename T>
+__global__ void k_leaf_cleanup(Pool<T> P) {
+    uint32_t n = P.ctl->n_new_leaves;
+    VX_LOOP(i, n) {
+        bool freed = false;
+        if (i < n) {
+            uint32_t idx = P.new_leaves[i];
+            if (P.refs[idx] == 0) {
+                freed = true;
+                P.leaf_map[(uint32_t)P.values[idx]] = 0;
+                P.values[idx] = 0;
+                uint32_t g = P.gens[idx] + 1u;
+                P.gens[idx] = (uint16_t)(g >= kMaxGeneration ? 0u : g);
+                uint32_t pos = atomicAdd(&P.ctl->free_top, 1u);
+                if (pos < P.capacity) P.free_stack[pos] = idx;
+                else atomicOr(&P.ctl->err, kErrPoison);
+            }
+        }
+        uint64_t b = __ballot(freed);
+        if (b && __lane_id() == (uint32_t)(__ffsll((unsigned long long)b) - 1))
+            atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)(-(long long)__popcll(b)));
+    }
+}
+
+// One round of dec_ref_recursive over all queued nodes (interner/mod.rs:449-527):
+// decrement; a node that reaches zero drops its pattern, is recycled
+// (mod.rs:566-625), and decrements its children, queueing those that reach zero.
+template <typename T>
+__global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uint32_t* next,
+                           bool decrement_self, uint64_t bound, uint64_t front_cap) {
+    uint32_t n = P.ctl->gc_count[round];
+    if (n > front_cap) n = (uint32_t)front_cap;
+    VX_LOOP(i, bound) {
+        bool freed = false, is_leaf = false;
+        uint32_t idx = 0;
+        if (i < n) {
+            idx = front[i];
+            if (idx != 0 && idx < P.capacity) {
+                uint32_t old = decrement_self ? atomicSub(&P.refs[idx], 1u) : 1u;
+                freed = old == 1u;
+            }
+        }
+        if (freed) {
+            uint32_t tm = P.tms[idx];
+            is_leaf = tm == 0;
+            if (is_leaf) {
+                P.leaf_map[(uint32_t)P.values[idx]] = 0;
+            } else {
+                uint64_t ch[8];
+                uint64_t* src = P.children + 8ull * idx;
+#pragma unroll
+                for (int j = 0; j < 8; j++) ch[j] = src[j];
+                // remove the pattern (mod.rs:518)
+                uint64_t h = branch_hash(ch, tm);
+                uint32_t tag = slot_tag(h);
+                uint64_t want = slot_make(tag, false, idx);
+                uint64_t s = h & P.tmask;
+                for (uint64_t guard = 0; guard <= P.tmask; guard++) {
+                    uint64_t e = P.tab[s];
+                    if (e == want) {
+                        P.tab[s] = kSlotTomb;
+                        break;
+                    }
+                    if (e == kSlotEmpty) break;
+                    s = (s + 1) & P.tmask;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    uint64_t c = ch[j];
+                    src[j] = kEmpty;
+                    if (c == kEmpty) continue;
+                    uint32_t ci = bid_index(c);
+                    uint32_t o = atomicSub(&P.refs[ci], 1u);
+                    if (o == 1u) {
+                        uint32_t pos = atomicAdd(&P.ctl->gc_count[round + 1], 1u);
+                        if (pos < front_cap) next[pos] = ci;
+                        else atomicOr(&P.ctl->err, kErrPoison);
+                    }
+                }
+                P.tms[idx] = 0;
+            }
+            P.values[idx] = 0;
+            P.refs[idx] = 0;
+            uint32_t g = P.gens[idx] + 1u;
+            P.gens[idx] = (uint16_t)(g >= kMaxGeneration ? 0u : g);
+            uint32_t pos = atomicAdd(&P.ctl->free_top, 1u);
+            if (pos < P.capacity) P.free_stack[pos] = idx;
+            else atomicOr(&P.ctl->err, kErrPoison);
+        }
+        wave_count(&P.ctl->st[ST_RECYCLED], freed);
+        uint64_t bl = __ballot(freed && is_leaf), bb = __ballot(freed && !is_leaf);
+        if (__lane_id() == 0) {
+            if (bl) atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)(-(long long)__popcll(bl)));
+            if (bb) {
+                atomicAdd((unsigned long long*)&P.ctl->live_branches, (unsigned long long)(-(long long)__popcll(bb)));
+                atomicAdd((unsigned long long*)&P.ctl->tombstones, (unsigned long long)__popcll(bb));
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ fill / clear
+
+// VoxOpsBulkWrite::fill/clear (voxtree.rs:263-292): queue the old root for
+// dec_ref_recursive first, then (after GC and leaf creation) install leaf(v).
+template <typename T>
+__global__ void k_fill_prepare(Pool<T> P, uint32_t n, const uint64_t* roots, const uint64_t* vals,
+                               uint32_t* gc_front) {
+    const uint32_t vmask = sizeof(T) == 1 ? 0xFFu : 0xFFFFu;
+    VX_LOOP(c, n) {
+        bool push = false;
+        uint64_t r = kEmpty;
+        if (c < n) {
+            r = roots[c];
+            push = r != kEmpty;
+            uint32_t v = (uint32_t)vals[c] & vmask;
+            if (v != 0) P.leaf_need[v] = 1;
+        }
+        uint32_t k = wave_append(&P.ctl->gc_count[0], push);
+        if (push) gc_front[k] = bid_index(r);
+    }
+}
+
+template <typename T>
+__global__ void k_fill_finish(Pool<T> P, uint32_t n, uint64_t* roots, const uint64_t* vals) {
+    const uint32_t vmask = sizeof(T) == 1 ? 0xFFu : 0xFFFFu;
+    VX_LOOP(c, n) {
+        if (c < n) {
+            uint32_t v = (uint32_t)vals[c] & vmask;
+            if (v == 0) {
+                roots[c] = kEmpty;
+            } else {
+                uint32_t idx = P.leaf_map[v];
+                if (idx != 0) {
+                    atomicAdd(&P.refs[idx], 1u);
+                    roots[c] = make_leaf(idx, P.gens[idx]);
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_leaf_alloc_only(Pool<T> P, uint32_t nvalues) {
+    __shared__ uint32_t created;
+    if (threadIdx.x == 0) created = 0;
+    __syncthreads();
+    for (uint32_t v = threadIdx.x; v < nvalues; v += blockDim.x) {
+        if (P.leaf_need[v]) {
+            P.leaf_need[v] = 0;
+            if (P.leaf_map[v] == 0) {
+                bool ok;
+                uint32_t idx = alloc_node(P, &ok);
+                if (ok) {
+                    P.values[idx] = (T)v;
+                    P.tms[idx] = 0;
+                    P.refs[idx] = 0;
+                    P.leaf_map[v] = idx;
+                    atomicAdd(&created, 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)created);
+        atomicAdd((unsigned long long*)&P.ctl->st[ST_LEAF_MISS], (unsigned long long)created);
+    }
+}
+
+// ------------------------------------------------------------------ reads
+
+// VoxOpsRead::get (voxtree.rs:145-159) -> get_at_depth (utils/common.rs:122-156),
+// one lane per query.
+template <typename T>
+__global__ void k_get(Pool<T> P, uint64_t nq, uint64_t ntrees, const uint64_t* roots,
+                      const uint8_t* depths, const uint32_t* chunk_of, const int32_t* xyz, T* out,
+                      uint8_t* present) {
+    VX_LOOP(i, nq) {
+        if (i < nq) {
+            uint32_t c = chunk_of[i];
+            bool tree_ok = c < ntrees;
+            uint64_t node = tree_ok ? roots[c] : kEmpty;
+            uint32_t D = tree_ok ? depths[c] : 0u;
+            if (D > (uint32_t)kMaxDepth) tree_ok = false;
+            int32_t x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+            int32_t lim = 1 << D;
+            uint32_t v = 0;
+            uint8_t pr = 0;
+            if (!tree_ok || x < 0 || y < 0 || z < 0 || x >= lim || y >= lim || z >= lim) {
+                atomicOr(&P.ctl->err, kErrRange);
+                pr = 0xFF;
+            } else {
+                uint32_t d = 0;
+                while (node != kEmpty) {
+                    uint32_t idx = bid_index(node);
+                    if (idx >= P.capacity) {
+                        atomicOr(&P.ctl->err, kErrPoison);
+                        break;
+                    }
+                    if (d >= D) {
+                        v = (uint32_t)P.values[idx];
+                        pr = v != 0;
+                        break;
+                    }
+                    if (!bid_is_leaf(node)) {
+                        uint32_t sh = D - d - 1;
+                        uint32_t ci = (((uint32_t)x >> sh) & 1u) | ((((uint32_t)y >> sh) & 1u) << 1) |
+                                      ((((uint32_t)z >> sh) & 1u) << 2);
+                        node = P.children[8ull * idx + ci];
+                        d++;
+                    } else {
+                        v = (uint32_t)P.values[idx];
+                        pr = 1;
+                        break;
+                    }
+                }
+                if (!pr) v = 0;
+            }
+            out[i] = (T)v;
+            if (present) present[i] = pr;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ canonical hash
+
+__device__ inline uint64_t canon_leaf(uint32_t v, uint64_t seed) {
+    return mix64(seed ^ mix64((uint64_t)v + 0x4C454146ull));
+}
+__device__ inline uint64_t canon_branch_init(uint32_t v, uint32_t tm, uint64_t seed) {
+    return mix64(seed ^ 0x4252414Eull ^ mix64((uint64_t)v) ^ ((uint64_t)tm << 32));
+}
+__device__ inline uint64_t canon_combine(uint64_t h, uint64_t child, uint32_t i) {
+    return mix64(h ^ (child + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1)));
+}
+
+// Merkle hash of a tree, seeds S0/S1; one lane per tree, iterative DFS.
+template <typename T>
+__global__ void k_canon(Pool<T> P, uint32_t n, const uint64_t* roots, uint64_t* out) {
+    const uint64_t S0 = 0x736F6D6570736575ull, S1 = 0x646F72616E646F6Dull;
+    VX_LOOP(c, n) {
+        if (c < n) {
+            uint64_t root = roots[c];
+            uint64_t r0 = 0, r1 = 0;
+            if (bid_index(root) >= P.capacity) root = kEmpty;
+            if (root != kEmpty) {
+                uint32_t ri = bid_index(root);
+                if (bid_is_leaf(root)) {
+                    r0 = canon_leaf((uint32_t)P.values[ri], S0);
+                    r1 = canon_leaf((uint32_t)P.values[ri], S1);
+                } else {
+                    uint64_t sid[kLevels + 2], sh0[kLevels + 2], sh1[kLevels + 2];
+                    uint32_t snext[kLevels + 2];
+                    int top = 0;
+                    sid[0] = root;
+                    snext[0] = 0;
+                    sh0[0] = canon_branch_init((uint32_t)P.values[ri], bid_tm(root), S0);
+                    sh1[0] = canon_branch_init((uint32_t)P.values[ri], bid_tm(root), S1);
+                    while (top >= 0) {
+                        if (snext[top] == 8) {
+                            uint64_t h0 = sh0[top], h1 = sh1[top];
+                            top--;
+                            if (top < 0) {
+                                r0 = h0;
+                                r1 = h1;
+                                break;
+                            }
+                            sh0[top] = canon_combine(sh0[top], h0, snext[top]);
+                            sh1[top] = canon_combine(sh1[top], h1, snext[top]);
+                            snext[top]++;
+                            continue;
+                        }
+                        uint64_t cid = P.children[8ull * bid_index(sid[top]) + snext[top]];
+                        if (bid_index(cid) >= P.capacity) cid = kEmpty;
+                        if (cid == kEmpty) {
+                            sh0[top] = canon_combine(sh0[top], 0, snext[top]);
+                            sh1[top] = canon_combine(sh1[top], 0, snext[top]);
+                            snext[top]++;
+                        } else if (bid_is_leaf(cid)) {
+                            uint32_t lv = (uint32_t)P.values[bid_index(cid)];
+                            sh0[top] = canon_combine(sh0[top], canon_leaf(lv, S0), snext[top]);
+                            sh1[top] = canon_combine(sh1[top], canon_leaf(lv, S1), snext[top]);
+                            snext[top]++;
+                        } else if (top + 1 < kLevels + 2) {
+                            top++;
+                            uint32_t vi = (uint32_t)P.values[bid_index(cid)];
+                            sid[top] = cid;
+                            snext[top] = 0;
+                            sh0[top] = canon_branch_init(vi, bid_tm(cid), S0);
+                            sh1[top] = canon_branch_init(vi, bid_tm(cid), S1);
+                        } else {
+                            snext[top]++;  // deeper than any valid tree: malformed, skip
+                        }
+                    }
+                }
+            }
+            out[2 * c] = r0;
+            out[2 * c + 1] = r1;
+        }
+    }
+}
+
+// explicit instantiations are produced by the host dispatch in vx_api.hip
+
+}  // namespace vx
