@@ -137,6 +137,16 @@ int vx_trees_fill(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* val
  * out: device, 2*n uint64.  Asynchronous. */
 int vx_trees_canonical_hash(vx_interner* h, size_t n, const uint64_t* roots, uint64_t* out);
 
+/* ---- measurement (no reference counterpart; the reference uses tracy spans) ---- */
+
+/* Time every kernel launch with HIP events on the interner's stream (0 = off). */
+int vx_interner_profile(vx_interner* h, int enable);
+/* Synchronizes; fills, for kernel id k < n, the milliseconds and launch count
+ * accumulated since the previous read, then resets them.  Returns the number of
+ * kernel ids (name of id k: vx_kernel_name(k)). */
+int vx_interner_profile_read(vx_interner* h, double* ms, uint64_t* launches, uint32_t n);
+const char* vx_kernel_name(uint32_t id);
+
 /* ---- device memory helpers for callers without a HIP allocator ---- */
 int vx_device_alloc(size_t bytes, void** ptr);
 int vx_device_free(void* ptr);

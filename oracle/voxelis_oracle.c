@@ -601,6 +601,19 @@ size_t vxo_batch_size(const vxo_batch* b) {
     return n;
 }
 int vxo_batch_has_patches(const vxo_batch* b) { return b->has_patches; }
+
+/* Load dense buffers recorded elsewhere (masks: 2*P bytes, values: 8*P voxels of
+ * vsize bytes) as if just_set had been called for them; has_patches follows. */
+void vxo_batch_load(vxo_batch* b, const uint8_t* masks, const void* values, uint32_t vsize,
+                    int has_patches) {
+    memcpy(b->masks, masks, b->bricks * 2);
+    for (size_t i = 0; i < b->bricks * 8; i++) {
+        uint64_t v = 0;
+        memcpy(&v, (const uint8_t*)values + i * vsize, vsize);
+        b->values[i] = v;
+    }
+    b->has_patches = has_patches;
+}
 size_t vxo_batch_bricks(const vxo_batch* b) { return b->bricks; }
 const uint8_t* vxo_batch_masks(const vxo_batch* b) { return b->masks; }
 const uint64_t* vxo_batch_values(const vxo_batch* b) { return b->values; }

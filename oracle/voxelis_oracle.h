@@ -106,6 +106,8 @@ void vxo_batch_just_fill(vxo_batch* b, uint64_t v);
 void vxo_batch_just_clear(vxo_batch* b);
 size_t vxo_batch_size(const vxo_batch* b);
 int vxo_batch_has_patches(const vxo_batch* b);
+void vxo_batch_load(vxo_batch* b, const uint8_t* masks, const void* values, uint32_t vsize,
+                    int has_patches);
 size_t vxo_batch_bricks(const vxo_batch* b);
 const uint8_t* vxo_batch_masks(const vxo_batch* b);   /* 2*P bytes, (set, clear) */
 const uint64_t* vxo_batch_values(const vxo_batch* b); /* 8*P values */

@@ -77,6 +77,7 @@ def lib():
         "vxo_batch_just_clear": (None, [vp]),
         "vxo_batch_size": (sz, [vp]),
         "vxo_batch_has_patches": (C.c_int, [vp]),
+        "vxo_batch_load": (None, [vp, vp, vp, u32, C.c_int]),
         "vxo_batch_bricks": (sz, [vp]),
         "vxo_batch_masks": (vp, [vp]),
         "vxo_batch_values": (vp, [vp]),
@@ -139,6 +140,12 @@ class Batch:
 
     def just_clear(self):
         lib().vxo_batch_just_clear(self.h)
+
+    def load_dense(self, masks, values):
+        masks = np.ascontiguousarray(masks, dtype=np.uint8)
+        values = np.ascontiguousarray(values)
+        lib().vxo_batch_load(self.h, masks.ctypes.data, values.ctypes.data, values.itemsize,
+                             1 if masks[:, 0].any() or masks[:, 1].any() else 0)
 
     def size(self):
         return lib().vxo_batch_size(self.h)

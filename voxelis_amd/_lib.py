@@ -47,6 +47,9 @@ SIGNATURES = {
     "vx_trees_get": (_int, [_vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
     "vx_trees_fill": (_int, [_vp, _sz, _vp, _vp]),
     "vx_trees_canonical_hash": (_int, [_vp, _sz, _vp, _vp]),
+    "vx_interner_profile": (_int, [_vp, _int]),
+    "vx_interner_profile_read": (_int, [_vp, C.POINTER(C.c_double), C.POINTER(_u64), _u32]),
+    "vx_kernel_name": (C.c_char_p, [_u32]),
     "vx_device_alloc": (_int, [_sz, C.POINTER(_vp)]),
     "vx_device_free": (_int, [_vp]),
     "vx_memcpy_h2d": (_int, [_vp, _vp, _sz]),

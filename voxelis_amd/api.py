@@ -117,6 +117,18 @@ class VoxInterner:
     def sync(self):
         check(lib().vx_interner_sync(self.h), "sync")
 
+    def profile(self, enable=True):
+        check(lib().vx_interner_profile(self.h, 1 if enable else 0), "profile")
+
+    def profile_read(self):
+        """{kernel name: (total ms, launches)} since the previous read (synchronizes)."""
+        ms = (C.c_double * 64)()
+        cnt = (C.c_uint64 * 64)()
+        n = lib().vx_interner_profile_read(self.h, ms, cnt, 64)
+        if n < 0:
+            check(n, "profile_read")
+        return {lib().vx_kernel_name(i).decode(): (ms[i], cnt[i]) for i in range(n) if cnt[i]}
+
 
 class Batch:
     """core/batch.rs Batch<T>: (set, clear) masks and 8 values per 2x2x2 brick."""

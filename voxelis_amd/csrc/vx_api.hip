@@ -62,7 +62,27 @@ struct Grow {
 };
 
 struct LevelBufs {
-    Grow res, key, hash, id, tslot, slot;
+    Grow res, key, hash, id, tslot, slot, tiles;
+};
+
+// Kernel ids of the per-kernel timer (vx_interner_profile_*).
+enum KernelId : uint32_t {
+    KID_BEGIN = 0, KID_PREPARE, KID_BRICK_SCAN, KID_LEAF_ALLOC, KID_BRICK_BUILD, KID_PROBE_BRICKS,
+    KID_COMMIT_BRICKS, KID_LEVEL_BUILD, KID_PROBE_UPPER, KID_COMMIT_UPPER, KID_FINALIZE,
+    KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_COUNT
+};
+const char* const kKernelNames[KID_COUNT] = {
+    "k_begin", "k_prepare", "k_brick_scan", "k_leaf_alloc", "k_brick_build", "k_probe[bricks]",
+    "k_commit[bricks]", "k_level_build", "k_probe[upper]", "k_commit[upper]", "k_finalize",
+    "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill"};
+
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<uint32_t, size_t>> pending;  // kernel id, index of its start event
+    double ms[KID_COUNT] = {0};
+    uint64_t launches[KID_COUNT] = {0};
 };
 
 }  // namespace
@@ -97,6 +117,7 @@ struct vx_interner {
     int ring_next = 0;
     int ring_cur = 0;
     void* h_pinned = nullptr;  // the slot of the current call
+    Prof prof;
 };
 
 namespace {
@@ -161,6 +182,34 @@ int err_to_status(uint32_t err) {
     } while (0)
 
 uint64_t slots_at(uint32_t d, uint32_t h) { return 1ull << (3 * (d - h)); }
+
+hipEvent_t prof_event(vx_interner* h) {
+    Prof& p = h->prof;
+    if (p.used == p.pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        p.pool.push_back(e);
+    }
+    return p.pool[p.used++];
+}
+
+// Launch a kernel on the interner's stream, bracketed by timing events when on.
+#define VX_LAUNCH(h, kid, kern, grid, block, ...)                                   \
+    do {                                                                            \
+        size_t ev0__ = 0;                                                           \
+        bool prof__ = (h)->prof.on;                                                 \
+        if (prof__) {                                                               \
+            ev0__ = (h)->prof.used;                                                 \
+            hipEvent_t e0__ = prof_event(h);                                        \
+            hipEvent_t e1__ = prof_event(h);                                        \
+            prof__ = e0__ && e1__ && hipEventRecord(e0__, (h)->stream) == hipSuccess; \
+        }                                                                           \
+        kern<<<(grid), (block), 0, (h)->stream>>>(__VA_ARGS__);                     \
+        if (prof__) {                                                               \
+            (void)hipEventRecord((h)->prof.pool[ev0__ + 1], (h)->stream);           \
+            (h)->prof.pending.push_back({(uint32_t)(kid), ev0__});                  \
+        }                                                                           \
+    } while (0)
 
 template <typename T>
 int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
@@ -244,12 +293,17 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
     uint64_t* cid_ptrs_h[kLevels + 1] = {nullptr};
     Cands K[kLevels + 1];
     std::memset(K, 0, sizeof(K));
+    uint32_t* tiles_h[kLevels + 1] = {nullptr};
+    uint64_t ntiles[kLevels + 1] = {0};
     for (uint32_t hh = 1; hh <= dmax; hh++) {
-        uint64_t ns = std::max<uint64_t>(L[hh].nslots, 1);
+        ntiles[hh] = std::max<uint64_t>((L[hh].nslots + kTile - 1) / kTile, 1);
+        uint64_t ns = ntiles[hh] * kTile;
         LevelBufs& B = h->lv[hh];
         if (!B.res.ensure(ns * 8) || !B.key.ensure(ns * 72) || !B.hash.ensure(ns * 8) ||
-            !B.id.ensure(ns * 8) || !B.tslot.ensure(ns * 8) || !B.slot.ensure(ns * 8))
+            !B.id.ensure(ns * 8) || !B.tslot.ensure(ns * 8) || !B.slot.ensure(ns * 8) ||
+            !B.tiles.ensure(ntiles[hh] * 4))
             return VX_E_HIP;
+        tiles_h[hh] = (uint32_t*)B.tiles.ptr;
         K[hh].key = (uint64_t*)B.key.ptr;
         K[hh].hash = (uint64_t*)B.hash.ptr;
         K[hh].id = (uint64_t*)B.id.ptr;
@@ -276,42 +330,42 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
                           hipMemcpyHostToDevice, s));
     pinned_release(h);
     uint32_t nvalues = 1u << (8 * sizeof(T));
-    uint32_t* ncand = &h->ctl->ncand[0];
     // ---- launch sequence
-    k_begin<<<1, 1, 0, s>>>(h->ctl);
-    k_prepare<T><<<grid_for(n), kBlock, 0, s>>>(P, dchunks, (uint32_t)n, roots);
+    VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
+    VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, grid_for(n), kBlock, P, dchunks, (uint32_t)n, roots);
     if (dmax >= 2) {
-        k_brick_scan<T><<<grid_for(L[1].nslots), kBlock, 0, s>>>(P, dchunks, L[1], res_ptrs_h[1]);
+        VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_scan<T>, grid_for(L[1].nslots), kBlock, P, dchunks, L[1],
+                  res_ptrs_h[1]);
     }
-    k_leaf_alloc<T><<<1, 1024, 0, s>>>(P, dchunks, (uint32_t)n, nvalues);
+    VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc<T>, 1, 1024, P, dchunks, (uint32_t)n, nvalues);
     if (dmax >= 2) {
-        k_brick_build<T><<<grid_for(L[1].nslots), kBlock, 0, s>>>(P, dchunks, L[1], res_ptrs_h[1],
-                                                                  K[1], ncand + 1);
-        k_probe<T><<<grid_for(L[1].nslots), kBlock, 0, s>>>(P, K[1], ncand + 1, res_ptrs_h[1],
-                                                            L[1].nslots);
-        k_commit<T><<<grid_for(L[1].nslots), kBlock, 0, s>>>(P, K[1], ncand + 1, L[1].nslots);
+        unsigned t1 = (unsigned)ntiles[1];
+        VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T>, t1, kTile, P, dchunks, L[1], res_ptrs_h[1], K[1],
+                  tiles_h[1]);
+        VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T>, t1, kTile, P, K[1], tiles_h[1], res_ptrs_h[1]);
+        VX_LAUNCH(h, KID_COMMIT_BRICKS, k_commit<T>, t1, kTile, P, K[1], tiles_h[1]);
         for (uint32_t hh = 2; hh <= dmax; hh++) {
-            k_level_build<T><<<grid_for(L[hh].nslots), kBlock, 0, s>>>(
-                P, dchunks, L[hh], doff + (size_t)(hh - 1) * n, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1],
-                res_ptrs_h[hh], K[hh], ncand + hh);
-            k_probe<T><<<grid_for(L[hh].nslots), kBlock, 0, s>>>(P, K[hh], ncand + hh, res_ptrs_h[hh],
-                                                                 L[hh].nslots);
-            k_commit<T><<<grid_for(L[hh].nslots), kBlock, 0, s>>>(P, K[hh], ncand + hh, L[hh].nslots);
+            unsigned th = (unsigned)ntiles[hh];
+            VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T>, th, kTile, P, dchunks, L[hh],
+                      doff + (size_t)(hh - 1) * n, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh],
+                      K[hh], tiles_h[hh]);
+            VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T>, th, kTile, P, K[hh], tiles_h[hh], res_ptrs_h[hh]);
+            VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T>, th, kTile, P, K[hh], tiles_h[hh]);
         }
     }
     uint32_t* gf0 = (uint32_t*)h->gc_front[0].ptr;
     uint32_t* gf1 = (uint32_t*)h->gc_front[1].ptr;
-    k_finalize<T><<<grid_for(n), kBlock, 0, s>>>(P, dchunks, (uint32_t)n, doff, (uint32_t)n, dres, dcid,
-                                                 roots, status, gf0);
-    k_end_alloc<<<1, 1, 0, s>>>(h->ctl);
-    k_leaf_cleanup<T><<<grid_for(nvalues), kBlock, 0, s>>>(P);
+    VX_LAUNCH(h, KID_FINALIZE, k_finalize<T>, grid_for(n), kBlock, P, dchunks, (uint32_t)n, doff,
+              (uint32_t)n, dres, dcid, roots, status, gf0);
+    VX_LAUNCH(h, KID_END_ALLOC, k_end_alloc, 1, 1, h->ctl);
+    VX_LAUNCH(h, KID_LEAF_CLEANUP, k_leaf_cleanup<T>, grid_for(nvalues), kBlock, P);
     // dec_ref_recursive of the replaced roots: depth <= 7 gives at most 8 rounds
     uint64_t gbound = std::min<uint64_t>(gcap, h->capacity + n);
     for (uint32_t r = 0; r < (uint32_t)kLevels + 1; r++) {
         uint32_t* fr = (r & 1) ? gf1 : gf0;
         uint32_t* nx = (r & 1) ? gf0 : gf1;
         uint64_t bound = r == 0 ? n : gbound;
-        k_gc_round<T><<<grid_for(bound), kBlock, 0, s>>>(P, r, fr, nx, r == 0, bound, gcap);
+        VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid_for(bound), kBlock, P, r, fr, nx, r == 0, bound, gcap);
     }
     if (hipGetLastError() != hipSuccess) return VX_E_HIP;
     return VX_OK;
@@ -319,26 +373,25 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
 
 template <typename T>
 int fill_impl(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* vals) {
-    hipStream_t s = h->stream;
     Pool<T> P = pool_of<T>(h);
     size_t gcap = (size_t)h->capacity + n + 16;
     if (!h->gc_front[0].ensure(gcap * 4) || !h->gc_front[1].ensure(gcap * 4)) return VX_E_HIP;
     uint32_t* gf0 = (uint32_t*)h->gc_front[0].ptr;
     uint32_t* gf1 = (uint32_t*)h->gc_front[1].ptr;
     uint32_t nvalues = 1u << (8 * sizeof(T));
-    k_begin<<<1, 1, 0, s>>>(h->ctl);
-    k_fill_prepare<T><<<grid_for(n), kBlock, 0, s>>>(P, (uint32_t)n, roots, vals, gf0);
+    VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
+    VX_LAUNCH(h, KID_FILL, k_fill_prepare<T>, grid_for(n), kBlock, P, (uint32_t)n, roots, vals, gf0);
     uint64_t gbound = std::min<uint64_t>(gcap, h->capacity + n);
     for (uint32_t r = 0; r < (uint32_t)kLevels + 1; r++) {
         uint32_t* fr = (r & 1) ? gf1 : gf0;
         uint32_t* nx = (r & 1) ? gf0 : gf1;
         uint64_t bound = r == 0 ? n : gbound;
-        k_gc_round<T><<<grid_for(bound), kBlock, 0, s>>>(P, r, fr, nx, r == 0, bound, gcap);
+        VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid_for(bound), kBlock, P, r, fr, nx, r == 0, bound, gcap);
     }
-    k_begin<<<1, 1, 0, s>>>(h->ctl);
-    k_leaf_alloc_only<T><<<1, 1024, 0, s>>>(P, nvalues);
-    k_fill_finish<T><<<grid_for(n), kBlock, 0, s>>>(P, (uint32_t)n, roots, vals);
-    k_end_alloc<<<1, 1, 0, s>>>(h->ctl);
+    VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
+    VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc_only<T>, 1, 1024, P, nvalues);
+    VX_LAUNCH(h, KID_FILL, k_fill_finish<T>, grid_for(n), kBlock, P, (uint32_t)n, roots, vals);
+    VX_LAUNCH(h, KID_END_ALLOC, k_end_alloc, 1, 1, h->ctl);
     if (hipGetLastError() != hipSuccess) return VX_E_HIP;
     return VX_OK;
 }
@@ -348,15 +401,15 @@ int get_impl(vx_interner* h, size_t nq, size_t ntrees, const uint64_t* roots,
              const uint8_t* depths, const uint32_t* chunk_of, const int32_t* xyz, void* out,
              uint8_t* present) {
     Pool<T> P = pool_of<T>(h);
-    k_get<T><<<grid_for(nq), kBlock, 0, h->stream>>>(P, nq, ntrees, roots, depths, chunk_of, xyz,
-                                                     (T*)out, present);
+    VX_LAUNCH(h, KID_GET, k_get<T>, grid_for(nq), kBlock, P, nq, ntrees, roots, depths, chunk_of, xyz,
+              (T*)out, present);
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
 }
 
 template <typename T>
 int canon_impl(vx_interner* h, size_t n, const uint64_t* roots, uint64_t* out) {
     Pool<T> P = pool_of<T>(h);
-    k_canon<T><<<grid_for(n), 64, 0, h->stream>>>(P, (uint32_t)n, roots, out);
+    VX_LAUNCH(h, KID_CANON, k_canon<T>, grid_for(n), 64, P, (uint32_t)n, roots, out);
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
 }
 
@@ -460,7 +513,9 @@ int vx_interner_destroy(vx_interner* h) {
         b.id.release();
         b.tslot.release();
         b.slot.release();
+        b.tiles.release();
     }
+    for (hipEvent_t e : h->prof.pool) (void)hipEventDestroy(e);
     for (int i = 0; i < vx_interner::kRing; i++) {
         if (h->h_pinned_ring[i]) (void)hipHostFree(h->h_pinned_ring[i]);
         if (h->ring_ev[i]) (void)hipEventDestroy(h->ring_ev[i]);
@@ -513,7 +568,11 @@ int vx_interner_stats(vx_interner* h, uint64_t* out, uint32_t n) {
     static const int map[VX_STAT_COUNT] = {ST_HIT, ST_MISS, ST_DUP, ST_LEAF_MISS, ST_COLLAPSE,
                                            ST_RECYCLED, ST_H2, ST_H3, ST_H4, ST_H5, ST_CAND,
                                            ST_PROBES};
-    for (uint32_t i = 0; i < n && i < (uint32_t)VX_STAT_COUNT; i++) out[i] = c.st[map[i]];
+    for (uint32_t i = 0; i < n && i < (uint32_t)VX_STAT_COUNT; i++) {
+        uint64_t sum = 0;
+        for (int sidx = 0; sidx < kStatStripes; sidx++) sum += c.st[sidx][map[i]];
+        out[i] = sum;
+    }
     return VX_OK;
 }
 
@@ -610,6 +669,36 @@ int vx_trees_canonical_hash(vx_interner* h, size_t n, const uint64_t* roots, uin
     return h->vsize == 1 ? canon_impl<uint8_t>(h, n, roots, out)
                          : canon_impl<uint16_t>(h, n, roots, out);
 }
+
+int vx_interner_profile(vx_interner* h, int enable) {
+    if (!h) return VX_E_INVALID_ARG;
+    h->prof.on = enable != 0;
+    return VX_OK;
+}
+
+int vx_interner_profile_read(vx_interner* h, double* ms, uint64_t* launches, uint32_t n) {
+    if (!h) return VX_E_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    Prof& p = h->prof;
+    for (auto& pr : p.pending) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, p.pool[pr.second], p.pool[pr.second + 1]) == hipSuccess) {
+            p.ms[pr.first] += t;
+            p.launches[pr.first] += 1;
+        }
+    }
+    p.pending.clear();
+    p.used = 0;
+    for (uint32_t i = 0; i < n && i < (uint32_t)KID_COUNT; i++) {
+        if (ms) ms[i] = p.ms[i];
+        if (launches) launches[i] = p.launches[i];
+        p.ms[i] = 0;
+        p.launches[i] = 0;
+    }
+    return (int)KID_COUNT;
+}
+
+const char* vx_kernel_name(uint32_t id) { return id < (uint32_t)KID_COUNT ? kKernelNames[id] : ""; }
 
 int vx_device_alloc(size_t bytes, void** ptr) {
     if (!ptr) return VX_E_INVALID_ARG;

@@ -78,6 +78,10 @@ __host__ __device__ inline uint32_t slot_payload(uint64_t s) { return (uint32_t)
 
 // Device control block: allocator cursors, live counters, errors, statistics.
 enum : uint32_t { kErrOom = 1u, kErrTableFull = 2u, kErrRange = 4u, kErrPoison = 8u };
+constexpr int kStatStripes = 64;
+// Build/probe/commit kernels work on tiles of kTile level slots (one block each);
+// a tile's candidates are compacted into its own region of the candidate arrays.
+constexpr int kTile = 256;
 
 struct Ctl {
     uint32_t free_top;    // entries on the free stack
@@ -94,7 +98,8 @@ struct Ctl {
     uint64_t pad1;
     uint32_t ncand[kLevels + 1];
     uint32_t gc_count[kLevels + 2];
-    uint64_t st[16];
+    // statistics, striped over 64 cache lines by block index (summed when read)
+    uint64_t st[kStatStripes][16];
 };
 
 // Per-tree record of one apply launch sequence (trees sorted by depth on the host).

@@ -48,6 +48,69 @@ __device__ inline void wave_count(uint64_t* st, bool pred) {
     if ((int)__lane_id() == leader) atomicAdd((unsigned long long*)st, (unsigned long long)__popcll(b));
 }
 
+// this block's stripe of a statistics counter
+__device__ inline uint64_t* stat(Ctl* ctl, int id) { return &ctl->st[blockIdx.x & (kStatStripes - 1)][id]; }
+
+__device__ inline uint64_t shfl64(uint64_t v, int src) {
+    uint32_t lo = __shfl((uint32_t)v, src);
+    uint32_t hi = __shfl((uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ inline void wave_add64(uint64_t* st, uint32_t v) {
+    uint32_t t = wave_sum(v);
+    if (__lane_id() == 0 && t) atomicAdd((unsigned long long*)st, (unsigned long long)t);
+}
+
+constexpr uint32_t kNoIdx = 0xFFFFFFFFu;
+constexpr int kMatchRounds = 4;
+
+// refs[idx] += 1 for every lane with idx != kNoIdx; lanes with equal idx share one
+// atomic (hot leaves are referenced from most new nodes).
+__device__ inline void wave_ref_inc(uint32_t* refs, uint32_t idx) {
+    bool pending = idx != kNoIdx;
+    for (int r = 0; r < kMatchRounds; r++) {
+        uint64_t act = __ballot(pending);
+        if (!act) return;
+        int leader = __ffsll((unsigned long long)act) - 1;
+        uint32_t li = __shfl(idx, leader);
+        bool same = pending && idx == li;
+        uint64_t sm = __ballot(same);
+        if ((int)__lane_id() == leader) atomicAdd(&refs[li], (uint32_t)__popcll(sm));
+        if (same) pending = false;
+    }
+    if (pending) atomicAdd(&refs[idx], 1u);
+}
+
+// refs[idx] -= 1 for every lane with idx != kNoIdx (aggregated like wave_ref_inc).
+// Returns true on exactly one lane per index whose count reached zero.
+__device__ inline bool wave_ref_dec_zero(uint32_t* refs, uint32_t idx) {
+    bool pending = idx != kNoIdx;
+    bool owner = false;
+    for (int r = 0; r < kMatchRounds; r++) {
+        uint64_t act = __ballot(pending);
+        if (!act) return owner;
+        int leader = __ffsll((unsigned long long)act) - 1;
+        uint32_t li = __shfl(idx, leader);
+        bool same = pending && idx == li;
+        uint64_t sm = __ballot(same);
+        if ((int)__lane_id() == leader) {
+            uint32_t k = (uint32_t)__popcll(sm);
+            uint32_t old = atomicSub(&refs[li], k);
+            owner = old == k;
+        }
+        if (same) pending = false;
+    }
+    if (pending) owner = atomicSub(&refs[idx], 1u) == 1u;
+    return owner;
+}
+
 template <typename T>
 __device__ inline void load8(const T* v, uint32_t out[8]) {
     if constexpr (sizeof(T) == 1) {
@@ -287,7 +350,7 @@ __global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t n
     __syncthreads();
     if (threadIdx.x == 0) {
         atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)created);
-        atomicAdd((unsigned long long*)&P.ctl->st[ST_LEAF_MISS], (unsigned long long)created);
+        atomicAdd((unsigned long long*)stat(P.ctl, ST_LEAF_MISS), (unsigned long long)created);
     }
     for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
         ChunkDev& C = chunks[c];
@@ -297,19 +360,19 @@ __global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t n
         C.init = make_leaf(idx, P.gens[idx]);
         if (C.has_patches) {
             atomicAdd(&P.refs[idx], 1u);
-            atomicAdd((unsigned long long*)&P.ctl->st[ST_H3], 1ull);
+            atomicAdd((unsigned long long*)stat(P.ctl, ST_H3), 1ull);
         }
     }
 }
 
 // ------------------------------------------------------------------ Phase 1 build
 
-template <typename T>
-__device__ inline void emit_candidate(const Cands& K, uint32_t* counter, bool pred,
-                                      const uint64_t ch[8], uint32_t tm, uint64_t slot,
-                                      uint64_t* st) {
-    uint32_t k = wave_append(counter, pred);
+// Append a branch key to this tile's region of the candidate arrays.
+__device__ inline void emit_candidate(const Cands& K, uint32_t* s_cnt, bool pred, const uint64_t ch[8],
+                                      uint32_t tm, uint64_t slot) {
+    uint32_t pos = wave_append(s_cnt, pred);
     if (pred) {
+        uint64_t k = (uint64_t)blockIdx.x * kTile + pos;
         uint64_t* dst = K.key + 9ull * k;
 #pragma unroll
         for (int i = 0; i < 8; i++) dst[i] = ch[i];
@@ -317,91 +380,113 @@ __device__ inline void emit_candidate(const Cands& K, uint32_t* counter, bool pr
         K.hash[k] = branch_hash(ch, tm);
         K.slot[k] = slot;
     }
-    wave_count(st, pred);
+}
+
+// Tile statistics in LDS, flushed with one striped atomic per counter per block.
+struct TileStats {
+    uint32_t v[8];
+};
+__device__ inline void tile_count(uint32_t* s, int id, bool pred) {
+    uint64_t b = __ballot(pred);
+    if (b && (int)__lane_id() == __ffsll((unsigned long long)b) - 1) atomicAdd(&s[id], (uint32_t)__popcll(b));
 }
 
 // Per brick (voxtree.rs:824-896): the all-same brick becomes a leaf; otherwise
 // the brick's children are built from the old node, a split leaf, or nothing, and
 // the changed slots get leaves.  Branch keys go to the level-1 candidate list.
 template <typename T>
-__global__ void k_brick_build(Pool<T> P, const ChunkDev* chunks, LevelInfo L, uint64_t* res1,
-                              Cands K, uint32_t* ncand) {
-    VX_LOOP(slot, L.nslots) {
-        bool valid = slot < L.nslots;
-        bool cand = false;
-        uint64_t ch[8];
-        uint32_t tm = 0;
-        if (valid) {
-            uint64_t base = res1[slot];
-            if (base != kInvalid) {
-                uint32_t g = group_of(L, slot);
-                uint64_t local = slot - L.slot_base[g];
-                uint32_t D = L.depth[g];
-                uint32_t sh = 3 * (D - 1);
-                uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
-                uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
-                const ChunkDev& C = chunks[c];
-                uint64_t out = kInvalid;
-                if (C.status >= 0) {
-                    uint32_t m = C.masks[2ull * p];
-                    uint32_t v[8];
-                    load8<T>((const T*)C.values + 8ull * p, v);
-                    bool all_same = m == 0xFFu;
+__global__ __launch_bounds__(kTile) void k_brick_build(Pool<T> P, const ChunkDev* chunks, LevelInfo L,
+                                                       uint64_t* res1, Cands K, uint32_t* tile_cnt) {
+    __shared__ uint32_t s_cnt;
+    __shared__ uint32_t s_st[8];
+    if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    uint64_t slot = (uint64_t)blockIdx.x * kTile + threadIdx.x;
+    bool valid = slot < L.nslots;
+    bool cand = false, collapsed = false, h2 = false, h3 = false, h4 = false;
+    uint64_t ch[8];
+    uint32_t tm = 0;
+    if (valid) {
+        uint64_t base = res1[slot];
+        if (base != kInvalid) {
+            uint32_t g = group_of(L, slot);
+            uint64_t local = slot - L.slot_base[g];
+            uint32_t D = L.depth[g];
+            uint32_t sh = 3 * (D - 1);
+            uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
+            uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
+            const ChunkDev& C = chunks[c];
+            uint64_t out = kInvalid;
+            if (C.status >= 0) {
+                uint32_t m = C.masks[2ull * p];
+                uint32_t v[8];
+                load8<T>((const T*)C.values + 8ull * p, v);
+                bool all_same = m == 0xFFu;
 #pragma unroll
-                    for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
-                    if (all_same) {
-                        out = leaf_id_of(P, v[0]);
-                        atomicAdd((unsigned long long*)&P.ctl->st[ST_COLLAPSE], 1ull);
+                for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
+                if (all_same) {
+                    out = leaf_id_of(P, v[0]);
+                    collapsed = true;
+                } else {
+                    if (base == kFillMark) base = C.init;
+                    bool split = bid_is_leaf(base);
+                    uint32_t types = 0, mask = 0;
+                    if (split) {
+                        for (int i = 0; i < 8; i++) ch[i] = base;
+                        types = mask = 0xFFu;
+                    } else if (base != kEmpty) {
+                        for (int i = 0; i < 8; i++) ch[i] = P.children[8ull * bid_index(base) + i];
+                        types = bid_types(base);
+                        mask = bid_mask(base);
                     } else {
-                        if (base == kFillMark) base = C.init;
-                        bool split = bid_is_leaf(base);
-                        uint32_t types = 0, mask = 0;
+                        for (int i = 0; i < 8; i++) ch[i] = kEmpty;
+                    }
+                    uint32_t modified = 0;
+                    for (int i = 0; i < 8; i++) {
+                        if (!(m & (1u << i))) continue;
+                        if (ch[i] != kEmpty && (uint32_t)P.values[bid_index(ch[i])] == v[i]) continue;
+                        ch[i] = leaf_id_of(P, v[i]);
+                        types |= 1u << i;
+                        mask |= 1u << i;
+                        modified |= 1u << i;
+                    }
+                    uint32_t pcm = (uint32_t)__popc(m), pcmod = (uint32_t)__popc(modified);
+                    if (modified == 0) {
+                        // no change: not pushed; the split leaf keeps 8-pc(s) refs (H3)
                         if (split) {
-                            for (int i = 0; i < 8; i++) ch[i] = base;
-                            types = mask = 0xFFu;
-                        } else if (base != kEmpty) {
-                            for (int i = 0; i < 8; i++) ch[i] = P.children[8ull * bid_index(base) + i];
-                            types = bid_types(base);
-                            mask = bid_mask(base);
-                        } else {
-                            for (int i = 0; i < 8; i++) ch[i] = kEmpty;
+                            atomicAdd(&P.refs[bid_index(base)], 8u - pcm);
+                            h3 = true;
                         }
-                        uint32_t modified = 0;
-                        for (int i = 0; i < 8; i++) {
-                            if (!(m & (1u << i))) continue;
-                            if (ch[i] != kEmpty && (uint32_t)P.values[bid_index(ch[i])] == v[i]) continue;
-                            ch[i] = leaf_id_of(P, v[i]);
-                            types |= 1u << i;
-                            mask |= 1u << i;
-                            modified |= 1u << i;
+                    } else {
+                        if (split && pcmod != pcm) {
+                            // slots equal to the split leaf were kept without a ref (H4)
+                            atomicAdd(&P.refs[bid_index(base)], pcmod - pcm);
+                            h4 = true;
                         }
-                        uint32_t pcm = (uint32_t)__popc(m), pcmod = (uint32_t)__popc(modified);
-                        if (modified == 0) {
-                            // no change: not pushed; the split leaf keeps 8-pc(s) refs (H3)
-                            if (split) {
-                                atomicAdd(&P.refs[bid_index(base)], 8u - pcm);
-                                atomicAdd((unsigned long long*)&P.ctl->st[ST_H3], 1ull);
-                            }
-                        } else {
-                            if (split && pcmod != pcm) {
-                                // slots equal to the split leaf were kept without a ref (H4)
-                                atomicAdd(&P.refs[bid_index(base)], pcmod - pcm);
-                                atomicAdd((unsigned long long*)&P.ctl->st[ST_H4], 1ull);
-                            }
-                            bool uniform = types == 0xFFu;
-                            for (int i = 1; i < 8; i++) uniform = uniform && ch[i] == ch[0];
-                            if (uniform) atomicAdd((unsigned long long*)&P.ctl->st[ST_H2], 1ull);
-                            tm = (types << 8) | mask;
-                            cand = true;
-                            out = res_pending(0xFFFFFFFFu);  // replaced by the probe
-                        }
+                        bool uniform = types == 0xFFu;
+                        for (int i = 1; i < 8; i++) uniform = uniform && ch[i] == ch[0];
+                        h2 = uniform;
+                        tm = (types << 8) | mask;
+                        cand = true;
+                        out = res_pending(0xFFFFFFFFu);  // replaced by the probe
                     }
                 }
-                res1[slot] = out;
             }
+            res1[slot] = out;
         }
-        emit_candidate<T>(K, ncand, cand, ch, tm, slot, &P.ctl->st[ST_CAND]);
     }
+    emit_candidate(K, &s_cnt, cand, ch, tm, slot);
+    tile_count(s_st, 0, cand);
+    tile_count(s_st, 1, collapsed);
+    tile_count(s_st, 2, h2);
+    tile_count(s_st, 3, h3);
+    tile_count(s_st, 4, h4);
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_cnt;
+    const int ids[5] = {ST_CAND, ST_COLLAPSE, ST_H2, ST_H3, ST_H4};
+    if (threadIdx.x < 5 && s_st[threadIdx.x])
+        atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
 }
 
 // ------------------------------------------------------------------ interning
@@ -426,135 +511,156 @@ __device__ inline bool key_matches_cand(const Cands& K, uint32_t k2, const uint6
 
 // get_or_create_branch (interner/mod.rs:716-829) for one launch's candidates:
 // committed keys resolve to the existing node, the first inserter of a new key
-// owns it (allocates its index), later equal keys point at the owner.
+// owns it (allocates its index), later equal keys point at the owner.  Equal keys
+// inside one wavefront are merged first, so only one lane per key probes.
+constexpr int kDedupRounds = 8;
+
 template <typename T>
-__global__ void k_probe(Pool<T> P, Cands K, const uint32_t* ncand, uint64_t* res, uint64_t bound) {
-    uint32_t n = *ncand;
-    VX_LOOP(k, bound) {
-        bool valid = k < n;
-        int kind = -1;  // 0 hit, 1 dup, 2 miss
-        uint32_t probes = 0;
-        if (valid) {
-            uint64_t key[9];
-#pragma unroll
-            for (int i = 0; i < 9; i++) key[i] = K.key[9ull * k + i];
-            uint64_t h = K.hash[k];
-            uint32_t tag = slot_tag(h);
-            uint32_t tm = (uint32_t)key[8];
-            uint64_t s = h & P.tmask;
-            uint64_t first_free = ~0ull;
-            uint64_t result = 0, tslot = ~0ull;
-            for (uint64_t guard = 0; guard <= 4 * (P.tmask + 1); guard++) {
-                uint64_t e = P.tab[s];
-                probes++;
-                bool resolved = false;
-                if (e == kSlotEmpty) {
-                    uint64_t target = first_free != ~0ull ? first_free : s;
-                    uint64_t expected = target == s ? kSlotEmpty : kSlotTomb;
-                    uint64_t desired = slot_make(tag, true, (uint32_t)k);
-                    uint64_t seen = expected;
-                    bool won = __hip_atomic_compare_exchange_strong(
-                        &P.tab[target], &seen, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                        __HIP_MEMORY_SCOPE_AGENT);
-                    if (won) {
-                        bool ok;
-                        uint32_t idx = alloc_node(P, &ok);
-                        if (ok) {
-                            result = make_branch(idx, P.gens[idx], tm);
-                            tslot = target;
-                        } else {
-                            result = kInvalid;
-                        }
-                        kind = 2;
-                        break;
-                    }
-                    e = seen;
-                    s = target;
-                    if (e != kSlotEmpty && e != kSlotTomb && slot_tag_of(e) == tag) {
-                        if (slot_pending(e)) {
-                            if (key_matches_cand(K, slot_payload(e), key)) {
-                                result = res_pending(slot_payload(e));
-                                kind = 1;
-                                resolved = true;
-                            }
-                        } else if (key_matches_node(P, slot_payload(e), key)) {
-                            uint32_t idx = slot_payload(e);
-                            result = make_branch(idx, P.gens[idx], tm);
-                            kind = 0;
-                            resolved = true;
-                        }
-                    }
-                    if (resolved) break;
-                    s = h & P.tmask;
-                    first_free = ~0ull;
-                    continue;
-                }
-                if (e == kSlotTomb) {
-                    if (first_free == ~0ull) first_free = s;
-                } else if (slot_tag_of(e) == tag) {
-                    if (slot_pending(e)) {
-                        if (key_matches_cand(K, slot_payload(e), key)) {
-                            result = res_pending(slot_payload(e));
-                            kind = 1;
-                            break;
-                        }
-                    } else if (key_matches_node(P, slot_payload(e), key)) {
-                        uint32_t idx = slot_payload(e);
-                        result = make_branch(idx, P.gens[idx], tm);
-                        kind = 0;
-                        break;
-                    }
-                }
-                s = (s + 1) & P.tmask;
+__device__ inline int probe_table(const Pool<T>& P, const Cands& K, uint64_t k, const uint64_t key[9],
+                                  uint64_t h, uint64_t* result, uint64_t* tslot, uint32_t* probes) {
+    uint32_t tag = slot_tag(h);
+    uint32_t tm = (uint32_t)key[8];
+    uint64_t s = h & P.tmask;
+    uint64_t first_free = ~0ull;
+    for (uint64_t guard = 0; guard <= 4 * (P.tmask + 1); guard++) {
+        uint64_t e = __hip_atomic_load(&P.tab[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        (*probes)++;
+        if (e == kSlotEmpty) {
+            uint64_t target = first_free != ~0ull ? first_free : s;
+            uint64_t expected = target == s ? kSlotEmpty : kSlotTomb;
+            uint64_t seen = expected;
+            bool won = __hip_atomic_compare_exchange_strong(
+                &P.tab[target], &seen, slot_make(tag, true, (uint32_t)k), __ATOMIC_RELAXED,
+                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (won) {
+                bool ok;
+                uint32_t idx = alloc_node(P, &ok);
+                *result = ok ? make_branch(idx, P.gens[idx], tm) : kInvalid;
+                *tslot = ok ? target : ~0ull;
+                return 2;
             }
-            if (kind < 0) {
-                atomicOr(&P.ctl->err, kErrTableFull | kErrPoison);
-                result = kInvalid;
+            e = seen;
+            if (e != kSlotEmpty && e != kSlotTomb && slot_tag_of(e) == tag) {
+                if (slot_pending(e)) {
+                    if (key_matches_cand(K, slot_payload(e), key)) {
+                        *result = res_pending(slot_payload(e));
+                        return 1;
+                    }
+                } else if (key_matches_node(P, slot_payload(e), key)) {
+                    *result = make_branch(slot_payload(e), P.gens[slot_payload(e)], tm);
+                    return 0;
+                }
             }
-            K.id[k] = result;
-            K.tslot[k] = tslot;
-            res[K.slot[k]] = result;
+            s = h & P.tmask;  // someone else inserted first: rescan the chain
+            first_free = ~0ull;
+            continue;
         }
-        wave_count(&P.ctl->st[ST_HIT], kind == 0);
-        wave_count(&P.ctl->st[ST_DUP], kind == 1);
-        wave_count(&P.ctl->st[ST_MISS], kind == 2);
-        if (valid) atomicAdd((unsigned long long*)&P.ctl->st[ST_PROBES], (unsigned long long)probes);
+        if (e == kSlotTomb) {
+            if (first_free == ~0ull) first_free = s;
+        } else if (slot_tag_of(e) == tag) {
+            if (slot_pending(e)) {
+                if (key_matches_cand(K, slot_payload(e), key)) {
+                    *result = res_pending(slot_payload(e));
+                    return 1;
+                }
+            } else if (key_matches_node(P, slot_payload(e), key)) {
+                *result = make_branch(slot_payload(e), P.gens[slot_payload(e)], tm);
+                return 0;
+            }
+        }
+        s = (s + 1) & P.tmask;
     }
+    atomicOr(&P.ctl->err, kErrTableFull | kErrPoison);
+    *result = kInvalid;
+    return -1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTile) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
+                                                 uint64_t* res) {
+    uint32_t n = tile_cnt[blockIdx.x];
+    if ((threadIdx.x & ~63u) >= n) return;  // whole wavefront without candidates
+    int lane = (int)__lane_id();
+    bool valid = threadIdx.x < n;
+    uint64_t k = (uint64_t)blockIdx.x * kTile + threadIdx.x;
+    uint64_t key[9], h = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) key[i] = valid ? K.key[9ull * k + i] : 0ull;
+    if (valid) h = K.hash[k];
+    // wavefront-level merge of equal keys: leader = lowest lane holding the key
+    int leader = lane;
+    bool open = valid;
+    for (int r = 0; r < kDedupRounds; r++) {
+        uint64_t act = __ballot(open);
+        if (!act) break;
+        int L = __ffsll((unsigned long long)act) - 1;
+        uint64_t lh = shfl64(h, L);
+        bool same = open && h == lh;
+#pragma unroll
+        for (int w = 0; w < 9; w++) {
+            uint64_t lw = shfl64(key[w], L);
+            same = same && key[w] == lw;
+        }
+        if (same) {
+            leader = L;
+            open = false;
+        }
+        if (lane == L) open = false;
+    }
+    uint64_t result = kInvalid, tslot = ~0ull;
+    int kind = -1;
+    uint32_t probes = 0;
+    if (valid && leader == lane) kind = probe_table(P, K, k, key, h, &result, &tslot, &probes);
+    uint64_t lres = shfl64(result, leader);
+    int lkind = __shfl(kind, leader);
+    if (valid && leader != lane) {
+        result = lres;
+        kind = lkind == 0 ? 0 : 1;  // a follower of an owner or of a duplicate is a duplicate
+    }
+    if (valid) {
+        K.id[k] = result;
+        K.tslot[k] = tslot;
+        res[K.slot[k]] = result;
+    }
+    wave_count(stat(P.ctl, ST_HIT), valid && kind == 0);
+    wave_count(stat(P.ctl, ST_DUP), valid && kind == 1);
+    wave_count(stat(P.ctl, ST_MISS), valid && kind == 2);
+    wave_add64(stat(P.ctl, ST_PROBES), probes);
 }
 
 // Owners write their node (children, LOD value, types/mask), publish the table
 // slot, and take one reference on each non-empty child.
 template <typename T>
-__global__ void k_commit(Pool<T> P, Cands K, const uint32_t* ncand, uint64_t bound) {
-    uint32_t n = *ncand;
-    VX_LOOP(k, bound) {
-        bool own = false;
-        if (k < n) {
-            uint64_t ts = K.tslot[k];
-            uint64_t id = K.id[k];
-            if (ts != ~0ull && id != kInvalid) {
-                own = true;
-                uint32_t idx = bid_index(id);
-                uint64_t key[9];
+__global__ __launch_bounds__(kTile) void k_commit(Pool<T> P, Cands K, const uint32_t* tile_cnt) {
+    uint32_t n = tile_cnt[blockIdx.x];
+    if ((threadIdx.x & ~63u) >= n) return;
+    uint64_t k = (uint64_t)blockIdx.x * kTile + threadIdx.x;
+    bool own = false;
+    uint64_t key[9];
+    if (threadIdx.x < n) {
+        uint64_t ts = K.tslot[k];
+        uint64_t id = K.id[k];
+        if (ts != ~0ull && id != kInvalid) {
+            own = true;
+            uint32_t idx = bid_index(id);
 #pragma unroll
-                for (int i = 0; i < 9; i++) key[i] = K.key[9ull * k + i];
-                uint32_t vals[8];
-                uint64_t* dst = P.children + 8ull * idx;
+            for (int i = 0; i < 9; i++) key[i] = K.key[9ull * k + i];
+            uint32_t vals[8];
+            uint64_t* dst = P.children + 8ull * idx;
 #pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    dst[i] = key[i];
-                    vals[i] = (uint32_t)P.values[bid_index(key[i])];
-                }
-                P.tms[idx] = (uint16_t)key[8];
-                P.values[idx] = (T)calc_average(vals);
-                P.tab[ts] = slot_make(slot_tag(K.hash[k]), false, idx);
-#pragma unroll
-                for (int i = 0; i < 8; i++)
-                    if (key[i] != kEmpty) atomicAdd(&P.refs[bid_index(key[i])], 1u);
+            for (int i = 0; i < 8; i++) {
+                dst[i] = key[i];
+                vals[i] = (uint32_t)P.values[bid_index(key[i])];
             }
+            P.tms[idx] = (uint16_t)key[8];
+            P.values[idx] = (T)calc_average(vals);
+            P.tab[ts] = slot_make(slot_tag(K.hash[k]), false, idx);
         }
-        wave_count(&P.ctl->live_branches, own);
     }
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        wave_ref_inc(P.refs, (own && key[i] != kEmpty) ? bid_index(key[i]) : kNoIdx);
+    wave_count(&P.ctl->live_branches, own);
 }
 
 // ------------------------------------------------------------------ Phase 2
@@ -567,13 +673,19 @@ __device__ inline uint64_t resolve(uint64_t r, const uint64_t* cand_id_prev) {
 // from height h-1, clone untouched ones from the old node at this position (or
 // fill them with the split leaf), collapse 8 equal leaves, else emit a key.
 template <typename T>
-__global__ void k_level_build(Pool<T> P, const ChunkDev* chunks, LevelInfo L,
-                              const uint64_t* slot_off_prev, const uint64_t* res_prev,
-                              const uint64_t* cand_id_prev, uint64_t* res, Cands K,
-                              uint32_t* ncand) {
-    VX_LOOP(slot, L.nslots) {
+__global__ __launch_bounds__(kTile) void k_level_build(
+    Pool<T> P, const ChunkDev* chunks, LevelInfo L, const uint64_t* slot_off_prev,
+    const uint64_t* res_prev, const uint64_t* cand_id_prev, uint64_t* res, Cands K,
+    uint32_t* tile_cnt) {
+    __shared__ uint32_t s_cnt;
+    __shared__ uint32_t s_st[8];
+    if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    {
+        uint64_t slot = (uint64_t)blockIdx.x * kTile + threadIdx.x;
         bool valid = slot < L.nslots;
-        bool cand = false;
+        bool cand = false, collapsed = false, h5 = false;
         uint64_t ch[8];
         uint32_t tm = 0;
         if (valid) {
@@ -644,13 +756,13 @@ __global__ void k_level_build(Pool<T> P, const ChunkDev* chunks, LevelInfo L,
                     for (int i = 1; i < 8; i++) all_same = all_same && ch[i] == ch[0];
                     if (all_same) {
                         out = ch[0];
-                        atomicAdd((unsigned long long*)&P.ctl->st[ST_COLLAPSE], 1ull);
+                        collapsed = true;
                         if (cloned) {
                             // dec_ref_by(children[0], min(pc(cloned),7)) instead of
                             // 7 - pc(cloned) (voxtree.rs:1066-1072, H5)
                             uint32_t pc = (uint32_t)__popc(cloned);
                             atomicAdd(&P.refs[bid_index(ch[0])], 7u - 2u * pc);
-                            atomicAdd((unsigned long long*)&P.ctl->st[ST_H5], 1ull);
+                            h5 = true;
                         }
                     } else {
                         tm = (types << 8) | mask;
@@ -661,8 +773,16 @@ __global__ void k_level_build(Pool<T> P, const ChunkDev* chunks, LevelInfo L,
             }
             res[slot] = out;
         }
-        emit_candidate<T>(K, ncand, cand, ch, tm, slot, &P.ctl->st[ST_CAND]);
+        emit_candidate(K, &s_cnt, cand, ch, tm, slot);
+        tile_count(s_st, 0, cand);
+        tile_count(s_st, 1, collapsed);
+        tile_count(s_st, 2, h5);
     }
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_cnt;
+    const int ids[3] = {ST_CAND, ST_COLLAPSE, ST_H5};
+    if (threadIdx.x < 3 && s_st[threadIdx.x])
+        atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
 }
 
 // ------------------------------------------------------------------ finalize + GC
@@ -738,35 +858,36 @@ __global__ void k_leaf_cleanup(Pool<T> P) {
 // One round of dec_ref_recursive over all queued nodes (interner/mod.rs:449-527):
 // decrement; a node that reaches zero drops its pattern, is recycled
 // (mod.rs:566-625), and decrements its children, queueing those that reach zero.
+// Round 0 holds the replaced roots (decremented here); later rounds hold nodes
+// whose count already reached zero.
 template <typename T>
 __global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uint32_t* next,
                            bool decrement_self, uint64_t bound, uint64_t front_cap) {
-    uint32_t n = P.ctl->gc_count[round];
-    if (n > front_cap) n = (uint32_t)front_cap;
-    VX_LOOP(i, bound) {
-        bool freed = false, is_leaf = false;
-        uint32_t idx = 0;
+    uint64_t n = P.ctl->gc_count[round];
+    if (n > front_cap) n = front_cap;
+    VX_LOOP(i, n) {
+        uint32_t idx = kNoIdx;
         if (i < n) {
-            idx = front[i];
-            if (idx != 0 && idx < P.capacity) {
-                uint32_t old = decrement_self ? atomicSub(&P.refs[idx], 1u) : 1u;
-                freed = old == 1u;
-            }
+            uint32_t v = front[i];
+            if (v != 0 && v < P.capacity) idx = v;
         }
+        bool freed = decrement_self ? wave_ref_dec_zero(P.refs, idx) : idx != kNoIdx;
+        bool is_leaf = false;
+        uint64_t ch[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) ch[j] = kEmpty;
         if (freed) {
             uint32_t tm = P.tms[idx];
             is_leaf = tm == 0;
             if (is_leaf) {
                 P.leaf_map[(uint32_t)P.values[idx]] = 0;
             } else {
-                uint64_t ch[8];
                 uint64_t* src = P.children + 8ull * idx;
 #pragma unroll
                 for (int j = 0; j < 8; j++) ch[j] = src[j];
                 // remove the pattern (mod.rs:518)
                 uint64_t h = branch_hash(ch, tm);
-                uint32_t tag = slot_tag(h);
-                uint64_t want = slot_make(tag, false, idx);
+                uint64_t want = slot_make(slot_tag(h), false, idx);
                 uint64_t s = h & P.tmask;
                 for (uint64_t guard = 0; guard <= P.tmask; guard++) {
                     uint64_t e = P.tab[s];
@@ -778,29 +899,30 @@ __global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uin
                     s = (s + 1) & P.tmask;
                 }
 #pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    uint64_t c = ch[j];
-                    src[j] = kEmpty;
-                    if (c == kEmpty) continue;
-                    uint32_t ci = bid_index(c);
-                    uint32_t o = atomicSub(&P.refs[ci], 1u);
-                    if (o == 1u) {
-                        uint32_t pos = atomicAdd(&P.ctl->gc_count[round + 1], 1u);
-                        if (pos < front_cap) next[pos] = ci;
-                        else atomicOr(&P.ctl->err, kErrPoison);
-                    }
-                }
+                for (int j = 0; j < 8; j++) src[j] = kEmpty;
                 P.tms[idx] = 0;
             }
             P.values[idx] = 0;
             P.refs[idx] = 0;
             uint32_t g = P.gens[idx] + 1u;
             P.gens[idx] = (uint16_t)(g >= kMaxGeneration ? 0u : g);
-            uint32_t pos = atomicAdd(&P.ctl->free_top, 1u);
-            if (pos < P.capacity) P.free_stack[pos] = idx;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            uint32_t ci = ch[j] != kEmpty ? bid_index(ch[j]) : kNoIdx;
+            bool push = wave_ref_dec_zero(P.refs, ci);
+            uint32_t pos = wave_append(&P.ctl->gc_count[round + 1], push);
+            if (push) {
+                if (pos < front_cap) next[pos] = ci;
+                else atomicOr(&P.ctl->err, kErrPoison);
+            }
+        }
+        uint32_t fpos = wave_append(&P.ctl->free_top, freed);
+        if (freed) {
+            if (fpos < P.capacity) P.free_stack[fpos] = idx;
             else atomicOr(&P.ctl->err, kErrPoison);
         }
-        wave_count(&P.ctl->st[ST_RECYCLED], freed);
+        wave_count(stat(P.ctl, ST_RECYCLED), freed);
         uint64_t bl = __ballot(freed && is_leaf), bb = __ballot(freed && !is_leaf);
         if (__lane_id() == 0) {
             if (bl) atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)(-(long long)__popcll(bl)));
@@ -877,7 +999,7 @@ __global__ void k_leaf_alloc_only(Pool<T> P, uint32_t nvalues) {
     __syncthreads();
     if (threadIdx.x == 0) {
         atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)created);
-        atomicAdd((unsigned long long*)&P.ctl->st[ST_LEAF_MISS], (unsigned long long)created);
+        atomicAdd((unsigned long long*)stat(P.ctl, ST_LEAF_MISS), (unsigned long long)created);
     }
 }
 
