@@ -72,7 +72,7 @@ enum KernelId : uint32_t {
     KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
-    "k_begin", "k_prepare", "k_brick_scan", "k_leaf_alloc", "k_brick_build", "k_probe[bricks]",
+    "k_begin", "k_prepare", "k_brick_retry", "k_leaf_alloc", "k_brick_build", "k_probe[bricks]",
     "k_commit[bricks]", "k_level_build", "k_probe[upper]", "k_commit[upper]", "k_finalize",
     "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill"};
 
@@ -106,7 +106,7 @@ struct vx_interner {
     bool own_stream = false;
     int poisoned = 0;
     // workspace
-    Grow chunks, slot_off, ptr_tables, gc_front[2], stage, scratch;
+    Grow chunks, slot_off, ptr_tables, gc_front[2], stage, scratch, deferred, drift;
     LevelBufs lv[kLevels + 1];
     // pinned staging ring: a slot is reused only after its copies completed
     static constexpr int kRing = 4;
@@ -330,27 +330,34 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
                           hipMemcpyHostToDevice, s));
     pinned_release(h);
     uint32_t nvalues = 1u << (8 * sizeof(T));
+    uint64_t nb = std::max<uint64_t>(dmax >= 2 ? L[1].nslots : 1, 1);
+    if (!h->deferred.ensure(nb * 4) || !h->drift.ensure(nb * sizeof(Drift))) return VX_E_HIP;
+    uint32_t* ddef = (uint32_t*)h->deferred.ptr;
+    Drift* ddrift = (Drift*)h->drift.ptr;
     // ---- launch sequence
     VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
     VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, grid_for(n), kBlock, P, dchunks, (uint32_t)n, roots);
-    if (dmax >= 2) {
-        VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_scan<T>, grid_for(L[1].nslots), kBlock, P, dchunks, L[1],
-                  res_ptrs_h[1]);
-    }
+    unsigned t1 = (unsigned)ntiles[1];
+    if (dmax >= 2)
+        VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T>, t1, kTile, P, dchunks, L[1], res_ptrs_h[1], K[1],
+                  tiles_h[1], ddef, ddrift);
     VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc<T>, 1, 1024, P, dchunks, (uint32_t)n, nvalues);
     if (dmax >= 2) {
-        unsigned t1 = (unsigned)ntiles[1];
-        VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T>, t1, kTile, P, dchunks, L[1], res_ptrs_h[1], K[1],
-                  tiles_h[1]);
-        VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T>, t1, kTile, P, K[1], tiles_h[1], res_ptrs_h[1]);
-        VX_LAUNCH(h, KID_COMMIT_BRICKS, k_commit<T>, t1, kTile, P, K[1], tiles_h[1]);
+        VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_retry<T>, 256, kBlock, P, dchunks, L[1], res_ptrs_h[1], K[1],
+                  tiles_h[1], ddef, ddrift);
+        VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T>, t1, kTile, P, K[1], tiles_h[1], res_ptrs_h[1], dchunks,
+                  L[1]);
+        VX_LAUNCH(h, KID_COMMIT_BRICKS, k_commit<T>, std::min<unsigned>(t1, 1024), kTile, P, K[1],
+                  tiles_h[1], (uint64_t)t1);
         for (uint32_t hh = 2; hh <= dmax; hh++) {
             unsigned th = (unsigned)ntiles[hh];
             VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T>, th, kTile, P, dchunks, L[hh],
                       doff + (size_t)(hh - 1) * n, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh],
                       K[hh], tiles_h[hh]);
-            VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T>, th, kTile, P, K[hh], tiles_h[hh], res_ptrs_h[hh]);
-            VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T>, th, kTile, P, K[hh], tiles_h[hh]);
+            VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T>, th, kTile, P, K[hh], tiles_h[hh], res_ptrs_h[hh],
+                      dchunks, L[hh]);
+            VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T>, std::min<unsigned>(th, 1024), kTile, P, K[hh],
+                      tiles_h[hh], (uint64_t)th);
         }
     }
     uint32_t* gf0 = (uint32_t*)h->gc_front[0].ptr;
@@ -360,12 +367,11 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
     VX_LAUNCH(h, KID_END_ALLOC, k_end_alloc, 1, 1, h->ctl);
     VX_LAUNCH(h, KID_LEAF_CLEANUP, k_leaf_cleanup<T>, grid_for(nvalues), kBlock, P);
     // dec_ref_recursive of the replaced roots: depth <= 7 gives at most 8 rounds
-    uint64_t gbound = std::min<uint64_t>(gcap, h->capacity + n);
     for (uint32_t r = 0; r < (uint32_t)kLevels + 1; r++) {
         uint32_t* fr = (r & 1) ? gf1 : gf0;
         uint32_t* nx = (r & 1) ? gf0 : gf1;
-        uint64_t bound = r == 0 ? n : gbound;
-        VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid_for(bound), kBlock, P, r, fr, nx, r == 0, bound, gcap);
+        unsigned grid = r == 0 ? grid_for(n) : 1024u;
+        VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid, kBlock, P, r, fr, nx, r == 0, (uint64_t)gcap);
     }
     if (hipGetLastError() != hipSuccess) return VX_E_HIP;
     return VX_OK;
@@ -381,12 +387,11 @@ int fill_impl(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* vals) {
     uint32_t nvalues = 1u << (8 * sizeof(T));
     VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
     VX_LAUNCH(h, KID_FILL, k_fill_prepare<T>, grid_for(n), kBlock, P, (uint32_t)n, roots, vals, gf0);
-    uint64_t gbound = std::min<uint64_t>(gcap, h->capacity + n);
     for (uint32_t r = 0; r < (uint32_t)kLevels + 1; r++) {
         uint32_t* fr = (r & 1) ? gf1 : gf0;
         uint32_t* nx = (r & 1) ? gf0 : gf1;
-        uint64_t bound = r == 0 ? n : gbound;
-        VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid_for(bound), kBlock, P, r, fr, nx, r == 0, bound, gcap);
+        unsigned grid = r == 0 ? grid_for(n) : 1024u;
+        VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid, kBlock, P, r, fr, nx, r == 0, (uint64_t)gcap);
     }
     VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
     VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc_only<T>, 1, 1024, P, nvalues);
@@ -506,6 +511,8 @@ int vx_interner_destroy(vx_interner* h) {
     h->gc_front[1].release();
     h->stage.release();
     h->scratch.release();
+    h->deferred.release();
+    h->drift.release();
     for (auto& b : h->lv) {
         b.res.release();
         b.key.release();

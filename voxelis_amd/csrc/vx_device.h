@@ -91,11 +91,12 @@ struct Ctl {
     uint32_t alloc_count;  // allocations in the current sequence
     uint32_t err;
     uint32_t n_new_leaves;
-    uint32_t pad0;
+    uint32_t n_deferred;  // bricks waiting for leaves created later in the sequence
     uint64_t live_leaves;
     uint64_t live_branches;
     uint64_t tombstones;
-    uint64_t pad1;
+    uint32_t n_drift;  // queued reference-count corrections of the reference's quirks
+    uint32_t pad1;
     uint32_t ncand[kLevels + 1];
     uint32_t gc_count[kLevels + 2];
     // statistics, striped over 64 cache lines by block index (summed when read)
@@ -126,6 +127,14 @@ struct LevelInfo {
     uint32_t ngroups;
     uint32_t h;
     uint64_t nslots;
+};
+
+// A queued reference-count correction (applied once the tree's status is final).
+struct Drift {
+    uint32_t chunk;
+    uint32_t idx;
+    int32_t delta;
+    uint32_t stat;
 };
 
 template <typename T>

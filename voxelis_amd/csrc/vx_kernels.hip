@@ -111,6 +111,32 @@ __device__ inline bool wave_ref_dec_zero(uint32_t* refs, uint32_t idx) {
     return owner;
 }
 
+// Block-local combining of reference-count updates: hot nodes (the few leaves
+// most new nodes point at) get one global atomic per block instead of one per
+// reference.  agg_add returns false when the table has no room.
+constexpr int kAggSlots = 512;
+
+__device__ inline void agg_init(uint32_t* akey, uint32_t* acnt) {
+    for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
+        akey[i] = kNoIdx;
+        acnt[i] = 0;
+    }
+}
+
+__device__ inline bool agg_add(uint32_t* akey, uint32_t* acnt, uint32_t idx, uint32_t n) {
+    uint32_t h = (idx * 2654435761u) >> 23;
+#pragma unroll 1
+    for (int p = 0; p < 16; p++) {
+        uint32_t s = (h + p) & (kAggSlots - 1);
+        uint32_t k = atomicCAS(&akey[s], kNoIdx, idx);
+        if (k == kNoIdx || k == idx) {
+            atomicAdd(&acnt[s], n);
+            return true;
+        }
+    }
+    return false;
+}
+
 template <typename T>
 __device__ inline void load8(const T* v, uint32_t out[8]) {
     if constexpr (sizeof(T) == 1) {
@@ -202,6 +228,8 @@ __global__ void k_begin(Ctl* ctl) {
         ctl->next0 = ctl->next_index;
         ctl->alloc_count = 0;
         ctl->n_new_leaves = 0;
+        ctl->n_deferred = 0;
+        ctl->n_drift = 0;
         for (int i = 0; i <= kLevels; i++) ctl->ncand[i] = 0;
         for (int i = 0; i < kLevels + 2; i++) ctl->gc_count[i] = 0;
     }
@@ -236,90 +264,7 @@ __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_
     }
 }
 
-// ------------------------------------------------------------------ Phase 1 scan
-
-// Per brick: descend the old tree (voxtree.rs:792-822), detect the leaf-brick
-// panic (H1), and mark the voxel values whose leaves get_or_create_leaf will need.
-// Leaves the Phase-1 base node (E or split leaf) in the brick's slot.
-template <typename T>
-__global__ void k_brick_scan(Pool<T> P, const ChunkDev* chunks, LevelInfo L, uint64_t* res1) {
-    VX_LOOP(slot, L.nslots) {
-        if (slot < L.nslots) {
-            uint32_t g = group_of(L, slot);
-            uint64_t local = slot - L.slot_base[g];
-            uint32_t D = L.depth[g];
-            uint32_t sh = 3 * (D - 1);
-            uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
-            uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
-            const ChunkDev& C = chunks[c];
-            uint64_t out = kInvalid;
-            uint32_t m = 0;
-            if (C.status >= 0 && C.has_patches) m = C.masks[2ull * p];
-            if (m != 0) {
-                uint32_t v[8];
-                load8<T>((const T*)C.values + 8ull * p, v);
-                bool all_same = m == 0xFFu;
-#pragma unroll
-                for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
-                if (all_same) {
-                    if (P.leaf_map[v[0]] == 0) P.leaf_need[v[0]] = 1;
-                    out = kEmpty;
-                } else {
-                    uint64_t E = kEmpty, Ls = kEmpty;
-                    bool fill_split = false;
-                    if (C.has_fill) {
-                        fill_split = true;
-                    } else {
-                        uint64_t cur = C.root_in;
-                        for (uint32_t cd = 0; cd + 1 < D; cd++) {
-                            if (!bid_is_leaf(cur)) {
-                                uint32_t digit = (p >> (3 * (D - 2 - cd))) & 7u;
-                                if (bid_index(cur) >= P.capacity) {
-                                    atomicOr(&P.ctl->err, kErrPoison);
-                                    cur = kEmpty;
-                                    break;
-                                }
-                                cur = P.children[8ull * bid_index(cur) + digit];
-                            } else {
-                                Ls = cur;
-                                cur = kEmpty;
-                            }
-                            if (cur == kEmpty) break;
-                        }
-                        E = cur;
-                    }
-                    if (E != kEmpty && bid_is_leaf(E)) {
-                        // BlockId::types() on a leaf panics (block_id.rs:273-276)
-                        ((ChunkDev*)chunks)[c].status = -5;
-                    } else {
-                        uint32_t ev[8];
-                        uint32_t have = 0;
-                        if (E != kEmpty) {
-                            for (int i = 0; i < 8; i++) {
-                                uint64_t ch = P.children[8ull * bid_index(E) + i];
-                                if (ch != kEmpty) {
-                                    have |= 1u << i;
-                                    ev[i] = (uint32_t)P.values[bid_index(ch)];
-                                }
-                            }
-                        } else if (fill_split || Ls != kEmpty) {
-                            uint32_t lv = fill_split ? (uint32_t)C.fill : (uint32_t)P.values[bid_index(Ls)];
-                            have = 0xFFu;
-                            for (int i = 0; i < 8; i++) ev[i] = lv;
-                        }
-                        for (int i = 0; i < 8; i++) {
-                            if (!(m & (1u << i))) continue;
-                            if ((have & (1u << i)) && ev[i] == v[i]) continue;
-                            if (P.leaf_map[v[i]] == 0) P.leaf_need[v[i]] = 1;
-                        }
-                        out = fill_split ? kFillMark : (E != kEmpty ? E : Ls);
-                    }
-                }
-            }
-            res1[slot] = out;
-        }
-    }
-}
+// ------------------------------------------------------------------ Phase 1
 
 // One block: create the leaves marked as needed (interner/mod.rs:669-708), then
 // fix each tree's Phase-1 start node.  The fill leaf of a batch that also has
@@ -391,25 +336,199 @@ __device__ inline void tile_count(uint32_t* s, int id, bool pred) {
     if (b && (int)__lane_id() == __ffsll((unsigned long long)b) - 1) atomicAdd(&s[id], (uint32_t)__popcll(b));
 }
 
-// Per brick (voxtree.rs:824-896): the all-same brick becomes a leaf; otherwise
-// the brick's children are built from the old node, a split leaf, or nothing, and
-// the changed slots get leaves.  Branch keys go to the level-1 candidate list.
+// Per brick (voxtree.rs:778-896): descend the old tree to the brick (792-822), then
+// the all-same brick becomes a leaf; otherwise the brick's children are built from
+// the old node, a split leaf, or nothing, and the changed slots get leaves.
+// Returns: 0 = not pushed, 1 = leaf result in *out, 2 = branch key in ch/tm,
+// 3 = a needed leaf does not exist yet (marked; the brick is retried later).
+// Reference-count corrections of the reference's split-leaf quirks are queued in
+// *drift_idx / *drift_delta (applied once the tree's status is final).
 template <typename T>
-__global__ __launch_bounds__(kTile) void k_brick_build(Pool<T> P, const ChunkDev* chunks, LevelInfo L,
-                                                       uint64_t* res1, Cands K, uint32_t* tile_cnt) {
+__device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m, const uint32_t v[8],
+                                 uint64_t base, uint64_t* out, uint64_t ch[8], uint32_t* tm,
+                                 uint32_t* drift_idx, int32_t* drift_delta, int* flags) {
+    *drift_idx = kNoIdx;
+    bool all_same = m == 0xFFu;
+#pragma unroll
+    for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
+    if (all_same) {
+        uint32_t li = P.leaf_map[v[0]];
+        if (li == 0) {
+            P.leaf_need[v[0]] = 1;
+            return 3;
+        }
+        *out = make_leaf(li, P.gens[li]);
+        *flags |= 1;  // collapsed
+        return 1;
+    }
+    // the split leaf's value is known without its id (the fill leaf may not exist yet)
+    bool missing = false;
+    if (base == kFillMark) {
+        uint32_t fi = P.leaf_map[(uint32_t)C.fill];
+        if (fi == 0) {
+            P.leaf_need[(uint32_t)C.fill] = 1;
+            missing = true;
+            base = make_leaf(0, 0);
+        } else {
+            base = make_leaf(fi, P.gens[fi]);
+        }
+    }
+    bool split = bid_is_leaf(base);
+    uint32_t split_value = 0;
+    if (split) split_value = missing ? (uint32_t)C.fill : (uint32_t)P.values[bid_index(base)];
+    uint32_t types = 0, mask = 0;
+    if (split) {
+        for (int i = 0; i < 8; i++) ch[i] = base;
+        types = mask = 0xFFu;
+    } else if (base != kEmpty) {
+        for (int i = 0; i < 8; i++) ch[i] = P.children[8ull * bid_index(base) + i];
+        types = bid_types(base);
+        mask = bid_mask(base);
+    } else {
+        for (int i = 0; i < 8; i++) ch[i] = kEmpty;
+    }
+    uint32_t modified = 0;
+    for (int i = 0; i < 8; i++) {
+        if (!(m & (1u << i))) continue;
+        if (ch[i] != kEmpty) {
+            uint32_t cur = split ? split_value : (uint32_t)P.values[bid_index(ch[i])];
+            if (cur == v[i]) continue;
+        }
+        uint32_t li = P.leaf_map[v[i]];
+        if (li == 0) {
+            P.leaf_need[v[i]] = 1;
+            missing = true;
+            continue;
+        }
+        ch[i] = make_leaf(li, P.gens[li]);
+        types |= 1u << i;
+        mask |= 1u << i;
+        modified |= 1u << i;
+    }
+    if (missing) return 3;
+    uint32_t pcm = (uint32_t)__popc(m), pcmod = (uint32_t)__popc(modified);
+    if (modified == 0) {
+        // no change: not pushed; the split leaf keeps its 8-pc(s) refs (H3)
+        if (split) {
+            *drift_idx = bid_index(base);
+            *drift_delta = (int32_t)(8u - pcm);
+            *flags |= 8;
+        }
+        return 0;
+    }
+    if (split && pcmod != pcm) {
+        // slots equal to the split leaf stay without a ref (H4)
+        *drift_idx = bid_index(base);
+        *drift_delta = (int32_t)pcmod - (int32_t)pcm;
+        *flags |= 16;
+    }
+    bool uniform = types == 0xFFu;
+    for (int i = 1; i < 8; i++) uniform = uniform && ch[i] == ch[0];
+    if (uniform) *flags |= 4;  // H2
+    *tm = (types << 8) | mask;
+    return 2;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTile) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
+                                                       uint64_t* res1, Cands K, uint32_t* tile_cnt,
+                                                       uint32_t* deferred, Drift* drift) {
     __shared__ uint32_t s_cnt;
     __shared__ uint32_t s_st[8];
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     uint64_t slot = (uint64_t)blockIdx.x * kTile + threadIdx.x;
-    bool valid = slot < L.nslots;
-    bool cand = false, collapsed = false, h2 = false, h3 = false, h4 = false;
+    bool cand = false, defer = false;
+    int flags = 0;
     uint64_t ch[8];
     uint32_t tm = 0;
-    if (valid) {
-        uint64_t base = res1[slot];
-        if (base != kInvalid) {
+    if (slot < L.nslots) {
+        uint32_t g = group_of(L, slot);
+        uint64_t local = slot - L.slot_base[g];
+        uint32_t D = L.depth[g];
+        uint32_t sh = 3 * (D - 1);
+        uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
+        uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
+        const ChunkDev& C = chunks[c];
+        uint64_t out = kInvalid;
+        uint32_t m = C.status >= 0 ? C.masks[2ull * p] : 0u;
+        if (m != 0) {
+            uint32_t v[8];
+            load8<T>((const T*)C.values + 8ull * p, v);
+            bool all_same = m == 0xFFu;
+#pragma unroll
+            for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
+            uint64_t base = kEmpty;
+            bool h1 = false;
+            if (!all_same) {
+                if (C.has_fill) {
+                    base = kFillMark;
+                } else {
+                    uint64_t cur = C.root_in, Ls = kEmpty;
+                    for (uint32_t cd = 0; cd + 1 < D; cd++) {
+                        if (!bid_is_leaf(cur)) {
+                            uint32_t digit = (p >> (3 * (D - 2 - cd))) & 7u;
+                            if (bid_index(cur) >= P.capacity) {
+                                atomicOr(&P.ctl->err, kErrPoison);
+                                cur = kEmpty;
+                                break;
+                            }
+                            cur = P.children[8ull * bid_index(cur) + digit];
+                        } else {
+                            Ls = cur;
+                            cur = kEmpty;
+                        }
+                        if (cur == kEmpty) break;
+                    }
+                    // BlockId::types() on a leaf brick panics (block_id.rs:273-276)
+                    h1 = cur != kEmpty && bid_is_leaf(cur);
+                    base = cur != kEmpty ? cur : Ls;
+                }
+            }
+            if (h1) {
+                chunks[c].status = -5;
+            } else {
+                uint32_t didx;
+                int32_t dd;
+                int r = brick_work(P, C, m, v, base, &out, ch, &tm, &didx, &dd, &flags);
+                if (didx != kNoIdx) {
+                    uint32_t pos = atomicAdd(&P.ctl->n_drift, 1u);
+                    drift[pos] = Drift{c, didx, dd, (uint32_t)(flags & 8 ? ST_H3 : ST_H4)};
+                }
+                if (r == 2) {
+                    cand = true;
+                    out = res_pending(0xFFFFFFFFu);  // replaced by the probe
+                } else if (r == 3) {
+                    defer = true;
+                    out = base;  // kept for the retry
+                }
+            }
+        }
+        res1[slot] = out;
+    }
+    emit_candidate(K, &s_cnt, cand, ch, tm, slot);
+    uint32_t dpos = wave_append(&P.ctl->n_deferred, defer);
+    if (defer) deferred[dpos] = (uint32_t)slot;
+    tile_count(s_st, 0, cand);
+    tile_count(s_st, 1, (flags & 1) != 0);
+    tile_count(s_st, 2, (flags & 4) != 0);
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_cnt;
+    const int ids[3] = {ST_CAND, ST_COLLAPSE, ST_H2};
+    if (threadIdx.x < 3 && s_st[threadIdx.x])
+        atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
+}
+
+// Bricks that waited for new leaves, and the queued reference-count corrections
+// of trees that did not fail (their status is final now).
+template <typename T>
+__global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, uint64_t* res1, Cands K,
+                              uint32_t* tile_cnt, const uint32_t* deferred, const Drift* drift) {
+    uint32_t nd = P.ctl->n_deferred;
+    VX_LOOP(i, nd) {
+        if (i < nd) {
+            uint64_t slot = deferred[i];
             uint32_t g = group_of(L, slot);
             uint64_t local = slot - L.slot_base[g];
             uint32_t D = L.depth[g];
@@ -422,71 +541,45 @@ __global__ __launch_bounds__(kTile) void k_brick_build(Pool<T> P, const ChunkDev
                 uint32_t m = C.masks[2ull * p];
                 uint32_t v[8];
                 load8<T>((const T*)C.values + 8ull * p, v);
-                bool all_same = m == 0xFFu;
-#pragma unroll
-                for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
-                if (all_same) {
-                    out = leaf_id_of(P, v[0]);
-                    collapsed = true;
-                } else {
-                    if (base == kFillMark) base = C.init;
-                    bool split = bid_is_leaf(base);
-                    uint32_t types = 0, mask = 0;
-                    if (split) {
-                        for (int i = 0; i < 8; i++) ch[i] = base;
-                        types = mask = 0xFFu;
-                    } else if (base != kEmpty) {
-                        for (int i = 0; i < 8; i++) ch[i] = P.children[8ull * bid_index(base) + i];
-                        types = bid_types(base);
-                        mask = bid_mask(base);
-                    } else {
-                        for (int i = 0; i < 8; i++) ch[i] = kEmpty;
-                    }
-                    uint32_t modified = 0;
-                    for (int i = 0; i < 8; i++) {
-                        if (!(m & (1u << i))) continue;
-                        if (ch[i] != kEmpty && (uint32_t)P.values[bid_index(ch[i])] == v[i]) continue;
-                        ch[i] = leaf_id_of(P, v[i]);
-                        types |= 1u << i;
-                        mask |= 1u << i;
-                        modified |= 1u << i;
-                    }
-                    uint32_t pcm = (uint32_t)__popc(m), pcmod = (uint32_t)__popc(modified);
-                    if (modified == 0) {
-                        // no change: not pushed; the split leaf keeps 8-pc(s) refs (H3)
-                        if (split) {
-                            atomicAdd(&P.refs[bid_index(base)], 8u - pcm);
-                            h3 = true;
-                        }
-                    } else {
-                        if (split && pcmod != pcm) {
-                            // slots equal to the split leaf were kept without a ref (H4)
-                            atomicAdd(&P.refs[bid_index(base)], pcmod - pcm);
-                            h4 = true;
-                        }
-                        bool uniform = types == 0xFFu;
-                        for (int i = 1; i < 8; i++) uniform = uniform && ch[i] == ch[0];
-                        h2 = uniform;
-                        tm = (types << 8) | mask;
-                        cand = true;
-                        out = res_pending(0xFFFFFFFFu);  // replaced by the probe
-                    }
+                uint64_t ch[8];
+                uint32_t tm = 0, didx;
+                int32_t dd;
+                int flags = 0;
+                int r = brick_work(P, C, m, v, res1[slot], &out, ch, &tm, &didx, &dd, &flags);
+                if (didx != kNoIdx) {
+                    atomicAdd(&P.refs[didx], (uint32_t)dd);
+                    atomicAdd((unsigned long long*)stat(P.ctl, flags & 8 ? ST_H3 : ST_H4), 1ull);
+                }
+                if (flags & 1) atomicAdd((unsigned long long*)stat(P.ctl, ST_COLLAPSE), 1ull);
+                if (r == 2) {
+                    uint64_t tile = slot / kTile;
+                    uint32_t pos = atomicAdd(&tile_cnt[tile], 1u);
+                    uint64_t k = tile * kTile + pos;
+                    uint64_t* dst = K.key + 9ull * k;
+                    for (int j = 0; j < 8; j++) dst[j] = ch[j];
+                    dst[8] = tm;
+                    K.hash[k] = branch_hash(ch, tm);
+                    K.slot[k] = slot;
+                    atomicAdd((unsigned long long*)stat(P.ctl, ST_CAND), 1ull);
+                    out = res_pending(0xFFFFFFFFu);
+                } else if (r == 3) {
+                    atomicOr(&P.ctl->err, kErrOom | kErrPoison);  // the leaf could not be created
+                    out = kInvalid;
                 }
             }
             res1[slot] = out;
         }
     }
-    emit_candidate(K, &s_cnt, cand, ch, tm, slot);
-    tile_count(s_st, 0, cand);
-    tile_count(s_st, 1, collapsed);
-    tile_count(s_st, 2, h2);
-    tile_count(s_st, 3, h3);
-    tile_count(s_st, 4, h4);
-    __syncthreads();
-    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_cnt;
-    const int ids[5] = {ST_CAND, ST_COLLAPSE, ST_H2, ST_H3, ST_H4};
-    if (threadIdx.x < 5 && s_st[threadIdx.x])
-        atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
+    uint32_t ndr = P.ctl->n_drift;
+    VX_LOOP(j, ndr) {
+        if (j < ndr) {
+            Drift d = drift[j];
+            if (chunks[d.chunk].status >= 0) {
+                atomicAdd(&P.refs[d.idx], (uint32_t)d.delta);
+                atomicAdd((unsigned long long*)stat(P.ctl, (int)d.stat), 1ull);
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------ interning
@@ -577,12 +670,24 @@ __device__ inline int probe_table(const Pool<T>& P, const Cands& K, uint64_t k, 
 
 template <typename T>
 __global__ __launch_bounds__(kTile) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
-                                                 uint64_t* res) {
+                                                 uint64_t* res, const ChunkDev* chunks, LevelInfo L) {
     uint32_t n = tile_cnt[blockIdx.x];
     if ((threadIdx.x & ~63u) >= n) return;  // whole wavefront without candidates
     int lane = (int)__lane_id();
     bool valid = threadIdx.x < n;
     uint64_t k = (uint64_t)blockIdx.x * kTile + threadIdx.x;
+    if (valid && L.h == 1) {
+        // a brick of a tree that hit the leaf-brick panic later in the same pass
+        uint64_t slot = K.slot[k];
+        uint32_t g = group_of(L, slot);
+        uint32_t c = L.chunk_base[g] + (uint32_t)((slot - L.slot_base[g]) >> (3 * (L.depth[g] - 1)));
+        if (chunks[c].status < 0) {
+            res[slot] = kInvalid;
+            K.id[k] = kInvalid;
+            K.tslot[k] = ~0ull;
+            valid = false;
+        }
+    }
     uint64_t key[9], h = 0;
 #pragma unroll
     for (int i = 0; i < 9; i++) key[i] = valid ? K.key[9ull * k + i] : 0ull;
@@ -631,36 +736,44 @@ __global__ __launch_bounds__(kTile) void k_probe(Pool<T> P, Cands K, const uint3
 // Owners write their node (children, LOD value, types/mask), publish the table
 // slot, and take one reference on each non-empty child.
 template <typename T>
-__global__ __launch_bounds__(kTile) void k_commit(Pool<T> P, Cands K, const uint32_t* tile_cnt) {
-    uint32_t n = tile_cnt[blockIdx.x];
-    if ((threadIdx.x & ~63u) >= n) return;
-    uint64_t k = (uint64_t)blockIdx.x * kTile + threadIdx.x;
-    bool own = false;
-    uint64_t key[9];
-    if (threadIdx.x < n) {
+__global__ __launch_bounds__(kTile) void k_commit(Pool<T> P, Cands K, const uint32_t* tile_cnt,
+                                                  uint64_t ntiles) {
+    __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
+    agg_init(akey, acnt);
+    __syncthreads();
+    uint32_t owned = 0;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint32_t n = tile_cnt[t];
+        if (threadIdx.x >= n) continue;
+        uint64_t k = t * kTile + threadIdx.x;
         uint64_t ts = K.tslot[k];
         uint64_t id = K.id[k];
-        if (ts != ~0ull && id != kInvalid) {
-            own = true;
-            uint32_t idx = bid_index(id);
+        if (ts == ~0ull || id == kInvalid) continue;
+        owned++;
+        uint32_t idx = bid_index(id);
+        uint64_t key[9];
 #pragma unroll
-            for (int i = 0; i < 9; i++) key[i] = K.key[9ull * k + i];
-            uint32_t vals[8];
-            uint64_t* dst = P.children + 8ull * idx;
+        for (int i = 0; i < 9; i++) key[i] = K.key[9ull * k + i];
+        uint32_t vals[8];
+        uint64_t* dst = P.children + 8ull * idx;
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
-                dst[i] = key[i];
-                vals[i] = (uint32_t)P.values[bid_index(key[i])];
-            }
-            P.tms[idx] = (uint16_t)key[8];
-            P.values[idx] = (T)calc_average(vals);
-            P.tab[ts] = slot_make(slot_tag(K.hash[k]), false, idx);
+        for (int i = 0; i < 8; i++) {
+            dst[i] = key[i];
+            vals[i] = (uint32_t)P.values[bid_index(key[i])];
+        }
+        P.tms[idx] = (uint16_t)key[8];
+        P.values[idx] = (T)calc_average(vals);
+        P.tab[ts] = slot_make(slot_tag(K.hash[k]), false, idx);
+        for (int i = 0; i < 8; i++) {
+            if (key[i] == kEmpty) continue;
+            uint32_t ci = bid_index(key[i]);
+            if (!agg_add(akey, acnt, ci, 1u)) atomicAdd(&P.refs[ci], 1u);
         }
     }
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-        wave_ref_inc(P.refs, (own && key[i] != kEmpty) ? bid_index(key[i]) : kNoIdx);
-    wave_count(&P.ctl->live_branches, own);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x)
+        if (akey[i] != kNoIdx && acnt[i]) atomicAdd(&P.refs[akey[i]], acnt[i]);
+    wave_add64(&P.ctl->live_branches, owned);
 }
 
 // ------------------------------------------------------------------ Phase 2
@@ -862,9 +975,13 @@ __global__ void k_leaf_cleanup(Pool<T> P) {
 // whose count already reached zero.
 template <typename T>
 __global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uint32_t* next,
-                           bool decrement_self, uint64_t bound, uint64_t front_cap) {
+                           bool decrement_self, uint64_t front_cap) {
+    __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
+    agg_init(akey, acnt);
+    __syncthreads();
     uint64_t n = P.ctl->gc_count[round];
     if (n > front_cap) n = front_cap;
+    uint32_t* next_count = &P.ctl->gc_count[round + 1];
     VX_LOOP(i, n) {
         uint32_t idx = kNoIdx;
         if (i < n) {
@@ -873,15 +990,13 @@ __global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uin
         }
         bool freed = decrement_self ? wave_ref_dec_zero(P.refs, idx) : idx != kNoIdx;
         bool is_leaf = false;
-        uint64_t ch[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) ch[j] = kEmpty;
         if (freed) {
             uint32_t tm = P.tms[idx];
             is_leaf = tm == 0;
             if (is_leaf) {
                 P.leaf_map[(uint32_t)P.values[idx]] = 0;
             } else {
+                uint64_t ch[8];
                 uint64_t* src = P.children + 8ull * idx;
 #pragma unroll
                 for (int j = 0; j < 8; j++) ch[j] = src[j];
@@ -901,21 +1016,21 @@ __global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uin
 #pragma unroll
                 for (int j = 0; j < 8; j++) src[j] = kEmpty;
                 P.tms[idx] = 0;
+                for (int j = 0; j < 8; j++) {
+                    if (ch[j] == kEmpty) continue;
+                    uint32_t ci = bid_index(ch[j]);
+                    if (agg_add(akey, acnt, ci, 1u)) continue;
+                    if (atomicSub(&P.refs[ci], 1u) == 1u) {
+                        uint32_t pos = atomicAdd(next_count, 1u);
+                        if (pos < front_cap) next[pos] = ci;
+                        else atomicOr(&P.ctl->err, kErrPoison);
+                    }
+                }
             }
             P.values[idx] = 0;
             P.refs[idx] = 0;
             uint32_t g = P.gens[idx] + 1u;
             P.gens[idx] = (uint16_t)(g >= kMaxGeneration ? 0u : g);
-        }
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            uint32_t ci = ch[j] != kEmpty ? bid_index(ch[j]) : kNoIdx;
-            bool push = wave_ref_dec_zero(P.refs, ci);
-            uint32_t pos = wave_append(&P.ctl->gc_count[round + 1], push);
-            if (push) {
-                if (pos < front_cap) next[pos] = ci;
-                else atomicOr(&P.ctl->err, kErrPoison);
-            }
         }
         uint32_t fpos = wave_append(&P.ctl->free_top, freed);
         if (freed) {
@@ -930,6 +1045,17 @@ __global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uin
                 atomicAdd((unsigned long long*)&P.ctl->live_branches, (unsigned long long)(-(long long)__popcll(bb)));
                 atomicAdd((unsigned long long*)&P.ctl->tombstones, (unsigned long long)__popcll(bb));
             }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
+        bool push = false;
+        uint32_t key = akey[i];
+        if (key != kNoIdx && acnt[i]) push = atomicSub(&P.refs[key], acnt[i]) == acnt[i];
+        uint32_t pos = wave_append(next_count, push);
+        if (push) {
+            if (pos < front_cap) next[pos] = key;
+            else atomicOr(&P.ctl->err, kErrPoison);
         }
     }
 }
