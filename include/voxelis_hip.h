@@ -49,7 +49,7 @@ typedef struct {
     uint8_t depth;         /* MaxDepth of the target tree */
     uint8_t has_fill;
     uint8_t has_patches;   /* Batch::has_patches (batch.rs:127-132) */
-    uint8_t reserved;
+    uint8_t reserved;      /* must be 0 */
 } vx_batch_view;
 
 /* ---- interner: interner/mod.rs ---- */
@@ -111,6 +111,19 @@ int vx_trees_apply_batches(vx_interner* h, size_t n, uint64_t* roots,
  * BlockId, updated in place; *changed receives apply_batch's bool.  Synchronous. */
 int vx_tree_apply_batch_host(vx_interner* h, uint64_t* root, const vx_batch_view* batch,
                              int* changed);
+
+/* Edit-list form of vx_trees_apply_batches: tree t's edits are
+ * xyz[3e..3e+3) / values[e] for e in [edit_offsets[t], edit_offsets[t+1]) in
+ * submission order, as if Batch::just_set (core/batch.rs:145-175) had been called
+ * for each on a fresh Batch of that tree; a tree with no edits is not applied
+ * (status 0).  The edits are radix-sorted by (tree, Morton path) on the device;
+ * the last write to a voxel wins.  depths and edit_offsets (n+1 entries) are
+ * host arrays; roots, xyz, values and status are device arrays.  A coordinate
+ * outside the tree fails that tree with VX_E_RANGE (the reference's Vec index
+ * panics).  Asynchronous. */
+int vx_trees_apply_edits(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* depths,
+                         const uint64_t* edit_offsets, const int32_t* xyz, const void* values,
+                         int32_t* status);
 
 /* ---- batched reads: VoxOpsRead::get (voxtree.rs:145-159, utils/common.rs:122-156) ---- */
 

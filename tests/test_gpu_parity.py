@@ -225,3 +225,100 @@ def test_terrain_small_grid_and_alternation(vx):
         assert_same(it, f, it_o, trees_o, full_reads=(step < 2), what=f"step {step}")
     s = it.stats()
     assert s["h4_undercount"] == 0 and s["h5_collapse_cloned"] == 0
+
+
+def test_uneven_u16_mixed_depths_with_depth7(vx):
+    """Config 3 in miniature: u16, depths 5/6/7, dense set_sum-style and sparse random
+    trees in one launch; depth 7 is compared with the oracle's extended depth."""
+    api, W = vx
+    spec = [(5, "dense", 0, 1), (6, "sparse", 0.01, 2), (7, "sparse", 0.001, 3), (6, "dense", 0, 4),
+            (5, "sparse", 0.1, 5), (7, "dense", 0, 6), (5, "sparse", 0.001, 7), (6, "sparse", 0.1, 8)]
+    batches = W.uneven_batches(spec)
+    it = api.VoxInterner(2, 256 * MB)
+    it_o = O.Interner(2, 256 * MB, extended_depth=True)
+    f = api.Forest(it, [d for d, _, _, _ in spec])
+    trees_o = []
+    for (d, _, _, _), (m, v, _) in zip(spec, batches):
+        t_o = O.Tree(d)
+        b = O.Batch(d)
+        b.load_dense(m, v)
+        assert t_o.apply_batch(it_o, b)
+        trees_o.append(t_o)
+    st = f.apply([api.to_device(m) for m, _, _ in batches], [api.to_device(v) for _, v, _ in batches])
+    assert (st == 1).all(), st
+    assert_same(it, f, it_o, trees_o, full_reads=False, what="uneven")
+    # full reads on the depth-5/6 trees, a random sample on the depth-7 ones
+    rng = np.random.default_rng(0)
+    for i, t_o in enumerate(trees_o):
+        if t_o.depth <= 6:
+            g = full_reads_gpu(f, i, t_o.depth)
+            o = O.all_coords(t_o.depth)
+            ov, op = t_o.get_many(it_o, o)
+            assert np.array_equal(g, np.where(op == 1, ov, 0)), i
+        else:
+            xyz = rng.integers(0, 128, size=(20000, 3)).astype(np.int32)
+            gv, gp = f.get(np.full(len(xyz), i, dtype=np.uint32), xyz)
+            ov, op = t_o.get_many(it_o, xyz)
+            assert np.array_equal(np.where(gp == 1, gv, 0), np.where(op == 1, ov, 0)), i
+    # replace every tree with a shifted dense fill (exercises GC of mixed depths)
+    batches2 = [W.dense_to_batch((W.uneven_dense(d) % 7 + 1).astype(np.uint16), d) for d, _, _, _ in spec]
+    for t_o, (m, v) in zip(trees_o, batches2):
+        b = O.Batch(t_o.depth)
+        b.load_dense(m, v)
+        assert t_o.apply_batch(it_o, b)
+    st = f.apply([api.to_device(m) for m, _ in batches2], [api.to_device(v) for _, v in batches2])
+    assert (st == 1).all(), st
+    assert_same(it, f, it_o, trees_o, full_reads=False, what="uneven replaced")
+
+
+def test_edit_lists_sorted_on_device(vx):
+    """vx_trees_apply_edits against the oracle's Batch::just_set in submission order:
+    duplicates (last write wins), default values (clears, ignored by apply), trees
+    without edits, mixed depths, u16, and an out-of-range coordinate."""
+    api, W = vx
+    rng = np.random.default_rng(42)
+    depths = [5, 2, 6, 5, 3, 7, 5, 4]
+    it = api.VoxInterner(2, 256 * MB)
+    it_o = O.Interner(2, 256 * MB, extended_depth=True)
+    f = api.Forest(it, depths)
+    trees_o = [O.Tree(d) for d in depths]
+    for rnd in range(3):
+        xyzs, vals, offs = [], [], [0]
+        expect = []
+        for t, d in enumerate(depths):
+            n = 0 if (t == 1 and rnd == 0) else int(rng.integers(1, 3000))
+            n_side = 1 << d
+            xyz = rng.integers(0, n_side, size=(n, 3)).astype(np.int32)
+            if n:
+                xyz[n // 2:] = xyz[: n - n // 2]  # many duplicate voxels
+            v = rng.integers(0, 9, size=n).astype(np.uint16)  # includes 0 = clear
+            bad = (t == 4 and rnd == 1)
+            if bad:
+                xyz[n // 3] = [n_side, 0, 0]
+            xyzs.append(xyz)
+            vals.append(v)
+            offs.append(offs[-1] + n)
+            expect.append(-4 if bad else None)
+        xyz_all = np.concatenate(xyzs)
+        val_all = np.concatenate(vals)
+        st = f.apply_edits(offs, api.to_device(xyz_all), api.to_device(val_all))
+        for t, d in enumerate(depths):
+            if expect[t] == -4:
+                assert st[t] == -4
+                continue
+            if len(xyzs[t]) == 0:
+                assert st[t] == 0
+                continue
+            b = O.Batch(d)
+            b.set_many(xyzs[t], vals[t].astype(np.uint64))
+            try:
+                ch = trees_o[t].apply_batch(it_o, b)
+                assert st[t] == (1 if ch else 0), (rnd, t, st[t], ch)
+            except O.OraclePanic as e:
+                assert (e.kind, st[t]) in (("SAME_ROOT", -6), ("LEAF_TYPES", -5)), (e.kind, st[t])
+        assert_same(it, f, it_o, trees_o, full_reads=False, what=f"edits round {rnd}")
+        for i, t_o in enumerate(trees_o):
+            if t_o.depth <= 5:
+                assert np.array_equal(full_reads_gpu(f, i, t_o.depth),
+                                      np.where(t_o.get_many(it_o, O.all_coords(t_o.depth))[1] == 1,
+                                               t_o.get_many(it_o, O.all_coords(t_o.depth))[0], 0))

@@ -44,6 +44,7 @@ SIGNATURES = {
     "vx_trees_apply_batches": (_int, [_vp, _sz, _vp, C.POINTER(BatchView), _vp]),
     "vx_tree_apply_batch_host": (_int, [_vp, C.POINTER(_u64), C.POINTER(BatchView),
                                         C.POINTER(_int)]),
+    "vx_trees_apply_edits": (_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp]),
     "vx_trees_get": (_int, [_vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
     "vx_trees_fill": (_int, [_vp, _sz, _vp, _vp]),
     "vx_trees_canonical_hash": (_int, [_vp, _sz, _vp, _vp]),

@@ -286,6 +286,21 @@ class Forest:
             return self.status.cpu().numpy()
         return None
 
+    def apply_edits(self, offsets, xyz_dev, values_dev, sync=True):
+        """vx_trees_apply_edits: tree t's edits are [offsets[t], offsets[t+1]) of the device
+        arrays xyz_dev (int32 [E,3]) and values_dev (value dtype [E]), in submission order."""
+        offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+        assert len(offs) == self.n + 1
+        check(lib().vx_trees_apply_edits(self.it.h, self.n, self.roots.data_ptr(),
+                                         self.depths_host.ctypes.data, offs.ctypes.data,
+                                         xyz_dev.data_ptr() if xyz_dev is not None else None,
+                                         values_dev.data_ptr() if values_dev is not None else None,
+                                         self.status.data_ptr()), "apply_edits")
+        if sync:
+            self.it.sync()
+            return self.status.cpu().numpy()
+        return None
+
     def get(self, chunk_of, xyz, sync=True):
         nq = len(chunk_of)
         dev = self.it.device

@@ -13,6 +13,7 @@
 
 #include "../../include/voxelis_hip.h"
 #include "vx_device.h"
+#include "vx_edits.hip"
 #include "vx_kernels.hip"
 
 using namespace vx;
@@ -107,6 +108,8 @@ struct vx_interner {
     int poisoned = 0;
     // workspace
     Grow chunks, slot_off, ptr_tables, gc_front[2], stage, scratch, deferred, drift;
+    // edit-list path: zero-kept dense Batch buffers, sort buffers, tree records
+    Grow edit_ws, edit_keys[2], edit_order[2], edit_tmp, edit_trees, edit_first, edit_status;
     LevelBufs lv[kLevels + 1];
     // pinned staging ring: a slot is reused only after its copies completed
     static constexpr int kRing = 4;
@@ -183,6 +186,9 @@ int err_to_status(uint32_t err) {
 
 uint64_t slots_at(uint32_t d, uint32_t h) { return 1ull << (3 * (d - h)); }
 
+// vx_batch_view.reserved value used internally for trees that are not applied
+constexpr uint8_t kViewSkip = 0x5A;
+
 hipEvent_t prof_event(vx_interner* h) {
     Prof& p = h->prof;
     if (p.used == p.pool.size()) {
@@ -213,7 +219,7 @@ hipEvent_t prof_event(vx_interner* h) {
 
 template <typename T>
 int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
-               int32_t* status) {
+               int32_t* status, const int32_t* pre_status = nullptr) {
     hipStream_t s = h->stream;
     Pool<T> P = pool_of<T>(h);
     // ---- plan: trees with patches grouped by depth (others first), dense slot
@@ -227,7 +233,7 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
     };
     auto sort_key = [&](uint32_t i) -> uint32_t {
         const vx_batch_view& b = batches[i];
-        return (b.has_patches && status_of(b) == 0) ? b.depth : 0u;
+        return (b.has_patches && status_of(b) == 0 && b.reserved != kViewSkip) ? b.depth : 0u;
     };
     std::vector<uint32_t> perm(n);
     for (size_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
@@ -252,8 +258,9 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
         C.orig = perm[c];
         C.depth = b.depth;
         C.has_fill = b.has_fill ? 1 : 0;
+        C.skip = b.reserved == kViewSkip ? 1 : 0;
         C.status = status_of(b);
-        C.has_patches = (b.has_patches && C.status == 0) ? 1 : 0;
+        C.has_patches = (b.has_patches && C.status == 0 && !C.skip) ? 1 : 0;
         if (!C.has_patches) continue;
         uint32_t d = b.depth;
         dmax = std::max<uint32_t>(dmax, d);
@@ -336,7 +343,8 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
     Drift* ddrift = (Drift*)h->drift.ptr;
     // ---- launch sequence
     VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
-    VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, grid_for(n), kBlock, P, dchunks, (uint32_t)n, roots);
+    VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, grid_for(n), kBlock, P, dchunks, (uint32_t)n, roots,
+              pre_status);
     unsigned t1 = (unsigned)ntiles[1];
     if (dmax >= 2)
         VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T>, t1, kTile, P, dchunks, L[1], res_ptrs_h[1], K[1],
@@ -375,6 +383,87 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
     }
     if (hipGetLastError() != hipSuccess) return VX_E_HIP;
     return VX_OK;
+}
+
+template <typename T>
+int apply_edits_impl(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* depths,
+                     const uint64_t* offsets, const int32_t* xyz, const void* values, int32_t* status) {
+    hipStream_t s = h->stream;
+    uint64_t E = offsets[n];
+    std::vector<EditTree> trees(n);
+    std::vector<uint64_t> first(n);
+    uint64_t mbytes = 0, vcount = 0;
+    for (size_t t = 0; t < n; t++) {
+        uint32_t d = depths[t];
+        if (offsets[t + 1] < offsets[t]) return VX_E_INVALID_ARG;
+        uint64_t P = d > 0 ? (1ull << (3 * (d - 1))) : 1ull;
+        trees[t].first = offsets[t];
+        trees[t].depth = d;
+        trees[t].status = 0;
+        bool has = offsets[t + 1] > offsets[t];
+        if (d > (uint32_t)kMaxDepth || (has && d < 2)) trees[t].status = VX_E_DEPTH;
+        trees[t].mbase = mbytes;
+        trees[t].vbase = vcount;
+        first[t] = offsets[t];
+        if (d <= (uint32_t)kMaxDepth) {
+            mbytes += ((2 * P + 255) / 256) * 256;
+            vcount += ((8 * P + 255) / 256) * 256;
+        }
+    }
+    size_t ws_bytes = mbytes + vcount * sizeof(T);
+    if (ws_bytes > h->edit_ws.cap) {
+        if (!h->edit_ws.ensure(ws_bytes)) return VX_E_HIP;
+        HIPCHK(hipMemsetAsync(h->edit_ws.ptr, 0, h->edit_ws.cap, s));  // kept zero from here on
+    }
+    uint8_t* mws = (uint8_t*)h->edit_ws.ptr;
+    T* vws = (T*)(mws + mbytes);
+    uint64_t En = std::max<uint64_t>(E, 1);
+    if (!h->edit_keys[0].ensure(En * 8) || !h->edit_keys[1].ensure(En * 8) ||
+        !h->edit_order[0].ensure(En * 4) || !h->edit_order[1].ensure(En * 4) ||
+        !h->edit_trees.ensure(n * sizeof(EditTree)) || !h->edit_first.ensure(n * 8) ||
+        !h->edit_status.ensure(n * 4))
+        return VX_E_HIP;
+    EditTree* dtrees = (EditTree*)h->edit_trees.ptr;
+    uint64_t* dfirst = (uint64_t*)h->edit_first.ptr;
+    int32_t* dpre = (int32_t*)h->edit_status.ptr;
+    HIPCHK(hipMemcpyAsync(dtrees, trees.data(), n * sizeof(EditTree), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dfirst, first.data(), n * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));  // the host vectors above go out of scope
+    uint64_t* k0 = (uint64_t*)h->edit_keys[0].ptr;
+    uint64_t* k1 = (uint64_t*)h->edit_keys[1].ptr;
+    uint32_t* o0 = (uint32_t*)h->edit_order[0].ptr;
+    uint32_t* o1 = (uint32_t*)h->edit_order[1].ptr;
+    unsigned tree_bits = 1;
+    while ((1ull << tree_bits) <= n) tree_bits++;
+    unsigned end_bit = kPathBits + tree_bits;
+    if (E > 0) {
+        k_edit_keys<<<grid_for(E), kBlock, 0, s>>>(E, dfirst, (uint32_t)n, dtrees, xyz, k0, o0);
+        size_t tmp_bytes = 0;
+        HIPCHK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, k0, k1, o0, o1, (size_t)E, 0, end_bit, s));
+        if (!h->edit_tmp.ensure(tmp_bytes + 16)) return VX_E_HIP;
+        HIPCHK(rocprim::radix_sort_pairs(h->edit_tmp.ptr, tmp_bytes, k0, k1, o0, o1, (size_t)E, 0, end_bit,
+                                         s));
+        k_edit_scatter<T><<<grid_for(E), kBlock, 0, s>>>(E, (uint32_t)n, dtrees, k1, o1, (const T*)values, mws,
+                                                         vws, false);
+    }
+    // per-tree failures recorded by k_edit_keys go in as pre-statuses
+    k_edit_status<<<grid_for(n), kBlock, 0, s>>>((uint32_t)n, dtrees, dpre);
+    std::vector<vx_batch_view> views(n);
+    for (size_t t = 0; t < n; t++) {
+        vx_batch_view& v = views[t];
+        std::memset(&v, 0, sizeof(v));
+        v.depth = depths[t];
+        v.has_patches = offsets[t + 1] > offsets[t] ? 1 : 0;
+        if (!v.has_patches) v.reserved = kViewSkip;
+        v.masks = mws + trees[t].mbase;
+        v.values = vws + trees[t].vbase;
+    }
+    int rc = apply_impl<T>(h, n, roots, views.data(), status, dpre);
+    if (rc) return rc;
+    if (E > 0)
+        k_edit_scatter<T><<<grid_for(E), kBlock, 0, s>>>(E, (uint32_t)n, dtrees, k1, o1, (const T*)values, mws,
+                                                         vws, true);
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
 }
 
 template <typename T>
@@ -513,6 +602,9 @@ int vx_interner_destroy(vx_interner* h) {
     h->scratch.release();
     h->deferred.release();
     h->drift.release();
+    for (Grow* g : {&h->edit_ws, &h->edit_keys[0], &h->edit_keys[1], &h->edit_order[0], &h->edit_order[1],
+                    &h->edit_tmp, &h->edit_trees, &h->edit_first, &h->edit_status})
+        g->release();
     for (auto& b : h->lv) {
         b.res.release();
         b.key.release();
@@ -612,6 +704,8 @@ int vx_trees_apply_batches(vx_interner* h, size_t n, uint64_t* roots, const vx_b
     if (h->poisoned) return VX_E_POISONED;
     if (n == 0) return VX_OK;
     if (n > 0x7FFFFFFFull) return VX_E_INVALID_ARG;
+    for (size_t i = 0; i < n; i++)
+        if (batches[i].reserved != 0) return VX_E_INVALID_ARG;
     return h->vsize == 1 ? apply_impl<uint8_t>(h, n, roots, batches, status)
                          : apply_impl<uint16_t>(h, n, roots, batches, status);
 }
@@ -650,6 +744,19 @@ int vx_tree_apply_batch_host(vx_interner* h, uint64_t* root, const vx_batch_view
     }
     *changed = st;
     return VX_OK;
+}
+
+int vx_trees_apply_edits(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* depths,
+                         const uint64_t* edit_offsets, const int32_t* xyz, const void* values,
+                         int32_t* status) {
+    if (!h || (n && (!roots || !depths || !edit_offsets || !status))) return VX_E_INVALID_ARG;
+    if (h->poisoned) return VX_E_POISONED;
+    if (n == 0) return VX_OK;
+    if (n > 0x7FFFFFFFull) return VX_E_INVALID_ARG;
+    if (edit_offsets[n] > 0 && (!xyz || !values)) return VX_E_INVALID_ARG;
+    return h->vsize == 1
+               ? apply_edits_impl<uint8_t>(h, n, roots, depths, edit_offsets, xyz, values, status)
+               : apply_edits_impl<uint16_t>(h, n, roots, depths, edit_offsets, xyz, values, status);
 }
 
 int vx_trees_get(vx_interner* h, size_t ntrees, const uint64_t* roots, const uint8_t* depths,

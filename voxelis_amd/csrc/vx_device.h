@@ -115,7 +115,8 @@ struct ChunkDev {
     uint8_t depth;
     uint8_t has_fill;
     uint8_t has_patches;
-    uint8_t pad[5];
+    uint8_t skip;  // not applied at all (an edit-list tree without edits)
+    uint8_t pad[4];
 };
 
 // Slot space of one level: groups of trees of equal depth, each tree owning

@@ -250,12 +250,14 @@ __global__ void k_end_alloc(Ctl* ctl) {
 
 // Read roots, mark the fill leaf as needed (Phase 0, voxtree.rs:742-754).
 template <typename T>
-__global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_t* roots) {
+__global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_t* roots,
+                          const int32_t* pre_status) {
     VX_LOOP(c, n) {
         if (c < n) {
             ChunkDev& C = chunks[c];
             C.root_in = roots[C.orig];
             C.init = C.root_in;
+            if (pre_status && pre_status[C.orig] < 0 && C.status >= 0) C.status = pre_status[C.orig];
             if (C.status >= 0 && C.has_fill) {
                 uint32_t v = (uint32_t)C.fill;
                 if (P.leaf_map[v] == 0) P.leaf_need[v] = 1;
@@ -913,7 +915,9 @@ __global__ void k_finalize(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64
         if (c < n) {
             ChunkDev& C = chunks[c];
             int32_t st = C.status;
-            if (st >= 0) {
+            if (C.skip) {
+                st = 0;
+            } else if (st >= 0) {
                 uint64_t nw;
                 if (!C.has_patches) {
                     nw = C.init;

@@ -174,3 +174,26 @@ def uneven_tree_edits(depth, kind, density, seed):
     xyz = rng.integers(0, n, size=(k, 3)).astype(np.int32)
     vals = rng.integers(1, 65536, size=k).astype(np.uint16)  # 0 excluded (a clear is a no-op)
     return xyz, vals
+
+
+def uneven_dense(depth):
+    """Dense (y,z,x) u16 voxels of a dense tree: ((x+y+z) mod 65535) + 1."""
+    n = 1 << depth
+    y, z, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    return (((x + y + z) % 65535) + 1).astype(np.uint16)
+
+
+def uneven_batches(spec):
+    """Per tree (masks [P,2] u8, values [P,8] u16, edits) for an uneven_spec list."""
+    from .morton import apply_edits_to_dense, dense_batch_arrays
+    out = []
+    for depth, kind, density, seed in spec:
+        if kind == "dense":
+            m, v = dense_to_batch(uneven_dense(depth), depth)
+            out.append((m, v, 1 << (3 * depth)))
+        else:
+            xyz, vals = uneven_tree_edits(depth, kind, density, seed)
+            m, v = dense_batch_arrays(depth, np.uint16)
+            apply_edits_to_dense(m, v, xyz, vals)
+            out.append((m, v, len(vals)))
+    return out
