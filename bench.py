@@ -42,6 +42,18 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--rates", default="200,1000,4000",
+                    help="serve: Poisson arrival rates (requests/s) to sweep")
+    ap.add_argument("--requests", type=int, default=1500, help="serve: requests per rate")
+    ap.add_argument("--train-requests", type=int, default=1500,
+                    help="serve: requests logged (with exact dry-run counts) to train the predictor")
+    ap.add_argument("--config", default="terrain", choices=["terrain", "plain", "uneven", "gets", "serve"],
+                    help="terrain: the headline (64x64 Perlin-dune trees); plain: one 32^3 tree; "
+                         "uneven: u16 trees of depth 5/6/7, dense or sparse; gets: batched reads")
+    ap.add_argument("--trees", type=int, default=4096)
+    ap.add_argument("--queries", type=int, default=1 << 24)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 1 GiB device copy (a known-size kernel for counter scaling)")
     return ap.parse_args()
 
 
@@ -59,13 +71,11 @@ def kernel_bytes(name, launches, st, counts):
     slots_up = counts["upper_slots"] * steps
     s = counts["value_size"]
     d = counts["depth"]
-    if name == "k_brick_scan":
-        # set masks (2 B/slot) + touched values (8s) + old-tree path (8 B per level) + slot write
-        return slots1 * 2 + touched * (8 * s + 8 * (d - 1)) + slots1 * 8
     if name == "k_brick_build":
-        # masks + slot read/write; touched: values + base node children (64 B);
-        # candidates: key (72 B) + hash (8) + slot index (8)
-        return slots1 * (2 + 16) + touched * (8 * s + 64) + cand1 * 88
+        # every brick slot: its set mask (2 B of the mask pair) and the slot write (8);
+        # touched bricks: 8 values; non-uniform bricks: old-tree path (8 B per level),
+        # base node children (64) and their values (8s), candidate key + hash + slot (88)
+        return slots1 * (2 + 8) + touched * 8 * s + cand1 * (8 * (d - 1) + 64 + 8 * s + 88)
     if name in ("k_probe[bricks]", "k_probe[upper]"):
         c = cand1 if name == "k_probe[bricks]" else cand_up
         # key + hash + slot read (88), one table slot (8), the matched node's key (66),
@@ -74,10 +84,17 @@ def kernel_bytes(name, launches, st, counts):
     if name == "k_level_build":
         # 8 child slots (64 B) + slot write (8) per slot; candidate key + hash + slot (88)
         return slots_up * 72 + cand_up * 88
+    if name in ("k_commit[bricks]", "k_commit[upper]"):
+        c = cand1 if name == "k_commit[bricks]" else cand_up
+        owners = st.get("branch_misses", 0)
+        # every candidate: owner slot + id (16); owners: key + hash (80), node write (64 + 2 + s),
+        # table slot (8), 8 child values (8s) and refcount read-modify-writes (8 x 8)
+        return c * 16 + owners * (80 + 66 + s + 8 + 8 * s + 64)
     if name == "k_gc_round":
         freed = st.get("recycled", 0)
-        # per freed node: children (64) + refcount RMW of children (8 x 8) + columns (16)
-        return freed * (64 + 64 + 16)
+        # per freed node: type + children read and clear (2 + 128), pattern slot read + tombstone (16),
+        # child refcount read-modify-writes (64), value/ref/gen/free-stack writes (s + 10)
+        return freed * (2 + 128 + 16 + 64 + s + 10)
     return None
 
 
@@ -87,6 +104,14 @@ def main():
         print(f"bench.py measures one GPU (this build has no multi-GPU path); got --gpus {a.gpus}",
               file=sys.stderr)
         sys.exit(2)
+    if a.config == "plain":
+        return bench_plain(a)
+    if a.config == "uneven":
+        return bench_uneven(a)
+    if a.config == "gets":
+        return bench_gets(a)
+    if a.config == "serve":
+        return bench_serve(a)
     import torch
     from voxelis_amd import api
     from voxelis_amd import workloads as W
@@ -120,10 +145,7 @@ def main():
     assert (st_w == 1).all(), f"warmup status {np.unique(st_w)}"
     print(f"[bench] warmup done ({a.warmup} steps)", file=sys.stderr, flush=True)
 
-    counts_before = it.stats()
-    if not a.no_profile:
-        it.profile(True)
-        it.profile_read()
+    # ---- timed steps (no per-kernel events: value and latency percentiles)
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(a.steps)]
     torch.cuda.synchronize()
@@ -136,12 +158,8 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     it.sync()
-    prof = it.profile_read() if not a.no_profile else {}
-    it.profile(False)
     st_t = f.status.cpu().numpy()
     assert (st_t == 1).all(), f"timed status {np.unique(st_t)}"
-    counts_after = it.stats()
-    st = {k: counts_after[k] - counts_before[k] for k in counts_after}
     lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
     wall = t1 - t0
     value = E * a.steps / wall
@@ -149,6 +167,22 @@ def main():
     print(f"[bench] batched: {value / 1e9:.2f} G edits/s, {1e3 * wall / a.steps:.3f} ms/step, "
           f"p50 {np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms; "
           f"patterns {leaves} leaves {branches} branches", file=sys.stderr, flush=True)
+
+    # ---- the same steps again with per-kernel HIP events on the library's stream
+    prof = {}
+    counts_before = it.stats()
+    if not a.no_profile:
+        it.profile(True)
+        for k in range(2):  # creates the event pool
+            f.apply_views(viewsA if (a.warmup + a.steps + k) % 2 == 0 else viewsB)
+        it.profile_read()
+        counts_before = it.stats()
+        for k in range(a.steps):
+            f.apply_views(viewsA if (a.warmup + a.steps + k) % 2 == 0 else viewsB)
+        prof = it.profile_read()
+        it.profile(False)
+    counts_after = it.stats()
+    st = {k: counts_after[k] - counts_before[k] for k in counts_after}
 
     # ---- roofline of the dominant kernel (HIP events on the library's stream)
     roofline = None
@@ -180,6 +214,61 @@ def main():
                         "traffic": None, "avg_launch_ms": round(ms / n, 4),
                         "algorithmic_bytes_per_launch": int(per_launch_bytes)}
         print(f"[bench] kernels: {json.dumps(kern)}", file=sys.stderr, flush=True)
+
+    # ---- hash-consing passes against HBM, by the single algorithmic-bytes formula:
+    # B_edit = E (4 + s); B_hc = N_cand (64+16+8) + N_leafcand (s+16+8)
+    #        + N_new (64+s+8+4+2+16) + N_ref 8, over the probe / commit / build passes
+    hc = None
+    if prof:
+        s_ = 1
+        n_cand = st["candidates"] / a.steps
+        n_leafcand = E  # every set voxel looks its leaf up
+        n_new = (st["branch_misses"] + st["leaf_misses"]) / a.steps
+        n_ref = 8 * st["branch_misses"] / a.steps + n_new  # child refs of new nodes + frees
+        b_edit = E * (4 + s_)
+        b_hc = n_cand * 88 + n_leafcand * (s_ + 24) + n_new * (64 + s_ + 8 + 4 + 2 + 16) + n_ref * 8
+        hc_kern = ("k_brick_build", "k_brick_retry", "k_probe[bricks]", "k_commit[bricks]", "k_level_build",
+                   "k_probe[upper]", "k_commit[upper]")
+        t_hc = sum(prof[k][0] for k in hc_kern if k in prof) / a.steps / 1e3
+        if t_hc > 0:
+            ach = (b_edit + b_hc) / t_hc / 1e9
+            hc = {"bytes_edit": int(b_edit), "bytes_hash_consing": int(b_hc), "kernel_s": t_hc,
+                  "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(ach / HBM_PEAK_GBS, 4), "kernels": list(hc_kern),
+                  "n_cand": int(n_cand), "n_new": int(n_new)}
+            print(f"[bench] hash-consing passes: {ach:.0f} GB/s = {100 * ach / HBM_PEAK_GBS:.1f} % of HBM peak "
+                  f"({(b_edit + b_hc) / 1e9:.2f} GB in {1e3 * t_hc:.3f} ms)", file=sys.stderr, flush=True)
+
+    # ---- atomic-throughput ceilings (project microbenchmark) next to the apply's rates
+    atomics = None
+    if not a.no_profile:
+        import ctypes as C
+        from voxelis_amd._lib import lib
+        slots = 1 << 27
+        buf = torch.zeros(slots, dtype=torch.int64, device=dev)
+        ms = C.c_float(0)
+        atomics = {}
+        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for kind, name in ((0, "cas64_random"), (1, "add32_random"), (2, "add32_64_hot")):
+            ops = 1 << 28
+            lib().vx_bench_atomics(stream, kind, buf.data_ptr(), slots, ops, C.byref(ms))  # warm
+            buf.zero_()
+            rc = lib().vx_bench_atomics(stream, kind, buf.data_ptr(), slots, ops, C.byref(ms))
+            if rc == 0 and ms.value > 0:
+                atomics[name + "_ops_per_s"] = ops / (ms.value / 1e3)
+        del buf
+        t_probe = sum(prof[k][0] for k in ("k_probe[bricks]", "k_probe[upper]") if k in prof) / a.steps / 1e3
+        if t_probe > 0:
+            atomics["apply_probe_cas_per_s"] = (st["branch_misses"] + st["branch_dups"]) / a.steps / t_probe
+        print(f"[bench] atomics: {json.dumps({k: f'{v:.3g}' for k, v in atomics.items()})}",
+              file=sys.stderr, flush=True)
+
+    if a.calib_copy:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ---- plain per-tree form on the same GPU (one apply call per tree)
     plain = None
@@ -246,9 +335,264 @@ def main():
         "patterns": {"leaves": leaves, "branches": branches},
         "roofline": roofline,
         "kernels": kern,
+        "hash_consing_roofline": hc,
+        "atomics": atomics,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def _timed_steps(f, views_cycle, steps, warmup, it, torch):
+    for w in range(warmup):
+        f.apply_views(views_cycle[w % len(views_cycle)])
+    it.sync()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        evs[k][0].record()
+        f.apply_views(views_cycle[(warmup + k) % len(views_cycle)])
+        evs[k][1].record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    it.sync()
+    lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+    return t1 - t0, lat
+
+
+def bench_plain(a):
+    """Config 1: one 32^3 u8 tree, v = ((x+y+z)&1)+1, apply then 32768 gets."""
+    import torch
+    from voxelis_amd import api
+    from voxelis_amd import workloads as W
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    dense = W.plain_checker(5)
+    mA, vA = W.dense_to_batch(dense, 5)
+    vB = (3 - vA.astype(np.int64)).astype(np.uint8)  # the other checker phase
+    it = api.VoxInterner(1, 256 << 20)
+    f = api.Forest(it, [5])
+    views = [f.make_views([api.to_device(mA)], [api.to_device(vA)]),
+             f.make_views([api.to_device(mA)], [api.to_device(vB)])]
+    wall, lat = _timed_steps(f, views, a.steps, a.warmup, it, torch)
+    xyz = O.all_coords(5)
+    chunk_t = torch.zeros(len(xyz), dtype=torch.int32, device="cuda")
+    xyz_t = torch.from_numpy(xyz.reshape(-1)).cuda()
+    out = torch.zeros(len(xyz), dtype=torch.uint8, device="cuda")
+    pres = torch.zeros(len(xyz), dtype=torch.uint8, device="cuda")
+    for _ in range(3):
+        f.get_device(chunk_t, xyz_t, out, pres)
+    torch.cuda.synchronize()
+    g0 = time.perf_counter()
+    for _ in range(a.steps):
+        f.get_device(chunk_t, xyz_t, out, pres)
+    torch.cuda.synchronize()
+    gets = len(xyz) * a.steps / (time.perf_counter() - g0)
+    # oracle: alternating applies on one tree and full reads
+    it_o = O.Interner(1, 256 << 20)
+    t_o = O.Tree(5)
+    bA, bB = O.Batch(5), O.Batch(5)
+    bA.load_dense(mA, vA)
+    bB.load_dense(mA, vB)
+    t_o.apply_batch(it_o, bA)
+    c0 = time.perf_counter()
+    n_o = 0
+    while time.perf_counter() - c0 < 2.0:
+        t_o.apply_batch(it_o, bB if n_o % 2 == 0 else bA)
+        n_o += 1
+    cpu_apply = 32768 * n_o / (time.perf_counter() - c0)
+    c0 = time.perf_counter()
+    t_o.get_many(it_o, xyz)
+    cpu_get = len(xyz) / (time.perf_counter() - c0)
+    print(json.dumps({
+        "metric": METRIC, "config": {"workload": "one 32^3 tree (MaxDepth 5), u8, v=((x+y+z)&1)+1 "
+                                                 "and its other phase applied alternately; 32768 gets"},
+        "value": 32768 * a.steps / wall, "unit": "edits/s", "ms_per_step": 1e3 * wall / a.steps,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "gets_per_s": gets, "counts": it.counts(),
+        "cpu_baseline": {"apply_edits_per_s": cpu_apply, "gets_per_s": cpu_get, "cores": 1,
+                         "kind": "port"}}))
+
+
+def bench_uneven(a):
+    """Config 3: u16 trees of depth 5/6/7 (uniform), dense ((x+y+z) mod 65535)+1 fills with
+    p = 0.5, else sparse uniform random edits at density 0.1/1/10 %; one launch per step;
+    steps alternate the values and the values shifted by one, so every tree is rebuilt."""
+    import torch
+    from voxelis_amd import api
+    from voxelis_amd import workloads as W
+    spec = W.uneven_spec(a.trees, depths=(5, 6, 7), p_dense=0.5, seed=a.seed + 2)
+    t_gen = time.perf_counter()
+    dev = torch.device("cuda", 0)
+    dense_buf = {}
+    mk, vA_list, vB_list, edits = [], [], [], []
+    for d, kind, dens, sd in spec:
+        if kind == "dense":
+            if d not in dense_buf:
+                m, v = W.dense_to_batch(W.uneven_dense(d), d)
+                vb = np.where(v == 65535, 1, v + 1).astype(np.uint16)
+                dense_buf[d] = (api.to_device(m, dev), api.to_device(v, dev), api.to_device(vb, dev))
+            m_t, vA_t, vB_t = dense_buf[d]
+            edits.append(1 << (3 * d))
+        else:
+            xyz, vals = W.uneven_tree_edits(d, kind, dens, sd)
+            from voxelis_amd.morton import apply_edits_to_dense, dense_batch_arrays
+            m, v = dense_batch_arrays(d, np.uint16)
+            apply_edits_to_dense(m, v, xyz, vals)
+            vb = np.where(v == 0, 0, np.where(v == 65535, 1, v + 1)).astype(np.uint16)
+            m_t, vA_t, vB_t = api.to_device(m, dev), api.to_device(v, dev), api.to_device(vb, dev)
+            edits.append(int(np.count_nonzero(m[:, 0] != 0) and np.unpackbits(m[:, 0]).sum()))
+        mk.append(m_t)
+        vA_list.append(vA_t)
+        vB_list.append(vB_t)
+    E = int(sum(edits))
+    depths = [d for d, _, _, _ in spec]
+    t_gen = time.perf_counter() - t_gen
+    print(f"[bench] uneven: {a.trees} trees, {E} edits per step, depths "
+          f"{dict(zip(*np.unique(depths, return_counts=True)))} (gen {t_gen:.1f}s)", file=sys.stderr, flush=True)
+    it = api.VoxInterner(2, a.budget_mib << 20, a.load_factor)
+    f = api.Forest(it, depths)
+    views = [f.make_views(mk, vA_list), f.make_views(mk, vB_list)]
+    if not a.no_profile:
+        it.profile(True)
+    wall, lat = _timed_steps(f, views, a.steps, a.warmup, it, torch)
+    prof = it.profile_read() if not a.no_profile else {}
+    st = f.status.cpu().numpy()
+    assert (st == 1).all(), np.unique(st)
+    e57 = sum(e for e, d in zip(edits, depths) if d <= 6)
+    out = {"metric": METRIC, "value": E * a.steps / wall, "unit": "edits/s",
+           "ms_per_step": 1e3 * wall / a.steps, "p50_ms": float(np.percentile(lat, 50)),
+           "p99_ms": float(np.percentile(lat, 99)), "dtype": "u16",
+           "config": {"workload": "uneven batched edits: depth 5/6/7 trees, dense p=0.5 or sparse "
+                                  "0.1/1/10 %, u16, one launch", "trees": a.trees, "edits_per_step": E,
+                      "edits_depth_le6": int(e57), "depth7_trees": int(sum(1 for d in depths if d == 7))},
+           "counts": it.counts(),
+           "kernels": {k: {"ms_per_step": v[0] / (a.steps + a.warmup)} for k, v in prof.items()}}
+    print(json.dumps(out))
+
+
+def bench_gets(a):
+    """Batched VoxOpsRead::get over the terrain world: uniform random voxels of random trees."""
+    import torch
+    from voxelis_amd import api
+    from voxelis_amd import workloads as W
+    grid = a.grid
+    T = grid * grid
+    mA, vA, edits, _ = W.terrain_batches(grid=grid, seed=a.seed)
+    it = api.VoxInterner(1, a.budget_mib << 20, a.load_factor)
+    f = api.Forest(it, [5] * T)
+    st = f.apply([api.to_device(mA[i]) for i in range(T)], [api.to_device(vA[i]) for i in range(T)])
+    assert (st == 1).all()
+    rng = np.random.default_rng(a.seed)
+    Q = a.queries
+    chunk = rng.integers(0, T, size=Q).astype(np.int32)
+    xyz = rng.integers(0, 32, size=(Q, 3)).astype(np.int32)
+    order = np.argsort(chunk, kind="stable")  # queries grouped by tree, as a caller batching per chunk would
+    res = {}
+    for name, idx in (("random", np.arange(Q)), ("grouped", order)):
+        chunk_t = torch.from_numpy(chunk[idx]).cuda()
+        xyz_t = torch.from_numpy(np.ascontiguousarray(xyz[idx]).reshape(-1)).cuda()
+        out = torch.zeros(Q, dtype=torch.uint8, device="cuda")
+        pres = torch.zeros(Q, dtype=torch.uint8, device="cuda")
+        it.profile(True)
+        for _ in range(3):
+            f.get_device(chunk_t, xyz_t, out, pres)
+        torch.cuda.synchronize()
+        it.profile_read()
+        g0 = time.perf_counter()
+        for _ in range(a.steps):
+            f.get_device(chunk_t, xyz_t, out, pres)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - g0
+        prof = it.profile_read()
+        ms, n = prof.get("k_get", (0.0, 1))
+        res[name] = {"gets_per_s": Q * a.steps / wall, "kernel_ms": ms / n,
+                     "kernel_gets_per_s": Q / (ms / n / 1e3)}
+    print(json.dumps({"metric": "batched get/s", "config": {"workload": f"{Q} uniform random voxel reads "
+                      f"over the 64x64 terrain world (4096 trees of 32^3)"}, "results": res}))
+
+
+def _serve_world(api, serve, W, shape, budget):
+    """u16 world of shape trees (32^3 each): dunes in the bottom layer, air above."""
+    world = serve.World(2, budget, shape, depth=5)
+    sx, sy, sz = shape
+    heights = W.terrain_heights(grid=max(sx, sz), chunk=32, seed=5)
+    trees = [world.tree_index(tx, 0, tz) for tz in range(sz) for tx in range(sx)]
+    ms, vs = [], []
+    for tz in range(sz):
+        for tx in range(sx):
+            m, v = W.dense_to_batch(W.terrain_dense(heights, tx, tz).astype(np.uint16), 5)
+            ms.append(api.to_device(m))
+            vs.append(api.to_device(v))
+    sub, idx = world.sub(trees)
+    sub.apply(ms, vs)
+    world.forest.roots[idx] = sub.roots
+    world.kind[trees] = 2
+    world.it.sync()
+    return world
+
+
+def bench_serve(a):
+    """Served stream: 16384-tree depth-5 u16 world resident in HBM (interner budget
+    64 GiB); Poisson arrivals at swept rates; 70 % get batches of 1-65536 uniform
+    random voxels, 25 % sphere edits (radius 2-40, random centre, u16 id or carving),
+    5 % whole-tree fill/clear; admission by memory budget with the learned predictor
+    (trained first on the same kind of stream, exact dry-run counts as targets)."""
+    import torch
+    from voxelis_amd import api, serve
+    from voxelis_amd import predictor as PR
+    from voxelis_amd import workloads as W
+    shape = (32, 16, 32)
+    budget = a.budget_mib << 20 if a.budget_mib != 2048 else 64 << 30
+    t0 = time.perf_counter()
+    world = _serve_world(api, serve, W, shape, budget)
+    print(f"[serve] world {world.n} trees, {world.it.alive_nodes()} nodes "
+          f"({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    # ---- predictor: log (features, exact growth) pairs, train on the GPU
+    t0 = time.perf_counter()
+    reqs = serve.make_requests(world, a.train_requests, rate=1e6, seed=a.seed + 50, carve=True, max_gets=1024)
+    logger = serve.Server(world, log_exact=True)
+    logger.serve(reqs, realtime=False)
+    X = np.stack([f for f, _ in logger.log])
+    y = np.array([g for _, g in logger.log], dtype=np.float64)
+    cut = int(0.8 * len(y))
+    pred, hist = PR.Predictor.fit(X[:cut], y[:cut], epochs=300, batch=256, lr=3e-3)
+    ph = pred.predict_features(X[cut:])
+    mae = float(np.mean(np.abs(ph - y[cut:])))
+    bound_mae = float(np.mean(np.abs(np.expm1(X[cut:, 15].astype(np.float64)) - y[cut:])))
+    under = float(np.mean(ph * 1.5 + 2 < y[cut:]))
+    print(f"[serve] predictor: {len(y)} pairs, held-out MAE {mae:.1f} nodes (hard bound {bound_mae:.1f}), "
+          f"under-predicted at margin 1.5: {100 * under:.2f} % ({time.perf_counter() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    results = []
+    for ri, rate in enumerate(float(x) for x in a.rates.split(",")):
+        reqs = serve.make_requests(world, a.requests, rate=rate, seed=a.seed + ri, carve=True)
+        srv = serve.Server(world, predictor=pred, margin=1.5)
+        torch.cuda.synchronize()
+        lat, wall = srv.serve(reqs, realtime=True)
+        edits = sum(r.edits for r in reqs if r.kind in ("sphere", "fill"))
+        gets = sum(len(r.payload[0]) for r in reqs if r.kind == "get")
+        n_edit = sum(1 for r in reqs if r.kind in ("sphere", "fill"))
+        span = max(r.done for r in reqs if r.done is not None)
+        res = {"rate": rate, "requests": len(reqs), "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+               "p99_ms": 1e3 * float(np.percentile(lat, 99)), "edits_per_s": edits / span,
+               "gets_per_s": gets / span, "cycles": srv.cycles, "wall_s": wall,
+               "rolled_back_applies": srv.fallbacks,
+               "fallback_rate": srv.fallback_requests / max(n_edit, 1), "exact_recounts": srv.recounts}
+        results.append(res)
+        print(f"[serve] rate {rate:.0f}/s: p50 {res['p50_ms']:.2f} ms p99 {res['p99_ms']:.2f} ms, "
+              f"{res['edits_per_s'] / 1e6:.1f} M edits/s, {res['gets_per_s'] / 1e6:.1f} M gets/s, "
+              f"{srv.cycles} cycles, fallback {100 * res['fallback_rate']:.2f} %", file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": "served requests: p50/p99 latency, edits/s, gets/s at swept Poisson rates",
+        "config": {"workload": "16384-tree depth-5 u16 world (32x16x32 trees, dunes in the bottom layer); "
+                               "70% gets of 1-65536 voxels, 25% spheres r 2-40 (fill or carve), 5% "
+                               "fill/clear; admission by memory budget with the learned predictor",
+                   "interner_budget_gib": budget / (1 << 30), "requests_per_rate": a.requests},
+        "predictor": {"pairs": len(y), "mae_nodes": mae, "hard_bound_mae_nodes": bound_mae,
+                      "under_prediction_rate_margin_1_5": under, "train_loss": hist[-1]},
+        "results": results}))
 
 
 if __name__ == "__main__":

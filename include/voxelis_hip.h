@@ -77,6 +77,12 @@ int vx_interner_get_children(vx_interner* h, uint64_t block_id, uint64_t childre
 int vx_interner_stats(vx_interner* h, uint64_t* out, uint32_t n);
 /* Use the caller's HIP stream (hipStream_t) instead of the interner's own. */
 int vx_interner_set_stream(vx_interner* h, void* hip_stream);
+/* Capture the launch sequence of vx_trees_apply_batches[_ex] as a HIP graph and
+ * replay it when a later call has the same arguments (same roots/status arrays,
+ * same batch views, same options): repeated batched applies skip planning and
+ * per-kernel launches.  Up to 4 sequences are kept; any workspace growth rebuilds
+ * them.  Off by default; profiling (vx_interner_profile) bypasses it. */
+int vx_interner_set_graphs(vx_interner* h, int enable);
 /* Wait for all work on the interner's stream; returns the first device error. */
 int vx_interner_sync(vx_interner* h);
 
@@ -125,6 +131,42 @@ int vx_trees_apply_edits(vx_interner* h, size_t n, uint64_t* roots, const uint8_
                          const uint64_t* edit_offsets, const int32_t* xyz, const void* values,
                          int32_t* status);
 
+/* Options of the _ex forms below (the plain forms behave as zeroed options).
+ *
+ * Allocation: an apply whose new nodes (leaves + branches) do not fit the
+ * interner's capacity, or exceed node_limit when that is nonzero, is rolled back
+ * as a whole: every tree of the call keeps its root and gets status VX_E_OOM
+ * (capacity; the reference panics, interner/macros.rs:37-38) or VX_E_BUDGET
+ * (node_limit), and the interner is left as before the call.  node_limit lets a
+ * caller reserve nodes per apply (admission by memory budget).
+ *
+ * VX_APPLY_DRY_RUN: run the apply's passes without changing anything (roots,
+ * nodes, reference counts and the pattern table are left as they were); status
+ * receives what the apply would return.  With counts, this is the exact count of
+ * the nodes the apply would create.
+ *
+ * counts (nullable, device, 2 + n uint32): [0] new branch nodes, [1] new leaf
+ * nodes, [2 + t] new branch nodes first created for tree t (a node shared by
+ * several trees of the call counts once, for one of them).  0xFFFFFFFF in [0]
+ * and [1] when the apply was rolled back. */
+enum { VX_APPLY_DRY_RUN = 1u };
+typedef struct {
+    uint32_t flags;
+    uint32_t reserved;   /* must be 0 */
+    uint64_t node_limit; /* 0 = none */
+    uint32_t* counts;
+} vx_apply_opts;
+
+/* vx_trees_apply_batches with options (no counterpart in the reference: the
+ * dry run and node_limit serve admission by memory budget). */
+int vx_trees_apply_batches_ex(vx_interner* h, size_t n, uint64_t* roots,
+                              const vx_batch_view* batches, int32_t* status,
+                              const vx_apply_opts* opts);
+/* vx_trees_apply_edits with options. */
+int vx_trees_apply_edits_ex(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* depths,
+                            const uint64_t* edit_offsets, const int32_t* xyz, const void* values,
+                            int32_t* status, const vx_apply_opts* opts);
+
 /* ---- batched reads: VoxOpsRead::get (voxtree.rs:145-159, utils/common.rs:122-156) ---- */
 
 /* For q in [0, nq): tree chunk_of_query[q] (< n_trees; roots/depths indexed by
@@ -136,6 +178,15 @@ int vx_trees_apply_edits(vx_interner* h, size_t n, uint64_t* roots, const uint8_
 int vx_trees_get(vx_interner* h, size_t n_trees, const uint64_t* roots, const uint8_t* depths,
                  size_t nq, const uint32_t* chunk_of_query, const int32_t* xyz, void* out_values,
                  uint8_t* out_present);
+
+/* vx_trees_get on the caller's HIP stream instead of the interner's.  Reads may
+ * run concurrently with an apply on the interner's stream as long as that apply
+ * does not edit the trees read (nodes reachable from a root that is not replaced
+ * are neither rewritten nor freed); the caller orders reads after the applies
+ * they depend on (e.g. with hipStreamWaitEvent). */
+int vx_trees_get_on(vx_interner* h, void* hip_stream, size_t n_trees, const uint64_t* roots,
+                    const uint8_t* depths, size_t nq, const uint32_t* chunk_of_query,
+                    const int32_t* xyz, void* out_values, uint8_t* out_present);
 
 /* ---- whole-tree ops: VoxOpsBulkWrite (voxtree.rs:263-292) ---- */
 
@@ -159,6 +210,27 @@ int vx_interner_profile(vx_interner* h, int enable);
  * kernel ids (name of id k: vx_kernel_name(k)). */
 int vx_interner_profile_read(vx_interner* h, double* ms, uint64_t* launches, uint32_t n);
 const char* vx_kernel_name(uint32_t id);
+
+/* ---- node-growth predictor (no reference counterpart: admission by memory budget) ----
+ * FP32 MLP 16 -> 64 -> 64 -> 1 with ReLU.  params (device, vx_mlp_param_count()
+ * floats): W1[64][16] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3.  x: n rows of 16
+ * features.  All pointers device; asynchronous on hip_stream (NULL = default). */
+int vx_mlp_param_count(void);
+int vx_mlp_forward(void* hip_stream, const float* params, const float* x, uint32_t n, float* y);
+/* Gradient of L = (1/n) sum (y - target)^2 with respect to params (grad, same
+ * layout), the outputs y, and L (loss, nullable).  workspace: n * 257 floats. */
+int vx_mlp_gradient(void* hip_stream, const float* params, const float* x, const float* target, uint32_t n,
+                    float* y, float* workspace, float* grad, float* loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8; step counts from 1). */
+int vx_mlp_adam(void* hip_stream, float* params, const float* grad, float* m, float* v, uint32_t step,
+                float lr);
+
+/* Atomic-throughput ceiling (no reference counterpart): `ops` operations on
+ * uniformly random addresses of buf (device, `slots` entries of 8 bytes for kind 0
+ * and 4 bytes otherwise, slots a power of two >= 64; kind 0: 64-bit CAS, 1: 32-bit
+ * add, 2: 32-bit add on 64 hot counters).  Kernel
+ * time in *ms.  Synchronizes the stream; buf contents are overwritten. */
+int vx_bench_atomics(void* hip_stream, int kind, void* buf, uint64_t slots, uint64_t ops, float* ms);
 
 /* ---- device memory helpers for callers without a HIP allocator ---- */
 int vx_device_alloc(size_t bytes, void** ptr);

@@ -322,3 +322,40 @@ def test_edit_lists_sorted_on_device(vx):
                 assert np.array_equal(full_reads_gpu(f, i, t_o.depth),
                                       np.where(t_o.get_many(it_o, O.all_coords(t_o.depth))[1] == 1,
                                                t_o.get_many(it_o, O.all_coords(t_o.depth))[0], 0))
+
+
+def test_graph_replay_matches_direct_launches(vx):
+    """Repeated batched applies replayed from captured HIP graphs give the same trees,
+    counts and statuses as launching every kernel directly, also with another call in
+    between whose larger workspace invalidates the captured sequences."""
+    api, W = vx
+    grid = 4
+    n = grid * grid
+    mA, vA, _, _ = W.terrain_batches(grid=grid, seed=31)
+    vB = vA.copy()
+    vB[vA == 1] = 2
+    vB[vA == 2] = 1
+    mC, vC, _, _ = W.terrain_batches(grid=2 * grid, seed=32)  # 4x the trees: workspace grows
+    dev = lambda arrs: [api.to_device(a) for a in arrs]
+    tA, tB = (dev(mA), dev(vA)), (dev(mA), dev(vB))
+    tC = (dev(mC), dev(vC))
+    runs = []
+    for graphs in (False, True):
+        it = api.VoxInterner(1, 64 * MB)
+        it.set_graphs(graphs)
+        f = api.Forest(it, [5] * n)
+        fc = api.Forest(it, [5] * (4 * n))
+        vA_, vB_ = f.make_views(*tA), f.make_views(*tB)
+        vC_ = fc.make_views(*tC)
+        log = []
+        for step in range(12):
+            if step == 6:
+                fc.apply_views(vC_)
+                it.sync()
+                log.append(("C", fc.status.cpu().numpy().tolist(), fc.canonical_hashes()))
+            f.apply_views(vA_ if step % 2 == 0 else vB_)
+            it.sync()
+            log.append((step, f.status.cpu().numpy().tolist(), f.canonical_hashes(), it.counts()))
+        runs.append(log)
+        it.close()
+    assert runs[0] == runs[1]

@@ -30,6 +30,14 @@ class BatchView(C.Structure):
                 ("reserved", C.c_uint8)]
 
 
+APPLY_DRY_RUN = 1
+
+
+class ApplyOpts(C.Structure):
+    _fields_ = [("flags", C.c_uint32), ("reserved", C.c_uint32), ("node_limit", C.c_uint64),
+                ("counts", C.c_void_p)]
+
+
 SIGNATURES = {
     "vx_interner_create": (_int, [_u32, _u64, C.c_double, C.POINTER(_vp)]),
     "vx_interner_destroy": (_int, [_vp]),
@@ -41,16 +49,26 @@ SIGNATURES = {
     "vx_interner_stats": (_int, [_vp, C.POINTER(_u64), _u32]),
     "vx_interner_set_stream": (_int, [_vp, _vp]),
     "vx_interner_sync": (_int, [_vp]),
+    "vx_interner_set_graphs": (_int, [_vp, _int]),
     "vx_trees_apply_batches": (_int, [_vp, _sz, _vp, C.POINTER(BatchView), _vp]),
     "vx_tree_apply_batch_host": (_int, [_vp, C.POINTER(_u64), C.POINTER(BatchView),
                                         C.POINTER(_int)]),
     "vx_trees_apply_edits": (_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "vx_trees_apply_batches_ex": (_int, [_vp, _sz, _vp, C.POINTER(BatchView), _vp,
+                                         C.POINTER(ApplyOpts)]),
+    "vx_trees_apply_edits_ex": (_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(ApplyOpts)]),
     "vx_trees_get": (_int, [_vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
+    "vx_trees_get_on": (_int, [_vp, _vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
     "vx_trees_fill": (_int, [_vp, _sz, _vp, _vp]),
     "vx_trees_canonical_hash": (_int, [_vp, _sz, _vp, _vp]),
     "vx_interner_profile": (_int, [_vp, _int]),
     "vx_interner_profile_read": (_int, [_vp, C.POINTER(C.c_double), C.POINTER(_u64), _u32]),
     "vx_kernel_name": (C.c_char_p, [_u32]),
+    "vx_bench_atomics": (_int, [_vp, _int, _vp, _u64, _u64, C.POINTER(C.c_float)]),
+    "vx_mlp_param_count": (_int, []),
+    "vx_mlp_forward": (_int, [_vp, _vp, _vp, _u32, _vp]),
+    "vx_mlp_gradient": (_int, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp]),
+    "vx_mlp_adam": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, C.c_float]),
     "vx_device_alloc": (_int, [_sz, C.POINTER(_vp)]),
     "vx_device_free": (_int, [_vp]),
     "vx_memcpy_h2d": (_int, [_vp, _vp, _sz]),

@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import morton
-from ._lib import STAT_NAMES, BatchView, VxError, check, lib
+from ._lib import APPLY_DRY_RUN, STAT_NAMES, ApplyOpts, BatchView, VxError, check, lib
 
 EMPTY = 0
 INVALID = (1 << 64) - 1
@@ -116,6 +116,10 @@ class VoxInterner:
 
     def sync(self):
         check(lib().vx_interner_sync(self.h), "sync")
+
+    def set_graphs(self, enable=True):
+        """Replay repeated batched applies from captured HIP graphs."""
+        check(lib().vx_interner_set_graphs(self.h, 1 if enable else 0), "set_graphs")
 
     def profile(self, enable=True):
         check(lib().vx_interner_profile(self.h, 1 if enable else 0), "profile")
@@ -272,30 +276,60 @@ class Forest:
             v.depth = int(self.depths_host[i])
         return views
 
-    def apply_views(self, views):
-        check(lib().vx_trees_apply_batches(self.it.h, self.n, self.roots.data_ptr(), views,
-                                           self.status.data_ptr()), "apply_batches")
+    def _opts(self, dry_run, node_limit, counts):
+        """vx_apply_opts; counts=True records new-node counts in self.counts (device)."""
+        if not dry_run and not node_limit and not counts:
+            return None
+        o = ApplyOpts()
+        o.flags = APPLY_DRY_RUN if dry_run else 0
+        o.node_limit = int(node_limit)
+        if counts:
+            self.counts = torch.zeros(2 + self.n, dtype=torch.int32, device=self.it.device)
+            o.counts = self.counts.data_ptr()
+        return o
 
-    def apply(self, masks, values, fills=None, has_patches=None, sync=True):
+    def counts_host(self):
+        """(new branches, new leaves, per-tree new branches) of the last apply with counts=True;
+        new branches/leaves are None when the apply was rolled back."""
+        c = self.counts.cpu().numpy().view(np.uint32).astype(np.int64)
+        if c[0] == 0xFFFFFFFF:
+            return None, None, c[2:]
+        return int(c[0]), int(c[1]), c[2:]
+
+    def apply_views(self, views, dry_run=False, node_limit=0, counts=False):
+        o = self._opts(dry_run, node_limit, counts)
+        if o is None:
+            check(lib().vx_trees_apply_batches(self.it.h, self.n, self.roots.data_ptr(), views,
+                                               self.status.data_ptr()), "apply_batches")
+        else:
+            check(lib().vx_trees_apply_batches_ex(self.it.h, self.n, self.roots.data_ptr(), views,
+                                                  self.status.data_ptr(), C.byref(o)), "apply_batches")
+
+    def apply(self, masks, values, fills=None, has_patches=None, sync=True, **opts):
+        """opts: dry_run, node_limit, counts (vx_apply_opts)."""
         views = self.make_views(masks, values, fills, has_patches)
         self._views = views
         self._keep = (masks, values)
-        self.apply_views(views)
+        self.apply_views(views, **opts)
         if sync:
             self.it.sync()
             return self.status.cpu().numpy()
         return None
 
-    def apply_edits(self, offsets, xyz_dev, values_dev, sync=True):
+    def apply_edits(self, offsets, xyz_dev, values_dev, sync=True, dry_run=False, node_limit=0,
+                    counts=False):
         """vx_trees_apply_edits: tree t's edits are [offsets[t], offsets[t+1]) of the device
         arrays xyz_dev (int32 [E,3]) and values_dev (value dtype [E]), in submission order."""
         offs = np.ascontiguousarray(offsets, dtype=np.uint64)
         assert len(offs) == self.n + 1
-        check(lib().vx_trees_apply_edits(self.it.h, self.n, self.roots.data_ptr(),
-                                         self.depths_host.ctypes.data, offs.ctypes.data,
-                                         xyz_dev.data_ptr() if xyz_dev is not None else None,
-                                         values_dev.data_ptr() if values_dev is not None else None,
-                                         self.status.data_ptr()), "apply_edits")
+        o = self._opts(dry_run, node_limit, counts)
+        args = [self.it.h, self.n, self.roots.data_ptr(), self.depths_host.ctypes.data, offs.ctypes.data,
+                xyz_dev.data_ptr() if xyz_dev is not None else None,
+                values_dev.data_ptr() if values_dev is not None else None, self.status.data_ptr()]
+        if o is None:
+            check(lib().vx_trees_apply_edits(*args), "apply_edits")
+        else:
+            check(lib().vx_trees_apply_edits_ex(*args, C.byref(o)), "apply_edits")
         if sync:
             self.it.sync()
             return self.status.cpu().numpy()
@@ -318,11 +352,14 @@ class Forest:
             check(rc, "get")
         return out.cpu().numpy().view(self.it.np_dtype).astype(np.uint64), p
 
-    def get_device(self, chunk_t, xyz_t, out, pres):
-        check(lib().vx_trees_get(self.it.h, self.n, self.roots.data_ptr(), self.depths.data_ptr(),
-                                 chunk_t.numel(), chunk_t.data_ptr(), xyz_t.data_ptr(),
-                                 out.data_ptr(), pres.data_ptr() if pres is not None else None),
-              "get")
+    def get_device(self, chunk_t, xyz_t, out, pres, stream=None):
+        """Device-to-device get; stream (a torch.cuda.Stream) runs it off the interner's stream."""
+        args = (self.n, self.roots.data_ptr(), self.depths.data_ptr(), chunk_t.numel(), chunk_t.data_ptr(),
+                xyz_t.data_ptr(), out.data_ptr(), pres.data_ptr() if pres is not None else None)
+        if stream is None:
+            check(lib().vx_trees_get(self.it.h, *args), "get")
+        else:
+            check(lib().vx_trees_get_on(self.it.h, C.c_void_p(stream.cuda_stream), *args), "get")
 
     def fill(self, values):
         vals = torch.from_numpy(np.asarray(values, dtype=np.uint64).view(np.int64)).to(self.it.device)

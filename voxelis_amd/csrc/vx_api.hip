@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -15,6 +16,8 @@
 #include "vx_device.h"
 #include "vx_edits.hip"
 #include "vx_kernels.hip"
+#include "vx_mlp.hip"
+#include "vx_microbench.hip"
 
 using namespace vx;
 
@@ -38,11 +41,16 @@ bool dmalloc(P** p, size_t bytes) {
     return true;
 }
 
+// Bumped whenever any workspace buffer moves: captured launch graphs hold its
+// addresses and are rebuilt after a move.
+static std::atomic<uint64_t> g_ws_gen{1};
+
 struct Grow {
     void* ptr = nullptr;
     size_t cap = 0;
-    bool ensure(size_t bytes) {
+    bool ensure(size_t bytes, bool tracked = true) {
         if (bytes <= cap) return true;
+        if (tracked) g_ws_gen.fetch_add(1);
         // queued kernels may still use the old buffer
         if (ptr) {
             (void)hipDeviceSynchronize();
@@ -70,12 +78,12 @@ struct LevelBufs {
 enum KernelId : uint32_t {
     KID_BEGIN = 0, KID_PREPARE, KID_BRICK_SCAN, KID_LEAF_ALLOC, KID_BRICK_BUILD, KID_PROBE_BRICKS,
     KID_COMMIT_BRICKS, KID_LEVEL_BUILD, KID_PROBE_UPPER, KID_COMMIT_UPPER, KID_FINALIZE,
-    KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_COUNT
+    KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_ROLLBACK, KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_begin", "k_prepare", "k_brick_retry", "k_leaf_alloc", "k_brick_build", "k_probe[bricks]",
     "k_commit[bricks]", "k_level_build", "k_probe[upper]", "k_commit[upper]", "k_finalize",
-    "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill"};
+    "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill", "k_rollback"};
 
 struct Prof {
     bool on = false;
@@ -84,6 +92,23 @@ struct Prof {
     std::vector<std::pair<uint32_t, size_t>> pending;  // kernel id, index of its start event
     double ms[KID_COUNT] = {0};
     uint64_t launches[KID_COUNT] = {0};
+};
+
+// A launch sequence captured as a HIP graph, replayed while the call's arguments
+// (and the workspace addresses it captured) are unchanged.
+struct Plan {
+    uint64_t key = 0;
+    uint64_t ws_gen = 0;
+    uint64_t last_use = 0;
+    hipGraphExec_t exec = nullptr;
+    Grow chunks, slot_off, tables;  // the call's records, owned by the plan
+    void release() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        chunks.release();
+        slot_off.release();
+        tables.release();
+    }
 };
 
 }  // namespace
@@ -107,7 +132,7 @@ struct vx_interner {
     bool own_stream = false;
     int poisoned = 0;
     // workspace
-    Grow chunks, slot_off, ptr_tables, gc_front[2], stage, scratch, deferred, drift;
+    Grow chunks, slot_off, ptr_tables, gc_front[2], stage, scratch, deferred, drift, undo;
     // edit-list path: zero-kept dense Batch buffers, sort buffers, tree records
     Grow edit_ws, edit_keys[2], edit_order[2], edit_tmp, edit_trees, edit_first, edit_status;
     LevelBufs lv[kLevels + 1];
@@ -121,6 +146,12 @@ struct vx_interner {
     int ring_cur = 0;
     void* h_pinned = nullptr;  // the slot of the current call
     Prof prof;
+    // captured launch sequences of repeated calls (vx_interner_set_graphs)
+    bool graphs = false;
+    hipStream_t cap_stream = nullptr;
+    static constexpr int kPlans = 4;
+    Plan plans[kPlans];
+    uint64_t plan_clock = 0;
 };
 
 namespace {
@@ -141,6 +172,12 @@ Pool<T> pool_of(vx_interner* h) {
     P.new_leaves = h->new_leaves;
     P.ctl = h->ctl;
     P.capacity = (uint32_t)h->capacity;
+    P.dry = 0;
+    P.alloc_limit = 0xFFFFFFFFu;
+    P.per_tree = nullptr;
+    P.levels = nullptr;
+    P.undo = nullptr;
+    P.undo_cap = 0;
     return P;
 }
 
@@ -217,11 +254,50 @@ hipEvent_t prof_event(vx_interner* h) {
         }                                                                           \
     } while (0)
 
+// Options of one apply (vx_apply_opts, validated by the caller).
+struct ApplyOpts {
+    bool dry = false;
+    uint32_t alloc_limit = 0xFFFFFFFFu;
+    uint32_t* counts = nullptr;  // device: new branches, new leaves, then per tree
+};
+
 template <typename T>
 int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
-               int32_t* status, const int32_t* pre_status = nullptr) {
+               int32_t* status, const int32_t* pre_status = nullptr, const ApplyOpts& opt = ApplyOpts()) {
     hipStream_t s = h->stream;
     Pool<T> P = pool_of<T>(h);
+    P.dry = opt.dry ? 1u : 0u;
+    P.alloc_limit = opt.alloc_limit;
+    P.per_tree = opt.counts ? opt.counts + 2 : nullptr;
+    // ---- a repeated call replays its captured launch sequence
+    Plan* plan = nullptr;
+    uint64_t key = 0;
+    if (h->graphs && !h->prof.on && !pre_status) {
+        key = mix64(0x5650u ^ (uint64_t)n ^ ((uint64_t)sizeof(T) << 40));
+        const uint64_t head[] = {(uint64_t)roots, (uint64_t)status, (uint64_t)opt.dry,
+                                 (uint64_t)opt.alloc_limit, (uint64_t)opt.counts};
+        for (uint64_t w : head) key = mix64(key ^ w);
+        for (size_t i = 0; i < n; i++) {
+            const vx_batch_view& b = batches[i];
+            key = mix64(key ^ (uint64_t)b.masks);
+            key = mix64(key ^ (uint64_t)b.values);
+            key = mix64(key ^ b.fill_value);
+            key = mix64(key ^ ((uint64_t)b.depth | (uint64_t)b.has_fill << 8 | (uint64_t)b.has_patches << 16 |
+                               (uint64_t)b.reserved << 24));
+        }
+        uint64_t gen = g_ws_gen.load();
+        for (Plan& pl : h->plans) {
+            if (pl.exec && pl.key == key && pl.ws_gen == gen) {
+                pl.last_use = ++h->plan_clock;
+                return hipGraphLaunch(pl.exec, s) == hipSuccess ? VX_OK : VX_E_HIP;
+            }
+        }
+        plan = &h->plans[0];
+        for (Plan& pl : h->plans)
+            if (pl.last_use < plan->last_use) plan = &pl;
+        if (plan->exec && hipStreamSynchronize(s) != hipSuccess) return VX_E_HIP;  // it may be running
+        plan->release();
+    }
     // ---- plan: trees with patches grouped by depth (others first), dense slot
     // spaces per height
     auto status_of = [&](const vx_batch_view& b) -> int32_t {
@@ -241,11 +317,12 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
                      [&](uint32_t a, uint32_t b) { return sort_key(a) < sort_key(b); });
     size_t chunk_bytes = n * sizeof(ChunkDev);
     size_t off_count = (size_t)(kLevels + 1) * n;
-    size_t stage_bytes = chunk_bytes + off_count * sizeof(uint64_t) + 2 * (kLevels + 1) * sizeof(void*);
+    size_t tab_bytes = 2 * (kLevels + 1) * sizeof(void*) + (kLevels + 1) * sizeof(Cands);
+    size_t stage_bytes = chunk_bytes + off_count * sizeof(uint64_t) + tab_bytes;
     if (!pinned_ensure(h, stage_bytes)) return VX_E_HIP;
     ChunkDev* hc = (ChunkDev*)h->h_pinned;
     uint64_t* hoff = (uint64_t*)((char*)h->h_pinned + chunk_bytes);
-    uint32_t dmax = 0;
+    uint32_t dmax = 0, dall = 0;  // deepest patched tree, deepest tree
     struct Run { uint32_t start, count, depth; };
     std::vector<Run> runs;
     for (size_t c = 0; c < n; c++) {
@@ -259,8 +336,9 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
         C.depth = b.depth;
         C.has_fill = b.has_fill ? 1 : 0;
         C.skip = b.reserved == kViewSkip ? 1 : 0;
-        C.status = status_of(b);
+        C.status = C.status0 = status_of(b);
         C.has_patches = (b.has_patches && C.status == 0 && !C.skip) ? 1 : 0;
+        if (C.status == 0 && !C.skip) dall = std::max<uint32_t>(dall, b.depth);
         if (!C.has_patches) continue;
         uint32_t d = b.depth;
         dmax = std::max<uint32_t>(dmax, d);
@@ -293,9 +371,12 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
         L[hh].slot_base[L[hh].ngroups] = base;
         L[hh].nslots = base;
     }
-    // ---- workspace
-    if (!h->chunks.ensure(chunk_bytes)) return VX_E_HIP;
-    if (!h->slot_off.ensure(off_count * sizeof(uint64_t))) return VX_E_HIP;
+    // ---- workspace (a captured call owns its records)
+    Grow& g_chunks = plan ? plan->chunks : h->chunks;
+    Grow& g_off = plan ? plan->slot_off : h->slot_off;
+    Grow& g_tables = plan ? plan->tables : h->ptr_tables;
+    if (!g_chunks.ensure(chunk_bytes, !plan)) return VX_E_HIP;
+    if (!g_off.ensure(off_count * sizeof(uint64_t), !plan)) return VX_E_HIP;
     uint64_t* res_ptrs_h[kLevels + 1] = {nullptr};
     uint64_t* cid_ptrs_h[kLevels + 1] = {nullptr};
     Cands K[kLevels + 1];
@@ -324,24 +405,42 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
         hptrs[i] = res_ptrs_h[i];
         hptrs[kLevels + 1 + i] = cid_ptrs_h[i];
     }
-    if (!h->ptr_tables.ensure(2 * (kLevels + 1) * sizeof(void*))) return VX_E_HIP;
+    std::memcpy(hptrs + 2 * (kLevels + 1), K, (kLevels + 1) * sizeof(Cands));
+    if (!g_tables.ensure(tab_bytes, !plan)) return VX_E_HIP;
     size_t gcap = (size_t)h->capacity + n + 16;
     if (!h->gc_front[0].ensure(gcap * 4) || !h->gc_front[1].ensure(gcap * 4)) return VX_E_HIP;
-    ChunkDev* dchunks = (ChunkDev*)h->chunks.ptr;
-    uint64_t* doff = (uint64_t*)h->slot_off.ptr;
-    uint64_t* const* dres = (uint64_t* const*)h->ptr_tables.ptr;
+    ChunkDev* dchunks = (ChunkDev*)g_chunks.ptr;
+    uint64_t* doff = (uint64_t*)g_off.ptr;
+    uint64_t* const* dres = (uint64_t* const*)g_tables.ptr;
     uint64_t* const* dcid = dres + (kLevels + 1);
     HIPCHK(hipMemcpyAsync(dchunks, hc, chunk_bytes, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(doff, hoff, off_count * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(h->ptr_tables.ptr, hptrs, 2 * (kLevels + 1) * sizeof(void*),
-                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(g_tables.ptr, hptrs, tab_bytes, hipMemcpyHostToDevice, s));
+    P.levels = (const Cands*)((void**)g_tables.ptr + 2 * (kLevels + 1));
     pinned_release(h);
     uint32_t nvalues = 1u << (8 * sizeof(T));
     uint64_t nb = std::max<uint64_t>(dmax >= 2 ? L[1].nslots : 1, 1);
     if (!h->deferred.ensure(nb * 4) || !h->drift.ensure(nb * sizeof(Drift))) return VX_E_HIP;
     uint32_t* ddef = (uint32_t*)h->deferred.ptr;
     Drift* ddrift = (Drift*)h->drift.ptr;
+    // undo log: one entry per deferred brick, collapsed upper node or fill leaf at most
+    uint64_t undo_cap = n;
+    LevelTiles LT;
+    std::memset(&LT, 0, sizeof(LT));
+    LT.dmax = dmax >= 2 ? dmax : 0;
+    for (uint32_t hh = 1; hh <= LT.dmax; hh++) {
+        undo_cap += L[hh].nslots;
+        LT.cnt[hh] = tiles_h[hh];
+        LT.start[hh + 1] = LT.start[hh] + ntiles[hh];
+    }
+    for (uint32_t hh = LT.dmax + 1; hh <= (uint32_t)kLevels; hh++) LT.start[hh + 1] = LT.start[hh];
+    if (undo_cap > 0xFFFFFFFFull) return VX_E_INVALID_ARG;
+    if (opt.dry && (uint64_t)h->capacity + 65536 + (undo_cap - n) > 0xFFFFFFFFull) return VX_E_INVALID_ARG;
+    if (!h->undo.ensure(std::max<uint64_t>(undo_cap, 1) * sizeof(Drift))) return VX_E_HIP;
+    P.undo = (Drift*)h->undo.ptr;
+    P.undo_cap = (uint32_t)undo_cap;
     // ---- launch sequence
+    auto launch_all = [&]() -> int {
     VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
     VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, grid_for(n), kBlock, P, dchunks, (uint32_t)n, roots,
               pre_status);
@@ -355,8 +454,9 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
                   tiles_h[1], ddef, ddrift);
         VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T>, t1, kTile, P, K[1], tiles_h[1], res_ptrs_h[1], dchunks,
                   L[1]);
-        VX_LAUNCH(h, KID_COMMIT_BRICKS, k_commit<T>, std::min<unsigned>(t1, 1024), kTile, P, K[1],
-                  tiles_h[1], (uint64_t)t1);
+        if (!opt.dry)
+            VX_LAUNCH(h, KID_COMMIT_BRICKS, k_commit<T>, std::min<unsigned>(t1, 1024), kTile, P, K[1],
+                      tiles_h[1], (uint64_t)t1);
         for (uint32_t hh = 2; hh <= dmax; hh++) {
             unsigned th = (unsigned)ntiles[hh];
             VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T>, th, kTile, P, dchunks, L[hh],
@@ -364,30 +464,67 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
                       K[hh], tiles_h[hh]);
             VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T>, th, kTile, P, K[hh], tiles_h[hh], res_ptrs_h[hh],
                       dchunks, L[hh]);
-            VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T>, std::min<unsigned>(th, 1024), kTile, P, K[hh],
-                      tiles_h[hh], (uint64_t)th);
+            if (!opt.dry)
+                VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T>, std::min<unsigned>(th, 1024), kTile, P, K[hh],
+                          tiles_h[hh], (uint64_t)th);
         }
+        // dry runs restore the table; an aborted apply takes its references back
+        unsigned rg = (unsigned)std::min<uint64_t>(LT.start[LT.dmax + 1], 1024);
+        VX_LAUNCH(h, KID_ROLLBACK, k_rollback<T>, std::max(rg, 1u), kTile, P, dchunks, LT, ddrift);
     }
     uint32_t* gf0 = (uint32_t*)h->gc_front[0].ptr;
     uint32_t* gf1 = (uint32_t*)h->gc_front[1].ptr;
     VX_LAUNCH(h, KID_FINALIZE, k_finalize<T>, grid_for(n), kBlock, P, dchunks, (uint32_t)n, doff,
               (uint32_t)n, dres, dcid, roots, status, gf0);
-    VX_LAUNCH(h, KID_END_ALLOC, k_end_alloc, 1, 1, h->ctl);
-    VX_LAUNCH(h, KID_LEAF_CLEANUP, k_leaf_cleanup<T>, grid_for(nvalues), kBlock, P);
-    // dec_ref_recursive of the replaced roots: depth <= 7 gives at most 8 rounds
-    for (uint32_t r = 0; r < (uint32_t)kLevels + 1; r++) {
-        uint32_t* fr = (r & 1) ? gf1 : gf0;
-        uint32_t* nx = (r & 1) ? gf0 : gf1;
-        unsigned grid = r == 0 ? grid_for(n) : 1024u;
-        VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid, kBlock, P, r, fr, nx, r == 0, (uint64_t)gcap);
+    VX_LAUNCH(h, KID_END_ALLOC, k_end_alloc<T>, 1, 256, P, true, opt.counts);
+    if (!opt.dry) {
+        VX_LAUNCH(h, KID_LEAF_CLEANUP, k_leaf_cleanup<T>, grid_for(nvalues), kBlock, P);
+        // dec_ref_recursive of the replaced roots: a tree of depth D has nodes at
+        // heights D..0, one round each
+        for (uint32_t r = 0; r <= dall; r++) {
+            uint32_t* fr = (r & 1) ? gf1 : gf0;
+            uint32_t* nx = (r & 1) ? gf0 : gf1;
+            unsigned grid = r == 0 ? grid_for(n) : 1024u;
+            VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid, kBlock, P, r, fr, nx, r == 0, (uint64_t)gcap);
+        }
     }
-    if (hipGetLastError() != hipSuccess) return VX_E_HIP;
-    return VX_OK;
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+    };
+    if (!plan) return launch_all();
+    // capture on a private stream, then replay on the interner's stream
+    if (!h->cap_stream && hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess)
+        return VX_E_HIP;
+    uint64_t gen = g_ws_gen.load();
+    hipStream_t real = h->stream;
+    h->stream = h->cap_stream;
+    if (hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        h->stream = real;
+        return VX_E_HIP;
+    }
+    int rc = launch_all();
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(h->cap_stream, &graph);
+    h->stream = real;
+    if (rc != VX_OK || e != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return VX_E_HIP;
+    }
+    e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        plan->exec = nullptr;
+        return VX_E_HIP;
+    }
+    plan->key = key;
+    plan->ws_gen = gen;
+    plan->last_use = ++h->plan_clock;
+    return hipGraphLaunch(plan->exec, s) == hipSuccess ? VX_OK : VX_E_HIP;
 }
 
 template <typename T>
 int apply_edits_impl(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* depths,
-                     const uint64_t* offsets, const int32_t* xyz, const void* values, int32_t* status) {
+                     const uint64_t* offsets, const int32_t* xyz, const void* values, int32_t* status,
+                     const ApplyOpts& opt = ApplyOpts()) {
     hipStream_t s = h->stream;
     uint64_t E = offsets[n];
     std::vector<EditTree> trees(n);
@@ -458,7 +595,7 @@ int apply_edits_impl(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* d
         v.masks = mws + trees[t].mbase;
         v.values = vws + trees[t].vbase;
     }
-    int rc = apply_impl<T>(h, n, roots, views.data(), status, dpre);
+    int rc = apply_impl<T>(h, n, roots, views.data(), status, dpre, opt);
     if (rc) return rc;
     if (E > 0)
         k_edit_scatter<T><<<grid_for(E), kBlock, 0, s>>>(E, (uint32_t)n, dtrees, k1, o1, (const T*)values, mws,
@@ -485,7 +622,7 @@ int fill_impl(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* vals) {
     VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
     VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc_only<T>, 1, 1024, P, nvalues);
     VX_LAUNCH(h, KID_FILL, k_fill_finish<T>, grid_for(n), kBlock, P, (uint32_t)n, roots, vals);
-    VX_LAUNCH(h, KID_END_ALLOC, k_end_alloc, 1, 1, h->ctl);
+    VX_LAUNCH(h, KID_END_ALLOC, k_end_alloc<T>, 1, 256, P, false, (uint32_t*)nullptr);
     if (hipGetLastError() != hipSuccess) return VX_E_HIP;
     return VX_OK;
 }
@@ -497,6 +634,15 @@ int get_impl(vx_interner* h, size_t nq, size_t ntrees, const uint64_t* roots,
     Pool<T> P = pool_of<T>(h);
     VX_LAUNCH(h, KID_GET, k_get<T>, grid_for(nq), kBlock, P, nq, ntrees, roots, depths, chunk_of, xyz,
               (T*)out, present);
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
+template <typename T>
+int get_on_impl(vx_interner* h, hipStream_t st, size_t nq, size_t ntrees, const uint64_t* roots,
+                const uint8_t* depths, const uint32_t* chunk_of, const int32_t* xyz, void* out,
+                uint8_t* present) {
+    Pool<T> P = pool_of<T>(h);
+    k_get<T><<<grid_for(nq), kBlock, 0, st>>>(P, nq, ntrees, roots, depths, chunk_of, xyz, (T*)out, present);
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
 }
 
@@ -602,6 +748,9 @@ int vx_interner_destroy(vx_interner* h) {
     h->scratch.release();
     h->deferred.release();
     h->drift.release();
+    h->undo.release();
+    for (Plan& pl : h->plans) pl.release();
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     for (Grow* g : {&h->edit_ws, &h->edit_keys[0], &h->edit_keys[1], &h->edit_order[0], &h->edit_order[1],
                     &h->edit_tmp, &h->edit_trees, &h->edit_first, &h->edit_status})
         g->release();
@@ -759,6 +908,50 @@ int vx_trees_apply_edits(vx_interner* h, size_t n, uint64_t* roots, const uint8_
                : apply_edits_impl<uint16_t>(h, n, roots, depths, edit_offsets, xyz, values, status);
 }
 
+namespace {
+int opts_of(const vx_interner* h, const vx_apply_opts* o, ApplyOpts* out) {
+    *out = ApplyOpts();
+    if (!o) return VX_OK;
+    if (o->reserved != 0 || (o->flags & ~(uint32_t)VX_APPLY_DRY_RUN)) return VX_E_INVALID_ARG;
+    out->dry = (o->flags & VX_APPLY_DRY_RUN) != 0;
+    if (o->node_limit != 0) out->alloc_limit = (uint32_t)std::min<uint64_t>(o->node_limit, 0xFFFFFFFEull);
+    out->counts = o->counts;
+    (void)h;
+    return VX_OK;
+}
+}  // namespace
+
+int vx_trees_apply_batches_ex(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
+                              int32_t* status, const vx_apply_opts* opts) {
+    if (!h || (n && (!roots || !batches || !status))) return VX_E_INVALID_ARG;
+    if (h->poisoned) return VX_E_POISONED;
+    if (n == 0) return VX_OK;
+    if (n > 0x7FFFFFFFull) return VX_E_INVALID_ARG;
+    for (size_t i = 0; i < n; i++)
+        if (batches[i].reserved != 0) return VX_E_INVALID_ARG;
+    ApplyOpts o;
+    int rc = opts_of(h, opts, &o);
+    if (rc) return rc;
+    return h->vsize == 1 ? apply_impl<uint8_t>(h, n, roots, batches, status, nullptr, o)
+                         : apply_impl<uint16_t>(h, n, roots, batches, status, nullptr, o);
+}
+
+int vx_trees_apply_edits_ex(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* depths,
+                            const uint64_t* edit_offsets, const int32_t* xyz, const void* values,
+                            int32_t* status, const vx_apply_opts* opts) {
+    if (!h || (n && (!roots || !depths || !edit_offsets || !status))) return VX_E_INVALID_ARG;
+    if (h->poisoned) return VX_E_POISONED;
+    if (n == 0) return VX_OK;
+    if (n > 0x7FFFFFFFull) return VX_E_INVALID_ARG;
+    if (edit_offsets[n] > 0 && (!xyz || !values)) return VX_E_INVALID_ARG;
+    ApplyOpts o;
+    int rc = opts_of(h, opts, &o);
+    if (rc) return rc;
+    return h->vsize == 1
+               ? apply_edits_impl<uint8_t>(h, n, roots, depths, edit_offsets, xyz, values, status, o)
+               : apply_edits_impl<uint16_t>(h, n, roots, depths, edit_offsets, xyz, values, status, o);
+}
+
 int vx_trees_get(vx_interner* h, size_t ntrees, const uint64_t* roots, const uint8_t* depths,
                  size_t nq, const uint32_t* chunk_of, const int32_t* xyz, void* out,
                  uint8_t* present) {
@@ -767,6 +960,17 @@ int vx_trees_get(vx_interner* h, size_t ntrees, const uint64_t* roots, const uin
     return h->vsize == 1
                ? get_impl<uint8_t>(h, nq, ntrees, roots, depths, chunk_of, xyz, out, present)
                : get_impl<uint16_t>(h, nq, ntrees, roots, depths, chunk_of, xyz, out, present);
+}
+
+int vx_trees_get_on(vx_interner* h, void* stream, size_t ntrees, const uint64_t* roots,
+                    const uint8_t* depths, size_t nq, const uint32_t* chunk_of, const int32_t* xyz,
+                    void* out, uint8_t* present) {
+    if (!h || (nq && (!roots || !depths || !chunk_of || !xyz || !out))) return VX_E_INVALID_ARG;
+    if (nq == 0) return VX_OK;
+    hipStream_t st = (hipStream_t)stream;
+    return h->vsize == 1
+               ? get_on_impl<uint8_t>(h, st, nq, ntrees, roots, depths, chunk_of, xyz, out, present)
+               : get_on_impl<uint16_t>(h, st, nq, ntrees, roots, depths, chunk_of, xyz, out, present);
 }
 
 int vx_trees_fill(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* values) {
@@ -782,6 +986,15 @@ int vx_trees_canonical_hash(vx_interner* h, size_t n, const uint64_t* roots, uin
     if (n == 0) return VX_OK;
     return h->vsize == 1 ? canon_impl<uint8_t>(h, n, roots, out)
                          : canon_impl<uint16_t>(h, n, roots, out);
+}
+
+int vx_interner_set_graphs(vx_interner* h, int enable) {
+    if (!h) return VX_E_INVALID_ARG;
+    (void)hipStreamSynchronize(h->stream);
+    h->graphs = enable != 0;
+    if (!h->graphs)
+        for (Plan& pl : h->plans) pl.release();
+    return VX_OK;
 }
 
 int vx_interner_profile(vx_interner* h, int enable) {
@@ -813,6 +1026,54 @@ int vx_interner_profile_read(vx_interner* h, double* ms, uint64_t* launches, uin
 }
 
 const char* vx_kernel_name(uint32_t id) { return id < (uint32_t)KID_COUNT ? kKernelNames[id] : ""; }
+
+int vx_bench_atomics(void* stream, int kind, void* buf, uint64_t slots, uint64_t ops, float* ms) {
+    if (!buf || !ms || kind < 0 || kind > 2 || slots < 64 || (slots & (slots - 1)) || ops == 0)
+        return VX_E_INVALID_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t per_lane = 64;
+    uint64_t lanes = (ops + per_lane - 1) / per_lane;
+    unsigned blocks = (unsigned)std::min<uint64_t>((lanes + 255) / 256, 1u << 20);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    k_atomic_bench<<<blocks, 256, 0, st>>>(buf, slots, per_lane, kind, (uint64_t*)buf);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
+int vx_mlp_param_count(void) { return kMlpParams; }
+
+int vx_mlp_forward(void* stream, const float* params, const float* x, uint32_t n, float* y) {
+    if (!params || (n && (!x || !y))) return VX_E_INVALID_ARG;
+    if (n == 0) return VX_OK;
+    unsigned grid = (unsigned)std::min<uint64_t>((n + kMlpWaves - 1) / kMlpWaves, 1024);
+    k_mlp_fwd<<<grid, 64 * kMlpWaves, 0, (hipStream_t)stream>>>(params, x, n, y, nullptr, nullptr);
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
+int vx_mlp_gradient(void* stream, const float* params, const float* x, const float* target, uint32_t n,
+                    float* y, float* workspace, float* grad, float* loss) {
+    if (!params || !grad || (n && (!x || !target || !y || !workspace))) return VX_E_INVALID_ARG;
+    if (n == 0) return VX_E_INVALID_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned grid = (unsigned)std::min<uint64_t>((n + kMlpWaves - 1) / kMlpWaves, 1024);
+    k_mlp_fwd<<<grid, 64 * kMlpWaves, 0, st>>>(params, x, n, y, target, workspace);
+    k_mlp_grad<<<(kMlpParams + 255) / 256, 256, 0, st>>>(x, target, y, n, workspace, grad, loss);
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
+int vx_mlp_adam(void* stream, float* params, const float* grad, float* m, float* v, uint32_t step, float lr) {
+    if (!params || !grad || !m || !v || step == 0) return VX_E_INVALID_ARG;
+    k_mlp_adam<<<(kMlpParams + 255) / 256, 256, 0, (hipStream_t)stream>>>(params, grad, m, v, step, lr, 0.9f,
+                                                                         0.999f, 1e-8f);
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
 
 int vx_device_alloc(size_t bytes, void** ptr) {
     if (!ptr) return VX_E_INVALID_ARG;

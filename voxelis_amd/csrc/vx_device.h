@@ -27,7 +27,8 @@ __host__ __device__ inline uint64_t res_pending(uint32_t cand) { return (kPendin
 // Phase-1 marker for "the fill leaf of this chunk" before its id is known.
 constexpr uint64_t kFillMark = (1ull << 63) | (0x7FFFull << 32);
 
-// Table slots: 0 = empty, 1 = tombstone, else tag(31) | pending(1) | payload(32).
+// Table slots: 0 = empty, 1 = tombstone, else tag(28) | level(3) | pending(1) | payload(32).
+// A pending slot's payload is a candidate index of level `level` (1..7).
 constexpr uint64_t kSlotEmpty = 0ull;
 constexpr uint64_t kSlotTomb = 1ull;
 
@@ -66,13 +67,15 @@ __host__ __device__ inline uint64_t branch_hash(const uint64_t ch[8], uint32_t t
 }
 
 __host__ __device__ inline uint32_t slot_tag(uint64_t h) {
-    uint32_t t = (uint32_t)(h >> 33);
+    uint32_t t = (uint32_t)(h >> 36);
     return t ? t : 1u;
 }
-__host__ __device__ inline uint64_t slot_make(uint32_t tag, bool pending, uint32_t payload) {
-    return ((uint64_t)tag << 33) | ((uint64_t)(pending ? 1 : 0) << 32) | payload;
+__host__ __device__ inline uint64_t slot_make(uint32_t tag, bool pending, uint32_t level, uint32_t payload) {
+    return ((uint64_t)tag << 36) | ((uint64_t)(level & 7u) << 33) | ((uint64_t)(pending ? 1 : 0) << 32) |
+           payload;
 }
-__host__ __device__ inline uint32_t slot_tag_of(uint64_t s) { return (uint32_t)(s >> 33); }
+__host__ __device__ inline uint32_t slot_tag_of(uint64_t s) { return (uint32_t)(s >> 36); }
+__host__ __device__ inline uint32_t slot_level(uint64_t s) { return (uint32_t)(s >> 33) & 7u; }
 __host__ __device__ inline bool slot_pending(uint64_t s) { return ((s >> 32) & 1ull) != 0; }
 __host__ __device__ inline uint32_t slot_payload(uint64_t s) { return (uint32_t)s; }
 
@@ -96,6 +99,10 @@ struct Ctl {
     uint64_t live_branches;
     uint64_t tombstones;
     uint32_t n_drift;  // queued reference-count corrections of the reference's quirks
+    uint32_t dry_owners;      // dry run: branch keys that would be new
+    uint32_t dry_new_leaves;  // dry run: leaves that would be new
+    uint32_t abort_;          // this apply could not allocate: 1 capacity, 2 reservation
+    uint32_t n_undo;          // entries of the undo log (reference-count deltas applied)
     uint32_t pad1;
     uint32_t ncand[kLevels + 1];
     uint32_t gc_count[kLevels + 2];
@@ -116,7 +123,7 @@ struct ChunkDev {
     uint8_t has_fill;
     uint8_t has_patches;
     uint8_t skip;  // not applied at all (an edit-list tree without edits)
-    uint8_t pad[4];
+    int32_t status0;  // status before the launch (a replayed launch starts from it)
 };
 
 // Slot space of one level: groups of trees of equal depth, each tree owning
@@ -128,6 +135,15 @@ struct LevelInfo {
     uint32_t ngroups;
     uint32_t h;
     uint64_t nslots;
+};
+
+// Candidate list of one level (compacted, order of discovery).
+struct Cands {
+    uint64_t* key;    // 9 words per candidate: children[8], tm
+    uint64_t* hash;
+    uint64_t* id;     // owner: the new node's id
+    uint64_t* tslot;  // owner: its table slot; ~0 otherwise
+    uint64_t* slot;   // the level slot this candidate resolves
 };
 
 // A queued reference-count correction (applied once the tree's status is final).
@@ -153,15 +169,17 @@ struct Pool {
     uint32_t* new_leaves;
     Ctl* ctl;
     uint32_t capacity;
+    uint32_t dry;          // 1: count what an apply would create, change nothing
+    uint32_t alloc_limit;  // allocations this apply may make (its reservation)
+    uint32_t* per_tree;    // optional: new branch nodes owned by each tree (caller's index)
+    const Cands* levels;   // candidate arrays of every level (pending keys of earlier levels)
+    Drift* undo;           // reference-count deltas applied outside commits (rolled back on abort)
+    uint32_t undo_cap;
 };
 
-// Candidate list of one level (compacted, order of discovery).
-struct Cands {
-    uint64_t* key;    // 9 words per candidate: children[8], tm
-    uint64_t* hash;
-    uint64_t* id;     // owner: the new node's id
-    uint64_t* tslot;  // owner: its table slot; ~0 otherwise
-    uint64_t* slot;   // the level slot this candidate resolves
-};
+// An owner's table slot with bit 63 set when the slot held a tombstone before.
+constexpr uint64_t kTslotTomb = 1ull << 63;
+__host__ __device__ inline uint64_t tslot_index(uint64_t ts) { return ts & ~kTslotTomb; }
+
 
 }  // namespace vx

@@ -199,6 +199,11 @@ __device__ inline uint64_t leaf_id_of(const Pool<T>& P, uint32_t v) {
 template <typename T>
 __device__ inline uint32_t alloc_node(const Pool<T>& P, bool* ok) {
     uint32_t k = atomicAdd(&P.ctl->alloc_count, 1u);
+    if (k >= P.alloc_limit) {
+        atomicOr(&P.ctl->abort_, 2u);  // beyond the caller's reservation
+        *ok = false;
+        return 0;
+    }
     uint32_t ft0 = P.ctl->free_top0;
     if (k < ft0) {
         *ok = true;
@@ -206,12 +211,23 @@ __device__ inline uint32_t alloc_node(const Pool<T>& P, bool* ok) {
     }
     uint64_t b = (uint64_t)P.ctl->next0 + (uint64_t)(k - ft0);
     if (b >= P.capacity) {
-        atomicOr(&P.ctl->err, kErrOom | kErrPoison);
+        atomicOr(&P.ctl->abort_, 1u);  // interner capacity (interner/macros.rs:37-38)
         *ok = false;
         return 0;
     }
     *ok = true;
     return (uint32_t)b;
+}
+
+// A reference-count delta applied outside a commit, logged so an aborted apply
+// can take it back.
+template <typename T>
+__device__ inline void ref_delta_logged(const Pool<T>& P, uint32_t chunk, uint32_t idx, int32_t delta,
+                                        uint32_t st) {
+    atomicAdd(&P.refs[idx], (uint32_t)delta);
+    uint32_t pos = atomicAdd(&P.ctl->n_undo, 1u);
+    if (pos < P.undo_cap) P.undo[pos] = Drift{chunk, idx, delta, st};
+    else atomicOr(&P.ctl->err, kErrPoison);
 }
 
 __device__ inline uint32_t group_of(const LevelInfo& L, uint64_t slot) {
@@ -230,14 +246,52 @@ __global__ void k_begin(Ctl* ctl) {
         ctl->n_new_leaves = 0;
         ctl->n_deferred = 0;
         ctl->n_drift = 0;
+        ctl->dry_owners = 0;
+        ctl->dry_new_leaves = 0;
+        ctl->abort_ = 0;
+        ctl->n_undo = 0;
         for (int i = 0; i <= kLevels; i++) ctl->ncand[i] = 0;
         for (int i = 0; i < kLevels + 2; i++) ctl->gc_count[i] = 0;
     }
 }
 
-__global__ void k_end_alloc(Ctl* ctl) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+// Close the allocation window of a launch sequence.  An aborted apply
+// (rollback = true) gives every allocation back instead: the allocator cursors
+// stay, reference counts of the taken indices go back to zero and new leaves
+// leave the value map.  counts (optional): new branches, new leaves.
+template <typename T>
+__global__ void k_end_alloc(Pool<T> P, bool rollback, uint32_t* counts) {
+    Ctl* ctl = P.ctl;
+    const bool abort = rollback && !P.dry && ctl->abort_ != 0;
+    if (abort) {
         uint32_t n = ctl->alloc_count;
+        uint32_t ft0 = ctl->free_top0;
+        uint64_t room = (uint64_t)ft0 + (P.capacity > ctl->next0 ? P.capacity - ctl->next0 : 0u);
+        uint64_t nok = n;
+        if (nok > P.alloc_limit) nok = P.alloc_limit;
+        if (nok > room) nok = room;
+        for (uint64_t i = threadIdx.x; i < nok; i += blockDim.x) {
+            uint32_t idx = i < ft0 ? P.free_stack[ft0 - 1 - i] : (uint32_t)(ctl->next0 + (i - ft0));
+            P.refs[idx] = 0;
+        }
+        uint32_t nl = ctl->n_new_leaves;
+        for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x)
+            P.leaf_map[(uint32_t)P.values[P.new_leaves[i]]] = 0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            ctl->live_leaves -= nl;
+            ctl->n_new_leaves = 0;
+            ctl->alloc_count = 0;
+            if (counts) counts[0] = counts[1] = 0xFFFFFFFFu;
+        }
+        return;
+    }
+    if (threadIdx.x == 0) {
+        uint32_t n = ctl->alloc_count;
+        if (counts) {
+            counts[0] = P.dry ? ctl->dry_owners : n - ctl->n_new_leaves;
+            counts[1] = P.dry ? ctl->dry_new_leaves : ctl->n_new_leaves;
+        }
         uint32_t used_free = n < ctl->free_top0 ? n : ctl->free_top0;
         ctl->free_top = ctl->free_top0 - used_free;
         uint64_t nx = (uint64_t)ctl->next0 + (n - used_free);
@@ -257,6 +311,8 @@ __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_
             ChunkDev& C = chunks[c];
             C.root_in = roots[C.orig];
             C.init = C.root_in;
+            C.status = C.status0;
+            if (P.per_tree) P.per_tree[C.orig] = 0;
             if (pre_status && pre_status[C.orig] < 0 && C.status >= 0) C.status = pre_status[C.orig];
             if (C.status >= 0 && C.has_fill) {
                 uint32_t v = (uint32_t)C.fill;
@@ -280,6 +336,10 @@ __global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t n
         if (P.leaf_need[v]) {
             P.leaf_need[v] = 0;
             if (P.leaf_map[v] == 0) {
+                if (P.dry) {
+                    atomicAdd(&created, 1u);
+                    continue;
+                }
                 bool ok;
                 uint32_t idx = alloc_node(P, &ok);
                 if (ok) {
@@ -296,17 +356,25 @@ __global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t n
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)created);
-        atomicAdd((unsigned long long*)stat(P.ctl, ST_LEAF_MISS), (unsigned long long)created);
+        if (P.dry) {
+            P.ctl->dry_new_leaves += created;
+        } else {
+            atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)created);
+            atomicAdd((unsigned long long*)stat(P.ctl, ST_LEAF_MISS), (unsigned long long)created);
+        }
     }
     for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
         ChunkDev& C = chunks[c];
         if (C.status < 0 || !C.has_fill) continue;
         uint32_t idx = P.leaf_map[(uint32_t)C.fill];
+        if (idx == 0 && P.dry) {
+            C.init = make_leaf(P.capacity + (uint32_t)C.fill, 0);
+            continue;
+        }
         if (idx == 0) continue;  // allocation failed (OOM flagged)
         C.init = make_leaf(idx, P.gens[idx]);
-        if (C.has_patches) {
-            atomicAdd(&P.refs[idx], 1u);
+        if (C.has_patches && !P.dry) {
+            ref_delta_logged(P, c, idx, 1, ST_H3);
             atomicAdd((unsigned long long*)stat(P.ctl, ST_H3), 1ull);
         }
     }
@@ -357,27 +425,34 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
         uint32_t li = P.leaf_map[v[0]];
         if (li == 0) {
             P.leaf_need[v[0]] = 1;
-            return 3;
+            if (!P.dry) return 3;
+            *out = make_leaf(P.capacity + v[0], 0);  // dry run: a placeholder for the new leaf
+        } else {
+            *out = make_leaf(li, P.gens[li]);
         }
-        *out = make_leaf(li, P.gens[li]);
         *flags |= 1;  // collapsed
         return 1;
     }
     // the split leaf's value is known without its id (the fill leaf may not exist yet)
     bool missing = false;
-    if (base == kFillMark) {
+    bool fill_split = base == kFillMark;
+    if (fill_split) {
         uint32_t fi = P.leaf_map[(uint32_t)C.fill];
         if (fi == 0) {
             P.leaf_need[(uint32_t)C.fill] = 1;
-            missing = true;
-            base = make_leaf(0, 0);
+            if (P.dry) {
+                base = make_leaf(P.capacity + (uint32_t)C.fill, 0);
+            } else {
+                missing = true;
+                base = make_leaf(0, 0);
+            }
         } else {
             base = make_leaf(fi, P.gens[fi]);
         }
     }
     bool split = bid_is_leaf(base);
     uint32_t split_value = 0;
-    if (split) split_value = missing ? (uint32_t)C.fill : (uint32_t)P.values[bid_index(base)];
+    if (split) split_value = fill_split ? (uint32_t)C.fill : (uint32_t)P.values[bid_index(base)];
     uint32_t types = 0, mask = 0;
     if (split) {
         for (int i = 0; i < 8; i++) ch[i] = base;
@@ -399,10 +474,14 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
         uint32_t li = P.leaf_map[v[i]];
         if (li == 0) {
             P.leaf_need[v[i]] = 1;
-            missing = true;
-            continue;
+            if (!P.dry) {
+                missing = true;
+                continue;
+            }
+            ch[i] = make_leaf(P.capacity + v[i], 0);
+        } else {
+            ch[i] = make_leaf(li, P.gens[li]);
         }
-        ch[i] = make_leaf(li, P.gens[li]);
         types |= 1u << i;
         mask |= 1u << i;
         modified |= 1u << i;
@@ -494,7 +573,7 @@ __global__ __launch_bounds__(kTile) void k_brick_build(Pool<T> P, ChunkDev* chun
                 uint32_t didx;
                 int32_t dd;
                 int r = brick_work(P, C, m, v, base, &out, ch, &tm, &didx, &dd, &flags);
-                if (didx != kNoIdx) {
+                if (didx != kNoIdx && !P.dry) {
                     uint32_t pos = atomicAdd(&P.ctl->n_drift, 1u);
                     drift[pos] = Drift{c, didx, dd, (uint32_t)(flags & 8 ? ST_H3 : ST_H4)};
                 }
@@ -549,7 +628,7 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
                 int flags = 0;
                 int r = brick_work(P, C, m, v, res1[slot], &out, ch, &tm, &didx, &dd, &flags);
                 if (didx != kNoIdx) {
-                    atomicAdd(&P.refs[didx], (uint32_t)dd);
+                    ref_delta_logged(P, c, didx, dd, flags & 8 ? ST_H3 : ST_H4);
                     atomicAdd((unsigned long long*)stat(P.ctl, flags & 8 ? ST_H3 : ST_H4), 1ull);
                 }
                 if (flags & 1) atomicAdd((unsigned long long*)stat(P.ctl, ST_COLLAPSE), 1ull);
@@ -565,7 +644,7 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
                     atomicAdd((unsigned long long*)stat(P.ctl, ST_CAND), 1ull);
                     out = res_pending(0xFFFFFFFFu);
                 } else if (r == 3) {
-                    atomicOr(&P.ctl->err, kErrOom | kErrPoison);  // the leaf could not be created
+                    atomicOr(&P.ctl->abort_, 1u);  // the leaf could not be created
                     out = kInvalid;
                 }
             }
@@ -611,7 +690,25 @@ __device__ inline bool key_matches_cand(const Cands& K, uint32_t k2, const uint6
 constexpr int kDedupRounds = 8;
 
 template <typename T>
-__device__ inline int probe_table(const Pool<T>& P, const Cands& K, uint64_t k, const uint64_t key[9],
+__device__ inline int match_slot(const Pool<T>& P, uint32_t level, uint64_t e, uint32_t tag,
+                                 const uint64_t key[9], uint64_t* result) {
+    if (e == kSlotEmpty || e == kSlotTomb || slot_tag_of(e) != tag) return -1;
+    uint32_t tm = (uint32_t)key[8];
+    if (slot_pending(e)) {
+        uint32_t lv = slot_level(e);
+        if (!key_matches_cand(P.levels[lv], slot_payload(e), key)) return -1;
+        // same level: point at the owner; an earlier level (dry runs keep those pending):
+        // its owner's id is final already
+        *result = lv == level ? res_pending(slot_payload(e)) : P.levels[lv].id[slot_payload(e)];
+        return lv == level ? 1 : 0;
+    }
+    if (!key_matches_node(P, slot_payload(e), key)) return -1;
+    *result = make_branch(slot_payload(e), P.gens[slot_payload(e)], tm);
+    return 0;
+}
+
+template <typename T>
+__device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, const uint64_t key[9],
                                   uint64_t h, uint64_t* result, uint64_t* tslot, uint32_t* probes) {
     uint32_t tag = slot_tag(h);
     uint32_t tm = (uint32_t)key[8];
@@ -625,43 +722,34 @@ __device__ inline int probe_table(const Pool<T>& P, const Cands& K, uint64_t k, 
             uint64_t expected = target == s ? kSlotEmpty : kSlotTomb;
             uint64_t seen = expected;
             bool won = __hip_atomic_compare_exchange_strong(
-                &P.tab[target], &seen, slot_make(tag, true, (uint32_t)k), __ATOMIC_RELAXED,
+                &P.tab[target], &seen, slot_make(tag, true, level, (uint32_t)k), __ATOMIC_RELAXED,
                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (won) {
+                if (P.dry) {
+                    // a placeholder id; the slot's previous value is restored after the run
+                    uint32_t serial = atomicAdd(&P.ctl->dry_owners, 1u);
+                    *result = make_branch(P.capacity + 65536u + serial, 0, tm);
+                    *tslot = target | (expected == kSlotTomb ? kTslotTomb : 0ull);
+                    return 2;
+                }
                 bool ok;
                 uint32_t idx = alloc_node(P, &ok);
                 *result = ok ? make_branch(idx, P.gens[idx], tm) : kInvalid;
-                *tslot = ok ? target : ~0ull;
+                // kept when the allocation failed too: the abort path restores the slot
+                *tslot = target | (expected == kSlotTomb ? kTslotTomb : 0ull);
                 return 2;
             }
-            e = seen;
-            if (e != kSlotEmpty && e != kSlotTomb && slot_tag_of(e) == tag) {
-                if (slot_pending(e)) {
-                    if (key_matches_cand(K, slot_payload(e), key)) {
-                        *result = res_pending(slot_payload(e));
-                        return 1;
-                    }
-                } else if (key_matches_node(P, slot_payload(e), key)) {
-                    *result = make_branch(slot_payload(e), P.gens[slot_payload(e)], tm);
-                    return 0;
-                }
-            }
+            int m = match_slot(P, level, seen, tag, key, result);
+            if (m >= 0) return m;
             s = h & P.tmask;  // someone else inserted first: rescan the chain
             first_free = ~0ull;
             continue;
         }
         if (e == kSlotTomb) {
             if (first_free == ~0ull) first_free = s;
-        } else if (slot_tag_of(e) == tag) {
-            if (slot_pending(e)) {
-                if (key_matches_cand(K, slot_payload(e), key)) {
-                    *result = res_pending(slot_payload(e));
-                    return 1;
-                }
-            } else if (key_matches_node(P, slot_payload(e), key)) {
-                *result = make_branch(slot_payload(e), P.gens[slot_payload(e)], tm);
-                return 0;
-            }
+        } else {
+            int m = match_slot(P, level, e, tag, key, result);
+            if (m >= 0) return m;
         }
         s = (s + 1) & P.tmask;
     }
@@ -678,12 +766,14 @@ __global__ __launch_bounds__(kTile) void k_probe(Pool<T> P, Cands K, const uint3
     int lane = (int)__lane_id();
     bool valid = threadIdx.x < n;
     uint64_t k = (uint64_t)blockIdx.x * kTile + threadIdx.x;
-    if (valid && L.h == 1) {
-        // a brick of a tree that hit the leaf-brick panic later in the same pass
+    uint32_t tree = kNoIdx;
+    if (valid && (L.h == 1 || P.per_tree)) {
         uint64_t slot = K.slot[k];
         uint32_t g = group_of(L, slot);
-        uint32_t c = L.chunk_base[g] + (uint32_t)((slot - L.slot_base[g]) >> (3 * (L.depth[g] - 1)));
-        if (chunks[c].status < 0) {
+        uint32_t c = L.chunk_base[g] + (uint32_t)((slot - L.slot_base[g]) >> (3 * (L.depth[g] - L.h)));
+        tree = chunks[c].orig;
+        // a brick of a tree that hit the leaf-brick panic later in the same pass
+        if (L.h == 1 && chunks[c].status < 0) {
             res[slot] = kInvalid;
             K.id[k] = kInvalid;
             K.tslot[k] = ~0ull;
@@ -717,7 +807,8 @@ __global__ __launch_bounds__(kTile) void k_probe(Pool<T> P, Cands K, const uint3
     uint64_t result = kInvalid, tslot = ~0ull;
     int kind = -1;
     uint32_t probes = 0;
-    if (valid && leader == lane) kind = probe_table(P, K, k, key, h, &result, &tslot, &probes);
+    if (valid && leader == lane) kind = probe_table(P, L.h, k, key, h, &result, &tslot, &probes);
+    if (kind == 2 && P.per_tree && tree != kNoIdx) atomicAdd(&P.per_tree[tree], 1u);
     uint64_t lres = shfl64(result, leader);
     int lkind = __shfl(kind, leader);
     if (valid && leader != lane) {
@@ -765,7 +856,7 @@ __global__ __launch_bounds__(kTile) void k_commit(Pool<T> P, Cands K, const uint
         }
         P.tms[idx] = (uint16_t)key[8];
         P.values[idx] = (T)calc_average(vals);
-        P.tab[ts] = slot_make(slot_tag(K.hash[k]), false, idx);
+        P.tab[tslot_index(ts)] = slot_make(slot_tag(K.hash[k]), false, 0, idx);
         for (int i = 0; i < 8; i++) {
             if (key[i] == kEmpty) continue;
             uint32_t ci = bid_index(key[i]);
@@ -815,11 +906,13 @@ __global__ __launch_bounds__(kTile) void k_level_build(
             if (C.status >= 0 && C.has_patches) {
                 const uint64_t* src = res_prev + slot_off_prev[c] + 8ull * q;
                 uint32_t types = 0, mask = 0;
+                bool failed = false;
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
                     uint64_t r = src[i];
                     if (r != kInvalid) {
                         r = resolve(r, cand_id_prev);
+                        failed = failed || r == kInvalid;  // its owner could not allocate
                         ch[i] = r;
                         mask |= 1u << i;
                         types |= (bid_is_leaf(r) ? 1u : 0u) << i;
@@ -827,6 +920,7 @@ __global__ __launch_bounds__(kTile) void k_level_build(
                         ch[i] = kEmpty;
                     }
                 }
+                if (failed) mask = 0;  // the apply aborts; nothing is built above
                 if (mask != 0) {
                     // equivalent node (voxtree.rs:930-952)
                     uint32_t ell = D - L.h;
@@ -876,7 +970,7 @@ __global__ __launch_bounds__(kTile) void k_level_build(
                             // dec_ref_by(children[0], min(pc(cloned),7)) instead of
                             // 7 - pc(cloned) (voxtree.rs:1066-1072, H5)
                             uint32_t pc = (uint32_t)__popc(cloned);
-                            atomicAdd(&P.refs[bid_index(ch[0])], 7u - 2u * pc);
+                            if (!P.dry) ref_delta_logged(P, c, bid_index(ch[0]), (int32_t)(7u - 2u * pc), ST_H5);
                             h5 = true;
                         }
                     } else {
@@ -900,6 +994,58 @@ __global__ __launch_bounds__(kTile) void k_level_build(
         atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
 }
 
+// Tiles of every level of one apply, flattened (levels 1..dmax).
+struct LevelTiles {
+    const uint32_t* cnt[kLevels + 1];
+    uint64_t start[kLevels + 2];  // first flattened tile of each level; start[dmax + 1] = total
+    uint32_t dmax;
+};
+
+// After the last commit: a dry run restores the table slots its owners claimed;
+// an apply that could not allocate (abort_) also takes back every reference its
+// commits and logged corrections added, so the interner is left as before.
+// Allocations themselves are undone by k_end_alloc.
+template <typename T>
+__global__ __launch_bounds__(kTile) void k_rollback(Pool<T> P, const ChunkDev* chunks, LevelTiles LT,
+                                                    const Drift* drift) {
+    const bool abort = !P.dry && P.ctl->abort_ != 0;
+    if (!P.dry && !abort) return;
+    uint32_t owned = 0;
+    uint64_t total = LT.start[LT.dmax + 1];
+    for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+        uint32_t h = 1;
+        while (h < LT.dmax && t >= LT.start[h + 1]) h++;
+        uint64_t lt = t - LT.start[h];
+        uint32_t n = LT.cnt[h][lt];
+        if (threadIdx.x >= n) continue;
+        const Cands& K = P.levels[h];
+        uint64_t k = lt * kTile + threadIdx.x;
+        uint64_t ts = K.tslot[k];
+        if (ts == ~0ull) continue;
+        P.tab[tslot_index(ts)] = (ts & kTslotTomb) ? kSlotTomb : kSlotEmpty;
+        uint64_t id = K.id[k];
+        if (P.dry || id == kInvalid) continue;
+        owned++;
+        for (int i = 0; i < 8; i++) {
+            uint64_t c = K.key[9ull * k + i];
+            if (c != kEmpty) atomicSub(&P.refs[bid_index(c)], 1u);
+        }
+    }
+    if (!abort) return;
+    uint32_t nu = P.ctl->n_undo < P.undo_cap ? P.ctl->n_undo : P.undo_cap;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nu; j += gridDim.x * blockDim.x)
+        atomicSub(&P.refs[P.undo[j].idx], (uint32_t)P.undo[j].delta);
+    // queued corrections applied by k_brick_retry (trees whose status was final)
+    uint32_t ndr = P.ctl->n_drift;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < ndr; j += gridDim.x * blockDim.x) {
+        Drift d = drift[j];
+        if (chunks[d.chunk].status >= 0) atomicSub(&P.refs[d.idx], (uint32_t)d.delta);
+    }
+    uint32_t tot = wave_sum(owned);
+    if (__lane_id() == 0 && tot)
+        atomicAdd((unsigned long long*)&P.ctl->live_branches, (unsigned long long)(-(long long)tot));
+}
+
 // ------------------------------------------------------------------ finalize + GC
 
 // voxtree.rs:303-328: adopt the new root, take the tree's reference on it, and
@@ -909,6 +1055,7 @@ __global__ void k_finalize(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64
                            uint32_t n_stride, uint64_t* const* res_levels,
                            uint64_t* const* cand_id_levels, uint64_t* roots, int32_t* status,
                            uint32_t* gc_front) {
+    const uint32_t ab = P.ctl->abort_;
     VX_LOOP(c, n) {
         bool push = false;
         uint64_t old = kEmpty;
@@ -917,6 +1064,8 @@ __global__ void k_finalize(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64
             int32_t st = C.status;
             if (C.skip) {
                 st = 0;
+            } else if (st >= 0 && ab) {
+                st = (ab & 2u) ? -8 : -2;  // VX_E_BUDGET / VX_E_OOM: the apply was rolled back
             } else if (st >= 0) {
                 uint64_t nw;
                 if (!C.has_patches) {
@@ -931,6 +1080,8 @@ __global__ void k_finalize(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64
                     st = 0;
                 } else if (old != kEmpty && nw == old) {
                     st = -6;  // assert_ne!(new_root_id, self.root_id) panics
+                } else if (P.dry) {
+                    st = 1;
                 } else {
                     if (nw != kEmpty) atomicAdd(&P.refs[bid_index(nw)], 1u);
                     push = old != kEmpty;
@@ -1006,7 +1157,7 @@ __global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uin
                 for (int j = 0; j < 8; j++) ch[j] = src[j];
                 // remove the pattern (mod.rs:518)
                 uint64_t h = branch_hash(ch, tm);
-                uint64_t want = slot_make(slot_tag(h), false, idx);
+                uint64_t want = slot_make(slot_tag(h), false, 0, idx);
                 uint64_t s = h & P.tmask;
                 for (uint64_t guard = 0; guard <= P.tmask; guard++) {
                     uint64_t e = P.tab[s];
@@ -1122,6 +1273,9 @@ __global__ void k_leaf_alloc_only(Pool<T> P, uint32_t nvalues) {
                     P.refs[idx] = 0;
                     P.leaf_map[v] = idx;
                     atomicAdd(&created, 1u);
+                } else {
+                    // fill released the old roots first: there is no state to return to
+                    atomicOr(&P.ctl->err, kErrOom | kErrPoison);
                 }
             }
         }

@@ -1,0 +1,516 @@
+"""Serving loop over a world of trees resident in HBM (continuous batching).
+
+Requests arrive over time (Poisson arrivals): get batches, sphere edits and whole-
+tree fills.  The server keeps admitting arrived requests into launch cycles while
+earlier cycles are still on the GPU:
+
+- reads run on their own stream, concurrently with applies of other trees;
+- a cycle's edits (one vx_trees_apply_edits over every admitted sphere, plus one
+  vx_trees_fill) run on the interner's stream after the cycle's reads;
+- a request may overtake an earlier one that is still waiting only when the two
+  commute (no tree that one edits and the other reads or edits), so every result
+  equals executing the requests one at a time in arrival order;
+- a request never depends on an edit that is not confirmed yet (its cycle has not
+  finished), because an apply can be rolled back.
+
+Admission by memory budget: every edit request holds a reservation of interner
+nodes; a cycle is admitted only while the reservations fit the free capacity, and
+its apply runs with node_limit = the sum of its reservations.  A reservation is
+the request's hard bound (every node position its edits touch, plus its new
+leaf), or, with a predictor, the predicted growth times a margin (capped by the
+bound).  An apply that exceeds its limit is rolled back by the library (state
+unchanged, status VX_E_BUDGET); its requests then get their exact growth from a
+dry run (the exact two-pass count) and are admitted again with that reservation.
+"""
+import collections
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import api
+
+VX_E_OOM = -2
+VX_E_BUDGET = -8
+
+
+class World:
+    """tx * ty * tz trees of one depth sharing an interner, roots resident on the device."""
+
+    def __init__(self, value_size, budget, shape, depth=5, load_factor=0.5):
+        self.it = api.VoxInterner(value_size, budget, load_factor)
+        self.shape = shape
+        self.depth = depth
+        self.side = 1 << depth
+        self.n = int(np.prod(shape))
+        self.forest = api.Forest(self.it, [depth] * self.n)
+        self.dev = self.it.device
+        # host-side facts about every tree (features of the growth predictor)
+        self.kind = np.zeros(self.n, dtype=np.int8)   # 0 empty, 1 single leaf (filled), 2 other
+        self.history = np.zeros(self.n, dtype=np.int64)  # edit requests applied so far
+
+    def tree_index(self, tx, ty, tz):
+        sx, sy, sz = self.shape
+        return (tz * sy + ty) * sx + tx
+
+    def split_world(self, xyz):
+        """World voxel coordinates [n,3] -> (tree index, local coordinates)."""
+        t = xyz // self.side
+        local = (xyz % self.side).astype(np.int32)
+        return self.tree_index(t[:, 0], t[:, 1], t[:, 2]).astype(np.int64), local
+
+    def sub(self, trees):
+        idx = torch.from_numpy(np.asarray(trees, dtype=np.int64)).to(self.dev)
+        sub = api.Forest(self.it, [self.depth] * len(trees))
+        sub.roots.copy_(self.forest.roots[idx])
+        return sub, idx
+
+    def subforest_apply_edits(self, trees, offsets, xyz_dev, val_dev, **opts):
+        """vx_trees_apply_edits on a subset of trees (roots gathered and scattered back);
+        the caller reads sub.status / sub.counts."""
+        sub, idx = self.sub(trees)
+        sub.apply_edits(offsets, xyz_dev, val_dev, sync=False, **opts)
+        if not opts.get("dry_run"):
+            self.forest.roots[idx] = sub.roots
+        return sub
+
+    def subforest_fill(self, trees, values):
+        from ._lib import check, lib
+        sub, idx = self.sub(trees)
+        vals = torch.from_numpy(np.asarray(values, dtype=np.uint64).view(np.int64)).to(self.dev)
+        check(lib().vx_trees_fill(self.it.h, len(trees), sub.roots.data_ptr(), vals.data_ptr()), "fill")
+        self.forest.roots[idx] = sub.roots
+        return sub
+
+
+class Request:
+    __slots__ = ("rid", "kind", "arrival", "trees", "payload", "edits", "done", "result", "bound",
+                 "pred", "reserve", "exact", "growth", "geom", "feat", "_tset")
+
+    def __init__(self, rid, kind, arrival):
+        self.rid = rid
+        self.kind = kind
+        self.arrival = arrival
+        self.trees = None
+        self.payload = None
+        self.edits = 0
+        self.done = None
+        self.result = None
+        self.bound = 0      # hard upper bound of new nodes
+        self.pred = None    # predicted new nodes
+        self.reserve = 0
+        self.exact = None   # exact new nodes (dry run), when measured
+        self.growth = None  # nodes its cycle created, shared out over its requests
+        self.geom = None    # per-level touched positions, covered-brick fraction
+        self.feat = None
+        self._tset = None
+
+    @property
+    def tset(self):
+        if self._tset is None:
+            self._tset = frozenset(int(x) for x in self.trees)
+        return self._tset
+
+
+def touched_positions(local, depth):
+    """Node positions at heights 1..depth that contain an edit of one tree, bounded by
+    the edits' bounding box (exact for boxes, an upper bound otherwise)."""
+    lo = local.min(axis=0).astype(np.int64)
+    hi = local.max(axis=0).astype(np.int64)
+    out = np.zeros(depth, dtype=np.int64)
+    for h in range(1, depth + 1):
+        out[h - 1] = int(np.prod((hi >> h) - (lo >> h) + 1))
+    return out
+
+
+def covered_bricks(local):
+    """(bricks touched, bricks whose 8 voxels are all written) of one tree's edits."""
+    b = local.astype(np.int64) >> 1
+    lo = b.min(axis=0)
+    ext = b.max(axis=0) - lo + 1
+    key = ((b[:, 0] - lo[0]) * ext[1] + (b[:, 1] - lo[1])) * ext[2] + (b[:, 2] - lo[2])
+    cnt = np.bincount(key, minlength=int(np.prod(ext)))
+    return int((cnt > 0).sum()), int((cnt == 8).sum())
+
+
+def sphere_edits(world, center, radius, value):
+    """Per-tree edit lists of a ball of `value`: {tree: (local xyz [k,3], values [k])}."""
+    r = int(radius)
+    lo = np.maximum(center - r, 0)
+    hi = np.minimum(center + r + 1, np.array(world.shape) * world.side)
+    if np.any(hi <= lo):
+        return {}
+    xs = np.arange(lo[0], hi[0])
+    ys = np.arange(lo[1], hi[1])
+    zs = np.arange(lo[2], hi[2])
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    inside = (X - center[0]) ** 2 + (Y - center[1]) ** 2 + (Z - center[2]) ** 2 <= radius * radius
+    pts = np.stack([X[inside], Y[inside], Z[inside]], axis=1)
+    if len(pts) == 0:
+        return {}
+    trees, local = world.split_world(pts)
+    order = np.argsort(trees, kind="stable")
+    trees, local = trees[order], local[order]
+    out = {}
+    cut = np.flatnonzero(np.diff(trees)) + 1
+    for seg_t, seg_l in zip(np.split(trees, cut), np.split(local, cut)):
+        out[int(seg_t[0])] = (seg_l, np.full(len(seg_l), value, dtype=np.uint16))
+    return out
+
+
+def describe(world, r):
+    """Fill r.bound and r.geom for an edit request (host geometry only)."""
+    if r.kind == "fill":
+        r.bound = 1
+        r.geom = (np.zeros(world.depth, dtype=np.int64), 1.0)
+        return
+    pos = np.zeros(world.depth, dtype=np.int64)
+    touched = covered = 0
+    for local, _ in r.payload.values():
+        pos += touched_positions(local, world.depth)
+        a, b = covered_bricks(local)
+        touched += a
+        covered += b
+    r.bound = int(pos.sum()) + 1  # + the request's one new leaf value
+    r.geom = (pos, covered / max(touched, 1))
+
+
+def make_requests(world, n, rate, seed, mix=(0.70, 0.25, 0.05), max_gets=65536, carve=False,
+                  radius=(2.0, 40.0)):
+    """Poisson arrivals at `rate` req/s; kinds get / sphere / fill per `mix`."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    world_size = np.array(world.shape) * world.side
+    reqs = []
+    for i in range(n):
+        u = rng.random()
+        if u < mix[0]:
+            r = Request(i, "get", t[i])
+            q = int(rng.integers(1, max_gets + 1))
+            pts = (rng.random((q, 3)) * world_size).astype(np.int64)
+            trees, local = world.split_world(pts)
+            r.trees = np.unique(trees)
+            r.payload = (trees.astype(np.uint32), local)
+        elif u < mix[0] + mix[1]:
+            r = Request(i, "sphere", t[i])
+            center = (rng.random(3) * world_size).astype(np.int64)
+            rad = float(rng.uniform(*radius))
+            value = 0 if (carve and rng.random() < 0.5) else int(rng.integers(1, 65536))
+            per_tree = sphere_edits(world, center, rad, value)
+            r.trees = np.array(sorted(per_tree), dtype=np.int64)
+            r.payload = per_tree
+            r.edits = int(sum(len(v[1]) for v in per_tree.values()))
+            if not per_tree:
+                r.kind = "noop"
+                r.trees = np.zeros(0, dtype=np.int64)
+        else:
+            r = Request(i, "fill", t[i])
+            tree = int(rng.integers(0, world.n))
+            value = 0 if rng.random() < 0.5 else int(rng.integers(1, 65536))
+            r.trees = np.array([tree], dtype=np.int64)
+            r.payload = value
+            r.edits = world.side ** 3
+        if r.kind != "get":
+            describe(world, r)
+        reqs.append(r)
+    return reqs
+
+
+class Cycle:
+    __slots__ = ("gets", "spheres", "fills", "reserve", "ev_r", "ev_e", "sub", "out", "pres", "edit_trees",
+                 "counts")
+
+    def __init__(self):
+        self.gets, self.spheres, self.fills = [], [], []
+        self.reserve = 0
+        self.ev_r = self.ev_e = self.sub = self.out = self.pres = None
+        self.edit_trees = set()
+        self.counts = None
+
+
+class Server:
+    """Continuous-batching executor of requests against one World.
+
+    predictor: None (reserve the hard bound) or an object with
+    predict(world, requests) -> predicted new nodes per request.
+    """
+
+    def __init__(self, world, predictor=None, margin=1.5, max_inflight=2, max_cycle_edits=1 << 21,
+                 max_cycle_gets=1 << 20, log_exact=False):
+        self.w = world
+        self.predictor = predictor
+        self.margin = margin
+        self.max_inflight = max_inflight
+        self.max_cycle_edits = max_cycle_edits
+        self.max_cycle_gets = max_cycle_gets
+        self.log_exact = log_exact  # dry-run every edit request at admission (training data)
+        self.R = torch.cuda.Stream(device=world.dev)
+        self.cycles = 0
+        self.fallbacks = 0        # applies rolled back for exceeding their reservation
+        self.fallback_requests = 0  # edit requests of those applies (counted exactly afterwards)
+        self.recounts = 0         # exact dry-run counts taken
+        self.admitted_edits = 0
+        self.log = []             # (features, exact growth) of measured requests
+        self.capacity = world.it.capacity()
+        self.free = None
+
+    # ---------------------------------------------------------------- budget
+
+    def _refresh_free(self):
+        self.free = self.capacity - self.w.it.alive_nodes()
+
+    def _exact(self, r):
+        """The exact two-pass count: a dry run of the request alone on the current state."""
+        self.recounts += 1
+        if r.kind == "fill":
+            r.exact = 1
+            return r.exact
+        offs, xyz_t, val_t, trees = self._pack([r])
+        sub = self.w.subforest_apply_edits(trees, offs, xyz_t, val_t, dry_run=True, counts=True)
+        b, l, _ = sub.counts_host()
+        r.exact = b + l
+        if self.log_exact:
+            from .predictor import features
+            self.log.append((features(self.w, r), r.exact))
+        return r.exact
+
+    def _reservation(self, r):
+        if r.exact is not None:
+            return r.exact
+        if r.pred is None:
+            return r.bound
+        return int(min(r.bound, math.ceil(r.pred * self.margin) + 2))
+
+    # ---------------------------------------------------------------- admission
+
+    def _admit(self, queue, now, unconfirmed):
+        """One cycle from the arrived requests, in arrival order, skipping requests
+        that do not commute with an earlier skipped one or depend on unconfirmed edits."""
+        cyc = Cycle()
+        def_edit, def_read = set(), set()
+        cyc_read = set()
+        free = self.free - sum(c.reserve for c in self._inflight)
+        n_edits = n_gets = 0
+        budget_blocked = False
+        taken = []
+        for k, r in enumerate(queue):
+            if r.arrival > now:
+                break
+            if r.kind == "noop":
+                r.result = np.zeros(0, dtype=np.int32)
+                taken.append(k)
+                continue
+            ts = r.tset
+            if r.kind == "get":
+                if ts & (unconfirmed | def_edit | cyc.edit_trees) or n_gets + len(r.payload[0]) > self.max_cycle_gets:
+                    def_read |= ts
+                    continue
+                cyc.gets.append(r)
+                cyc_read |= ts
+                n_gets += len(r.payload[0])
+                taken.append(k)
+                continue
+            if ts & (unconfirmed | def_edit | def_read | cyc.edit_trees) or (
+                    n_edits and n_edits + r.edits > self.max_cycle_edits):
+                def_edit |= ts
+                continue
+            if self.log_exact:
+                self._exact(r)  # training pair: features and exact growth on the current state
+            need = self._reservation(r)
+            if need > free - cyc.reserve:
+                def_edit |= ts
+                budget_blocked = True
+                continue
+            r.reserve = need
+            cyc.reserve += need
+            (cyc.fills if r.kind == "fill" else cyc.spheres).append(r)
+            cyc.edit_trees |= ts
+            n_edits += r.edits
+            taken.append(k)
+        for k in reversed(taken):
+            del queue[k]
+        return cyc, budget_blocked
+
+    def _pack(self, spheres):
+        trees, offs, xyzs, vals = [], [0], [], []
+        for r in spheres:
+            for t, (local, v) in sorted(r.payload.items()):
+                trees.append(t)
+                xyzs.append(local)
+                vals.append(v)
+                offs.append(offs[-1] + len(v))
+        xyz_t = api.to_device(np.concatenate(xyzs).astype(np.int32))
+        val_t = api.to_device(np.concatenate(vals).astype(np.uint16))
+        return offs, xyz_t, val_t, trees
+
+    # ---------------------------------------------------------------- launch / retire
+
+    def _launch(self, cyc):
+        dev = self.w.dev
+        if cyc.gets:
+            chunks = np.concatenate([r.payload[0] for r in cyc.gets])
+            xyz = np.concatenate([r.payload[1] for r in cyc.gets]).astype(np.int32)
+            with torch.cuda.stream(self.R):
+                chunk_t = torch.from_numpy(chunks.view(np.int32)).to(dev, non_blocking=False)
+                xyz_t = torch.from_numpy(xyz.reshape(-1)).to(dev, non_blocking=False)
+                cyc.out = torch.empty(len(chunks) * self.w.it.value_size, dtype=torch.uint8, device=dev)
+                cyc.pres = torch.empty(len(chunks), dtype=torch.uint8, device=dev)
+                self.w.forest.get_device(chunk_t, xyz_t, cyc.out, cyc.pres, stream=self.R)
+                cyc.ev_r = torch.cuda.Event()
+                cyc.ev_r.record(self.R)
+        E = torch.cuda.current_stream()
+        if cyc.spheres or cyc.fills:
+            if cyc.ev_r is not None:
+                E.wait_event(cyc.ev_r)  # this cycle's reads see the state before its edits
+            elif self._last_r is not None:
+                E.wait_event(self._last_r)  # earlier cycles' reads of these trees
+            if cyc.spheres:
+                offs, xyz_t, val_t, trees = self._pack(cyc.spheres)
+                cyc.sub = self.w.subforest_apply_edits(trees, offs, xyz_t, val_t,
+                                                       node_limit=max(cyc.reserve, 1), counts=True)
+            if cyc.fills:
+                self.w.subforest_fill([int(r.trees[0]) for r in cyc.fills], [r.payload for r in cyc.fills])
+            cyc.ev_e = torch.cuda.Event()
+            cyc.ev_e.record(E)
+        if cyc.ev_r is not None:
+            self._last_r = cyc.ev_r
+
+    def _retire(self, cyc, t0, lat, requeue):
+        if cyc.ev_r is not None:
+            cyc.ev_r.synchronize()
+            with torch.cuda.stream(self.R):
+                vals = cyc.out.cpu().numpy().view(self.w.it.np_dtype)
+                p = cyc.pres.cpu().numpy()
+            o = 0
+            for r in cyc.gets:
+                q = len(r.payload[0])
+                r.result = np.where(p[o:o + q] == 1, vals[o:o + q], 0)
+                o += q
+        rolled_back = False
+        if cyc.ev_e is not None:
+            cyc.ev_e.synchronize()
+            if cyc.spheres:
+                st = cyc.sub.status.cpu().numpy()
+                if (st == VX_E_BUDGET).any() or (st == VX_E_OOM).any():
+                    rolled_back = True
+                    self.fallbacks += 1
+                    self.fallback_requests += len(cyc.spheres)
+                    for r in cyc.spheres:
+                        r.exact = None
+                        r.pred = None
+                        requeue.append(r)
+                else:
+                    b, l, per_tree = cyc.sub.counts_host()
+                    self.free -= b + l  # nodes freed by its GC are found at the next refresh
+                    o = 0
+                    for r in cyc.spheres:
+                        k = len(r.payload)
+                        r.result = st[o:o + k]
+                        r.growth = int(per_tree[o:o + k].sum())
+                        o += k
+                        for t in r.trees:
+                            self.w.kind[t] = 2
+                            self.w.history[t] += 1
+            self.free -= len(cyc.fills)
+            for r in cyc.fills:
+                r.result = 1
+                self.w.kind[int(r.trees[0])] = 0 if r.payload == 0 else 1
+                self.w.history[int(r.trees[0])] += 1
+        t_done = time.perf_counter() - t0
+        done = cyc.gets + cyc.fills + ([] if rolled_back else cyc.spheres)
+        for r in done:
+            r.done = t_done
+            lat.append(t_done - r.arrival)
+        self.admitted_edits += 0 if rolled_back else len(cyc.spheres) + len(cyc.fills)
+        return rolled_back
+
+    def _drain(self, t0, lat, requeue):
+        while self._inflight:
+            self._retire(self._inflight.popleft(), t0, lat, requeue)
+
+    def _ready(self, c):
+        return (c.ev_e is None or c.ev_e.query()) and (c.ev_r is None or c.ev_r.query())
+
+    def serve(self, reqs, realtime=True):
+        """Run all requests; returns (latencies in s, wall time).  realtime=False treats
+        every request as arrived at time 0."""
+        queue = list(reqs)
+        if not realtime:
+            for r in queue:
+                r.arrival = 0.0
+        self._inflight = collections.deque()
+        self._last_r = None
+        lat = []
+        requeue = []
+        self._refresh_free()
+        fresh = True  # self.free was read with nothing in flight since the last change
+        t0 = time.perf_counter()
+        while queue or self._inflight:
+            while self._inflight and self._ready(self._inflight[0]):
+                self._retire(self._inflight.popleft(), t0, lat, requeue)
+                fresh = False
+            if requeue:
+                # rolled-back edits: drain, then give them exact counts (arrival order kept)
+                self._drain(t0, lat, requeue)
+                self._refresh_free()
+                for r in requeue:
+                    self._exact(r)
+                queue = sorted(queue + requeue, key=lambda r: (r.arrival, r.rid))
+                requeue = []
+                fresh = True
+            now = time.perf_counter() - t0 if realtime else 0.0
+            budget_blocked = False
+            if len(self._inflight) < self.max_inflight:
+                if self.predictor is not None:
+                    # (re)predict the edits next in line, with the trees' current features
+                    window = []
+                    for r in queue:
+                        if r.arrival > now or len(window) >= 256:
+                            break
+                        if r.kind in ("sphere", "fill") and r.exact is None:
+                            window.append(r)
+                    if window:
+                        for r, p in zip(window, self.predictor.predict(self.w, window)):
+                            r.pred = float(p)
+                unconfirmed = set()
+                for c in self._inflight:
+                    unconfirmed |= c.edit_trees
+                cyc, budget_blocked = self._admit(queue, now, unconfirmed)
+                if cyc.gets or cyc.spheres or cyc.fills:
+                    self._launch(cyc)
+                    self._inflight.append(cyc)
+                    self.cycles += 1
+                    fresh = False
+                    continue
+            if self._inflight:
+                self._retire(self._inflight.popleft(), t0, lat, requeue)
+                fresh = False
+                continue
+            if not queue:
+                break
+            if queue[0].arrival > now:
+                time.sleep(min(queue[0].arrival - now, 0.0005))
+                continue
+            if not fresh:
+                self._refresh_free()  # GC of the applied cycles may have freed nodes
+                fresh = True
+                continue
+            # nothing in flight and the free capacity is current: the first waiting edit
+            # does not fit its reservation.  Count it exactly; if even that does not fit,
+            # it cannot run (the reference would panic with out-of-memory).
+            r = next((x for x in queue if x.arrival <= now and x.kind in ("sphere", "fill")), None)
+            if r is None or not budget_blocked:
+                raise RuntimeError("admission made no progress")
+            if r.exact is None:
+                self._exact(r)
+                continue
+            if r.exact > self.free:
+                r.result = np.full(max(len(r.trees), 1), VX_E_OOM, dtype=np.int32)
+                r.done = time.perf_counter() - t0
+                lat.append(r.done - r.arrival)
+                queue.remove(r)
+                continue
+            raise RuntimeError("admission made no progress")
+        self.w.it.sync()
+        return np.array(lat), time.perf_counter() - t0
