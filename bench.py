@@ -42,12 +42,15 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="replay the captured launch sequence of repeated applies (HIP graphs)")
+    ap.add_argument("--tune-workload", default="terrain", choices=["terrain", "uneven"])
     ap.add_argument("--rates", default="200,1000,4000",
                     help="serve: Poisson arrival rates (requests/s) to sweep")
     ap.add_argument("--requests", type=int, default=1500, help="serve: requests per rate")
     ap.add_argument("--train-requests", type=int, default=1500,
                     help="serve: requests logged (with exact dry-run counts) to train the predictor")
-    ap.add_argument("--config", default="terrain", choices=["terrain", "plain", "uneven", "gets", "serve"],
+    ap.add_argument("--config", default="terrain", choices=["terrain", "plain", "uneven", "gets", "serve", "calib", "tune"],
                     help="terrain: the headline (64x64 Perlin-dune trees); plain: one 32^3 tree; "
                          "uneven: u16 trees of depth 5/6/7, dense or sparse; gets: batched reads")
     ap.add_argument("--trees", type=int, default=4096)
@@ -112,6 +115,10 @@ def main():
         return bench_gets(a)
     if a.config == "serve":
         return bench_serve(a)
+    if a.config == "calib":
+        return bench_calib(a)
+    if a.config == "tune":
+        return bench_tune(a)
     import torch
     from voxelis_amd import api
     from voxelis_amd import workloads as W
@@ -134,6 +141,7 @@ def main():
     vA_t = api.to_device(vA, dev)
     vB_t = api.to_device(vB, dev)
     it = api.VoxInterner(1, a.budget_mib << 20, a.load_factor)
+    it.set_graphs(bool(a.graphs))
     f = api.Forest(it, [5] * T)
     viewsA = f.make_views([mA_t[i] for i in range(T)], [vA_t[i] for i in range(T)])
     viewsB = f.make_views([mA_t[i] for i in range(T)], [vB_t[i] for i in range(T)])
@@ -328,7 +336,9 @@ def main():
                                "one batch per tree, all trees in one launch sequence, ids "
                                "alternating 1/2 and 2/1 per step",
                    "trees": T, "edits_per_step": E, "interner_budget_mib": a.budget_mib,
-                   "load_factor": a.load_factor, "parallelism": "single GPU"},
+                   "load_factor": a.load_factor, "parallelism": "single GPU",
+                   "launch": "HIP-graph replay of the captured launch sequence" if a.graphs
+                   else "kernels launched one by one"},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "plain_edits_per_s": plain,
         "batched_over_plain": (value / plain) if plain else None,
@@ -422,6 +432,30 @@ def bench_uneven(a):
     import torch
     from voxelis_amd import api
     from voxelis_amd import workloads as W
+    depths, mk, vA_list, vB_list, edits, E = _uneven_setup(a, api, W, torch)
+    it = api.VoxInterner(2, a.budget_mib << 20, a.load_factor)
+    f = api.Forest(it, depths)
+    views = [f.make_views(mk, vA_list), f.make_views(mk, vB_list)]
+    if not a.no_profile:
+        it.profile(True)
+    wall, lat = _timed_steps(f, views, a.steps, a.warmup, it, torch)
+    prof = it.profile_read() if not a.no_profile else {}
+    st = f.status.cpu().numpy()
+    assert (st == 1).all(), np.unique(st)
+    e57 = sum(e for e, d in zip(edits, depths) if d <= 6)
+    out = {"metric": METRIC, "value": E * a.steps / wall, "unit": "edits/s",
+           "ms_per_step": 1e3 * wall / a.steps, "p50_ms": float(np.percentile(lat, 50)),
+           "p99_ms": float(np.percentile(lat, 99)), "dtype": "u16",
+           "config": {"workload": "uneven batched edits: depth 5/6/7 trees, dense p=0.5 or sparse "
+                                  "0.1/1/10 %, u16, one launch", "trees": a.trees, "edits_per_step": E,
+                      "edits_depth_le6": int(e57), "depth7_trees": int(sum(1 for d in depths if d == 7))},
+           "counts": it.counts(),
+           "kernels": {k: {"ms_per_step": v[0] / (a.steps + a.warmup)} for k, v in prof.items()}}
+    print(json.dumps(out))
+
+
+def _uneven_setup(a, api, W, torch):
+    """Device buffers of the uneven workload: per tree masks, values, values shifted by one."""
     spec = W.uneven_spec(a.trees, depths=(5, 6, 7), p_dense=0.5, seed=a.seed + 2)
     t_gen = time.perf_counter()
     dev = torch.device("cuda", 0)
@@ -451,25 +485,45 @@ def bench_uneven(a):
     t_gen = time.perf_counter() - t_gen
     print(f"[bench] uneven: {a.trees} trees, {E} edits per step, depths "
           f"{dict(zip(*np.unique(depths, return_counts=True)))} (gen {t_gen:.1f}s)", file=sys.stderr, flush=True)
-    it = api.VoxInterner(2, a.budget_mib << 20, a.load_factor)
-    f = api.Forest(it, depths)
-    views = [f.make_views(mk, vA_list), f.make_views(mk, vB_list)]
-    if not a.no_profile:
-        it.profile(True)
-    wall, lat = _timed_steps(f, views, a.steps, a.warmup, it, torch)
-    prof = it.profile_read() if not a.no_profile else {}
-    st = f.status.cpu().numpy()
-    assert (st == 1).all(), np.unique(st)
-    e57 = sum(e for e, d in zip(edits, depths) if d <= 6)
-    out = {"metric": METRIC, "value": E * a.steps / wall, "unit": "edits/s",
-           "ms_per_step": 1e3 * wall / a.steps, "p50_ms": float(np.percentile(lat, 50)),
-           "p99_ms": float(np.percentile(lat, 99)), "dtype": "u16",
-           "config": {"workload": "uneven batched edits: depth 5/6/7 trees, dense p=0.5 or sparse "
-                                  "0.1/1/10 %, u16, one launch", "trees": a.trees, "edits_per_step": E,
-                      "edits_depth_le6": int(e57), "depth7_trees": int(sum(1 for d in depths if d == 7))},
-           "counts": it.counts(),
-           "kernels": {k: {"ms_per_step": v[0] / (a.steps + a.warmup)} for k, v in prof.items()}}
-    print(json.dumps(out))
+    return depths, mk, vA_list, vB_list, edits, E
+
+
+def bench_tune(a):
+    """Autotuner sweep (tile x probe rounds x load factor, then grid sizes) on the terrain
+    or the uneven workload; prints every point and the best."""
+    import torch
+    from voxelis_amd import api
+    from voxelis_amd import workloads as W
+    from voxelis_amd.autotune import autotune
+    dev = torch.device("cuda", 0)
+    if a.tune_workload == "terrain":
+        T = a.grid * a.grid
+        mA, vA, edits, _ = W.terrain_batches(grid=a.grid, seed=a.seed)
+        vB = vA.copy()
+        vB[vA == 1] = 2
+        vB[vA == 2] = 1
+        m_t, vA_t, vB_t = api.to_device(mA, dev), api.to_device(vA, dev), api.to_device(vB, dev)
+        depths = [5] * T
+        mk, va, vb = [m_t[i] for i in range(T)], [vA_t[i] for i in range(T)], [vB_t[i] for i in range(T)]
+        vsize, E = 1, int(edits.sum())
+    else:
+        depths, mk, va, vb, _, E = _uneven_setup(a, api, W, torch)
+        vsize = 2
+
+    def make_case(lf):
+        it = api.VoxInterner(vsize, a.budget_mib << 20, lf)
+        f = api.Forest(it, depths)
+        views = [f.make_views(mk, va), f.make_views(mk, vb)]
+        return it, (lambda k: f.apply_views(views[k % 2])), it.close
+
+    def log(pt):
+        print(f"[tune] {json.dumps(pt)}  ({E / pt['s_per_step'] / 1e9:.2f} G edits/s)", file=sys.stderr,
+              flush=True)
+
+    best, points = autotune(make_case, steps=a.steps, warmup=a.warmup, log=log)
+    print(json.dumps({"metric": "autotuned apply launch shape", "workload": a.tune_workload,
+                      "edits_per_step": E, "best": best,
+                      "best_edits_per_s": E / best["s_per_step"], "points": points}))
 
 
 def bench_gets(a):
@@ -511,6 +565,91 @@ def bench_gets(a):
                      "kernel_gets_per_s": Q / (ms / n / 1e3)}
     print(json.dumps({"metric": "batched get/s", "config": {"workload": f"{Q} uniform random voxel reads "
                       f"over the 64x64 terrain world (4096 trees of 32^3)"}, "results": res}))
+
+
+# Published single-thread apply_batch / get rates of the reference (Apple M3 Max,
+# i32, criterion mean; edits or gets per second derived from the batch size).
+_PUBLISHED = {("set_uniform", 32): 1.418e9, ("set_uniform", 64): 1.366e9,
+              ("set_sum", 32): 168.8e6, ("set_sum", 64): 170.6e6,
+              ("set_checkerboard", 32): 140.4e6, ("set_checkerboard", 64): 139.8e6,
+              ("set_sparse_fill", 32): 30.3e6, ("set_sparse_fill", 64): 27.5e6,
+              ("get_full_sum", 32): 178.5e6, ("get_full_sum", 64): 158.6e6}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def bench_calib(a):
+    """The oracle (single thread) on the reference's own criterion shapes (i32, two
+    prebuilt batches applied alternately to one tree, voxtree_bench.rs:553-579), next to
+    the published M3 Max rates: how this host's CPU baseline relates to the reference's."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    def shape_batches(name, n):
+        y, z, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+        xyz = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.int32)  # y, z, x loop order
+        s = xyz.sum(axis=1).astype(np.int64)
+        if name == "set_uniform":
+            return xyz, np.ones(len(s)), np.full(len(s), 2)
+        if name == "set_sum":
+            return xyz, s + 1, s + 1000
+        if name == "set_checkerboard":
+            keep = s % 2 == 0
+            return xyz[keep], np.ones(keep.sum()), np.full(keep.sum(), 2)
+        keep = (xyz % 4 == 0).all(axis=1)  # set_sparse_fill: every 4th on each axis
+        return xyz[keep], np.ones(keep.sum()), np.full(keep.sum(), 2)
+
+    rows = []
+    per = max(a.cpu_seconds / 10.0, 0.3)
+    for name in ("set_uniform", "set_sum", "set_checkerboard", "set_sparse_fill", "get_full_sum"):
+        for n in (32, 64):
+            d = int(np.log2(n))
+            it = O.Interner(4, 24 << 20)
+            t = O.Tree(d)
+            if name == "get_full_sum":
+                xyz, v0, _ = shape_batches("set_sum", n)
+                b = O.Batch(d)
+                b.set_many(xyz, v0.astype(np.uint64))
+                t.apply_batch(it, b)
+                c0 = time.perf_counter()
+                k = 0
+                while time.perf_counter() - c0 < per:
+                    t.get_many(it, xyz)
+                    k += 1
+                rate = k * len(xyz) / (time.perf_counter() - c0)
+                edits = len(xyz)
+            else:
+                xyz, v0, v1 = shape_batches(name, n)
+                bs = []
+                for vals in (v0, v1):
+                    b = O.Batch(d)
+                    b.set_many(xyz, vals.astype(np.uint64))
+                    bs.append(b)
+                t.apply_batch(it, bs[0])
+                k = 1
+                c0 = time.perf_counter()
+                while time.perf_counter() - c0 < per:
+                    t.apply_batch(it, bs[k % 2])
+                    k += 1
+                rate = (k - 1) * len(xyz) / (time.perf_counter() - c0)
+                edits = len(xyz)
+            pub = _PUBLISHED[(name, n)]
+            rows.append({"bench": name, "size": n, "per_batch": edits, "rate_per_s": rate,
+                         "published_m3max_per_s": pub, "ratio_to_published": rate / pub})
+            print(f"[calib] {name} {n}^3: {rate / 1e6:.1f} M/s (M3 Max {pub / 1e6:.1f} M/s, "
+                  f"ratio {rate / pub:.2f})", file=sys.stderr, flush=True)
+    ratios = [r["ratio_to_published"] for r in rows]
+    print(json.dumps({"metric": "CPU-baseline calibration: oracle vs published reference rates",
+                      "cpu": _cpu_model(), "nproc": os.cpu_count(), "threads": 1,
+                      "geomean_ratio": float(np.exp(np.mean(np.log(ratios)))), "rows": rows}))
 
 
 def _serve_world(api, serve, W, shape, budget):

@@ -77,6 +77,21 @@ int vx_interner_get_children(vx_interner* h, uint64_t block_id, uint64_t childre
 int vx_interner_stats(vx_interner* h, uint64_t* out, uint32_t n);
 /* Use the caller's HIP stream (hipStream_t) instead of the interner's own. */
 int vx_interner_set_stream(vx_interner* h, void* hip_stream);
+/* Launch shape of the apply (no reference counterpart; picked by the autotuner in
+ * voxelis_amd/autotune.py).  tile: lanes per block of the build / probe / commit
+ * passes (128, 256 or 512; a block owns one tile of level slots and its candidate
+ * region); dedup_rounds: wavefront key-merge rounds before the table probe (0-16);
+ * commit_grid / gc_grid: blocks of the commit pass and of each GC round.
+ * Defaults {256, 8, 1024, 1024}.  Synchronizes the interner's stream. */
+typedef struct {
+    uint32_t tile;
+    uint32_t dedup_rounds;
+    uint32_t commit_grid;
+    uint32_t gc_grid;
+} vx_tuning;
+int vx_interner_set_tuning(vx_interner* h, const vx_tuning* tuning);
+int vx_interner_get_tuning(const vx_interner* h, vx_tuning* tuning);
+
 /* Capture the launch sequence of vx_trees_apply_batches[_ex] as a HIP graph and
  * replay it when a later call has the same arguments (same roots/status arrays,
  * same batch views, same options): repeated batched applies skip planning and
@@ -193,6 +208,17 @@ int vx_trees_get_on(vx_interner* h, void* hip_stream, size_t n_trees, const uint
 /* fill(value) for value != 0, clear() for value == 0, on trees[i] (device
  * roots).  Asynchronous. */
 int vx_trees_fill(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* values);
+
+/* ---- dense extraction: to_vec (utils/common.rs:158-246) ---- */
+
+/* Tree t written densely at out + t * out_stride voxels (device), N = 2^(depth[t] -
+ * lod) per axis (MaxDepth::for_lod, core/max_depth.rs:137-140), index y*N*N + z*N +
+ * x.  A branch reached at the reduced depth contributes its LOD value
+ * (core/voxel.rs:96-141); lod 0 is the full-resolution to_vec.  depths: device.  A
+ * tree whose N^3 exceeds out_stride is skipped and the next sync reports VX_E_RANGE.
+ * Asynchronous. */
+int vx_trees_to_vec(vx_interner* h, size_t n, const uint64_t* roots, const uint8_t* depths, uint32_t lod,
+                    uint64_t out_stride, void* out);
 
 /* ---- verification: canonical structural hash (no reference counterpart) ---- */
 

@@ -293,10 +293,12 @@ class Tree:
             self.dirty = True
         return bool(ch.value)
 
-    def to_vec(self, it):
-        n = 1 << self.depth
+    def to_vec(self, it, lod=0):
+        """utils/common.rs to_vec with MaxDepth::for_lod(lod) (saturating)."""
+        d = max(self.depth - lod, 0)
+        n = 1 << d
         out = np.zeros(n * n * n, dtype=np.uint64)
-        _check(lib().vxo_tree_to_vec(it.h, self.root, self.depth, out.ctypes.data))
+        _check(lib().vxo_tree_to_vec(it.h, self.root, d, out.ctypes.data))
         return out
 
     def canonical_hash(self, it):

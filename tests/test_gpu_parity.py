@@ -359,3 +359,53 @@ def test_graph_replay_matches_direct_launches(vx):
         runs.append(log)
         it.close()
     assert runs[0] == runs[1]
+
+
+@pytest.mark.parametrize("tile,dedup,grid_blocks", [(128, 0, 64), (512, 2, 4096)])
+def test_launch_shapes_match_oracle(vx, tile, dedup, grid_blocks):
+    """Every tile size / probe schedule / grid size the autotuner may pick gives the
+    oracle's trees (two applies, so the second replaces and reclaims the first)."""
+    api, W = vx
+    grid = 4
+    n = grid * grid
+    mA, vA, _, _ = W.terrain_batches(grid=grid, seed=41)
+    mB, vB, _, _ = W.terrain_batches(grid=grid, seed=42, below=2, surface=3)
+    it = api.VoxInterner(1, 64 * MB)
+    it.set_tuning(tile=tile, dedup_rounds=dedup, commit_grid=grid_blocks, gc_grid=grid_blocks)
+    assert it.get_tuning() == {"tile": tile, "dedup_rounds": dedup, "commit_grid": grid_blocks,
+                               "gc_grid": grid_blocks}
+    it_o = O.Interner(1, 64 * MB)
+    f = api.Forest(it, [5] * n)
+    trees_o = [O.Tree(5) for _ in range(n)]
+    for m, v in ((mA, vA), (mB, vB)):
+        st = f.apply([api.to_device(x) for x in m], [api.to_device(x) for x in v])
+        assert (st == 1).all(), st
+        for i in range(n):
+            oracle_apply_dense(it_o, trees_o[i], m[i], v[i])
+    assert_same(it, f, it_o, trees_o, full_reads=True, what=f"tile {tile}")
+
+
+@pytest.mark.parametrize("lod", [0, 1, 3])
+def test_to_vec_with_lod_matches_oracle(vx, lod):
+    """Dense extraction of every tree (full resolution and reduced LOD depths, where
+    branches contribute their LOD values) against the oracle's to_vec."""
+    api, W = vx
+    grid = 3
+    n = grid * grid
+    m, v, _, _ = W.terrain_batches(grid=grid, seed=51, below=3, surface=7)
+    spec = [(5, "dense", 0, 1), (6, "sparse", 0.01, 2), (6, "dense", 0, 4), (5, "sparse", 0.1, 5)]
+    ub = W.uneven_batches(spec)
+    for vsize, depths, batches in ((1, [5] * n, [(m[i], v[i]) for i in range(n)]),
+                                   (2, [d for d, _, _, _ in spec], [(mm, vv) for mm, vv, _ in ub])):
+        it = api.VoxInterner(vsize, 64 * MB)
+        it_o = O.Interner(vsize, 64 * MB)
+        f = api.Forest(it, depths)
+        st = f.apply([api.to_device(a) for a, _ in batches], [api.to_device(b) for _, b in batches])
+        assert (st == 1).all()
+        got = f.to_vec(lod)
+        for i, (d, (mm, vv)) in enumerate(zip(depths, batches)):
+            t = O.Tree(d)
+            oracle_apply_dense(it_o, t, mm, vv)
+            exp = t.to_vec(it_o, lod)
+            assert np.array_equal(got[i].reshape(-1).astype(np.uint64), exp), (vsize, i, lod)
+        it.close()

@@ -33,6 +33,11 @@ class BatchView(C.Structure):
 APPLY_DRY_RUN = 1
 
 
+class Tuning(C.Structure):
+    _fields_ = [("tile", C.c_uint32), ("dedup_rounds", C.c_uint32), ("commit_grid", C.c_uint32),
+                ("gc_grid", C.c_uint32)]
+
+
 class ApplyOpts(C.Structure):
     _fields_ = [("flags", C.c_uint32), ("reserved", C.c_uint32), ("node_limit", C.c_uint64),
                 ("counts", C.c_void_p)]
@@ -50,6 +55,8 @@ SIGNATURES = {
     "vx_interner_set_stream": (_int, [_vp, _vp]),
     "vx_interner_sync": (_int, [_vp]),
     "vx_interner_set_graphs": (_int, [_vp, _int]),
+    "vx_interner_set_tuning": (_int, [_vp, C.POINTER(Tuning)]),
+    "vx_interner_get_tuning": (_int, [_vp, C.POINTER(Tuning)]),
     "vx_trees_apply_batches": (_int, [_vp, _sz, _vp, C.POINTER(BatchView), _vp]),
     "vx_tree_apply_batch_host": (_int, [_vp, C.POINTER(_u64), C.POINTER(BatchView),
                                         C.POINTER(_int)]),
@@ -61,6 +68,7 @@ SIGNATURES = {
     "vx_trees_get_on": (_int, [_vp, _vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
     "vx_trees_fill": (_int, [_vp, _sz, _vp, _vp]),
     "vx_trees_canonical_hash": (_int, [_vp, _sz, _vp, _vp]),
+    "vx_trees_to_vec": (_int, [_vp, _sz, _vp, _vp, _u32, _u64, _vp]),
     "vx_interner_profile": (_int, [_vp, _int]),
     "vx_interner_profile_read": (_int, [_vp, C.POINTER(C.c_double), C.POINTER(_u64), _u32]),
     "vx_kernel_name": (C.c_char_p, [_u32]),

@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import morton
-from ._lib import APPLY_DRY_RUN, STAT_NAMES, ApplyOpts, BatchView, VxError, check, lib
+from ._lib import APPLY_DRY_RUN, STAT_NAMES, ApplyOpts, BatchView, Tuning, VxError, check, lib
 
 EMPTY = 0
 INVALID = (1 << 64) - 1
@@ -116,6 +116,22 @@ class VoxInterner:
 
     def sync(self):
         check(lib().vx_interner_sync(self.h), "sync")
+
+    def set_tuning(self, tile=None, dedup_rounds=None, commit_grid=None, gc_grid=None):
+        """Launch shape of the apply (vx_tuning); None keeps a field."""
+        t = self.get_tuning()
+        for k, v in (("tile", tile), ("dedup_rounds", dedup_rounds), ("commit_grid", commit_grid),
+                     ("gc_grid", gc_grid)):
+            if v is not None:
+                t[k] = int(v)
+        tt = Tuning(t["tile"], t["dedup_rounds"], t["commit_grid"], t["gc_grid"])
+        check(lib().vx_interner_set_tuning(self.h, C.byref(tt)), "set_tuning")
+
+    def get_tuning(self):
+        tt = Tuning()
+        check(lib().vx_interner_get_tuning(self.h, C.byref(tt)), "get_tuning")
+        return {"tile": tt.tile, "dedup_rounds": tt.dedup_rounds, "commit_grid": tt.commit_grid,
+                "gc_grid": tt.gc_grid}
 
     def set_graphs(self, enable=True):
         """Replay repeated batched applies from captured HIP graphs."""
@@ -365,6 +381,22 @@ class Forest:
         vals = torch.from_numpy(np.asarray(values, dtype=np.uint64).view(np.int64)).to(self.it.device)
         check(lib().vx_trees_fill(self.it.h, self.n, self.roots.data_ptr(), vals.data_ptr()), "fill")
         self.it.sync()
+
+    def to_vec(self, lod=0):
+        """to_vec of every tree at LOD `lod`: a list of numpy arrays [N, N, N] indexed
+        [y, z, x] (N = 2^(depth - lod))."""
+        d = np.maximum(self.depths_host.astype(np.int64) - lod, 0)
+        stride = int((1 << (3 * int(d.max()))) if self.n else 1)
+        out = torch.zeros(self.n * stride * self.it.value_size, dtype=torch.uint8, device=self.it.device)
+        check(lib().vx_trees_to_vec(self.it.h, self.n, self.roots.data_ptr(), self.depths.data_ptr(), int(lod),
+                                    stride, out.data_ptr()), "to_vec")
+        self.it.sync()
+        flat = out.cpu().numpy().view(self.it.np_dtype).reshape(self.n, stride)
+        res = []
+        for t in range(self.n):
+            N = 1 << int(d[t])
+            res.append(flat[t, :N ** 3].reshape(N, N, N))
+        return res
 
     def canonical_hashes(self):
         out = torch.zeros(2 * self.n, dtype=torch.int64, device=self.it.device)

@@ -1,0 +1,65 @@
+"""Autotuner of the apply's launch shape: tile size (lanes per block of the build /
+probe / commit passes), wavefront key-merge rounds before the table probe (the
+probe schedule), commit and GC grid sizes, and the pattern table's load factor.
+
+The search is exhaustive over tile x dedup rounds for every load factor, then
+coordinate descent over the two grid sizes from the best point.  Each point times
+`steps` applies (after `warmup`) of the caller's workload with the device clock.
+"""
+import time
+
+import torch
+
+
+def _time(step, steps, warmup):
+    for k in range(warmup):
+        step(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(warmup + k)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def autotune(make_case, load_factors=(0.5, 0.6, 0.7, 0.85), tiles=(128, 256, 512), dedups=(0, 2, 4, 8),
+             commit_grids=(256, 1024, 4096), gc_grids=(256, 1024, 4096), steps=6, warmup=2, log=None):
+    """make_case(load_factor) -> (interner, step(k), close()); returns (best point, all points).
+
+    A point is {"load_factor", "tile", "dedup_rounds", "commit_grid", "gc_grid", "s_per_step"}."""
+    points = []
+    best = None
+    for lf in load_factors:
+        it, step, close = make_case(lf)
+        try:
+            for tile in tiles:
+                for dd in dedups:
+                    it.set_tuning(tile=tile, dedup_rounds=dd, commit_grid=1024, gc_grid=1024)
+                    t = _time(step, steps, warmup)
+                    pt = {"load_factor": lf, "tile": tile, "dedup_rounds": dd, "commit_grid": 1024,
+                          "gc_grid": 1024, "s_per_step": t}
+                    points.append(pt)
+                    if log:
+                        log(pt)
+                    if best is None or t < best["s_per_step"]:
+                        best = dict(pt)
+        finally:
+            close()
+    # coordinate descent over the grid sizes at the best (load factor, tile, rounds)
+    it, step, close = make_case(best["load_factor"])
+    try:
+        for field, choices in (("commit_grid", commit_grids), ("gc_grid", gc_grids)):
+            for v in choices:
+                cand = dict(best)
+                cand[field] = v
+                it.set_tuning(tile=cand["tile"], dedup_rounds=cand["dedup_rounds"],
+                              commit_grid=cand["commit_grid"], gc_grid=cand["gc_grid"])
+                cand["s_per_step"] = _time(step, steps, warmup)
+                points.append(cand)
+                if log:
+                    log(cand)
+                if cand["s_per_step"] < best["s_per_step"]:
+                    best = cand
+    finally:
+        close()
+    return best, points

@@ -78,12 +78,12 @@ struct LevelBufs {
 enum KernelId : uint32_t {
     KID_BEGIN = 0, KID_PREPARE, KID_BRICK_SCAN, KID_LEAF_ALLOC, KID_BRICK_BUILD, KID_PROBE_BRICKS,
     KID_COMMIT_BRICKS, KID_LEVEL_BUILD, KID_PROBE_UPPER, KID_COMMIT_UPPER, KID_FINALIZE,
-    KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_ROLLBACK, KID_COUNT
+    KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_ROLLBACK, KID_TO_VEC, KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_begin", "k_prepare", "k_brick_retry", "k_leaf_alloc", "k_brick_build", "k_probe[bricks]",
     "k_commit[bricks]", "k_level_build", "k_probe[upper]", "k_commit[upper]", "k_finalize",
-    "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill", "k_rollback"};
+    "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill", "k_rollback", "k_to_vec"};
 
 struct Prof {
     bool on = false;
@@ -146,6 +146,7 @@ struct vx_interner {
     int ring_cur = 0;
     void* h_pinned = nullptr;  // the slot of the current call
     Prof prof;
+    vx_tuning tune = {256, 8, 1024, 1024};
     // captured launch sequences of repeated calls (vx_interner_set_graphs)
     bool graphs = false;
     hipStream_t cap_stream = nullptr;
@@ -178,6 +179,7 @@ Pool<T> pool_of(vx_interner* h) {
     P.levels = nullptr;
     P.undo = nullptr;
     P.undo_cap = 0;
+    P.dedup_rounds = h->tune.dedup_rounds;
     return P;
 }
 
@@ -236,6 +238,7 @@ hipEvent_t prof_event(vx_interner* h) {
     return p.pool[p.used++];
 }
 
+#define VX_COMMA ,
 // Launch a kernel on the interner's stream, bracketed by timing events when on.
 #define VX_LAUNCH(h, kid, kern, grid, block, ...)                                   \
     do {                                                                            \
@@ -261,9 +264,9 @@ struct ApplyOpts {
     uint32_t* counts = nullptr;  // device: new branches, new leaves, then per tree
 };
 
-template <typename T>
-int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
-               int32_t* status, const int32_t* pre_status = nullptr, const ApplyOpts& opt = ApplyOpts()) {
+template <typename T, int TILE>
+int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
+                int32_t* status, const int32_t* pre_status, const ApplyOpts& opt) {
     hipStream_t s = h->stream;
     Pool<T> P = pool_of<T>(h);
     P.dry = opt.dry ? 1u : 0u;
@@ -275,7 +278,9 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
     if (h->graphs && !h->prof.on && !pre_status) {
         key = mix64(0x5650u ^ (uint64_t)n ^ ((uint64_t)sizeof(T) << 40));
         const uint64_t head[] = {(uint64_t)roots, (uint64_t)status, (uint64_t)opt.dry,
-                                 (uint64_t)opt.alloc_limit, (uint64_t)opt.counts};
+                                 (uint64_t)opt.alloc_limit, (uint64_t)opt.counts, (uint64_t)TILE,
+                                 (uint64_t)h->tune.dedup_rounds, (uint64_t)h->tune.commit_grid,
+                                 (uint64_t)h->tune.gc_grid};
         for (uint64_t w : head) key = mix64(key ^ w);
         for (size_t i = 0; i < n; i++) {
             const vx_batch_view& b = batches[i];
@@ -384,8 +389,8 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
     uint32_t* tiles_h[kLevels + 1] = {nullptr};
     uint64_t ntiles[kLevels + 1] = {0};
     for (uint32_t hh = 1; hh <= dmax; hh++) {
-        ntiles[hh] = std::max<uint64_t>((L[hh].nslots + kTile - 1) / kTile, 1);
-        uint64_t ns = ntiles[hh] * kTile;
+        ntiles[hh] = std::max<uint64_t>((L[hh].nslots + TILE - 1) / TILE, 1);
+        uint64_t ns = ntiles[hh] * TILE;
         LevelBufs& B = h->lv[hh];
         if (!B.res.ensure(ns * 8) || !B.key.ensure(ns * 72) || !B.hash.ensure(ns * 8) ||
             !B.id.ensure(ns * 8) || !B.tslot.ensure(ns * 8) || !B.slot.ensure(ns * 8) ||
@@ -446,31 +451,31 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
               pre_status);
     unsigned t1 = (unsigned)ntiles[1];
     if (dmax >= 2)
-        VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T>, t1, kTile, P, dchunks, L[1], res_ptrs_h[1], K[1],
+        VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE>, t1, TILE, P, dchunks, L[1], res_ptrs_h[1], K[1],
                   tiles_h[1], ddef, ddrift);
     VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc<T>, 1, 1024, P, dchunks, (uint32_t)n, nvalues);
     if (dmax >= 2) {
-        VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_retry<T>, 256, kBlock, P, dchunks, L[1], res_ptrs_h[1], K[1],
+        VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_retry<T VX_COMMA TILE>, 256, kBlock, P, dchunks, L[1], res_ptrs_h[1], K[1],
                   tiles_h[1], ddef, ddrift);
-        VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T>, t1, kTile, P, K[1], tiles_h[1], res_ptrs_h[1], dchunks,
+        VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T VX_COMMA TILE>, t1, TILE, P, K[1], tiles_h[1], res_ptrs_h[1], dchunks,
                   L[1]);
         if (!opt.dry)
-            VX_LAUNCH(h, KID_COMMIT_BRICKS, k_commit<T>, std::min<unsigned>(t1, 1024), kTile, P, K[1],
+            VX_LAUNCH(h, KID_COMMIT_BRICKS, k_commit<T VX_COMMA TILE>, std::min<unsigned>(t1, h->tune.commit_grid), TILE, P, K[1],
                       tiles_h[1], (uint64_t)t1);
         for (uint32_t hh = 2; hh <= dmax; hh++) {
             unsigned th = (unsigned)ntiles[hh];
-            VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T>, th, kTile, P, dchunks, L[hh],
+            VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T VX_COMMA TILE>, th, TILE, P, dchunks, L[hh],
                       doff + (size_t)(hh - 1) * n, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh],
                       K[hh], tiles_h[hh]);
-            VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T>, th, kTile, P, K[hh], tiles_h[hh], res_ptrs_h[hh],
+            VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T VX_COMMA TILE>, th, TILE, P, K[hh], tiles_h[hh], res_ptrs_h[hh],
                       dchunks, L[hh]);
             if (!opt.dry)
-                VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T>, std::min<unsigned>(th, 1024), kTile, P, K[hh],
+                VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T VX_COMMA TILE>, std::min<unsigned>(th, h->tune.commit_grid), TILE, P, K[hh],
                           tiles_h[hh], (uint64_t)th);
         }
         // dry runs restore the table; an aborted apply takes its references back
         unsigned rg = (unsigned)std::min<uint64_t>(LT.start[LT.dmax + 1], 1024);
-        VX_LAUNCH(h, KID_ROLLBACK, k_rollback<T>, std::max(rg, 1u), kTile, P, dchunks, LT, ddrift);
+        VX_LAUNCH(h, KID_ROLLBACK, k_rollback<T VX_COMMA TILE>, std::max(rg, 1u), TILE, P, dchunks, LT, ddrift);
     }
     uint32_t* gf0 = (uint32_t*)h->gc_front[0].ptr;
     uint32_t* gf1 = (uint32_t*)h->gc_front[1].ptr;
@@ -484,7 +489,7 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
         for (uint32_t r = 0; r <= dall; r++) {
             uint32_t* fr = (r & 1) ? gf1 : gf0;
             uint32_t* nx = (r & 1) ? gf0 : gf1;
-            unsigned grid = r == 0 ? grid_for(n) : 1024u;
+            unsigned grid = r == 0 ? grid_for(n) : h->tune.gc_grid;
             VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid, kBlock, P, r, fr, nx, r == 0, (uint64_t)gcap);
         }
     }
@@ -519,6 +524,16 @@ int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* b
     plan->ws_gen = gen;
     plan->last_use = ++h->plan_clock;
     return hipGraphLaunch(plan->exec, s) == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
+template <typename T>
+int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
+               int32_t* status, const int32_t* pre_status = nullptr, const ApplyOpts& opt = ApplyOpts()) {
+    switch (h->tune.tile) {
+        case 128: return apply_tiled<T, 128>(h, n, roots, batches, status, pre_status, opt);
+        case 512: return apply_tiled<T, 512>(h, n, roots, batches, status, pre_status, opt);
+        default: return apply_tiled<T, 256>(h, n, roots, batches, status, pre_status, opt);
+    }
 }
 
 template <typename T>
@@ -643,6 +658,16 @@ int get_on_impl(vx_interner* h, hipStream_t st, size_t nq, size_t ntrees, const 
                 uint8_t* present) {
     Pool<T> P = pool_of<T>(h);
     k_get<T><<<grid_for(nq), kBlock, 0, st>>>(P, nq, ntrees, roots, depths, chunk_of, xyz, (T*)out, present);
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
+template <typename T>
+int to_vec_impl(vx_interner* h, size_t n, const uint64_t* roots, const uint8_t* depths, uint32_t lod,
+                uint64_t stride, void* out) {
+    Pool<T> P = pool_of<T>(h);
+    uint64_t total = (uint64_t)n * stride;
+    VX_LAUNCH(h, KID_TO_VEC, k_to_vec<T>, grid_for(total), kBlock, P, (uint64_t)n, roots, depths, lod, stride,
+              (T*)out);
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
 }
 
@@ -981,11 +1006,37 @@ int vx_trees_fill(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* val
                          : fill_impl<uint16_t>(h, n, roots, values);
 }
 
+int vx_trees_to_vec(vx_interner* h, size_t n, const uint64_t* roots, const uint8_t* depths, uint32_t lod,
+                    uint64_t out_stride, void* out) {
+    if (!h || (n && (!roots || !depths || !out || out_stride == 0))) return VX_E_INVALID_ARG;
+    if (n == 0) return VX_OK;
+    if (lod > (uint32_t)kMaxDepth || (uint64_t)n * out_stride / out_stride != (uint64_t)n) return VX_E_INVALID_ARG;
+    return h->vsize == 1 ? to_vec_impl<uint8_t>(h, n, roots, depths, lod, out_stride, out)
+                         : to_vec_impl<uint16_t>(h, n, roots, depths, lod, out_stride, out);
+}
+
 int vx_trees_canonical_hash(vx_interner* h, size_t n, const uint64_t* roots, uint64_t* out) {
     if (!h || (n && (!roots || !out))) return VX_E_INVALID_ARG;
     if (n == 0) return VX_OK;
     return h->vsize == 1 ? canon_impl<uint8_t>(h, n, roots, out)
                          : canon_impl<uint16_t>(h, n, roots, out);
+}
+
+int vx_interner_set_tuning(vx_interner* h, const vx_tuning* t) {
+    if (!h || !t) return VX_E_INVALID_ARG;
+    if (t->tile != 128 && t->tile != 256 && t->tile != 512) return VX_E_INVALID_ARG;
+    if (t->dedup_rounds > 16 || t->commit_grid == 0 || t->commit_grid > (1u << 20) || t->gc_grid == 0 ||
+        t->gc_grid > (1u << 20))
+        return VX_E_INVALID_ARG;
+    (void)hipStreamSynchronize(h->stream);
+    h->tune = *t;
+    return VX_OK;
+}
+
+int vx_interner_get_tuning(const vx_interner* h, vx_tuning* t) {
+    if (!h || !t) return VX_E_INVALID_ARG;
+    *t = h->tune;
+    return VX_OK;
 }
 
 int vx_interner_set_graphs(vx_interner* h, int enable) {

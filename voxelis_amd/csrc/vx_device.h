@@ -82,9 +82,9 @@ __host__ __device__ inline uint32_t slot_payload(uint64_t s) { return (uint32_t)
 // Device control block: allocator cursors, live counters, errors, statistics.
 enum : uint32_t { kErrOom = 1u, kErrTableFull = 2u, kErrRange = 4u, kErrPoison = 8u };
 constexpr int kStatStripes = 64;
-// Build/probe/commit kernels work on tiles of kTile level slots (one block each);
-// a tile's candidates are compacted into its own region of the candidate arrays.
-constexpr int kTile = 256;
+// Build/probe/commit kernels work on tiles of TILE level slots (one block each,
+// TILE = 128, 256 or 512, a tuning choice); a tile's candidates are compacted into
+// its own region of the candidate arrays.
 
 struct Ctl {
     uint32_t free_top;    // entries on the free stack
@@ -171,6 +171,7 @@ struct Pool {
     uint32_t capacity;
     uint32_t dry;          // 1: count what an apply would create, change nothing
     uint32_t alloc_limit;  // allocations this apply may make (its reservation)
+    uint32_t dedup_rounds;  // wavefront key-merge rounds before the table probe
     uint32_t* per_tree;    // optional: new branch nodes owned by each tree (caller's index)
     const Cands* levels;   // candidate arrays of every level (pending keys of earlier levels)
     Drift* undo;           // reference-count deltas applied outside commits (rolled back on abort)

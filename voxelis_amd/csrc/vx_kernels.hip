@@ -383,11 +383,12 @@ __global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t n
 // ------------------------------------------------------------------ Phase 1 build
 
 // Append a branch key to this tile's region of the candidate arrays.
+template <int TILE>
 __device__ inline void emit_candidate(const Cands& K, uint32_t* s_cnt, bool pred, const uint64_t ch[8],
                                       uint32_t tm, uint64_t slot) {
     uint32_t pos = wave_append(s_cnt, pred);
     if (pred) {
-        uint64_t k = (uint64_t)blockIdx.x * kTile + pos;
+        uint64_t k = (uint64_t)blockIdx.x * TILE + pos;
         uint64_t* dst = K.key + 9ull * k;
 #pragma unroll
         for (int i = 0; i < 8; i++) dst[i] = ch[i];
@@ -510,8 +511,8 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
     return 2;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kTile) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
                                                        uint32_t* deferred, Drift* drift) {
     __shared__ uint32_t s_cnt;
@@ -519,7 +520,7 @@ __global__ __launch_bounds__(kTile) void k_brick_build(Pool<T> P, ChunkDev* chun
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
-    uint64_t slot = (uint64_t)blockIdx.x * kTile + threadIdx.x;
+    uint64_t slot = (uint64_t)blockIdx.x * TILE + threadIdx.x;
     bool cand = false, defer = false;
     int flags = 0;
     uint64_t ch[8];
@@ -588,7 +589,7 @@ __global__ __launch_bounds__(kTile) void k_brick_build(Pool<T> P, ChunkDev* chun
         }
         res1[slot] = out;
     }
-    emit_candidate(K, &s_cnt, cand, ch, tm, slot);
+    emit_candidate<TILE>(K, &s_cnt, cand, ch, tm, slot);
     uint32_t dpos = wave_append(&P.ctl->n_deferred, defer);
     if (defer) deferred[dpos] = (uint32_t)slot;
     tile_count(s_st, 0, cand);
@@ -603,7 +604,7 @@ __global__ __launch_bounds__(kTile) void k_brick_build(Pool<T> P, ChunkDev* chun
 
 // Bricks that waited for new leaves, and the queued reference-count corrections
 // of trees that did not fail (their status is final now).
-template <typename T>
+template <typename T, int TILE>
 __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, uint64_t* res1, Cands K,
                               uint32_t* tile_cnt, const uint32_t* deferred, const Drift* drift) {
     uint32_t nd = P.ctl->n_deferred;
@@ -633,9 +634,9 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
                 }
                 if (flags & 1) atomicAdd((unsigned long long*)stat(P.ctl, ST_COLLAPSE), 1ull);
                 if (r == 2) {
-                    uint64_t tile = slot / kTile;
+                    uint64_t tile = slot / TILE;
                     uint32_t pos = atomicAdd(&tile_cnt[tile], 1u);
-                    uint64_t k = tile * kTile + pos;
+                    uint64_t k = tile * TILE + pos;
                     uint64_t* dst = K.key + 9ull * k;
                     for (int j = 0; j < 8; j++) dst[j] = ch[j];
                     dst[8] = tm;
@@ -687,7 +688,6 @@ __device__ inline bool key_matches_cand(const Cands& K, uint32_t k2, const uint6
 // committed keys resolve to the existing node, the first inserter of a new key
 // owns it (allocates its index), later equal keys point at the owner.  Equal keys
 // inside one wavefront are merged first, so only one lane per key probes.
-constexpr int kDedupRounds = 8;
 
 template <typename T>
 __device__ inline int match_slot(const Pool<T>& P, uint32_t level, uint64_t e, uint32_t tag,
@@ -758,14 +758,14 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
     return -1;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kTile) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
                                                  uint64_t* res, const ChunkDev* chunks, LevelInfo L) {
     uint32_t n = tile_cnt[blockIdx.x];
     if ((threadIdx.x & ~63u) >= n) return;  // whole wavefront without candidates
     int lane = (int)__lane_id();
     bool valid = threadIdx.x < n;
-    uint64_t k = (uint64_t)blockIdx.x * kTile + threadIdx.x;
+    uint64_t k = (uint64_t)blockIdx.x * TILE + threadIdx.x;
     uint32_t tree = kNoIdx;
     if (valid && (L.h == 1 || P.per_tree)) {
         uint64_t slot = K.slot[k];
@@ -787,7 +787,7 @@ __global__ __launch_bounds__(kTile) void k_probe(Pool<T> P, Cands K, const uint3
     // wavefront-level merge of equal keys: leader = lowest lane holding the key
     int leader = lane;
     bool open = valid;
-    for (int r = 0; r < kDedupRounds; r++) {
+    for (uint32_t r = 0; r < P.dedup_rounds; r++) {
         uint64_t act = __ballot(open);
         if (!act) break;
         int L = __ffsll((unsigned long long)act) - 1;
@@ -828,8 +828,8 @@ __global__ __launch_bounds__(kTile) void k_probe(Pool<T> P, Cands K, const uint3
 
 // Owners write their node (children, LOD value, types/mask), publish the table
 // slot, and take one reference on each non-empty child.
-template <typename T>
-__global__ __launch_bounds__(kTile) void k_commit(Pool<T> P, Cands K, const uint32_t* tile_cnt,
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_commit(Pool<T> P, Cands K, const uint32_t* tile_cnt,
                                                   uint64_t ntiles) {
     __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
     agg_init(akey, acnt);
@@ -838,7 +838,7 @@ __global__ __launch_bounds__(kTile) void k_commit(Pool<T> P, Cands K, const uint
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         uint32_t n = tile_cnt[t];
         if (threadIdx.x >= n) continue;
-        uint64_t k = t * kTile + threadIdx.x;
+        uint64_t k = t * TILE + threadIdx.x;
         uint64_t ts = K.tslot[k];
         uint64_t id = K.id[k];
         if (ts == ~0ull || id == kInvalid) continue;
@@ -878,8 +878,8 @@ __device__ inline uint64_t resolve(uint64_t r, const uint64_t* cand_id_prev) {
 // One node at height h >= 2 (voxtree.rs:913-1106): gather the touched children
 // from height h-1, clone untouched ones from the old node at this position (or
 // fill them with the split leaf), collapse 8 equal leaves, else emit a key.
-template <typename T>
-__global__ __launch_bounds__(kTile) void k_level_build(
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_level_build(
     Pool<T> P, const ChunkDev* chunks, LevelInfo L, const uint64_t* slot_off_prev,
     const uint64_t* res_prev, const uint64_t* cand_id_prev, uint64_t* res, Cands K,
     uint32_t* tile_cnt) {
@@ -889,7 +889,7 @@ __global__ __launch_bounds__(kTile) void k_level_build(
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     {
-        uint64_t slot = (uint64_t)blockIdx.x * kTile + threadIdx.x;
+        uint64_t slot = (uint64_t)blockIdx.x * TILE + threadIdx.x;
         bool valid = slot < L.nslots;
         bool cand = false, collapsed = false, h5 = false;
         uint64_t ch[8];
@@ -982,7 +982,7 @@ __global__ __launch_bounds__(kTile) void k_level_build(
             }
             res[slot] = out;
         }
-        emit_candidate(K, &s_cnt, cand, ch, tm, slot);
+        emit_candidate<TILE>(K, &s_cnt, cand, ch, tm, slot);
         tile_count(s_st, 0, cand);
         tile_count(s_st, 1, collapsed);
         tile_count(s_st, 2, h5);
@@ -1005,8 +1005,8 @@ struct LevelTiles {
 // an apply that could not allocate (abort_) also takes back every reference its
 // commits and logged corrections added, so the interner is left as before.
 // Allocations themselves are undone by k_end_alloc.
-template <typename T>
-__global__ __launch_bounds__(kTile) void k_rollback(Pool<T> P, const ChunkDev* chunks, LevelTiles LT,
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_rollback(Pool<T> P, const ChunkDev* chunks, LevelTiles LT,
                                                     const Drift* drift) {
     const bool abort = !P.dry && P.ctl->abort_ != 0;
     if (!P.dry && !abort) return;
@@ -1019,7 +1019,7 @@ __global__ __launch_bounds__(kTile) void k_rollback(Pool<T> P, const ChunkDev* c
         uint32_t n = LT.cnt[h][lt];
         if (threadIdx.x >= n) continue;
         const Cands& K = P.levels[h];
-        uint64_t k = lt * kTile + threadIdx.x;
+        uint64_t k = lt * TILE + threadIdx.x;
         uint64_t ts = K.tslot[k];
         if (ts == ~0ull) continue;
         P.tab[tslot_index(ts)] = (ts & kTslotTomb) ? kSlotTomb : kSlotEmpty;
@@ -1355,6 +1355,48 @@ __device__ inline uint64_t canon_combine(uint64_t h, uint64_t child, uint32_t i)
 }
 
 // Merkle hash of a tree, seeds S0/S1; one lane per tree, iterative DFS.
+// to_vec (utils/common.rs:158-246) at a reduced depth (MaxDepth::for_lod,
+// core/max_depth.rs:137-140): one lane per output voxel, output in y, z, x order.
+// A branch reached at the reduced depth gives its LOD value; an empty child gives
+// the default value; a leaf covers its whole cube.
+template <typename T>
+__global__ void k_to_vec(Pool<T> P, uint64_t n, const uint64_t* roots, const uint8_t* depths, uint32_t lod,
+                         uint64_t stride, T* out) {
+    const uint64_t total = n * stride;
+    VX_LOOP(i, total) {
+        if (i < total) {
+            uint64_t t = i / stride, v = i - t * stride;
+            uint32_t D = depths[t];
+            if (D > (uint32_t)kMaxDepth) {
+                atomicOr(&P.ctl->err, kErrRange);
+                continue;
+            }
+            uint32_t Dl = D > lod ? D - lod : 0u;
+            uint64_t N = 1ull << Dl;
+            if (N * N * N > stride) {
+                if (v == 0) atomicOr(&P.ctl->err, kErrRange);
+                continue;
+            }
+            if (v >= N * N * N) continue;
+            uint32_t x = (uint32_t)(v % N), z = (uint32_t)((v / N) % N), y = (uint32_t)(v / (N * N));
+            uint64_t node = roots[t];
+            for (uint32_t d = 0; d < Dl && node != kEmpty && !bid_is_leaf(node); d++) {
+                if (bid_index(node) >= P.capacity) {
+                    atomicOr(&P.ctl->err, kErrPoison);
+                    node = kEmpty;
+                    break;
+                }
+                uint32_t s = Dl - 1 - d;
+                uint32_t ci = ((x >> s) & 1u) | (((y >> s) & 1u) << 1) | (((z >> s) & 1u) << 2);
+                node = P.children[8ull * bid_index(node) + ci];
+            }
+            T val = 0;
+            if (node != kEmpty && bid_index(node) < P.capacity) val = P.values[bid_index(node)];
+            out[i] = val;
+        }
+    }
+}
+
 template <typename T>
 __global__ void k_canon(Pool<T> P, uint32_t n, const uint64_t* roots, uint64_t* out) {
     const uint64_t S0 = 0x736F6D6570736575ull, S1 = 0x646F72616E646F6Dull;
