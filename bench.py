@@ -693,21 +693,15 @@ def bench_serve(a):
     reqs = serve.make_requests(world, a.train_requests, rate=1e6, seed=a.seed + 50, carve=True, max_gets=1024)
     logger = serve.Server(world, log_exact=True)
     logger.serve(reqs, realtime=False)
-    X = np.stack([f for f, _ in logger.log])
-    y = np.array([g for _, g in logger.log], dtype=np.float64)
-    cut = int(0.8 * len(y))
-    pred, hist = PR.Predictor.fit(X[:cut], y[:cut], epochs=300, batch=256, lr=3e-3)
-    ph = pred.predict_features(X[cut:])
-    mae = float(np.mean(np.abs(ph - y[cut:])))
-    bound_mae = float(np.mean(np.abs(np.expm1(X[cut:, 15].astype(np.float64)) - y[cut:])))
-    under = float(np.mean(ph * 1.5 + 2 < y[cut:]))
-    print(f"[serve] predictor: {len(y)} pairs, held-out MAE {mae:.1f} nodes (hard bound {bound_mae:.1f}), "
-          f"under-predicted at margin 1.5: {100 * under:.2f} % ({time.perf_counter() - t0:.1f}s)",
-          file=sys.stderr, flush=True)
+    pred, margin, rep = PR.Predictor.fit_from_log(logger.log, holdout=0.3, quantile=0.995)
+    print(f"[serve] predictor: {rep['pairs']} (request, tree) pairs from {rep['requests']} requests, "
+          f"held-out MAE {rep['mae_nodes']:.1f} nodes (hard bound {rep['hard_bound_mae_nodes']:.1f}), "
+          f"margin {margin:.2f}, held-out under-prediction {100 * rep['heldout_under_rate']:.2f} % "
+          f"({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
     results = []
     for ri, rate in enumerate(float(x) for x in a.rates.split(",")):
         reqs = serve.make_requests(world, a.requests, rate=rate, seed=a.seed + ri, carve=True)
-        srv = serve.Server(world, predictor=pred, margin=1.5)
+        srv = serve.Server(world, predictor=pred, margin=margin)
         torch.cuda.synchronize()
         lat, wall = srv.serve(reqs, realtime=True)
         edits = sum(r.edits for r in reqs if r.kind in ("sphere", "fill"))
@@ -729,8 +723,7 @@ def bench_serve(a):
                                "70% gets of 1-65536 voxels, 25% spheres r 2-40 (fill or carve), 5% "
                                "fill/clear; admission by memory budget with the learned predictor",
                    "interner_budget_gib": budget / (1 << 30), "requests_per_rate": a.requests},
-        "predictor": {"pairs": len(y), "mae_nodes": mae, "hard_bound_mae_nodes": bound_mae,
-                      "under_prediction_rate_margin_1_5": under, "train_loss": hist[-1]},
+        "predictor": rep,
         "results": results}))
 
 

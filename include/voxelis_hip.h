@@ -198,10 +198,12 @@ int vx_trees_get(vx_interner* h, size_t n_trees, const uint64_t* roots, const ui
  * run concurrently with an apply on the interner's stream as long as that apply
  * does not edit the trees read (nodes reachable from a root that is not replaced
  * are neither rewritten nor freed); the caller orders reads after the applies
- * they depend on (e.g. with hipStreamWaitEvent). */
+ * they depend on (e.g. with hipStreamWaitEvent).  out_stop (nullable, device, nq
+ * bytes): the depth where the traversal stopped, + 0x80 when it stopped at a leaf
+ * (a leaf at depth d covers a cube of side 2^(depth - d)). */
 int vx_trees_get_on(vx_interner* h, void* hip_stream, size_t n_trees, const uint64_t* roots,
                     const uint8_t* depths, size_t nq, const uint32_t* chunk_of_query,
-                    const int32_t* xyz, void* out_values, uint8_t* out_present);
+                    const int32_t* xyz, void* out_values, uint8_t* out_present, uint8_t* out_stop);
 
 /* ---- whole-tree ops: VoxOpsBulkWrite (voxtree.rs:263-292) ---- */
 

@@ -324,6 +324,92 @@ void vxo_interner_free(vxo_interner* o) {
     free(o);
 }
 
+/* Snapshot of the mutable interner state, for comparing against a device path whose
+ * policy for a reference panic (the process aborts there) is "the call changed
+ * nothing": take before an apply, restore if it panicked.  Test infrastructure. */
+struct vxo_snapshot {
+    uint32_t next_index;
+    size_t free_len;
+    uint32_t* free_indices;
+    uint32_t* ref_counts;
+    uint16_t* generations;
+    uint64_t (*children)[8];
+    uint64_t* values;
+    uint64_t* hashes;
+    pmap maps[2];
+    uint64_t stats[VXO_ST_COUNT];
+};
+
+vxo_snapshot* vxo_snapshot_take(const vxo_interner* o) {
+    vxo_snapshot* s = (vxo_snapshot*)calloc(1, sizeof(vxo_snapshot));
+    size_t n = o->next_index;
+    s->next_index = o->next_index;
+    s->free_len = o->free_len;
+    s->free_indices = (uint32_t*)malloc(sizeof(uint32_t) * (o->free_len + 1));
+    memcpy(s->free_indices, o->free_indices, sizeof(uint32_t) * o->free_len);
+    s->ref_counts = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    memcpy(s->ref_counts, o->ref_counts, sizeof(uint32_t) * n);
+    s->generations = (uint16_t*)malloc(sizeof(uint16_t) * n);
+    memcpy(s->generations, o->generations, sizeof(uint16_t) * n);
+    s->children = (uint64_t(*)[8])malloc(sizeof(uint64_t[8]) * n);
+    memcpy(s->children, o->children, sizeof(uint64_t[8]) * n);
+    s->values = (uint64_t*)malloc(sizeof(uint64_t) * n);
+    memcpy(s->values, o->values, sizeof(uint64_t) * n);
+    s->hashes = (uint64_t*)malloc(sizeof(uint64_t) * n);
+    memcpy(s->hashes, o->hashes, sizeof(uint64_t) * n);
+    for (int i = 0; i < 2; i++) {
+        const pmap* m = &o->maps[i];
+        s->maps[i] = *m;
+        s->maps[i].hash = (uint64_t*)malloc(sizeof(uint64_t) * m->cap);
+        s->maps[i].id = (uint64_t*)malloc(sizeof(uint64_t) * m->cap);
+        memcpy(s->maps[i].hash, m->hash, sizeof(uint64_t) * m->cap);
+        memcpy(s->maps[i].id, m->id, sizeof(uint64_t) * m->cap);
+    }
+    memcpy(s->stats, o->stats, sizeof(s->stats));
+    return s;
+}
+
+void vxo_snapshot_restore(vxo_interner* o, const vxo_snapshot* s) {
+    size_t n = s->next_index, cur = o->next_index;
+    memcpy(o->ref_counts, s->ref_counts, sizeof(uint32_t) * n);
+    memcpy(o->generations, s->generations, sizeof(uint16_t) * n);
+    memcpy(o->children, s->children, sizeof(uint64_t[8]) * n);
+    memcpy(o->values, s->values, sizeof(uint64_t) * n);
+    memcpy(o->hashes, s->hashes, sizeof(uint64_t) * n);
+    if (cur > n) { /* slots first taken by the undone call go back to never-used */
+        memset(o->ref_counts + n, 0, sizeof(uint32_t) * (cur - n));
+        memset(o->generations + n, 0, sizeof(uint16_t) * (cur - n));
+        memset(o->children + n, 0, sizeof(uint64_t[8]) * (cur - n));
+        memset(o->values + n, 0, sizeof(uint64_t) * (cur - n));
+        memset(o->hashes + n, 0, sizeof(uint64_t) * (cur - n));
+    }
+    o->next_index = s->next_index;
+    o->free_len = s->free_len;
+    memcpy(o->free_indices, s->free_indices, sizeof(uint32_t) * s->free_len);
+    for (int i = 0; i < 2; i++) {
+        pmap_free(&o->maps[i]);
+        o->maps[i] = s->maps[i];
+        o->maps[i].hash = (uint64_t*)malloc(sizeof(uint64_t) * s->maps[i].cap);
+        o->maps[i].id = (uint64_t*)malloc(sizeof(uint64_t) * s->maps[i].cap);
+        memcpy(o->maps[i].hash, s->maps[i].hash, sizeof(uint64_t) * s->maps[i].cap);
+        memcpy(o->maps[i].id, s->maps[i].id, sizeof(uint64_t) * s->maps[i].cap);
+    }
+    memcpy(o->stats, s->stats, sizeof(s->stats));
+}
+
+void vxo_snapshot_free(vxo_snapshot* s) {
+    if (!s) return;
+    free(s->free_indices);
+    free(s->ref_counts);
+    free(s->generations);
+    free(s->children);
+    free(s->values);
+    free(s->hashes);
+    pmap_free(&s->maps[0]);
+    pmap_free(&s->maps[1]);
+    free(s);
+}
+
 void vxo_set_extended_depth(vxo_interner* o, int enable) { o->extended_depth = enable; }
 uint64_t vxo_capacity(const vxo_interner* o) { return o->capacity; }
 

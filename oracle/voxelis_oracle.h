@@ -84,6 +84,12 @@ uint64_t vxo_calc_average(const uint64_t* values, size_t n);
 vxo_interner* vxo_interner_new(uint32_t value_size, uint64_t budget_bytes, int* err);
 void vxo_interner_free(vxo_interner* o);
 void vxo_set_extended_depth(vxo_interner* o, int enable); /* allow D == 7 */
+/* Snapshot / restore of the whole mutable interner state (test infrastructure: the
+ * device path leaves everything unchanged when the reference would panic). */
+typedef struct vxo_snapshot vxo_snapshot;
+vxo_snapshot* vxo_snapshot_take(const vxo_interner* o);
+void vxo_snapshot_restore(vxo_interner* o, const vxo_snapshot* s);
+void vxo_snapshot_free(vxo_snapshot* s);
 uint64_t vxo_capacity(const vxo_interner* o);
 uint64_t vxo_node_size(uint32_t value_size);
 void vxo_counts(const vxo_interner* o, uint64_t* leaf_patterns, uint64_t* branch_patterns);

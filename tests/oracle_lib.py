@@ -87,6 +87,9 @@ def lib():
         "vxo_tree_fill": (C.c_int, [vp, P64, u8, u64]),
         "vxo_tree_clear": (C.c_int, [vp, P64]),
         "vxo_tree_apply_batch": (C.c_int, [vp, P64, u8, vp, C.POINTER(C.c_int)]),
+        "vxo_snapshot_take": (vp, [vp]),
+        "vxo_snapshot_restore": (None, [vp, vp]),
+        "vxo_snapshot_free": (None, [vp]),
         "vxo_tree_to_vec": (C.c_int, [vp, u64, u8, vp]),
         "vxo_tree_canonical_hash": (C.c_int, [vp, u64, u8, P64]),
     }
@@ -292,6 +295,18 @@ class Tree:
         if ch.value:
             self.dirty = True
         return bool(ch.value)
+
+    def apply_batch_atomic(self, it, batch):
+        """apply_batch with the device path's policy for a reference panic: the tree and
+        the interner are left exactly as before the call (the panic is re-raised)."""
+        snap = lib().vxo_snapshot_take(it.h)
+        try:
+            return self.apply_batch(it, batch)
+        except OraclePanic:
+            lib().vxo_snapshot_restore(it.h, snap)
+            raise
+        finally:
+            lib().vxo_snapshot_free(snap)
 
     def to_vec(self, it, lod=0):
         """utils/common.rs to_vec with MaxDepth::for_lod(lod) (saturating)."""

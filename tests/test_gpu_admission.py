@@ -29,11 +29,15 @@ def vx():
 
 
 def _batches(api, W, grid):
-    """Step A: terrain; step B: another terrain, with fills on some trees
+    """Step A: terrain; step B: the same terrain with the ids swapped (so no batch edits
+    part of a collapsed brick, which the reference rejects), and fills on some trees
     (fill + patches on trees 0-3, fill only on 4-6)."""
     n = grid * grid
     mA, vA, _, _ = W.terrain_batches(grid=grid, seed=21)
-    mB, vB, _, _ = W.terrain_batches(grid=grid, seed=22, below=3, surface=1)
+    mB = mA
+    vB = vA.copy()
+    vB[vA == 1] = 2
+    vB[vA == 2] = 1
     fills = [None] * n
     patches = [True] * n
     for i in range(4):

@@ -1,6 +1,8 @@
 """The serving loop (continuous batching) against the oracle executing the same
-requests one at a time in arrival order: every read result, every tree's
-canonical hash and the interner's pattern counts must match."""
+requests one at a time in arrival order: every read result, every status, every
+tree's canonical hash and the interner's pattern counts must match.  Requests the
+reference rejects with a panic (H1, H6) leave the tree and the interner unchanged on
+both sides (the oracle restores a snapshot)."""
 import numpy as np
 import pytest
 
@@ -64,7 +66,9 @@ def _oracle_run(req, world, it_o, trees_o):
         b = O.Batch(5)
         b.set_many(local, vals.astype(np.uint64))
         try:
-            st.append(1 if trees_o[t].apply_batch(it_o, b) else 0)
+            # a reference panic leaves the tree and the interner as they were (the device
+            # path's policy; the reference aborts the process)
+            st.append(1 if trees_o[t].apply_batch_atomic(it_o, b) else 0)
         except O.OraclePanic as e:
             st.append({"SAME_ROOT": -6, "LEAF_TYPES": -5}[e.kind])
     return np.array(st)
@@ -133,7 +137,10 @@ def test_admission_by_budget_defers_but_completes(mods):
     shape = (2, 1, 2)
     world, _, _ = _base_world(api, serve, W, shape, 1 * MB)
     _, it_o, trees_o = _base_world(api, serve, W, shape, 64 * MB)
-    reqs = serve.make_requests(world, 30, rate=1e6, seed=3, max_gets=512, mix=(0.5, 0.4, 0.1))
+    # big spheres: their hard bounds (every touched node position) exceed the free
+    # capacity, their exact growth does not
+    reqs = serve.make_requests(world, 30, rate=1e6, seed=3, max_gets=512, mix=(0.5, 0.4, 0.1),
+                               radius=(30.0, 40.0))
     srv = serve.Server(world)
     srv.serve(reqs, realtime=False)
     assert srv.recounts > 0

@@ -463,3 +463,47 @@ def test_depth_seven_rejected_unless_extended():  # max_depth.rs:77-83
         t.apply_batch(it, b)
     it2 = O.Interner(2, MB, extended_depth=True)
     assert t.apply_batch(it2, b) and t.get(it2, 0, 0, 0) == 1
+
+
+def test_snapshot_restore_undoes_a_panicking_apply():
+    """The comparison mode for reference panics: an apply that mutates the interner and
+    then hits H1 is undone completely (counts, refcounts, allocator, later behaviour)."""
+    def build(it):
+        t = O.Tree(4)
+        b = O.Batch(4)
+        g = np.stack(np.meshgrid(*[np.arange(14, 16)] * 3, indexing="ij"), -1).reshape(-1, 3).astype(np.int32)
+        b.set_many(g, np.full(len(g), 5, dtype=np.uint64))  # a uniform brick: a leaf at brick level
+        b.set_many(np.array([[13, 15, 15], [9, 9, 9], [12, 3, 7]], dtype=np.int32),
+                   np.array([3, 2, 3], dtype=np.uint64))  # its parent stays a branch
+        t.apply_batch(it, b)
+        return t
+
+    def later(it, t):
+        b = O.Batch(4)
+        b.set_many(np.array([[11, 12, 12], [1, 14, 2]], dtype=np.int32), np.array([7, 8], dtype=np.uint64))
+        t.apply_batch(it, b)
+
+    it_a, it_b = O.Interner(1, 1 << 20), O.Interner(1, 1 << 20)
+    ta, tb = build(it_a), build(it_b)
+    before = (it_a.counts(), it_a.alive_nodes(), it_a.get_ref(ta.root), list(ta.to_vec(it_a)))
+    bad = O.Batch(4)
+    # bricks earlier in Morton order create new nodes, then the last brick is a partial
+    # edit of the collapsed one
+    bad.set_many(np.array([[0, 0, 0], [2, 2, 2], [15, 15, 15]], dtype=np.int32),
+                 np.array([4, 6, 9], dtype=np.uint64))
+    # without the snapshot the panicking call leaves mutations behind
+    it_c = O.Interner(1, 1 << 20)
+    tc = build(it_c)
+    with pytest.raises(O.OraclePanic):
+        tc.apply_batch(it_c, bad)
+    assert (it_c.counts(), it_c.alive_nodes()) != before[:2]
+    root0 = ta.root
+    with pytest.raises(O.OraclePanic) as e:
+        ta.apply_batch_atomic(it_a, bad)
+    assert e.value.kind == "LEAF_TYPES"
+    assert ta.root == root0
+    assert (it_a.counts(), it_a.alive_nodes(), it_a.get_ref(ta.root), list(ta.to_vec(it_a))) == before
+    later(it_a, ta)
+    later(it_b, tb)
+    assert it_a.counts() == it_b.counts() and it_a.alive_nodes() == it_b.alive_nodes()
+    assert ta.root == tb.root and list(ta.to_vec(it_a)) == list(tb.to_vec(it_b))

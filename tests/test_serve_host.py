@@ -1,5 +1,7 @@
 """Host-side pieces of the serving loop (CPU): sphere edit lists, the hard bound of a
-request's new nodes, and the predictor's features.
+request's new nodes, the predictor's features, and the scheduler itself on a fake
+device (every request completes, rolled-back applies are retried, each tree sees its
+edits in arrival order).
 
 The hard bound is what admission reserves without a predictor, so it must never be
 below the nodes an apply creates: checked against the oracle's get_or_create misses
@@ -109,6 +111,149 @@ def test_features_shape_and_flags():
     r.edits = sum(len(v) for _, v in r.payload.values())
     serve.describe(w, r)
     f = predictor.features(w, r)
-    assert f.shape == (16,) and f.dtype == np.float32
-    assert f[12] == 1.0 and f[14] == 0.0 and f[9] == 1.0  # carving, not a fill, empty trees
-    assert np.isclose(f[15], np.log1p(r.bound))
+    k = len(r.payload)
+    assert f.shape == (k, 16) and f.dtype == np.float32
+    assert (f[:, 12] == 1).all()  # carving
+    assert (f[:, 9] == 0).all() and (f[:, 10] == 0).all() and (f[:, 14] == 0).all()  # not sampled yet
+    assert np.isclose(np.expm1(f[:, 15].astype(np.float64)).sum() + 1, r.bound, rtol=1e-5)
+    assert np.allclose(np.expm1(f[:, 0].astype(np.float64)), [len(v) for _, (_, v) in sorted(r.payload.items())],
+                       rtol=1e-5)
+    r.occ = (np.full(k, 0.25), np.full(k, 0.5), np.ones(k), np.full(k, 0.125))
+    f = predictor.features(w, r)
+    assert (f[:, 9] == 0.25).all() and (f[:, 10] == 0.5).all() and (f[:, 14] == 1).all()
+    assert (f[:, 13] == 0.125).all()
+    seg = r.sample[3]
+    assert len(seg) == k + 1 and (np.diff(seg) <= serve.SAMPLES_PER_TREE).all() and r.sample[2] == 0
+    # one voxel of every partly written brick (a sphere's boundary), at even coordinates
+    pseg = r.sample[6]
+    assert len(pseg) == k + 1 and (r.sample[5] % 2 == 0).all()
+    for i, (t, (local, _)) in enumerate(sorted(r.payload.items())):
+        b, cnt = np.unique(local // 2, axis=0, return_counts=True)
+        exp = {tuple(x * 2) for x in b[cnt < 8]}
+        assert {tuple(x) for x in r.sample[5][pseg[i]:pseg[i + 1]]} == exp
+    fill = serve.Request(1, "fill", 0.0)
+    fill.payload, fill.trees = 3, np.array([2])
+    serve.describe(w, fill)
+    assert fill.bound == 1 and predictor.features(w, fill).shape == (0, 16)
+
+
+# ---------------------------------------------------------------- scheduler on a fake device
+
+class _FakeDevice:
+    """Stands in for the GPU: an apply's exact growth is one node per 50 edits of each
+    tree (at least one); an apply beyond its node_limit is rolled back (VX_E_BUDGET)."""
+
+    def __init__(self, capacity):
+        self.cap = capacity
+        self.alive = 100
+
+    @staticmethod
+    def per_tree(trees, offs):
+        return [max(1, (offs[i + 1] - offs[i]) // 50) for i in range(len(trees))]
+
+
+def _fake_device_world(monkeypatch, capacity):
+    import contextlib
+    import torch
+
+    class Ev:
+        def record(self, s=None):
+            pass
+
+        def query(self):
+            return True
+
+        def synchronize(self):
+            pass
+
+    class St:
+        cuda_stream = 0
+
+        def wait_event(self, e):
+            pass
+
+    monkeypatch.setattr(torch.cuda, "Stream", lambda device=None: St())
+    monkeypatch.setattr(torch.cuda, "Event", lambda *a, **k: Ev())
+    monkeypatch.setattr(torch.cuda, "current_stream", lambda: St())
+    monkeypatch.setattr(torch.cuda, "stream", lambda s: contextlib.nullcontext())
+    monkeypatch.setattr(serve.api, "to_device", lambda a, device=None: a)
+    w = _fake_world(shape=(4, 2, 4))
+    dev = _FakeDevice(capacity)
+
+    class Sub:
+        def __init__(self, n, st, b, per):
+            self.status = torch.tensor(st, dtype=torch.int32)
+            self._c = (b, 0, np.asarray(per))
+
+        def counts_host(self):
+            return self._c
+
+    def apply_edits(trees, offs, xyz, vals, dry_run=False, node_limit=0, counts=False):
+        per = dev.per_tree(trees, offs)
+        g = sum(per)
+        if dry_run:
+            return Sub(len(trees), [1] * len(trees), g, per)
+        if node_limit and g > node_limit:
+            return Sub(len(trees), [serve.VX_E_BUDGET] * len(trees), g, per)
+        dev.alive += g
+        return Sub(len(trees), [1] * len(trees), g, per)
+
+    def fill(trees, vals):
+        dev.alive += 1
+
+    class It:
+        value_size = 2
+        np_dtype = np.uint16
+
+        def capacity(self):
+            return dev.cap
+
+        def alive_nodes(self):
+            return dev.alive
+
+        def sync(self):
+            pass
+
+    class Forest:
+        def get_device(self, chunk_t, xyz_t, out, pres, stream=None, stop=None):
+            pass
+
+    w.it = It()
+    w.dev = "cpu"
+    w.forest = Forest()
+    w.subforest_apply_edits = apply_edits
+    w.subforest_fill = fill
+    return w, dev
+
+
+class _Under:
+    def predict(self, world, reqs):
+        return [0.0] * len(reqs)
+
+
+def test_scheduler_completes_and_keeps_per_tree_order(monkeypatch):
+    for predictor, cap in ((None, 10 ** 9), (None, 2500), (_Under(), 10 ** 9)):
+        w, dev = _fake_device_world(monkeypatch, cap)
+        reqs = serve.make_requests(w, 150, rate=1e6, seed=4, mix=(0.0, 0.9, 0.1), radius=(2.0, 20.0))
+        srv = serve.Server(w, predictor=predictor, margin=1.0)
+        order = []
+        orig = srv._launch
+
+        def launch(cyc):
+            order.append([r.rid for r in cyc.spheres])
+            orig(cyc)
+        srv._launch = launch
+        srv.serve(reqs, realtime=False)
+        assert all(r.done is not None for r in reqs if r.kind != "noop")
+        if predictor is not None:
+            assert srv.fallbacks > 0 and srv.recounts > 0
+        # every tree saw its edit requests in arrival order
+        applied = {}
+        for cyc_rids in order:
+            for rid in cyc_rids:
+                r = reqs[rid]
+                if np.asarray(r.result).size and (np.asarray(r.result) != serve.VX_E_BUDGET).all():
+                    for t in r.trees:
+                        applied.setdefault(int(t), []).append(rid)
+        for t, rids in applied.items():
+            assert rids == sorted(rids), (t, rids)

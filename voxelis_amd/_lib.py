@@ -65,7 +65,7 @@ SIGNATURES = {
                                          C.POINTER(ApplyOpts)]),
     "vx_trees_apply_edits_ex": (_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(ApplyOpts)]),
     "vx_trees_get": (_int, [_vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
-    "vx_trees_get_on": (_int, [_vp, _vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
+    "vx_trees_get_on": (_int, [_vp, _vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _vp]),
     "vx_trees_fill": (_int, [_vp, _sz, _vp, _vp]),
     "vx_trees_canonical_hash": (_int, [_vp, _sz, _vp, _vp]),
     "vx_trees_to_vec": (_int, [_vp, _sz, _vp, _vp, _u32, _u64, _vp]),

@@ -368,14 +368,16 @@ class Forest:
             check(rc, "get")
         return out.cpu().numpy().view(self.it.np_dtype).astype(np.uint64), p
 
-    def get_device(self, chunk_t, xyz_t, out, pres, stream=None):
-        """Device-to-device get; stream (a torch.cuda.Stream) runs it off the interner's stream."""
+    def get_device(self, chunk_t, xyz_t, out, pres, stream=None, stop=None):
+        """Device-to-device get; stream (a torch.cuda.Stream) runs it off the interner's stream;
+        stop (uint8 [nq], needs stream) receives the traversal's stop depth (+0x80 at a leaf)."""
         args = (self.n, self.roots.data_ptr(), self.depths.data_ptr(), chunk_t.numel(), chunk_t.data_ptr(),
                 xyz_t.data_ptr(), out.data_ptr(), pres.data_ptr() if pres is not None else None)
-        if stream is None:
+        if stream is None and stop is None:
             check(lib().vx_trees_get(self.it.h, *args), "get")
         else:
-            check(lib().vx_trees_get_on(self.it.h, C.c_void_p(stream.cuda_stream), *args), "get")
+            sp = C.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+            check(lib().vx_trees_get_on(self.it.h, sp, *args, stop.data_ptr() if stop is not None else None), "get")
 
     def fill(self, values):
         vals = torch.from_numpy(np.asarray(values, dtype=np.uint64).view(np.int64)).to(self.it.device)

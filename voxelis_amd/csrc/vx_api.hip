@@ -655,9 +655,10 @@ int get_impl(vx_interner* h, size_t nq, size_t ntrees, const uint64_t* roots,
 template <typename T>
 int get_on_impl(vx_interner* h, hipStream_t st, size_t nq, size_t ntrees, const uint64_t* roots,
                 const uint8_t* depths, const uint32_t* chunk_of, const int32_t* xyz, void* out,
-                uint8_t* present) {
+                uint8_t* present, uint8_t* stop) {
     Pool<T> P = pool_of<T>(h);
-    k_get<T><<<grid_for(nq), kBlock, 0, st>>>(P, nq, ntrees, roots, depths, chunk_of, xyz, (T*)out, present);
+    k_get<T><<<grid_for(nq), kBlock, 0, st>>>(P, nq, ntrees, roots, depths, chunk_of, xyz, (T*)out, present,
+                                              stop);
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
 }
 
@@ -989,13 +990,13 @@ int vx_trees_get(vx_interner* h, size_t ntrees, const uint64_t* roots, const uin
 
 int vx_trees_get_on(vx_interner* h, void* stream, size_t ntrees, const uint64_t* roots,
                     const uint8_t* depths, size_t nq, const uint32_t* chunk_of, const int32_t* xyz,
-                    void* out, uint8_t* present) {
+                    void* out, uint8_t* present, uint8_t* stop) {
     if (!h || (nq && (!roots || !depths || !chunk_of || !xyz || !out))) return VX_E_INVALID_ARG;
     if (nq == 0) return VX_OK;
     hipStream_t st = (hipStream_t)stream;
     return h->vsize == 1
-               ? get_on_impl<uint8_t>(h, st, nq, ntrees, roots, depths, chunk_of, xyz, out, present)
-               : get_on_impl<uint16_t>(h, st, nq, ntrees, roots, depths, chunk_of, xyz, out, present);
+               ? get_on_impl<uint8_t>(h, st, nq, ntrees, roots, depths, chunk_of, xyz, out, present, stop)
+               : get_on_impl<uint16_t>(h, st, nq, ntrees, roots, depths, chunk_of, xyz, out, present, stop);
 }
 
 int vx_trees_fill(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* values) {

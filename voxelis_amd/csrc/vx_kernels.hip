@@ -511,8 +511,10 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
     return 2;
 }
 
+// Register budget for 8 waves per SIMD (fits without spilling): the pass is bound by
+// the latency of its old-tree descent, so more waves in flight hide more of it.
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
                                                        uint32_t* deferred, Drift* drift) {
     __shared__ uint32_t s_cnt;
@@ -1294,7 +1296,7 @@ __global__ void k_leaf_alloc_only(Pool<T> P, uint32_t nvalues) {
 template <typename T>
 __global__ void k_get(Pool<T> P, uint64_t nq, uint64_t ntrees, const uint64_t* roots,
                       const uint8_t* depths, const uint32_t* chunk_of, const int32_t* xyz, T* out,
-                      uint8_t* present) {
+                      uint8_t* present, uint8_t* stop = nullptr) {
     VX_LOOP(i, nq) {
         if (i < nq) {
             uint32_t c = chunk_of[i];
@@ -1306,11 +1308,12 @@ __global__ void k_get(Pool<T> P, uint64_t nq, uint64_t ntrees, const uint64_t* r
             int32_t lim = 1 << D;
             uint32_t v = 0;
             uint8_t pr = 0;
+            uint32_t d = 0;
+            bool at_leaf = false;
             if (!tree_ok || x < 0 || y < 0 || z < 0 || x >= lim || y >= lim || z >= lim) {
                 atomicOr(&P.ctl->err, kErrRange);
                 pr = 0xFF;
             } else {
-                uint32_t d = 0;
                 while (node != kEmpty) {
                     uint32_t idx = bid_index(node);
                     if (idx >= P.capacity) {
@@ -1331,6 +1334,7 @@ __global__ void k_get(Pool<T> P, uint64_t nq, uint64_t ntrees, const uint64_t* r
                     } else {
                         v = (uint32_t)P.values[idx];
                         pr = 1;
+                        at_leaf = true;
                         break;
                     }
                 }
@@ -1338,6 +1342,7 @@ __global__ void k_get(Pool<T> P, uint64_t nq, uint64_t ntrees, const uint64_t* r
             }
             out[i] = (T)v;
             if (present) present[i] = pr;
+            if (stop) stop[i] = (uint8_t)(d | (at_leaf ? 0x80u : 0u));
         }
     }
 }

@@ -1,25 +1,30 @@
 """Learned node-growth predictor for admission by memory budget.
 
-An FP32 MLP 16 -> 64 -> 64 -> 1 (ReLU) predicts log(1 + new interner nodes) of an
-edit request from 16 features of its geometry and of the trees it touches.  It is
-trained on the GPU by the library's own kernels (vx_mlp_gradient + vx_mlp_adam) on
-(features, exact growth) pairs that the serving loop logs with dry runs
-(Server(log_exact=True)).  The server reserves ceil(prediction * margin) + 2 nodes
-per request (never more than the request's hard bound); an apply whose requests
-need more is rolled back by the library and the exact dry-run count takes over.
+An FP32 MLP 16 -> 64 -> 64 -> 1 (ReLU) predicts log(1 + new branch nodes) that an
+edit request creates in one tree, from 16 features of the request's edits in that
+tree and of the tree's current state; a request's prediction is the sum over its
+trees plus its one new leaf.  It is trained on the GPU by the library's own kernels
+(vx_mlp_gradient + vx_mlp_adam) on (features, exact growth) pairs that the serving
+loop logs with dry runs (Server(log_exact=True)).  The server reserves
+ceil(prediction * margin) + 2 nodes per request (never more than the request's hard
+bound); an apply whose requests need more is rolled back by the library and the
+exact dry-run count takes over.  The margin is calibrated on held-out requests.
 
-Features (index: meaning):
-  0   log1p(voxels edited)
-  1   log1p(trees touched)
-  2-7 log1p(node positions touched at height 1..6, summed over trees; 0 above depth)
+Features of (request, tree) (index: meaning):
+  0   log1p(voxels the request edits in the tree)
+  1   log1p(trees the request touches)
+  2-6 log1p(node positions touched at height 1..5)
+  7   log1p(node positions touched at heights 6 and 7)
   8   fraction of the touched 2x2x2 bricks written completely (uniform bricks)
-  9   fraction of the touched trees that are empty
-  10  fraction of the touched trees that are one leaf (filled)
-  11  log1p(mean edit requests applied so far to the touched trees)
+  9   fraction of sampled edited voxels that are non-empty now (batched get)
+  10  fraction of sampled edited voxels that already hold the written value
+  11  log1p(edit requests applied so far to the tree)
   12  1 if the request writes the default value (carving)
-  13  tree depth / 8
-  14  1 for a whole-tree fill
-  15  log1p(hard bound of the request's new nodes)
+  13  fraction of the partly written bricks that lie inside a leaf above brick level
+  14  1 if a partly written brick is a leaf at brick level now (the reference panics, H1)
+  15  log1p(hard bound of the tree's new branch nodes)
+Features 9, 10, 13 and 14 come from one batched read of the touched trees
+(vx_trees_get_on with stop depths) at admission time.
 """
 import ctypes as C
 import math
@@ -33,25 +38,25 @@ N_FEATURES = 16
 
 
 def features(world, r):
-    f = np.zeros(N_FEATURES, dtype=np.float32)
-    pos, cov = r.geom
-    f[0] = math.log1p(r.edits)
-    f[1] = math.log1p(len(r.trees))
-    for h in range(min(len(pos), 6)):
-        f[2 + h] = math.log1p(int(pos[h]))
-    f[8] = cov
-    kinds = world.kind[r.trees]
-    f[9] = float(np.mean(kinds == 0))
-    f[10] = float(np.mean(kinds == 1))
-    f[11] = math.log1p(float(np.mean(world.history[r.trees])))
-    if r.kind == "sphere":
-        v = next(iter(r.payload.values()))[1]
-        f[12] = 1.0 if len(v) and int(v[0]) == 0 else 0.0
-    else:
-        f[12] = 1.0 if r.payload == 0 else 0.0
-    f[13] = world.depth / 8.0
-    f[14] = 1.0 if r.kind == "fill" else 0.0
-    f[15] = math.log1p(r.bound)
+    """[trees touched, 16] feature rows of an edit request (sphere); fills have none."""
+    if r.kind != "sphere":
+        return np.zeros((0, N_FEATURES), dtype=np.float32)
+    edits, pos, cov = r.geom
+    k = len(edits)
+    trees = np.asarray(sorted(r.payload), dtype=np.int64)
+    f = np.zeros((k, N_FEATURES), dtype=np.float32)
+    f[:, 0] = np.log1p(edits)
+    f[:, 1] = math.log1p(k)
+    d = pos.shape[1]
+    f[:, 2:2 + min(d, 5)] = np.log1p(pos[:, :5])
+    if d > 5:
+        f[:, 7] = np.log1p(pos[:, 5:].sum(axis=1))
+    f[:, 8] = cov
+    if r.occ is not None:
+        f[:, 9], f[:, 10], f[:, 14], f[:, 13] = r.occ
+    f[:, 11] = np.log1p(world.history[trees])
+    f[:, 12] = 1.0 if r.sample[2] == 0 else 0.0
+    f[:, 15] = np.log1p(pos.sum(axis=1))
     return f
 
 
@@ -135,24 +140,83 @@ class Predictor:
         self.mu = np.asarray(mu, dtype=np.float32)
         self.sd = np.asarray(sd, dtype=np.float32)
 
-    def predict_features(self, X):
-        z = (np.asarray(X, dtype=np.float32) - self.mu) / self.sd
-        y = self.model.forward(torch.from_numpy(z).to(self.model.device))
-        return np.maximum(np.expm1(y.cpu().numpy().astype(np.float64)), 0.0)
+    def predict_features(self, X, bound=None):
+        """Predicted new nodes; features outside the training range are clipped (at 8
+        standard deviations), and a prediction never exceeds the hard bound (feature 15)."""
+        X = np.asarray(X, dtype=np.float32)
+        z = np.clip((X - self.mu) / self.sd, -8.0, 8.0).astype(np.float32)
+        y = self.model.forward(torch.from_numpy(z).to(self.model.device)).cpu().numpy().astype(np.float64)
+        cap = X[:, 15].astype(np.float64) if bound is None else np.log1p(np.asarray(bound, dtype=np.float64))
+        return np.expm1(np.clip(y, 0.0, cap))
 
     def predict(self, world, reqs):
-        return self.predict_features(np.stack([features(world, r) for r in reqs]))
+        """Predicted new nodes per request: per-tree branch predictions summed, plus the
+        request's new leaf; a fill creates at most its leaf."""
+        rows = [features(world, r) for r in reqs]
+        n = [len(x) for x in rows]
+        out = np.ones(len(reqs), dtype=np.float64)
+        if sum(n):
+            per = self.predict_features(np.concatenate(rows))
+            o = 0
+            for i, k in enumerate(n):
+                out[i] += per[o:o + k].sum()
+                o += k
+        return np.minimum(out, [r.bound for r in reqs])
+
+    @staticmethod
+    def calibrate_margin(pred, exact, quantile=0.995, lo=1.0, hi=8.0):
+        """Smallest margin m (in [lo, hi]) with exact <= ceil(pred * m) + 2 for `quantile`
+        of the pairs."""
+        pred = np.asarray(pred, dtype=np.float64)
+        exact = np.asarray(exact, dtype=np.float64)
+        need = np.where(exact <= 2, 0.0, (exact - 2) / np.maximum(pred, 1e-9))
+        return float(np.clip(np.quantile(need, quantile), lo, hi))
 
     @classmethod
     def fit(cls, X, growth, epochs=300, batch=512, lr=3e-3, seed=0, device=None):
         X = np.asarray(X, dtype=np.float32)
         mu = X.mean(axis=0)
-        sd = X.std(axis=0) + 1e-3
+        sd = np.maximum(X.std(axis=0), 0.05)  # a feature constant in training stays near 0
         model = GrowthMLP(device=device, seed=seed)
         xt = torch.from_numpy((X - mu) / sd).to(model.device)
         tt = torch.from_numpy(np.log1p(np.asarray(growth, dtype=np.float64)).astype(np.float32)).to(model.device)
         hist = model.train(xt, tt, epochs=epochs, batch=batch, lr=lr, seed=seed)
         return cls(model, mu, sd), hist
+
+    def predict_rows(self, rows_per_request):
+        """Request-level predictions from per-request feature rows (sum + one leaf)."""
+        n = [len(x) for x in rows_per_request]
+        out = np.ones(len(n), dtype=np.float64)
+        if sum(n):
+            per = self.predict_features(np.concatenate(rows_per_request))
+            o = 0
+            for i, k in enumerate(n):
+                out[i] += per[o:o + k].sum()
+                o += k
+        return out
+
+    @classmethod
+    def fit_from_log(cls, log, holdout=0.3, quantile=0.995, epochs=300, batch=512, lr=3e-3, seed=0):
+        """Train on a serving log [(rows [k,16], per-tree growth [k], exact request growth)],
+        holding out the last `holdout` of the requests to measure the error and to
+        calibrate the margin.  Returns (predictor, margin, report)."""
+        log = [e for e in log if len(e[0])]
+        cut = int((1.0 - holdout) * len(log))
+        train, test = log[:cut], log[cut:]
+        X = np.concatenate([e[0] for e in train])
+        y = np.concatenate([e[1] for e in train])
+        pred, hist = cls.fit(X, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        p = pred.predict_rows([e[0] for e in test])
+        exact = np.array([e[2] for e in test], dtype=np.float64)
+        bound = np.array([np.expm1(e[0][:, 15].astype(np.float64)).sum() + 1 for e in test])
+        margin = cls.calibrate_margin(p, exact, quantile=quantile)
+        report = {"pairs": int(len(y)), "requests": len(log), "heldout_requests": len(test),
+                  "mae_nodes": float(np.mean(np.abs(p - exact))),
+                  "hard_bound_mae_nodes": float(np.mean(np.abs(bound - exact))),
+                  "mean_growth": float(exact.mean()) if len(exact) else 0.0, "margin": margin,
+                  "heldout_under_rate": float(np.mean(np.ceil(p * margin) + 2 < exact)) if len(exact) else 0.0,
+                  "train_loss": hist[-1] if hist else None}
+        return pred, margin, report
 
     def save(self, path):
         np.savez(path, params=self.model.p.cpu().numpy(), mu=self.mu, sd=self.sd)

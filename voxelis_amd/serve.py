@@ -86,7 +86,7 @@ class World:
 
 class Request:
     __slots__ = ("rid", "kind", "arrival", "trees", "payload", "edits", "done", "result", "bound",
-                 "pred", "reserve", "exact", "growth", "geom", "feat", "_tset")
+                 "pred", "reserve", "exact", "growth", "geom", "feat", "_tset", "occ", "sample")
 
     def __init__(self, rid, kind, arrival):
         self.rid = rid
@@ -105,6 +105,8 @@ class Request:
         self.geom = None    # per-level touched positions, covered-brick fraction
         self.feat = None
         self._tset = None
+        self.occ = None     # per touched tree: sampled non-empty / equal fractions, H1 flag, split fraction
+        self.sample = None  # sampled edits and partly written bricks, with per-tree segments
 
     @property
     def tset(self):
@@ -124,14 +126,23 @@ def touched_positions(local, depth):
     return out
 
 
-def covered_bricks(local):
-    """(bricks touched, bricks whose 8 voxels are all written) of one tree's edits."""
+def covered_bricks(local, with_partial=False):
+    """(bricks touched, bricks whose 8 voxels are all written) of one tree's edits, and
+    with_partial: the first voxel of every partly written brick [m, 3]."""
     b = local.astype(np.int64) >> 1
     lo = b.min(axis=0)
     ext = b.max(axis=0) - lo + 1
     key = ((b[:, 0] - lo[0]) * ext[1] + (b[:, 1] - lo[1])) * ext[2] + (b[:, 2] - lo[2])
     cnt = np.bincount(key, minlength=int(np.prod(ext)))
-    return int((cnt > 0).sum()), int((cnt == 8).sum())
+    res = (int((cnt > 0).sum()), int((cnt == 8).sum()))
+    if not with_partial:
+        return res
+    k = np.flatnonzero((cnt > 0) & (cnt < 8))
+    bz = k % ext[2]
+    by = (k // ext[2]) % ext[1]
+    bx = k // (ext[2] * ext[1])
+    partial = (np.stack([bx, by, bz], axis=1) + lo) * 2
+    return res + (partial.astype(np.int32),)
 
 
 def sphere_edits(world, center, radius, value):
@@ -159,21 +170,48 @@ def sphere_edits(world, center, radius, value):
     return out
 
 
+SAMPLES_PER_TREE = 64
+PARTIAL_PER_TREE = 2048
+
+
 def describe(world, r):
-    """Fill r.bound and r.geom for an edit request (host geometry only)."""
+    """Fill r.bound and r.geom for an edit request (host geometry only).
+
+    r.geom, per touched tree in sorted tree order: (edits [k], touched node positions
+    [k, depth] at heights 1..depth, fraction of touched bricks written completely [k]);
+    r.sample: up to SAMPLES_PER_TREE evenly strided edited positions of every tree."""
     if r.kind == "fill":
         r.bound = 1
-        r.geom = (np.zeros(world.depth, dtype=np.int64), 1.0)
+        r.geom = None
         return
-    pos = np.zeros(world.depth, dtype=np.int64)
-    touched = covered = 0
-    for local, _ in r.payload.values():
-        pos += touched_positions(local, world.depth)
-        a, b = covered_bricks(local)
-        touched += a
-        covered += b
+    items = sorted(r.payload.items())
+    k = len(items)
+    edits = np.zeros(k, dtype=np.int64)
+    pos = np.zeros((k, world.depth), dtype=np.int64)
+    cov = np.zeros(k, dtype=np.float64)
+    s_tree, s_local, s_seg = [], [], [0]
+    p_tree, p_local, p_seg = [], [], [0]
+    for i, (t, (local, v)) in enumerate(items):
+        edits[i] = len(v)
+        pos[i] = touched_positions(local, world.depth)
+        a, b, partial = covered_bricks(local, with_partial=True)
+        cov[i] = b / max(a, 1)
+        step = max(1, len(v) // SAMPLES_PER_TREE)
+        pick = local[::step][:SAMPLES_PER_TREE]
+        s_tree.append(np.full(len(pick), t, dtype=np.uint32))
+        s_local.append(pick)
+        s_seg.append(s_seg[-1] + len(pick))
+        pstep = max(1, -(-len(partial) // PARTIAL_PER_TREE))
+        partial = partial[::pstep]
+        p_tree.append(np.full(len(partial), t, dtype=np.uint32))
+        p_local.append(partial)
+        p_seg.append(p_seg[-1] + len(partial))
     r.bound = int(pos.sum()) + 1  # + the request's one new leaf value
-    r.geom = (pos, covered / max(touched, 1))
+    r.geom = (edits, pos, cov)
+    value = int(items[0][1][1][0])
+    # sampled edited voxels (occupancy) and partly written bricks (one voxel each)
+    r.sample = (np.concatenate(s_tree), np.concatenate(s_local).astype(np.int32), value, np.array(s_seg),
+                np.concatenate(p_tree), np.concatenate(p_local).astype(np.int32), np.array(p_seg))
 
 
 def make_requests(world, n, rate, seed, mix=(0.70, 0.25, 0.05), max_gets=65536, carve=False,
@@ -268,12 +306,68 @@ class Server:
             return r.exact
         offs, xyz_t, val_t, trees = self._pack([r])
         sub = self.w.subforest_apply_edits(trees, offs, xyz_t, val_t, dry_run=True, counts=True)
-        b, l, _ = sub.counts_host()
+        b, l, per_tree = sub.counts_host()
         r.exact = b + l
         if self.log_exact:
             from .predictor import features
-            self.log.append((features(self.w, r), r.exact))
+            self._occupancy([r])
+            # one pair per touched tree: its new branch nodes (a node shared by two trees
+            # of the request counts for one of them)
+            self.log.append((features(self.w, r), np.asarray(per_tree, dtype=np.float64), r.exact))
         return r.exact
+
+    def _occupancy(self, reqs):
+        """Sampled reads of the voxels each request edits (one batched get on the read
+        stream): fraction non-empty and fraction already equal to the written value.
+        Only called for requests whose trees no unconfirmed apply touches."""
+        todo = [r for r in reqs if r.kind == "sphere" and r.sample is not None]
+        for r in reqs:
+            if r.kind != "sphere":
+                r.occ = None
+        if not todo:
+            return
+        chunks = np.concatenate([np.concatenate([r.sample[0], r.sample[4]]) for r in todo])
+        xyz = np.concatenate([np.concatenate([r.sample[1], r.sample[5]]) for r in todo]).astype(np.int32)
+        dev = self.w.dev
+        with torch.cuda.stream(self.R):
+            chunk_t = torch.from_numpy(chunks.view(np.int32)).to(dev)
+            xyz_t = torch.from_numpy(xyz.reshape(-1)).to(dev)
+            out = torch.empty(len(chunks) * self.w.it.value_size, dtype=torch.uint8, device=dev)
+            pres = torch.empty(len(chunks), dtype=torch.uint8, device=dev)
+            stop = torch.empty(len(chunks), dtype=torch.uint8, device=dev)
+            self.w.forest.get_device(chunk_t, xyz_t, out, pres, stream=self.R, stop=stop)
+            vals = out.cpu().numpy().view(self.w.it.np_dtype)
+            p = pres.cpu().numpy()
+            sd = stop.cpu().numpy()
+        brick_level = self.w.depth - 1
+        o = 0
+        for r in todo:
+            k = len(r.sample[0])
+            v = np.where(p[o:o + k] == 1, vals[o:o + k], 0).astype(np.int64)
+            seg = r.sample[3]
+            nonempty = np.add.reduceat((v != 0).astype(np.float64), seg[:-1]) / np.maximum(np.diff(seg), 1)
+            equal = np.add.reduceat((v == r.sample[2]).astype(np.float64), seg[:-1]) / np.maximum(np.diff(seg), 1)
+            o += k
+            m = len(r.sample[4])
+            st = sd[o:o + m].astype(np.int64)
+            o += m
+            pseg = r.sample[6]
+            at_leaf = (st & 0x80) != 0
+            depth = st & 0x7F
+            # a partly written brick whose node is a leaf at brick level: the apply of
+            # that tree panics (H1); a leaf above brick level is split along the path
+            h1_hits = (at_leaf & (depth == brick_level)).astype(np.float64)
+            split = (at_leaf & (depth < brick_level)).astype(np.float64)
+            n_p = np.maximum(np.diff(pseg), 1)
+            if m:
+                h1 = np.maximum.reduceat(h1_hits, np.minimum(pseg[:-1], m - 1)) * (np.diff(pseg) > 0)
+                sp = np.add.reduceat(split, np.minimum(pseg[:-1], m - 1)) * (np.diff(pseg) > 0) / n_p
+            else:
+                h1 = np.zeros(len(pseg) - 1)
+                sp = np.zeros(len(pseg) - 1)
+            if r.sample[2] == 0:
+                h1 = np.zeros_like(h1)  # writing the default value records no set bits
+            r.occ = (nonempty, equal, h1, sp)  # per touched tree
 
     def _reservation(self, r):
         if r.exact is not None:
@@ -446,7 +540,7 @@ class Server:
         self._refresh_free()
         fresh = True  # self.free was read with nothing in flight since the last change
         t0 = time.perf_counter()
-        while queue or self._inflight:
+        while queue or self._inflight or requeue:
             while self._inflight and self._ready(self._inflight[0]):
                 self._retire(self._inflight.popleft(), t0, lat, requeue)
                 fresh = False
@@ -462,20 +556,22 @@ class Server:
             now = time.perf_counter() - t0 if realtime else 0.0
             budget_blocked = False
             if len(self._inflight) < self.max_inflight:
+                unconfirmed = set()
+                for c in self._inflight:
+                    unconfirmed |= c.edit_trees
                 if self.predictor is not None:
-                    # (re)predict the edits next in line, with the trees' current features
+                    # (re)predict the edits next in line whose trees no unconfirmed apply
+                    # touches, with the trees' current features
                     window = []
                     for r in queue:
                         if r.arrival > now or len(window) >= 256:
                             break
-                        if r.kind in ("sphere", "fill") and r.exact is None:
+                        if r.kind in ("sphere", "fill") and r.exact is None and not (r.tset & unconfirmed):
                             window.append(r)
                     if window:
+                        self._occupancy(window)
                         for r, p in zip(window, self.predictor.predict(self.w, window)):
                             r.pred = float(p)
-                unconfirmed = set()
-                for c in self._inflight:
-                    unconfirmed |= c.edit_trees
                 cyc, budget_blocked = self._admit(queue, now, unconfirmed)
                 if cyc.gets or cyc.spheres or cyc.fills:
                     self._launch(cyc)
@@ -488,7 +584,7 @@ class Server:
                 fresh = False
                 continue
             if not queue:
-                break
+                continue  # rolled-back edits may be waiting in requeue
             if queue[0].arrival > now:
                 time.sleep(min(queue[0].arrival - now, 0.0005))
                 continue
