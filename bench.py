@@ -219,7 +219,7 @@ def main():
             ach = per_launch_bytes / avg_s / 1e9
             roofline = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": None, "avg_launch_ms": round(ms / n, 4),
+                        "traffic": _committed_traffic(name, a.grid), "avg_launch_ms": round(ms / n, 4),
                         "algorithmic_bytes_per_launch": int(per_launch_bytes)}
         print(f"[bench] kernels: {json.dumps(kern)}", file=sys.stderr, flush=True)
 
@@ -272,10 +272,16 @@ def main():
               file=sys.stderr, flush=True)
 
     if a.calib_copy:
-        src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        # the reference kernel for the memory counters: exactly 1 GiB read and written
+        import ctypes as C
+        from voxelis_amd._lib import lib
+        src = torch.ones(1 << 30, dtype=torch.uint8, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        ms = C.c_float(0)
+        lib().vx_bench_copy(C.c_void_p(torch.cuda.current_stream().cuda_stream), dst.data_ptr(), src.data_ptr(),
+                            1 << 30, C.byref(ms))
+        print(f"[bench] copy 1 GiB: {ms.value:.3f} ms = {2 * (1 << 30) / ms.value / 1e6:.0f} GB/s",
+              file=sys.stderr, flush=True)
         del src, dst
 
     # ---- plain per-tree form on the same GPU (one apply call per tree)
@@ -350,6 +356,19 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def _committed_traffic(kernel, grid):
+    """HBM bytes per launch of `kernel` on the default terrain workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_terrain_4096x32_u8.json")
+    if grid != 64 or not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    for k, v in d["kernels"].items():
+        if k.startswith(kernel + "<"):
+            return int(v["hbm_bytes_scaled"] / max(v["dispatches"], 1))
+    return None
 
 
 def _timed_steps(f, views_cycle, steps, warmup, it, torch):

@@ -259,6 +259,9 @@ int vx_mlp_adam(void* hip_stream, float* params, const float* grad, float* m, fl
  * add, 2: 32-bit add on 64 hot counters).  Kernel
  * time in *ms.  Synchronizes the stream; buf contents are overwritten. */
 int vx_bench_atomics(void* hip_stream, int kind, void* buf, uint64_t slots, uint64_t ops, float* ms);
+/* Plain copy of `bytes` (a multiple of 16, 16-byte aligned buffers) with the kernel
+ * time in *ms: the reference kernel for scaling memory-traffic counters.  Synchronizes. */
+int vx_bench_copy(void* hip_stream, void* dst, const void* src, uint64_t bytes, float* ms);
 
 /* ---- device memory helpers for callers without a HIP allocator ---- */
 int vx_device_alloc(size_t bytes, void** ptr);

@@ -20,6 +20,7 @@ def _fake_world(shape=(2, 2, 2), depth=5):
     w.n = int(np.prod(shape))
     w.kind = np.zeros(w.n, dtype=np.int8)
     w.history = np.zeros(w.n, dtype=np.int64)
+    w.version = np.zeros(w.n, dtype=np.int64)
     return w
 
 

@@ -73,6 +73,7 @@ SIGNATURES = {
     "vx_interner_profile_read": (_int, [_vp, C.POINTER(C.c_double), C.POINTER(_u64), _u32]),
     "vx_kernel_name": (C.c_char_p, [_u32]),
     "vx_bench_atomics": (_int, [_vp, _int, _vp, _u64, _u64, C.POINTER(C.c_float)]),
+    "vx_bench_copy": (_int, [_vp, _vp, _vp, _u64, C.POINTER(C.c_float)]),
     "vx_mlp_param_count": (_int, []),
     "vx_mlp_forward": (_int, [_vp, _vp, _vp, _u32, _vp]),
     "vx_mlp_gradient": (_int, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp]),

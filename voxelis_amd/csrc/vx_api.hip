@@ -1099,6 +1099,23 @@ int vx_bench_atomics(void* stream, int kind, void* buf, uint64_t slots, uint64_t
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
 }
 
+int vx_bench_copy(void* stream, void* dst, const void* src, uint64_t bytes, float* ms) {
+    if (!dst || !src || !ms || bytes == 0 || (bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return VX_E_INVALID_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    k_copy_bench<<<2048, 256, 0, st>>>((const uint4*)src, (uint4*)dst, bytes / 16);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
 int vx_mlp_param_count(void) { return kMlpParams; }
 
 int vx_mlp_forward(void* stream, const float* params, const float* x, uint32_t n, float* y) {

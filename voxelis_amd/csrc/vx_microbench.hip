@@ -34,4 +34,11 @@ __global__ void k_atomic_bench(void* buf, uint64_t slots, uint32_t per_lane, int
     if (acc == 0x5A5A5A5A5A5A5A5Aull) sink[0] = acc;  // keeps the returns live
 }
 
+// A plain streaming copy (16 B per lane per step): the reference point that scales
+// the memory counters (bytes read and written are exactly `bytes` each).
+__global__ void k_copy_bench(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace vx

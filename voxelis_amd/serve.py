@@ -49,6 +49,7 @@ class World:
         # host-side facts about every tree (features of the growth predictor)
         self.kind = np.zeros(self.n, dtype=np.int8)   # 0 empty, 1 single leaf (filled), 2 other
         self.history = np.zeros(self.n, dtype=np.int64)  # edit requests applied so far
+        self.version = np.zeros(self.n, dtype=np.int64)  # bumped whenever a tree changes
 
     def tree_index(self, tx, ty, tz):
         sx, sy, sz = self.shape
@@ -86,7 +87,7 @@ class World:
 
 class Request:
     __slots__ = ("rid", "kind", "arrival", "trees", "payload", "edits", "done", "result", "bound",
-                 "pred", "reserve", "exact", "growth", "geom", "feat", "_tset", "occ", "sample")
+                 "pred", "reserve", "exact", "growth", "geom", "feat", "occ", "sample", "_sel")
 
     def __init__(self, rid, kind, arrival):
         self.rid = rid
@@ -103,16 +104,11 @@ class Request:
         self.exact = None   # exact new nodes (dry run), when measured
         self.growth = None  # nodes its cycle created, shared out over its requests
         self.geom = None    # per-level touched positions, covered-brick fraction
-        self.feat = None
-        self._tset = None
+        self.feat = None    # version stamp of the trees when last predicted
+        self._sel = None
         self.occ = None     # per touched tree: sampled non-empty / equal fractions, H1 flag, split fraction
         self.sample = None  # sampled edits and partly written bricks, with per-tree segments
 
-    @property
-    def tset(self):
-        if self._tset is None:
-            self._tset = frozenset(int(x) for x in self.trees)
-        return self._tset
 
 
 def touched_positions(local, depth):
@@ -255,6 +251,29 @@ def make_requests(world, n, rate, seed, mix=(0.70, 0.25, 0.05), max_gets=65536, 
     return reqs
 
 
+def _selector(r, n):
+    """The request's trees as an index array, or as a bool mask when it touches many
+    (a mask tests and merges in one vector pass)."""
+    if r._sel is None:
+        r._sel = r.trees
+        if len(r.trees) * 32 > n:
+            m = np.zeros(n, dtype=bool)
+            m[r.trees] = True
+            r._sel = m
+    return r._sel
+
+
+def _hit(mask, sel):
+    return bool(np.logical_and(mask, sel).any()) if sel.dtype == bool else bool(mask[sel].any())
+
+
+def _mark(mask, sel):
+    if sel.dtype == bool:
+        np.logical_or(mask, sel, out=mask)
+    else:
+        mask[sel] = True
+
+
 class Cycle:
     __slots__ = ("gets", "spheres", "fills", "reserve", "ev_r", "ev_e", "sub", "out", "pres", "edit_trees",
                  "counts")
@@ -263,7 +282,7 @@ class Cycle:
         self.gets, self.spheres, self.fills = [], [], []
         self.reserve = 0
         self.ev_r = self.ev_e = self.sub = self.out = self.pres = None
-        self.edit_trees = set()
+        self.edit_trees = None  # bool mask over the world's trees
         self.counts = None
 
 
@@ -275,13 +294,14 @@ class Server:
     """
 
     def __init__(self, world, predictor=None, margin=1.5, max_inflight=2, max_cycle_edits=1 << 21,
-                 max_cycle_gets=1 << 20, log_exact=False):
+                 max_cycle_gets=1 << 20, log_exact=False, max_scan=1024):
         self.w = world
         self.predictor = predictor
         self.margin = margin
         self.max_inflight = max_inflight
         self.max_cycle_edits = max_cycle_edits
         self.max_cycle_gets = max_cycle_gets
+        self.max_scan = max_scan  # requests examined per admission pass
         self.log_exact = log_exact  # dry-run every edit request at admission (training data)
         self.R = torch.cuda.Stream(device=world.dev)
         self.cycles = 0
@@ -382,44 +402,51 @@ class Server:
         """One cycle from the arrived requests, in arrival order, skipping requests
         that do not commute with an earlier skipped one or depend on unconfirmed edits."""
         cyc = Cycle()
-        def_edit, def_read = set(), set()
-        cyc_read = set()
+        n = self.w.n
+        # tree masks: edited / read by requests skipped in this pass, edited in this cycle
+        def_edit = np.zeros(n, dtype=bool)
+        def_read = np.zeros(n, dtype=bool)
+        cyc.edit_trees = np.zeros(n, dtype=bool)
+        no_edit = unconfirmed | cyc.edit_trees  # kept equal to unconfirmed | def_edit | cycle edits
         free = self.free - sum(c.reserve for c in self._inflight)
         n_edits = n_gets = 0
         budget_blocked = False
         taken = []
+        scan = 0
         for k, r in enumerate(queue):
-            if r.arrival > now:
+            if r.arrival > now or scan >= self.max_scan:
                 break
+            scan += 1
             if r.kind == "noop":
                 r.result = np.zeros(0, dtype=np.int32)
                 taken.append(k)
                 continue
-            ts = r.tset
+            ts = _selector(r, n)
             if r.kind == "get":
-                if ts & (unconfirmed | def_edit | cyc.edit_trees) or n_gets + len(r.payload[0]) > self.max_cycle_gets:
-                    def_read |= ts
+                if _hit(no_edit, ts) or n_gets + len(r.payload[0]) > self.max_cycle_gets:
+                    _mark(def_read, ts)
                     continue
                 cyc.gets.append(r)
-                cyc_read |= ts
                 n_gets += len(r.payload[0])
                 taken.append(k)
                 continue
-            if ts & (unconfirmed | def_edit | def_read | cyc.edit_trees) or (
-                    n_edits and n_edits + r.edits > self.max_cycle_edits):
-                def_edit |= ts
+            if _hit(no_edit, ts) or _hit(def_read, ts) or (n_edits and n_edits + r.edits > self.max_cycle_edits):
+                _mark(def_edit, ts)
+                _mark(no_edit, ts)
                 continue
             if self.log_exact:
                 self._exact(r)  # training pair: features and exact growth on the current state
             need = self._reservation(r)
             if need > free - cyc.reserve:
-                def_edit |= ts
+                _mark(def_edit, ts)
+                _mark(no_edit, ts)
                 budget_blocked = True
                 continue
             r.reserve = need
             cyc.reserve += need
             (cyc.fills if r.kind == "fill" else cyc.spheres).append(r)
-            cyc.edit_trees |= ts
+            _mark(cyc.edit_trees, ts)
+            _mark(no_edit, ts)
             n_edits += r.edits
             taken.append(k)
         for k in reversed(taken):
@@ -503,14 +530,15 @@ class Server:
                         r.result = st[o:o + k]
                         r.growth = int(per_tree[o:o + k].sum())
                         o += k
-                        for t in r.trees:
-                            self.w.kind[t] = 2
-                            self.w.history[t] += 1
+                        self.w.kind[r.trees] = 2
+                        self.w.history[r.trees] += 1
+                        self.w.version[r.trees] += 1
             self.free -= len(cyc.fills)
             for r in cyc.fills:
                 r.result = 1
                 self.w.kind[int(r.trees[0])] = 0 if r.payload == 0 else 1
                 self.w.history[int(r.trees[0])] += 1
+                self.w.version[int(r.trees[0])] += 1
         t_done = time.perf_counter() - t0
         done = cyc.gets + cyc.fills + ([] if rolled_back else cyc.spheres)
         for r in done:
@@ -556,18 +584,22 @@ class Server:
             now = time.perf_counter() - t0 if realtime else 0.0
             budget_blocked = False
             if len(self._inflight) < self.max_inflight:
-                unconfirmed = set()
+                unconfirmed = np.zeros(self.w.n, dtype=bool)
                 for c in self._inflight:
-                    unconfirmed |= c.edit_trees
+                    if c.edit_trees is not None:
+                        unconfirmed |= c.edit_trees
                 if self.predictor is not None:
                     # (re)predict the edits next in line whose trees no unconfirmed apply
                     # touches, with the trees' current features
                     window = []
-                    for r in queue:
+                    for r in queue[:self.max_scan]:
                         if r.arrival > now or len(window) >= 256:
                             break
-                        if r.kind in ("sphere", "fill") and r.exact is None and not (r.tset & unconfirmed):
+                        if (r.kind in ("sphere", "fill") and r.exact is None and not unconfirmed[r.trees].any()
+                                and r.feat != int(self.w.version[r.trees].sum())):
                             window.append(r)
+                    for r in window:
+                        r.feat = int(self.w.version[r.trees].sum())  # trees' state the prediction saw
                     if window:
                         self._occupancy(window)
                         for r, p in zip(window, self.predictor.predict(self.w, window)):
