@@ -82,7 +82,7 @@ enum KernelId : uint32_t {
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_begin", "k_prepare", "k_brick_retry", "k_leaf_alloc", "k_brick_build", "k_probe[bricks]",
-    "k_commit[bricks]", "k_level_build", "k_probe[upper]", "k_commit[upper]", "k_finalize",
+    "k_commit[bricks]", "k_level_build", "k_probe[upper]", "k_commit[upper]", "k_finish",
     "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill", "k_rollback", "k_to_vec"};
 
 struct Prof {
@@ -446,7 +446,6 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     P.undo_cap = (uint32_t)undo_cap;
     // ---- launch sequence
     auto launch_all = [&]() -> int {
-    VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
     VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, grid_for(n), kBlock, P, dchunks, (uint32_t)n, roots,
               pre_status);
     unsigned t1 = (unsigned)ntiles[1];
@@ -457,40 +456,41 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     if (dmax >= 2) {
         VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_retry<T VX_COMMA TILE>, 256, kBlock, P, dchunks, L[1], res_ptrs_h[1], K[1],
                   tiles_h[1], ddef, ddrift);
-        VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T VX_COMMA TILE>, t1, TILE, P, K[1], tiles_h[1], res_ptrs_h[1], dchunks,
-                  L[1]);
-        if (!opt.dry)
-            VX_LAUNCH(h, KID_COMMIT_BRICKS, k_commit<T VX_COMMA TILE>, std::min<unsigned>(t1, h->tune.commit_grid), TILE, P, K[1],
-                      tiles_h[1], (uint64_t)t1);
+        VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T VX_COMMA TILE>, t1, TILE, P, K[1], tiles_h[1], res_ptrs_h[1],
+                  dchunks, L[1]);
         for (uint32_t hh = 2; hh <= dmax; hh++) {
             unsigned th = (unsigned)ntiles[hh];
-            VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T VX_COMMA TILE>, th, TILE, P, dchunks, L[hh],
+            // the commit of height hh-1 rides along in extra blocks (none in a dry run)
+            unsigned tc = opt.dry ? 0u : std::min<unsigned>((unsigned)ntiles[hh - 1], h->tune.commit_grid);
+            VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T VX_COMMA TILE>, th + tc, TILE, P, dchunks, L[hh],
                       doff + (size_t)(hh - 1) * n, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh],
-                      K[hh], tiles_h[hh]);
+                      K[hh], tiles_h[hh], th, K[hh - 1], tiles_h[hh - 1], (uint64_t)ntiles[hh - 1]);
             VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T VX_COMMA TILE>, th, TILE, P, K[hh], tiles_h[hh], res_ptrs_h[hh],
                       dchunks, L[hh]);
-            if (!opt.dry)
-                VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T VX_COMMA TILE>, std::min<unsigned>(th, h->tune.commit_grid), TILE, P, K[hh],
-                          tiles_h[hh], (uint64_t)th);
         }
-        // dry runs restore the table; an aborted apply takes its references back
-        unsigned rg = (unsigned)std::min<uint64_t>(LT.start[LT.dmax + 1], 1024);
-        VX_LAUNCH(h, KID_ROLLBACK, k_rollback<T VX_COMMA TILE>, std::max(rg, 1u), TILE, P, dchunks, LT, ddrift);
+        if (!opt.dry) {
+            unsigned td = (unsigned)ntiles[dmax];
+            VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T VX_COMMA TILE>, std::min<unsigned>(td, h->tune.commit_grid),
+                      TILE, P, K[dmax], tiles_h[dmax], (uint64_t)td);
+        }
     }
     uint32_t* gf0 = (uint32_t*)h->gc_front[0].ptr;
     uint32_t* gf1 = (uint32_t*)h->gc_front[1].ptr;
-    VX_LAUNCH(h, KID_FINALIZE, k_finalize<T>, grid_for(n), kBlock, P, dchunks, (uint32_t)n, doff,
-              (uint32_t)n, dres, dcid, roots, status, gf0);
+    // the rollback (dry runs restore the table; an aborted apply takes its references
+    // back) and the root swap
+    unsigned rg = (unsigned)std::min<uint64_t>(std::max<uint64_t>(LT.start[LT.dmax + 1], 1), 1024);
+    unsigned fg = std::max(rg, (unsigned)((n + TILE - 1) / TILE));
+    VX_LAUNCH(h, KID_FINALIZE, k_finish<T VX_COMMA TILE>, std::min(fg, 65535u), TILE, P, dchunks, LT, ddrift,
+              (uint32_t)n, doff, (uint32_t)n, dres, dcid, roots, status, gf0);
     VX_LAUNCH(h, KID_END_ALLOC, k_end_alloc<T>, 1, 256, P, true, opt.counts);
     if (!opt.dry) {
-        VX_LAUNCH(h, KID_LEAF_CLEANUP, k_leaf_cleanup<T>, grid_for(nvalues), kBlock, P);
         // dec_ref_recursive of the replaced roots: a tree of depth D has nodes at
-        // heights D..0, one round each
+        // heights D..0, one round each; the first round also releases unused new leaves
         for (uint32_t r = 0; r <= dall; r++) {
             uint32_t* fr = (r & 1) ? gf1 : gf0;
             uint32_t* nx = (r & 1) ? gf0 : gf1;
             unsigned grid = r == 0 ? grid_for(n) : h->tune.gc_grid;
-            VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid, kBlock, P, r, fr, nx, r == 0, (uint64_t)gcap);
+            VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid, kBlock, P, r, fr, nx, r == 0, (uint64_t)gcap, r == 0);
         }
     }
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
@@ -632,7 +632,7 @@ int fill_impl(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* vals) {
         uint32_t* fr = (r & 1) ? gf1 : gf0;
         uint32_t* nx = (r & 1) ? gf0 : gf1;
         unsigned grid = r == 0 ? grid_for(n) : 1024u;
-        VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid, kBlock, P, r, fr, nx, r == 0, (uint64_t)gcap);
+        VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid, kBlock, P, r, fr, nx, r == 0, (uint64_t)gcap, false);
     }
     VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
     VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc_only<T>, 1, 1024, P, nvalues);

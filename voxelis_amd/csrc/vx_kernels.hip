@@ -238,8 +238,9 @@ __device__ inline uint32_t group_of(const LevelInfo& L, uint64_t slot) {
 
 // ------------------------------------------------------------------ launch setup
 
-__global__ void k_begin(Ctl* ctl) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+// Start of a launch sequence: allocator snapshot, per-sequence counters cleared.
+__device__ inline void ctl_begin(Ctl* ctl) {
+    {
         ctl->free_top0 = ctl->free_top;
         ctl->next0 = ctl->next_index;
         ctl->alloc_count = 0;
@@ -253,6 +254,10 @@ __global__ void k_begin(Ctl* ctl) {
         for (int i = 0; i <= kLevels; i++) ctl->ncand[i] = 0;
         for (int i = 0; i < kLevels + 2; i++) ctl->gc_count[i] = 0;
     }
+}
+
+__global__ void k_begin(Ctl* ctl) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl_begin(ctl);
 }
 
 // Close the allocation window of a launch sequence.  An aborted apply
@@ -306,6 +311,8 @@ __global__ void k_end_alloc(Pool<T> P, bool rollback, uint32_t* counts) {
 template <typename T>
 __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_t* roots,
                           const int32_t* pre_status) {
+    // the sequence's control-block reset rides in block 0 (nothing else here reads it)
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl_begin(P.ctl);
     VX_LOOP(c, n) {
         if (c < n) {
             ChunkDev& C = chunks[c];
@@ -383,12 +390,14 @@ __global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t n
 // ------------------------------------------------------------------ Phase 1 build
 
 // Append a branch key to this tile's region of the candidate arrays.
+// Compaction of a tile's candidate keys into the tile's region of K (order of
+// discovery).  Called by every thread of the block.
 template <int TILE>
 __device__ inline void emit_candidate(const Cands& K, uint32_t* s_cnt, bool pred, const uint64_t ch[8],
-                                      uint32_t tm, uint64_t slot) {
+                                      uint32_t tm, uint64_t slot, uint32_t blk) {
     uint32_t pos = wave_append(s_cnt, pred);
     if (pred) {
-        uint64_t k = (uint64_t)blockIdx.x * TILE + pos;
+        uint64_t k = (uint64_t)blk * TILE + pos;
         uint64_t* dst = K.key + 9ull * k;
 #pragma unroll
         for (int i = 0; i < 8; i++) dst[i] = ch[i];
@@ -523,6 +532,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     uint64_t slot = (uint64_t)blockIdx.x * TILE + threadIdx.x;
+    uint64_t out = kInvalid;
     bool cand = false, defer = false;
     int flags = 0;
     uint64_t ch[8];
@@ -535,7 +545,6 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
         uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
         uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
         const ChunkDev& C = chunks[c];
-        uint64_t out = kInvalid;
         uint32_t m = C.status >= 0 ? C.masks[2ull * p] : 0u;
         if (m != 0) {
             uint32_t v[8];
@@ -589,9 +598,9 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                 }
             }
         }
-        res1[slot] = out;
     }
-    emit_candidate<TILE>(K, &s_cnt, cand, ch, tm, slot);
+    if (slot < L.nslots) res1[slot] = out;
+    emit_candidate<TILE>(K, &s_cnt, cand, ch, tm, slot, blockIdx.x);
     uint32_t dpos = wave_append(&P.ctl->n_deferred, defer);
     if (defer) deferred[dpos] = (uint32_t)slot;
     tile_count(s_st, 0, cand);
@@ -830,14 +839,15 @@ __global__ __launch_bounds__(TILE) void k_probe(Pool<T> P, Cands K, const uint32
 
 // Owners write their node (children, LOD value, types/mask), publish the table
 // slot, and take one reference on each non-empty child.
+// Block `blk` of `nblk` (a grid-stride loop over the tiles).
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) void k_commit(Pool<T> P, Cands K, const uint32_t* tile_cnt,
-                                                  uint64_t ntiles) {
+__device__ inline void commit_body(const Pool<T>& P, const Cands& K, const uint32_t* tile_cnt, uint64_t ntiles,
+                                   uint32_t blk, uint32_t nblk) {
     __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
     agg_init(akey, acnt);
     __syncthreads();
     uint32_t owned = 0;
-    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (uint64_t t = blk; t < ntiles; t += nblk) {
         uint32_t n = tile_cnt[t];
         if (threadIdx.x >= n) continue;
         uint64_t k = t * TILE + threadIdx.x;
@@ -871,8 +881,15 @@ __global__ __launch_bounds__(TILE) void k_commit(Pool<T> P, Cands K, const uint3
     wave_add64(&P.ctl->live_branches, owned);
 }
 
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_commit(Pool<T> P, Cands K, const uint32_t* tile_cnt,
+                                                  uint64_t ntiles) {
+    commit_body<T, TILE>(P, K, tile_cnt, ntiles, blockIdx.x, gridDim.x);
+}
+
 // ------------------------------------------------------------------ Phase 2
 
+// A pending result points at the candidate of the previous height that owns its key.
 __device__ inline uint64_t resolve(uint64_t r, const uint64_t* cand_id_prev) {
     return res_is_pending(r) ? cand_id_prev[(uint32_t)r] : r;
 }
@@ -881,18 +898,19 @@ __device__ inline uint64_t resolve(uint64_t r, const uint64_t* cand_id_prev) {
 // from height h-1, clone untouched ones from the old node at this position (or
 // fill them with the split leaf), collapse 8 equal leaves, else emit a key.
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) void k_level_build(
-    Pool<T> P, const ChunkDev* chunks, LevelInfo L, const uint64_t* slot_off_prev,
-    const uint64_t* res_prev, const uint64_t* cand_id_prev, uint64_t* res, Cands K,
-    uint32_t* tile_cnt) {
+__device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks, const LevelInfo& L,
+                                        const uint64_t* slot_off_prev, const uint64_t* res_prev,
+                                        const uint64_t* cand_id_prev, uint64_t* res, const Cands& K,
+                                        uint32_t* tile_cnt, uint32_t blk) {
     __shared__ uint32_t s_cnt;
     __shared__ uint32_t s_st[8];
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     {
-        uint64_t slot = (uint64_t)blockIdx.x * TILE + threadIdx.x;
+        uint64_t slot = (uint64_t)blk * TILE + threadIdx.x;
         bool valid = slot < L.nslots;
+        uint64_t out = kInvalid;
         bool cand = false, collapsed = false, h5 = false;
         uint64_t ch[8];
         uint32_t tm = 0;
@@ -904,7 +922,6 @@ __global__ __launch_bounds__(TILE) void k_level_build(
             uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
             uint32_t q = (uint32_t)(local & ((1ull << sh) - 1));
             const ChunkDev& C = chunks[c];
-            uint64_t out = kInvalid;
             if (C.status >= 0 && C.has_patches) {
                 const uint64_t* src = res_prev + slot_off_prev[c] + 8ull * q;
                 uint32_t types = 0, mask = 0;
@@ -982,18 +999,33 @@ __global__ __launch_bounds__(TILE) void k_level_build(
                     }
                 }
             }
-            res[slot] = out;
         }
-        emit_candidate<TILE>(K, &s_cnt, cand, ch, tm, slot);
+        if (valid) res[slot] = out;
+        emit_candidate<TILE>(K, &s_cnt, cand, ch, tm, slot, blk);
         tile_count(s_st, 0, cand);
         tile_count(s_st, 1, collapsed);
         tile_count(s_st, 2, h5);
     }
     __syncthreads();
-    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_cnt;
+    if (threadIdx.x == 0) tile_cnt[blk] = s_cnt;
     const int ids[3] = {ST_CAND, ST_COLLAPSE, ST_H5};
     if (threadIdx.x < 3 && s_st[threadIdx.x])
         atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
+}
+
+// Height h >= 2, plus (blocks nbuild..) the commit of height h-1: the commit writes
+// only nodes created at h-1 and the table, which the build of h does not read, and
+// the probe of h follows this launch.
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_level_build(
+    Pool<T> P, const ChunkDev* chunks, LevelInfo L, const uint64_t* slot_off_prev,
+    const uint64_t* res_prev, const uint64_t* cand_id_prev, uint64_t* res, Cands K,
+    uint32_t* tile_cnt, uint32_t nbuild, Cands Kc, const uint32_t* tile_cnt_c, uint64_t ntiles_c) {
+    if (blockIdx.x < nbuild)
+        level_build_body<T, TILE>(P, chunks, L, slot_off_prev, res_prev, cand_id_prev, res, K, tile_cnt,
+                                  blockIdx.x);
+    else
+        commit_body<T, TILE>(P, Kc, tile_cnt_c, ntiles_c, blockIdx.x - nbuild, gridDim.x - nbuild);
 }
 
 // Tiles of every level of one apply, flattened (levels 1..dmax).
@@ -1008,8 +1040,8 @@ struct LevelTiles {
 // commits and logged corrections added, so the interner is left as before.
 // Allocations themselves are undone by k_end_alloc.
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) void k_rollback(Pool<T> P, const ChunkDev* chunks, LevelTiles LT,
-                                                    const Drift* drift) {
+__device__ inline void rollback_body(const Pool<T>& P, const ChunkDev* chunks, const LevelTiles& LT,
+                                     const Drift* drift) {
     const bool abort = !P.dry && P.ctl->abort_ != 0;
     if (!P.dry && !abort) return;
     uint32_t owned = 0;
@@ -1053,16 +1085,16 @@ __global__ __launch_bounds__(TILE) void k_rollback(Pool<T> P, const ChunkDev* ch
 // voxtree.rs:303-328: adopt the new root, take the tree's reference on it, and
 // queue the old root for dec_ref_recursive.
 template <typename T>
-__global__ void k_finalize(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_t* slot_off_all,
-                           uint32_t n_stride, uint64_t* const* res_levels,
-                           uint64_t* const* cand_id_levels, uint64_t* roots, int32_t* status,
-                           uint32_t* gc_front) {
+__device__ inline void finalize_body(const Pool<T>& P, const ChunkDev* chunks, uint32_t n,
+                                     const uint64_t* slot_off_all, uint32_t n_stride,
+                                     uint64_t* const* res_levels, uint64_t* const* cand_id_levels,
+                                     uint64_t* roots, int32_t* status, uint32_t* gc_front) {
     const uint32_t ab = P.ctl->abort_;
     VX_LOOP(c, n) {
         bool push = false;
         uint64_t old = kEmpty;
         if (c < n) {
-            ChunkDev& C = chunks[c];
+            const ChunkDev& C = chunks[c];
             int32_t st = C.status;
             if (C.skip) {
                 st = 0;
@@ -1075,7 +1107,7 @@ __global__ void k_finalize(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64
                 } else {
                     uint32_t D = C.depth;
                     uint64_t r = res_levels[D][slot_off_all[(uint64_t)D * n_stride + c]];
-                    nw = (r != kInvalid && res_is_pending(r)) ? cand_id_levels[D][(uint32_t)r] : r;
+                    nw = r != kInvalid ? resolve(r, cand_id_levels[D]) : r;
                 }
                 old = C.root_in;
                 if (nw == kInvalid) {
@@ -1091,18 +1123,31 @@ __global__ void k_finalize(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64
                     st = 1;
                 }
             }
-            C.status = st;
-            status[C.orig] = st;
+            status[C.orig] = st;  // the record keeps its status (the rollback may read it)
         }
         uint32_t k = wave_append(&P.ctl->gc_count[0], push);
         if (push) gc_front[k] = bid_index(old);
     }
 }
 
+// After the last probe: the rollback (dry runs: restore the table; aborted applies:
+// take references back) and the root swap, in one launch.  The swap does not touch
+// the table or reference counts of an aborted apply, and leaves the per-tree records
+// (read by the rollback) unchanged.
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_finish(Pool<T> P, const ChunkDev* chunks, LevelTiles LT,
+                                                  const Drift* drift, uint32_t n, const uint64_t* slot_off_all,
+                                                  uint32_t n_stride, uint64_t* const* res_levels,
+                                                  uint64_t* const* cand_id_levels, uint64_t* roots,
+                                                  int32_t* status, uint32_t* gc_front) {
+    rollback_body<T, TILE>(P, chunks, LT, drift);
+    finalize_body<T>(P, chunks, n, slot_off_all, n_stride, res_levels, cand_id_levels, roots, status, gc_front);
+}
+
 // Leaves created in this sequence that nothing references (their trees hit an
 // error) are released again so the interner is left as before for those trees.
 template <typename T>
-__global__ void k_leaf_cleanup(Pool<T> P) {
+__device__ inline void leaf_cleanup_body(const Pool<T>& P) {
     uint32_t n = P.ctl->n_new_leaves;
     VX_LOOP(i, n) {
         bool freed = false;
@@ -1132,10 +1177,13 @@ __global__ void k_leaf_cleanup(Pool<T> P) {
 // whose count already reached zero.
 template <typename T>
 __global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uint32_t* next,
-                           bool decrement_self, uint64_t front_cap) {
+                           bool decrement_self, uint64_t front_cap, bool leaf_cleanup) {
     __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
     agg_init(akey, acnt);
     __syncthreads();
+    // new leaves nothing references (their trees failed) are released in the first
+    // round; no node that this GC visits references them
+    if (leaf_cleanup) leaf_cleanup_body(P);
     uint64_t n = P.ctl->gc_count[round];
     if (n > front_cap) n = front_cap;
     uint32_t* next_count = &P.ctl->gc_count[round + 1];
