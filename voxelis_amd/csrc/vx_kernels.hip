@@ -423,22 +423,32 @@ __device__ inline void tile_count(uint32_t* s, int id, bool pred) {
 // 3 = a needed leaf does not exist yet (marked; the brick is retried later).
 // Reference-count corrections of the reference's split-leaf quirks are queued in
 // *drift_idx / *drift_delta (applied once the tree's status is final).
+// The leaf id of a value, or kEmpty when no leaf holds it.  leaf_ids (LDS, u8 only)
+// caches it for every value; otherwise the direct-mapped table is read.
+template <typename T>
+__device__ inline uint64_t leaf_lookup(const Pool<T>& P, const uint64_t* leaf_ids, uint32_t v) {
+    if (leaf_ids) return leaf_ids[v];
+    uint32_t li = P.leaf_map[v];
+    return li ? make_leaf(li, P.gens[li]) : kEmpty;
+}
+
 template <typename T>
 __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m, const uint32_t v[8],
                                  uint64_t base, uint64_t* out, uint64_t ch[8], uint32_t* tm,
-                                 uint32_t* drift_idx, int32_t* drift_delta, int* flags) {
+                                 uint32_t* drift_idx, int32_t* drift_delta, int* flags,
+                                 const uint64_t* leaf_ids) {
     *drift_idx = kNoIdx;
     bool all_same = m == 0xFFu;
 #pragma unroll
     for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
     if (all_same) {
-        uint32_t li = P.leaf_map[v[0]];
-        if (li == 0) {
+        uint64_t lid = leaf_lookup(P, leaf_ids, v[0]);
+        if (lid == kEmpty) {
             P.leaf_need[v[0]] = 1;
             if (!P.dry) return 3;
             *out = make_leaf(P.capacity + v[0], 0);  // dry run: a placeholder for the new leaf
         } else {
-            *out = make_leaf(li, P.gens[li]);
+            *out = lid;
         }
         *flags |= 1;  // collapsed
         return 1;
@@ -477,12 +487,14 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
     uint32_t modified = 0;
     for (int i = 0; i < 8; i++) {
         if (!(m & (1u << i))) continue;
+        // a live leaf is the one leaf of its value: the child holds v[i] exactly when it
+        // is that leaf (the children of a brick are leaves)
+        uint64_t lid = leaf_lookup(P, leaf_ids, v[i]);
         if (ch[i] != kEmpty) {
-            uint32_t cur = split ? split_value : (uint32_t)P.values[bid_index(ch[i])];
-            if (cur == v[i]) continue;
+            bool same = split ? split_value == v[i] : (lid != kEmpty && ch[i] == lid);
+            if (same) continue;
         }
-        uint32_t li = P.leaf_map[v[i]];
-        if (li == 0) {
+        if (lid == kEmpty) {
             P.leaf_need[v[i]] = 1;
             if (!P.dry) {
                 missing = true;
@@ -490,7 +502,7 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
             }
             ch[i] = make_leaf(P.capacity + v[i], 0);
         } else {
-            ch[i] = make_leaf(li, P.gens[li]);
+            ch[i] = lid;
         }
         types |= 1u << i;
         mask |= 1u << i;
@@ -528,6 +540,14 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                                                        uint32_t* deferred, Drift* drift) {
     __shared__ uint32_t s_cnt;
     __shared__ uint32_t s_st[8];
+    // u8: the leaf id of every value, from LDS (the value map does not change here)
+    __shared__ uint64_t s_leaf[sizeof(T) == 1 ? 256 : 1];
+    if (sizeof(T) == 1)
+        for (uint32_t v = threadIdx.x; v < 256; v += blockDim.x) {
+            uint32_t li = P.leaf_map[v];
+            s_leaf[v] = li ? make_leaf(li, P.gens[li]) : kEmpty;
+        }
+    const uint64_t* leaf_ids = sizeof(T) == 1 ? s_leaf : nullptr;
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
@@ -584,7 +604,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
             } else {
                 uint32_t didx;
                 int32_t dd;
-                int r = brick_work(P, C, m, v, base, &out, ch, &tm, &didx, &dd, &flags);
+                int r = brick_work(P, C, m, v, base, &out, ch, &tm, &didx, &dd, &flags, leaf_ids);
                 if (didx != kNoIdx && !P.dry) {
                     uint32_t pos = atomicAdd(&P.ctl->n_drift, 1u);
                     drift[pos] = Drift{c, didx, dd, (uint32_t)(flags & 8 ? ST_H3 : ST_H4)};
@@ -638,7 +658,7 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
                 uint32_t tm = 0, didx;
                 int32_t dd;
                 int flags = 0;
-                int r = brick_work(P, C, m, v, res1[slot], &out, ch, &tm, &didx, &dd, &flags);
+                int r = brick_work(P, C, m, v, res1[slot], &out, ch, &tm, &didx, &dd, &flags, (const uint64_t*)nullptr);
                 if (didx != kNoIdx) {
                     ref_delta_logged(P, c, didx, dd, flags & 8 ? ST_H3 : ST_H4);
                     atomicAdd((unsigned long long*)stat(P.ctl, flags & 8 ? ST_H3 : ST_H4), 1ull);
