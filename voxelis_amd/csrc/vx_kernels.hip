@@ -57,6 +57,12 @@ __device__ inline uint64_t shfl64(uint64_t v, int src) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+__device__ inline uint64_t readlane64(uint64_t v, int lane) {
+    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, lane);
+    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ inline uint32_t wave_sum(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -738,6 +744,13 @@ __device__ inline int match_slot(const Pool<T>& P, uint32_t level, uint64_t e, u
     return 0;
 }
 
+// The scan reads the table with plain loads.  A slot changes at most once during
+// one probe launch (empty or tombstone -> this launch's pending key; commits and GC
+// run in other launches, and a launch starts with coherent caches), so a plain load
+// returns either the slot's value at launch start or its final value.  Every claim of
+// a slot that looked free is an atomicCAS, which returns the slot's true value; after
+// a failed claim the scan continues behind that slot, whose value is now known.
+// (A tombstone is claimed on the same terms: it cannot turn back into a tombstone.)
 template <typename T>
 __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, const uint64_t key[9],
                                   uint64_t h, uint64_t* result, uint64_t* tslot, uint32_t* probes) {
@@ -746,7 +759,7 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
     uint64_t s = h & P.tmask;
     uint64_t first_free = ~0ull;
     for (uint64_t guard = 0; guard <= 4 * (P.tmask + 1); guard++) {
-        uint64_t e = __hip_atomic_load(&P.tab[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t e = P.tab[s];
         (*probes)++;
         if (e == kSlotEmpty) {
             uint64_t target = first_free != ~0ull ? first_free : s;
@@ -772,7 +785,7 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
             }
             int m = match_slot(P, level, seen, tag, key, result);
             if (m >= 0) return m;
-            s = h & P.tmask;  // someone else inserted first: rescan the chain
+            s = (target + 1) & P.tmask;  // another key took the slot: go on behind it
             first_free = ~0ull;
             continue;
         }
@@ -821,14 +834,10 @@ __global__ __launch_bounds__(TILE) void k_probe(Pool<T> P, Cands K, const uint32
     for (uint32_t r = 0; r < P.dedup_rounds; r++) {
         uint64_t act = __ballot(open);
         if (!act) break;
-        int L = __ffsll((unsigned long long)act) - 1;
-        uint64_t lh = shfl64(h, L);
-        bool same = open && h == lh;
+        int L = __ffsll((unsigned long long)act) - 1;  // wave-uniform: a scalar broadcast
+        bool same = open && h == readlane64(h, L);
 #pragma unroll
-        for (int w = 0; w < 9; w++) {
-            uint64_t lw = shfl64(key[w], L);
-            same = same && key[w] == lw;
-        }
+        for (int w = 0; w < 9; w++) same = same && key[w] == readlane64(key[w], L);
         if (same) {
             leader = L;
             open = false;
