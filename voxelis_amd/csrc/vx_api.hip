@@ -71,7 +71,7 @@ struct Grow {
 };
 
 struct LevelBufs {
-    Grow res, key, hash, id, tslot, slot, tiles;
+    Grow res, key, hash, id, tslot, slot, tiles, old;
 };
 
 // Kernel ids of the per-kernel timer (vx_interner_profile_*).
@@ -388,14 +388,17 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     std::memset(K, 0, sizeof(K));
     uint32_t* tiles_h[kLevels + 1] = {nullptr};
     uint64_t ntiles[kLevels + 1] = {0};
+    OldNodes O;
+    std::memset(&O, 0, sizeof(O));
     for (uint32_t hh = 1; hh <= dmax; hh++) {
         ntiles[hh] = std::max<uint64_t>((L[hh].nslots + TILE - 1) / TILE, 1);
         uint64_t ns = ntiles[hh] * TILE;
         LevelBufs& B = h->lv[hh];
         if (!B.res.ensure(ns * 8) || !B.key.ensure(ns * 72) || !B.hash.ensure(ns * 8) ||
             !B.id.ensure(ns * 8) || !B.tslot.ensure(ns * 8) || !B.slot.ensure(ns * 8) ||
-            !B.tiles.ensure(ntiles[hh] * 4))
+            !B.tiles.ensure(ntiles[hh] * 4) || (hh >= 2 && !B.old.ensure(ns * 8)))
             return VX_E_HIP;
+        if (hh >= 2) O.at[hh] = (uint64_t*)B.old.ptr;
         tiles_h[hh] = (uint32_t*)B.tiles.ptr;
         K[hh].key = (uint64_t*)B.key.ptr;
         K[hh].hash = (uint64_t*)B.hash.ptr;
@@ -446,12 +449,15 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     P.undo_cap = (uint32_t)undo_cap;
     // ---- launch sequence
     auto launch_all = [&]() -> int {
-    VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, grid_for(n), kBlock, P, dchunks, (uint32_t)n, roots,
-              pre_status);
+    // chunk set-up, plus the old-tree descent over the level-2 slots in extra blocks
+    unsigned np = grid_for(n);
+    unsigned nd = L[2].nslots ? grid_for(L[2].nslots) : 0u;
+    VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, np + nd, kBlock, P, dchunks, (uint32_t)n, roots, pre_status, np,
+              L[2], doff, O);
     unsigned t1 = (unsigned)ntiles[1];
     if (dmax >= 2)
         VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE>, t1, TILE, P, dchunks, L[1], res_ptrs_h[1], K[1],
-                  tiles_h[1], ddef, ddrift);
+                  tiles_h[1], ddef, ddrift, O.at[2], doff + (size_t)2 * n);
     VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc<T>, 1, 1024, P, dchunks, (uint32_t)n, nvalues);
     if (dmax >= 2) {
         VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_retry<T VX_COMMA TILE>, 256, kBlock, P, dchunks, L[1], res_ptrs_h[1], K[1],
@@ -464,7 +470,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
             unsigned tc = opt.dry ? 0u : std::min<unsigned>((unsigned)ntiles[hh - 1], h->tune.commit_grid);
             VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T VX_COMMA TILE>, th + tc, TILE, P, dchunks, L[hh],
                       doff + (size_t)(hh - 1) * n, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh],
-                      K[hh], tiles_h[hh], th, K[hh - 1], tiles_h[hh - 1], (uint64_t)ntiles[hh - 1]);
+                      K[hh], tiles_h[hh], th, K[hh - 1], tiles_h[hh - 1], (uint64_t)ntiles[hh - 1], O.at[hh]);
             VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T VX_COMMA TILE>, th, TILE, P, K[hh], tiles_h[hh], res_ptrs_h[hh],
                       dchunks, L[hh]);
         }
@@ -788,6 +794,7 @@ int vx_interner_destroy(vx_interner* h) {
         b.tslot.release();
         b.slot.release();
         b.tiles.release();
+        b.old.release();
     }
     for (hipEvent_t e : h->prof.pool) (void)hipEventDestroy(e);
     for (int i = 0; i < vx_interner::kRing; i++) {

@@ -137,6 +137,13 @@ struct LevelInfo {
     uint64_t nslots;
 };
 
+// Per height h >= 2: the previous tree's node at every level-h slot (a branch,
+// the leaf that covers the position from above, or empty), written by one
+// descent pass at the start of an apply for trees without a fill.
+struct OldNodes {
+    uint64_t* at[kLevels + 1];
+};
+
 // Candidate list of one level (compacted, order of discovery).
 struct Cands {
     uint64_t* key;    // 9 words per candidate: children[8], tm

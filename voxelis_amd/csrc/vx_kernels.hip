@@ -236,10 +236,24 @@ __device__ inline void ref_delta_logged(const Pool<T>& P, uint32_t chunk, uint32
     else atomicOr(&P.ctl->err, kErrPoison);
 }
 
-__device__ inline uint32_t group_of(const LevelInfo& L, uint64_t slot) {
-    uint32_t g = 0;
-    while (g + 1 < L.ngroups && slot >= L.slot_base[g + 1]) g++;
-    return g;
+// The group of a level slot (the last group whose first slot is <= slot): its
+// first slot, tree depth and first chunk.  Every read of the kernel argument has a
+// uniform index, so the selection costs scalar loads and per-lane selects only.
+struct GroupPos {
+    uint64_t base;
+    uint32_t depth;
+    uint32_t chunk_base;
+};
+
+__device__ inline GroupPos group_pos(const LevelInfo& L, uint64_t slot) {
+    GroupPos r{L.slot_base[0], L.depth[0], L.chunk_base[0]};
+    for (uint32_t k = 1; k < L.ngroups; k++) {
+        bool in = slot >= L.slot_base[k];
+        r.base = in ? L.slot_base[k] : r.base;
+        r.depth = in ? L.depth[k] : r.depth;
+        r.chunk_base = in ? L.chunk_base[k] : r.chunk_base;
+    }
+    return r;
 }
 
 // ------------------------------------------------------------------ launch setup
@@ -313,14 +327,55 @@ __global__ void k_end_alloc(Pool<T> P, bool rollback, uint32_t* counts) {
     }
 }
 
-// Read roots, mark the fill leaf as needed (Phase 0, voxtree.rs:742-754).
+// The previous tree's nodes on the path to one height-2 position (level-2 slot
+// `slot`), for the equivalent-node lookups of the brick and level passes
+// (voxtree.rs:930-952): the node at height h is recorded by the lane that owns
+// the first height-2 position under it.  Below a leaf every height records that
+// leaf (it covers the position); below an empty child, empty.  Trees with a fill
+// start from the fill leaf instead and are skipped.
+template <typename T>
+__device__ inline void descend_body(const Pool<T>& P, const ChunkDev* chunks, const LevelInfo& L2,
+                                    const uint64_t* doff, uint32_t n, const OldNodes& O,
+                                    const uint64_t* roots, uint64_t slot) {
+    const GroupPos G = group_pos(L2, slot);
+    uint64_t local = slot - G.base;
+    uint32_t D = G.depth;
+    uint32_t sh = 3 * (D - 2);
+    uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
+    uint32_t q = (uint32_t)(local & ((1ull << sh) - 1));
+    const ChunkDev& C = chunks[c];
+    if (C.has_fill) return;
+    uint64_t cur = roots[C.orig];
+    for (int hh = (int)D; hh >= 2; hh--) {
+        if (hh < (int)D && cur != kEmpty && !bid_is_leaf(cur)) {
+            if (bid_index(cur) >= P.capacity) {
+                atomicOr(&P.ctl->err, kErrPoison);
+                cur = kEmpty;
+            } else {
+                cur = P.children[8ull * bid_index(cur) + ((q >> (3 * (hh - 2))) & 7u)];
+            }
+        }
+        uint32_t sub = 3 * (uint32_t)(hh - 2);
+        if ((q & ((1u << sub) - 1u)) == 0) O.at[hh][doff[(size_t)hh * n + c] + (q >> sub)] = cur;
+    }
+}
+
+// Read roots, mark the fill leaf as needed (Phase 0, voxtree.rs:742-754).  Blocks
+// nprep.. run the old-tree descent over the level-2 slots instead.
 template <typename T>
 __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_t* roots,
-                          const int32_t* pre_status) {
+                          const int32_t* pre_status, uint32_t nprep, LevelInfo L2, const uint64_t* doff,
+                          OldNodes O) {
+    if (blockIdx.x >= nprep) {
+        uint64_t stride = (uint64_t)(gridDim.x - nprep) * blockDim.x;
+        for (uint64_t s = (uint64_t)(blockIdx.x - nprep) * blockDim.x + threadIdx.x; s < L2.nslots; s += stride)
+            descend_body(P, chunks, L2, doff, n, O, roots, s);
+        return;
+    }
     // the sequence's control-block reset rides in block 0 (nothing else here reads it)
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl_begin(P.ctl);
-    VX_LOOP(c, n) {
-        if (c < n) {
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (uint64_t)nprep * blockDim.x) {
+        {
             ChunkDev& C = chunks[c];
             C.root_in = roots[C.orig];
             C.init = C.root_in;
@@ -543,66 +598,80 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
 template <typename T, int TILE>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
-                                                       uint32_t* deferred, Drift* drift) {
+                                                       uint32_t* deferred, Drift* drift,
+                                                       const uint64_t* old2, const uint64_t* off2) {
     __shared__ uint32_t s_cnt;
     __shared__ uint32_t s_st[8];
     // u8: the leaf id of every value, from LDS (the value map does not change here)
     __shared__ uint64_t s_leaf[sizeof(T) == 1 ? 256 : 1];
+    uint32_t lm[sizeof(T) == 1 ? (256 + TILE - 1) / TILE : 1];
     if (sizeof(T) == 1)
-        for (uint32_t v = threadIdx.x; v < 256; v += blockDim.x) {
-            uint32_t li = P.leaf_map[v];
-            s_leaf[v] = li ? make_leaf(li, P.gens[li]) : kEmpty;
-        }
+        for (uint32_t i = 0, v = threadIdx.x; v < 256; i++, v += TILE) lm[i] = P.leaf_map[v];
+    // This lane's brick: chunk fields, mask, values and the previous tree's node at
+    // height 2 above it, in two rounds of independent loads without branches (lanes
+    // past the end load the last slot), issued before the block's barrier so their
+    // latency overlaps the table's.
+    uint64_t slot = (uint64_t)blockIdx.x * TILE + threadIdx.x;
+    const bool live = slot < L.nslots;
+    const uint64_t sl = live ? slot : L.nslots - 1;
+    const GroupPos G = group_pos(L, sl);
+    const uint64_t local = sl - G.base;
+    const uint32_t D = G.depth;
+    const uint32_t sh = 3 * (D - 1);
+    const uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
+    const uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
+    const ChunkDev* Cp = chunks + c;
+    const uint8_t* cmask = Cp->masks;
+    const T* cval = (const T*)Cp->values;
+    const int32_t cstatus = Cp->status;
+    const bool cfill = Cp->has_fill != 0;
+    const uint64_t croot = Cp->root_in;
+    const uint64_t o2 = off2[c];  // a depth-1 tree has no height-2 slot: not used then
+    uint32_t m = cmask[2ull * p];
+    uint32_t v[8];
+    load8<T>(cval + 8ull * p, v);
+    const uint64_t up = old2[D >= 2 ? o2 + (p >> 3) : 0];  // unused for a tree with a fill
+    if (!live || cstatus < 0) m = 0;
+    if (sizeof(T) == 1)
+        for (uint32_t i = 0, vv = threadIdx.x; vv < 256; i++, vv += TILE)
+            s_leaf[vv] = lm[i] ? make_leaf(lm[i], P.gens[lm[i]]) : kEmpty;
     const uint64_t* leaf_ids = sizeof(T) == 1 ? s_leaf : nullptr;
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
-    uint64_t slot = (uint64_t)blockIdx.x * TILE + threadIdx.x;
     uint64_t out = kInvalid;
     bool cand = false, defer = false;
     int flags = 0;
     uint64_t ch[8];
     uint32_t tm = 0;
-    if (slot < L.nslots) {
-        uint32_t g = group_of(L, slot);
-        uint64_t local = slot - L.slot_base[g];
-        uint32_t D = L.depth[g];
-        uint32_t sh = 3 * (D - 1);
-        uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
-        uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
-        const ChunkDev& C = chunks[c];
-        uint32_t m = C.status >= 0 ? C.masks[2ull * p] : 0u;
+    {
+        const ChunkDev& C = *Cp;
         if (m != 0) {
-            uint32_t v[8];
-            load8<T>((const T*)C.values + 8ull * p, v);
             bool all_same = m == 0xFFu;
 #pragma unroll
             for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
             uint64_t base = kEmpty;
             bool h1 = false;
             if (!all_same) {
-                if (C.has_fill) {
+                if (cfill) {
                     base = kFillMark;
                 } else {
-                    uint64_t cur = C.root_in, Ls = kEmpty;
-                    for (uint32_t cd = 0; cd + 1 < D; cd++) {
-                        if (!bid_is_leaf(cur)) {
-                            uint32_t digit = (p >> (3 * (D - 2 - cd))) & 7u;
-                            if (bid_index(cur) >= P.capacity) {
-                                atomicOr(&P.ctl->err, kErrPoison);
-                                cur = kEmpty;
-                                break;
-                            }
-                            cur = P.children[8ull * bid_index(cur) + digit];
-                        } else {
-                            Ls = cur;
+                    // the equivalent node at brick level: a leaf above it splits;
+                    // a leaf at brick level is the reference's panic
+                    uint64_t cur = D >= 2 ? up : croot;
+                    if (D >= 2 && cur != kEmpty && !bid_is_leaf(cur)) {
+                        if (bid_index(cur) >= P.capacity) {
+                            atomicOr(&P.ctl->err, kErrPoison);
                             cur = kEmpty;
+                        } else {
+                            cur = P.children[8ull * bid_index(cur) + (p & 7u)];
                         }
-                        if (cur == kEmpty) break;
+                        // BlockId::types() on a leaf brick panics (block_id.rs:273-276)
+                        h1 = cur != kEmpty && bid_is_leaf(cur);
+                    } else if (D == 1) {
+                        h1 = cur != kEmpty && bid_is_leaf(cur);
                     }
-                    // BlockId::types() on a leaf brick panics (block_id.rs:273-276)
-                    h1 = cur != kEmpty && bid_is_leaf(cur);
-                    base = cur != kEmpty ? cur : Ls;
+                    base = cur;
                 }
             }
             if (h1) {
@@ -625,7 +694,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
             }
         }
     }
-    if (slot < L.nslots) res1[slot] = out;
+    if (live) res1[slot] = out;
     emit_candidate<TILE>(K, &s_cnt, cand, ch, tm, slot, blockIdx.x);
     uint32_t dpos = wave_append(&P.ctl->n_deferred, defer);
     if (defer) deferred[dpos] = (uint32_t)slot;
@@ -648,11 +717,11 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
     VX_LOOP(i, nd) {
         if (i < nd) {
             uint64_t slot = deferred[i];
-            uint32_t g = group_of(L, slot);
-            uint64_t local = slot - L.slot_base[g];
-            uint32_t D = L.depth[g];
+            const GroupPos G = group_pos(L, slot);
+            uint64_t local = slot - G.base;
+            uint32_t D = G.depth;
             uint32_t sh = 3 * (D - 1);
-            uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
+            uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
             uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
             const ChunkDev& C = chunks[c];
             uint64_t out = kInvalid;
@@ -813,8 +882,8 @@ __global__ __launch_bounds__(TILE) void k_probe(Pool<T> P, Cands K, const uint32
     uint32_t tree = kNoIdx;
     if (valid && (L.h == 1 || P.per_tree)) {
         uint64_t slot = K.slot[k];
-        uint32_t g = group_of(L, slot);
-        uint32_t c = L.chunk_base[g] + (uint32_t)((slot - L.slot_base[g]) >> (3 * (L.depth[g] - L.h)));
+        const GroupPos G = group_pos(L, slot);
+        uint32_t c = G.chunk_base + (uint32_t)((slot - G.base) >> (3 * (G.depth - L.h)));
         tree = chunks[c].orig;
         // a brick of a tree that hit the leaf-brick panic later in the same pass
         if (L.h == 1 && chunks[c].status < 0) {
@@ -930,7 +999,7 @@ template <typename T, int TILE>
 __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks, const LevelInfo& L,
                                         const uint64_t* slot_off_prev, const uint64_t* res_prev,
                                         const uint64_t* cand_id_prev, uint64_t* res, const Cands& K,
-                                        uint32_t* tile_cnt, uint32_t blk) {
+                                        uint32_t* tile_cnt, uint32_t blk, const uint64_t* old_h) {
     __shared__ uint32_t s_cnt;
     __shared__ uint32_t s_st[8];
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
@@ -944,13 +1013,14 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
         uint64_t ch[8];
         uint32_t tm = 0;
         if (valid) {
-            uint32_t g = group_of(L, slot);
-            uint64_t local = slot - L.slot_base[g];
-            uint32_t D = L.depth[g];
+            const GroupPos G = group_pos(L, slot);
+            uint64_t local = slot - G.base;
+            uint32_t D = G.depth;
             uint32_t sh = 3 * (D - L.h);
-            uint32_t c = L.chunk_base[g] + (uint32_t)(local >> sh);
+            uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
             uint32_t q = (uint32_t)(local & ((1ull << sh) - 1));
             const ChunkDev& C = chunks[c];
+            uint64_t recorded = old_h[slot];  // issued early; used for trees without a fill
             if (C.status >= 0 && C.has_patches) {
                 const uint64_t* src = res_prev + slot_off_prev[c] + 8ull * q;
                 uint32_t types = 0, mask = 0;
@@ -973,6 +1043,10 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
                     // equivalent node (voxtree.rs:930-952)
                     uint32_t ell = D - L.h;
                     uint64_t eq = C.init, lq = kEmpty;
+                    if (ell > 0 && !C.has_fill) {
+                        eq = recorded;  // the descent pass's node at this slot
+                        ell = 0;
+                    }
                     for (uint32_t cd = 0; cd < ell; cd++) {
                         if (bid_is_leaf(eq)) {
                             lq = eq;
@@ -1049,10 +1123,11 @@ template <typename T, int TILE>
 __global__ __launch_bounds__(TILE) void k_level_build(
     Pool<T> P, const ChunkDev* chunks, LevelInfo L, const uint64_t* slot_off_prev,
     const uint64_t* res_prev, const uint64_t* cand_id_prev, uint64_t* res, Cands K,
-    uint32_t* tile_cnt, uint32_t nbuild, Cands Kc, const uint32_t* tile_cnt_c, uint64_t ntiles_c) {
+    uint32_t* tile_cnt, uint32_t nbuild, Cands Kc, const uint32_t* tile_cnt_c, uint64_t ntiles_c,
+    const uint64_t* old_h) {
     if (blockIdx.x < nbuild)
         level_build_body<T, TILE>(P, chunks, L, slot_off_prev, res_prev, cand_id_prev, res, K, tile_cnt,
-                                  blockIdx.x);
+                                  blockIdx.x, old_h);
     else
         commit_body<T, TILE>(P, Kc, tile_cnt_c, ntiles_c, blockIdx.x - nbuild, gridDim.x - nbuild);
 }
