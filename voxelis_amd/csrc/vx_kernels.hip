@@ -1006,100 +1006,124 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     {
+        // Loads in three rounds without branches between them (lanes past the end
+        // load the last slot): the chunk's fields, its level h-1 offset and the
+        // recorded old node; the 8 results of height h-1 and the children of the
+        // equivalent node; the owners of results still pending.
         uint64_t slot = (uint64_t)blk * TILE + threadIdx.x;
-        bool valid = slot < L.nslots;
+        const bool valid = slot < L.nslots;
+        const uint64_t sl = valid ? slot : L.nslots - 1;
+        const GroupPos G = group_pos(L, sl);
+        const uint64_t local = sl - G.base;
+        const uint32_t D = G.depth;
+        const uint32_t sh = 3 * (D - L.h);
+        const uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
+        const uint32_t q = (uint32_t)(local & ((1ull << sh) - 1));
+        const ChunkDev* Cp = chunks + c;
+        const int32_t cstatus = Cp->status;
+        const bool cpatch = Cp->has_patches != 0;
+        const bool cfill = Cp->has_fill != 0;
+        const uint64_t cinit = Cp->init;
+        const uint64_t soff = slot_off_prev[c];
+        const uint64_t recorded = old_h[sl];
+        // equivalent node (voxtree.rs:930-952): the descent pass's node at this slot;
+        // a tree with a fill descends from its fill leaf
+        uint32_t ell = D - L.h;
+        uint64_t eq = cinit;
+        if (ell > 0 && !cfill) {
+            eq = recorded;
+            ell = 0;
+        }
+        uint64_t ch[8], ec[8];
+        {
+            const uint64_t* src = res_prev + soff + 8ull * q;
+            const bool eqb = eq != kEmpty && !bid_is_leaf(eq) && bid_index(eq) < P.capacity;
+            const uint64_t* ecp = P.children + 8ull * (eqb ? bid_index(eq) : 0u);
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                ch[i] = src[i];
+                ec[i] = ecp[i];
+            }
+        }
+        uint32_t touched = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            touched |= (ch[i] != kInvalid ? 1u : 0u) << i;
+            if (res_is_pending(ch[i])) ch[i] = cand_id_prev[(uint32_t)ch[i]];
+        }
         uint64_t out = kInvalid;
         bool cand = false, collapsed = false, h5 = false;
-        uint64_t ch[8];
         uint32_t tm = 0;
-        if (valid) {
-            const GroupPos G = group_pos(L, slot);
-            uint64_t local = slot - G.base;
-            uint32_t D = G.depth;
-            uint32_t sh = 3 * (D - L.h);
-            uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
-            uint32_t q = (uint32_t)(local & ((1ull << sh) - 1));
-            const ChunkDev& C = chunks[c];
-            uint64_t recorded = old_h[slot];  // issued early; used for trees without a fill
-            if (C.status >= 0 && C.has_patches) {
-                const uint64_t* src = res_prev + slot_off_prev[c] + 8ull * q;
-                uint32_t types = 0, mask = 0;
-                bool failed = false;
+        if (valid && cstatus >= 0 && cpatch) {
+            uint32_t types = 0, mask = 0;
+            bool failed = false;
 #pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    uint64_t r = src[i];
-                    if (r != kInvalid) {
-                        r = resolve(r, cand_id_prev);
-                        failed = failed || r == kInvalid;  // its owner could not allocate
-                        ch[i] = r;
-                        mask |= 1u << i;
-                        types |= (bid_is_leaf(r) ? 1u : 0u) << i;
-                    } else {
-                        ch[i] = kEmpty;
-                    }
+            for (int i = 0; i < 8; i++) {
+                uint64_t r = ch[i];
+                if (touched & (1u << i)) {
+                    failed = failed || r == kInvalid;  // its owner could not allocate
+                    mask |= 1u << i;
+                    types |= (bid_is_leaf(r) ? 1u : 0u) << i;
+                } else {
+                    ch[i] = kEmpty;
                 }
-                if (failed) mask = 0;  // the apply aborts; nothing is built above
-                if (mask != 0) {
-                    // equivalent node (voxtree.rs:930-952)
-                    uint32_t ell = D - L.h;
-                    uint64_t eq = C.init, lq = kEmpty;
-                    if (ell > 0 && !C.has_fill) {
-                        eq = recorded;  // the descent pass's node at this slot
-                        ell = 0;
-                    }
-                    for (uint32_t cd = 0; cd < ell; cd++) {
-                        if (bid_is_leaf(eq)) {
-                            lq = eq;
-                            eq = kEmpty;
-                            break;
-                        }
-                        uint32_t digit = (q >> (3 * (ell - 1 - cd))) & 7u;
-                        if (bid_index(eq) >= P.capacity) {
-                            atomicOr(&P.ctl->err, kErrPoison);
-                            eq = kEmpty;
-                            break;
-                        }
-                        eq = P.children[8ull * bid_index(eq) + digit];
-                        if (eq == kEmpty) break;
-                    }
+            }
+            if (failed) mask = 0;  // the apply aborts; nothing is built above
+            if (mask != 0) {
+                uint64_t lq = kEmpty;
+                bool walked = ell > 0;
+                for (uint32_t cd = 0; cd < ell; cd++) {
                     if (bid_is_leaf(eq)) {
                         lq = eq;
                         eq = kEmpty;
+                        break;
                     }
-                    uint32_t cloned = bid_mask(eq) & ~mask & 0xFFu;
-                    if (mask != 0xFFu) {
-                        if (cloned) {
-                            for (int i = 0; i < 8; i++) {
-                                if (!(cloned & (1u << i))) continue;
-                                ch[i] = P.children[8ull * bid_index(eq) + i];
-                                types |= (bid_is_leaf(ch[i]) ? 1u : 0u) << i;
-                                mask |= 1u << i;
-                            }
-                        } else if (lq != kEmpty) {
-                            uint32_t inv = ~mask & 0xFFu;
-                            types |= inv;
-                            mask |= inv;
-                            for (int i = 0; i < 8; i++)
-                                if (inv & (1u << i)) ch[i] = lq;
+                    uint32_t digit = (q >> (3 * (ell - 1 - cd))) & 7u;
+                    if (bid_index(eq) >= P.capacity) {
+                        atomicOr(&P.ctl->err, kErrPoison);
+                        eq = kEmpty;
+                        break;
+                    }
+                    eq = P.children[8ull * bid_index(eq) + digit];
+                    if (eq == kEmpty) break;
+                }
+                if (bid_is_leaf(eq)) {
+                    lq = eq;
+                    eq = kEmpty;
+                }
+                uint32_t cloned = bid_mask(eq) & ~mask & 0xFFu;
+                if (mask != 0xFFu) {
+                    if (cloned) {
+                        for (int i = 0; i < 8; i++) {
+                            if (!(cloned & (1u << i))) continue;
+                            ch[i] = walked ? P.children[8ull * bid_index(eq) + i] : ec[i];
+                            types |= (bid_is_leaf(ch[i]) ? 1u : 0u) << i;
+                            mask |= 1u << i;
                         }
+                    } else if (lq != kEmpty) {
+                        uint32_t inv = ~mask & 0xFFu;
+                        types |= inv;
+                        mask |= inv;
+                        for (int i = 0; i < 8; i++)
+                            if (inv & (1u << i)) ch[i] = lq;
                     }
-                    bool all_same = types == 0xFFu;
-                    for (int i = 1; i < 8; i++) all_same = all_same && ch[i] == ch[0];
-                    if (all_same) {
-                        out = ch[0];
-                        collapsed = true;
-                        if (cloned) {
-                            // dec_ref_by(children[0], min(pc(cloned),7)) instead of
-                            // 7 - pc(cloned) (voxtree.rs:1066-1072, H5)
-                            uint32_t pc = (uint32_t)__popc(cloned);
-                            if (!P.dry) ref_delta_logged(P, c, bid_index(ch[0]), (int32_t)(7u - 2u * pc), ST_H5);
-                            h5 = true;
-                        }
-                    } else {
-                        tm = (types << 8) | mask;
-                        cand = true;
-                        out = res_pending(0xFFFFFFFFu);
+                }
+                bool all_same = types == 0xFFu;
+                for (int i = 1; i < 8; i++) all_same = all_same && ch[i] == ch[0];
+                if (all_same) {
+                    out = ch[0];
+                    collapsed = true;
+                    if (cloned) {
+                        // dec_ref_by(children[0], min(pc(cloned),7)) instead of
+                        // 7 - pc(cloned) (voxtree.rs:1066-1072, H5)
+                        uint32_t pc = (uint32_t)__popc(cloned);
+                        if (!P.dry) ref_delta_logged(P, c, bid_index(ch[0]), (int32_t)(7u - 2u * pc), ST_H5);
+                        h5 = true;
                     }
+                } else {
+                    tm = (types << 8) | mask;
+                    cand = true;
+                    out = res_pending(0xFFFFFFFFu);
                 }
             }
         }
