@@ -203,8 +203,17 @@ __device__ inline uint64_t leaf_id_of(const Pool<T>& P, uint32_t v) {
 }
 
 template <typename T>
+__device__ inline uint32_t alloc_at(const Pool<T>& P, uint32_t k, bool* ok);
+
+template <typename T>
 __device__ inline uint32_t alloc_node(const Pool<T>& P, bool* ok) {
-    uint32_t k = atomicAdd(&P.ctl->alloc_count, 1u);
+    return alloc_at(P, atomicAdd(&P.ctl->alloc_count, 1u), ok);
+}
+
+// The index of the launch sequence's k-th allocation: the free stack first, then
+// fresh indices.
+template <typename T>
+__device__ inline uint32_t alloc_at(const Pool<T>& P, uint32_t k, bool* ok) {
     if (k >= P.alloc_limit) {
         atomicOr(&P.ctl->abort_, 2u);  // beyond the caller's reservation
         *ok = false;
@@ -808,8 +817,9 @@ __device__ inline int match_slot(const Pool<T>& P, uint32_t level, uint64_t e, u
         *result = lv == level ? res_pending(slot_payload(e)) : P.levels[lv].id[slot_payload(e)];
         return lv == level ? 1 : 0;
     }
+    const uint32_t gen = P.gens[slot_payload(e)];  // loaded with the node's key
     if (!key_matches_node(P, slot_payload(e), key)) return -1;
-    *result = make_branch(slot_payload(e), P.gens[slot_payload(e)], tm);
+    *result = make_branch(slot_payload(e), gen, tm);
     return 0;
 }
 
@@ -822,13 +832,14 @@ __device__ inline int match_slot(const Pool<T>& P, uint32_t level, uint64_t e, u
 // (A tombstone is claimed on the same terms: it cannot turn back into a tombstone.)
 template <typename T>
 __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, const uint64_t key[9],
-                                  uint64_t h, uint64_t* result, uint64_t* tslot, uint32_t* probes) {
+                                  uint64_t h, uint64_t e0, uint64_t* result, uint64_t* tslot,
+                                  uint32_t* probes) {
     uint32_t tag = slot_tag(h);
-    uint32_t tm = (uint32_t)key[8];
     uint64_t s = h & P.tmask;
     uint64_t first_free = ~0ull;
     for (uint64_t guard = 0; guard <= 4 * (P.tmask + 1); guard++) {
-        uint64_t e = P.tab[s];
+        // e0: the first slot as loaded by the caller, if it did
+        uint64_t e = guard == 0 && e0 != kSlotNone ? e0 : P.tab[s];
         (*probes)++;
         if (e == kSlotEmpty) {
             uint64_t target = first_free != ~0ull ? first_free : s;
@@ -838,17 +849,9 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
                 &P.tab[target], &seen, slot_make(tag, true, level, (uint32_t)k), __ATOMIC_RELAXED,
                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (won) {
-                if (P.dry) {
-                    // a placeholder id; the slot's previous value is restored after the run
-                    uint32_t serial = atomicAdd(&P.ctl->dry_owners, 1u);
-                    *result = make_branch(P.capacity + 65536u + serial, 0, tm);
-                    *tslot = target | (expected == kSlotTomb ? kTslotTomb : 0ull);
-                    return 2;
-                }
-                bool ok;
-                uint32_t idx = alloc_node(P, &ok);
-                *result = ok ? make_branch(idx, P.gens[idx], tm) : kInvalid;
-                // kept when the allocation failed too: the abort path restores the slot
+                // the caller allocates the owner's node; the slot is kept when the
+                // allocation fails too (the abort path restores it), and a dry run
+                // restores it after the run
                 *tslot = target | (expected == kSlotTomb ? kTslotTomb : 0ull);
                 return 2;
             }
@@ -871,36 +874,15 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
     return -1;
 }
 
-template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
-                                                 uint64_t* res, const ChunkDev* chunks, LevelInfo L) {
-    uint32_t n = tile_cnt[blockIdx.x];
-    if ((threadIdx.x & ~63u) >= n) return;  // whole wavefront without candidates
+// Wavefront-level merge of equal keys over `rounds` rounds: each round takes the
+// lowest open lane's key as the reference and closes every open lane holding it.
+// Returns the lane's leader (the lowest lane holding its key, or itself when its
+// key was not reached).
+__device__ inline int wave_merge_keys(uint32_t rounds, bool valid, const uint64_t key[9], uint64_t h) {
     int lane = (int)__lane_id();
-    bool valid = threadIdx.x < n;
-    uint64_t k = (uint64_t)blockIdx.x * TILE + threadIdx.x;
-    uint32_t tree = kNoIdx;
-    if (valid && (L.h == 1 || P.per_tree)) {
-        uint64_t slot = K.slot[k];
-        const GroupPos G = group_pos(L, slot);
-        uint32_t c = G.chunk_base + (uint32_t)((slot - G.base) >> (3 * (G.depth - L.h)));
-        tree = chunks[c].orig;
-        // a brick of a tree that hit the leaf-brick panic later in the same pass
-        if (L.h == 1 && chunks[c].status < 0) {
-            res[slot] = kInvalid;
-            K.id[k] = kInvalid;
-            K.tslot[k] = ~0ull;
-            valid = false;
-        }
-    }
-    uint64_t key[9], h = 0;
-#pragma unroll
-    for (int i = 0; i < 9; i++) key[i] = valid ? K.key[9ull * k + i] : 0ull;
-    if (valid) h = K.hash[k];
-    // wavefront-level merge of equal keys: leader = lowest lane holding the key
     int leader = lane;
     bool open = valid;
-    for (uint32_t r = 0; r < P.dedup_rounds; r++) {
+    for (uint32_t r = 0; r < rounds; r++) {
         uint64_t act = __ballot(open);
         if (!act) break;
         int L = __ffsll((unsigned long long)act) - 1;  // wave-uniform: a scalar broadcast
@@ -913,10 +895,66 @@ __global__ __launch_bounds__(TILE) void k_probe(Pool<T> P, Cands K, const uint32
         }
         if (lane == L) open = false;
     }
+    return leader;
+}
+
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
+                                                 uint64_t* res, const ChunkDev* chunks, LevelInfo L) {
+    uint32_t n = tile_cnt[blockIdx.x];
+    if ((threadIdx.x & ~63u) >= n) return;  // whole wavefront without candidates
+    int lane = (int)__lane_id();
+    bool valid = threadIdx.x < n;
+    uint64_t k = (uint64_t)blockIdx.x * TILE + threadIdx.x;
+    // The key, its hash and its level slot (lanes past the tile's count load the
+    // tile's first candidate, which exists).
+    const uint64_t kl = valid ? k : (uint64_t)blockIdx.x * TILE;
+    uint64_t key[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) key[i] = K.key[9ull * kl + i];
+    const uint64_t h = K.hash[kl];
+    const uint64_t slot = K.slot[kl];
+    int leader = wave_merge_keys(P.dedup_rounds, valid, key, h);
+    // the table slot the key hashes to, read by leaders only (duplicates of a hot key
+    // would all read one line), together with the tree's status
+    uint64_t e0 = valid && leader == lane ? P.tab[h & P.tmask] : kSlotNone;
+    uint32_t tree = kNoIdx;
+    if (L.h == 1 || P.per_tree) {
+        const GroupPos G = group_pos(L, slot);
+        uint32_t c = G.chunk_base + (uint32_t)((slot - G.base) >> (3 * (G.depth - L.h)));
+        const ChunkDev* Cp = chunks + c;
+        const int32_t cstatus = Cp->status;
+        tree = Cp->orig;
+        // a brick of a tree that hit the leaf-brick panic later in the same pass
+        bool drop = valid && L.h == 1 && cstatus < 0;
+        if (drop) {
+            res[slot] = kInvalid;
+            K.id[k] = kInvalid;
+            K.tslot[k] = ~0ull;
+            valid = false;
+        }
+        if (__ballot(drop)) {  // rare: merge again without the dropped lanes
+            leader = wave_merge_keys(P.dedup_rounds, valid, key, h);
+            e0 = kSlotNone;
+        }
+    }
     uint64_t result = kInvalid, tslot = ~0ull;
     int kind = -1;
     uint32_t probes = 0;
-    if (valid && leader == lane) kind = probe_table(P, L.h, k, key, h, &result, &tslot, &probes);
+    if (valid && leader == lane) kind = probe_table(P, L.h, k, key, h, e0, &result, &tslot, &probes);
+    // the owners' nodes: one counter update per wavefront
+    if (P.dry) {
+        // a placeholder id (the table slot's previous value is restored after the run)
+        uint32_t serial = wave_append(&P.ctl->dry_owners, kind == 2);
+        if (kind == 2) result = make_branch(P.capacity + 65536u + serial, 0, (uint32_t)key[8]);
+    } else {
+        uint32_t ka = wave_append(&P.ctl->alloc_count, kind == 2);
+        if (kind == 2) {
+            bool ok;
+            uint32_t idx = alloc_at(P, ka, &ok);
+            result = ok ? make_branch(idx, P.gens[idx], (uint32_t)key[8]) : kInvalid;
+        }
+    }
     if (kind == 2 && P.per_tree && tree != kNoIdx) atomicAdd(&P.per_tree[tree], 1u);
     uint64_t lres = shfl64(result, leader);
     int lkind = __shfl(kind, leader);
@@ -927,7 +965,7 @@ __global__ __launch_bounds__(TILE) void k_probe(Pool<T> P, Cands K, const uint32
     if (valid) {
         K.id[k] = result;
         K.tslot[k] = tslot;
-        res[K.slot[k]] = result;
+        res[slot] = result;
     }
     wave_count(stat(P.ctl, ST_HIT), valid && kind == 0);
     wave_count(stat(P.ctl, ST_DUP), valid && kind == 1);
