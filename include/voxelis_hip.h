@@ -81,13 +81,19 @@ int vx_interner_set_stream(vx_interner* h, void* hip_stream);
  * voxelis_amd/autotune.py).  tile: lanes per block of the build / probe / commit
  * passes (128, 256 or 512; a block owns one tile of level slots and its candidate
  * region); dedup_rounds: wavefront key-merge rounds before the table probe (0-16);
- * commit_grid / gc_grid: blocks of the commit pass and of each GC round.
- * Defaults {256, 8, 1024, 1024}.  Synchronizes the interner's stream. */
+ * commit_grid / gc_grid: blocks of the commit pass and of each GC round;
+ * probe_blocks: a probe launch over more tiles than this packs the candidates of
+ * up to 16 tiles into one block, keeping at least this many blocks (0: one tile
+ * per block), separately for the brick level (probe_blocks_bricks) and the levels
+ * above.  Defaults {256, 8, 1024, 1024, 32768, 512}.  Synchronizes the interner's
+ * stream. */
 typedef struct {
     uint32_t tile;
     uint32_t dedup_rounds;
     uint32_t commit_grid;
     uint32_t gc_grid;
+    uint32_t probe_blocks_bricks;
+    uint32_t probe_blocks;
 } vx_tuning;
 int vx_interner_set_tuning(vx_interner* h, const vx_tuning* tuning);
 int vx_interner_get_tuning(const vx_interner* h, vx_tuning* tuning);
@@ -114,6 +120,8 @@ enum {
     VX_STAT_H5_COLLAPSE_CLONED,
     VX_STAT_CANDIDATES,        /* branch keys probed */
     VX_STAT_PROBES,            /* table slots inspected */
+    VX_STAT_CLAIMS,            /* atomic claims of a free table slot */
+    VX_STAT_CLAIMS_LOST,       /* claims that found the slot taken meanwhile */
     VX_STAT_COUNT
 };
 

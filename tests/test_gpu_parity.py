@@ -361,10 +361,12 @@ def test_graph_replay_matches_direct_launches(vx):
     assert runs[0] == runs[1]
 
 
-@pytest.mark.parametrize("tile,dedup,grid_blocks", [(128, 0, 64), (512, 2, 4096)])
-def test_launch_shapes_match_oracle(vx, tile, dedup, grid_blocks):
-    """Every tile size / probe schedule / grid size the autotuner may pick gives the
-    oracle's trees (two applies, so the second replaces and reclaims the first)."""
+@pytest.mark.parametrize("tile,dedup,grid_blocks,pb", [(128, 0, 64, 0), (512, 2, 4096, 1), (256, 8, 1024, 3)])
+def test_launch_shapes_match_oracle(vx, tile, dedup, grid_blocks, pb):
+    """Every tile size / probe schedule / grid size / probe tile grouping the
+    autotuner may pick gives the oracle's trees (two applies, so the second replaces
+    and reclaims the first).  probe_blocks 1 packs 16 tiles into every probe block
+    (several passes per block), 3 packs 16 at the brick level and fewer above."""
     api, W = vx
     grid = 4
     n = grid * grid
@@ -374,9 +376,10 @@ def test_launch_shapes_match_oracle(vx, tile, dedup, grid_blocks):
     vB[vA == 1] = 3
     vB[vA == 2] = 1
     it = api.VoxInterner(1, 64 * MB)
-    it.set_tuning(tile=tile, dedup_rounds=dedup, commit_grid=grid_blocks, gc_grid=grid_blocks)
+    it.set_tuning(tile=tile, dedup_rounds=dedup, commit_grid=grid_blocks, gc_grid=grid_blocks,
+                  probe_blocks_bricks=pb, probe_blocks=pb)
     assert it.get_tuning() == {"tile": tile, "dedup_rounds": dedup, "commit_grid": grid_blocks,
-                               "gc_grid": grid_blocks}
+                               "gc_grid": grid_blocks, "probe_blocks_bricks": pb, "probe_blocks": pb}
     it_o = O.Interner(1, 64 * MB)
     f = api.Forest(it, [5] * n)
     trees_o = [O.Tree(5) for _ in range(n)]

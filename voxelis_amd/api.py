@@ -117,21 +117,22 @@ class VoxInterner:
     def sync(self):
         check(lib().vx_interner_sync(self.h), "sync")
 
-    def set_tuning(self, tile=None, dedup_rounds=None, commit_grid=None, gc_grid=None):
+    def set_tuning(self, tile=None, dedup_rounds=None, commit_grid=None, gc_grid=None,
+                   probe_blocks_bricks=None, probe_blocks=None):
         """Launch shape of the apply (vx_tuning); None keeps a field."""
         t = self.get_tuning()
         for k, v in (("tile", tile), ("dedup_rounds", dedup_rounds), ("commit_grid", commit_grid),
-                     ("gc_grid", gc_grid)):
+                     ("gc_grid", gc_grid), ("probe_blocks_bricks", probe_blocks_bricks),
+                     ("probe_blocks", probe_blocks)):
             if v is not None:
                 t[k] = int(v)
-        tt = Tuning(t["tile"], t["dedup_rounds"], t["commit_grid"], t["gc_grid"])
+        tt = Tuning(*(t[f] for f, _ in Tuning._fields_))
         check(lib().vx_interner_set_tuning(self.h, C.byref(tt)), "set_tuning")
 
     def get_tuning(self):
         tt = Tuning()
         check(lib().vx_interner_get_tuning(self.h, C.byref(tt)), "get_tuning")
-        return {"tile": tt.tile, "dedup_rounds": tt.dedup_rounds, "commit_grid": tt.commit_grid,
-                "gc_grid": tt.gc_grid}
+        return {f: getattr(tt, f) for f, _ in Tuning._fields_}
 
     def set_graphs(self, enable=True):
         """Replay repeated batched applies from captured HIP graphs."""

@@ -1,10 +1,12 @@
 """Autotuner of the apply's launch shape: tile size (lanes per block of the build /
 probe / commit passes), wavefront key-merge rounds before the table probe (the
-probe schedule), commit and GC grid sizes, and the pattern table's load factor.
+probe schedule), commit and GC grid sizes, how many tiles a probe block packs
+(at the brick level and above), and the pattern table's load factor.
 
 The search is exhaustive over tile x dedup rounds for every load factor, then
-coordinate descent over the two grid sizes from the best point.  Each point times
-`steps` applies (after `warmup`) of the caller's workload with the device clock.
+coordinate descent over the grid sizes and the probe packing from the best point.
+Each point times `steps` applies (after `warmup`) of the caller's workload with
+the device clock.
 """
 import time
 
@@ -23,10 +25,13 @@ def _time(step, steps, warmup):
 
 
 def autotune(make_case, load_factors=(0.5, 0.6, 0.7, 0.85), tiles=(128, 256, 512), dedups=(0, 2, 4, 8),
-             commit_grids=(256, 1024, 4096), gc_grids=(256, 1024, 4096), steps=6, warmup=2, log=None):
+             commit_grids=(256, 1024, 4096), gc_grids=(256, 1024, 4096),
+             probe_blocks_bricks=(0, 8192, 16384, 32768), probe_blocks=(0, 256, 512, 1024), steps=6, warmup=2,
+             log=None):
     """make_case(load_factor) -> (interner, step(k), close()); returns (best point, all points).
 
-    A point is {"load_factor", "tile", "dedup_rounds", "commit_grid", "gc_grid", "s_per_step"}."""
+    A point is {"load_factor", "tile", "dedup_rounds", "commit_grid", "gc_grid",
+    "probe_blocks_bricks", "probe_blocks", "s_per_step"}."""
     points = []
     best = None
     for lf in load_factors:
@@ -34,10 +39,11 @@ def autotune(make_case, load_factors=(0.5, 0.6, 0.7, 0.85), tiles=(128, 256, 512
         try:
             for tile in tiles:
                 for dd in dedups:
-                    it.set_tuning(tile=tile, dedup_rounds=dd, commit_grid=1024, gc_grid=1024)
+                    it.set_tuning(tile=tile, dedup_rounds=dd, commit_grid=1024, gc_grid=1024,
+                                  probe_blocks_bricks=32768, probe_blocks=512)
                     t = _time(step, steps, warmup)
                     pt = {"load_factor": lf, "tile": tile, "dedup_rounds": dd, "commit_grid": 1024,
-                          "gc_grid": 1024, "s_per_step": t}
+                          "gc_grid": 1024, "probe_blocks_bricks": 32768, "probe_blocks": 512, "s_per_step": t}
                     points.append(pt)
                     if log:
                         log(pt)
@@ -45,15 +51,17 @@ def autotune(make_case, load_factors=(0.5, 0.6, 0.7, 0.85), tiles=(128, 256, 512
                         best = dict(pt)
         finally:
             close()
-    # coordinate descent over the grid sizes at the best (load factor, tile, rounds)
+    # coordinate descent over the grid sizes and the probe packing at the best
+    # (load factor, tile, rounds)
     it, step, close = make_case(best["load_factor"])
     try:
-        for field, choices in (("commit_grid", commit_grids), ("gc_grid", gc_grids)):
+        for field, choices in (("commit_grid", commit_grids), ("gc_grid", gc_grids),
+                               ("probe_blocks_bricks", probe_blocks_bricks), ("probe_blocks", probe_blocks)):
             for v in choices:
                 cand = dict(best)
                 cand[field] = v
-                it.set_tuning(tile=cand["tile"], dedup_rounds=cand["dedup_rounds"],
-                              commit_grid=cand["commit_grid"], gc_grid=cand["gc_grid"])
+                it.set_tuning(**{f: cand[f] for f in ("tile", "dedup_rounds", "commit_grid", "gc_grid",
+                                                       "probe_blocks_bricks", "probe_blocks")})
                 cand["s_per_step"] = _time(step, steps, warmup)
                 points.append(cand)
                 if log:

@@ -26,6 +26,14 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint64_t kMaxGrid = 8192;
 
+// Tiles per probe block: candidates of sparse tiles are packed into full
+// wavefronts, and the owners' allocation takes one update of the shared counter per
+// block; at least `blocks` blocks stay so the launch still spreads over the chip.
+inline uint32_t probe_group(uint32_t ntiles, uint32_t blocks) {
+    if (blocks == 0) return 1;
+    return std::max<uint32_t>(1u, std::min<uint32_t>(16u, ntiles / blocks));
+}
+
 inline unsigned grid_for(uint64_t work) {
     uint64_t g = (work + kBlock - 1) / kBlock;
     if (g < 1) g = 1;
@@ -146,7 +154,7 @@ struct vx_interner {
     int ring_cur = 0;
     void* h_pinned = nullptr;  // the slot of the current call
     Prof prof;
-    vx_tuning tune = {256, 8, 1024, 1024};
+    vx_tuning tune = {256, 8, 1024, 1024, 32768, 512};
     // captured launch sequences of repeated calls (vx_interner_set_graphs)
     bool graphs = false;
     hipStream_t cap_stream = nullptr;
@@ -280,7 +288,8 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
         const uint64_t head[] = {(uint64_t)roots, (uint64_t)status, (uint64_t)opt.dry,
                                  (uint64_t)opt.alloc_limit, (uint64_t)opt.counts, (uint64_t)TILE,
                                  (uint64_t)h->tune.dedup_rounds, (uint64_t)h->tune.commit_grid,
-                                 (uint64_t)h->tune.gc_grid};
+                                 (uint64_t)h->tune.gc_grid, (uint64_t)h->tune.probe_blocks_bricks,
+                                 (uint64_t)h->tune.probe_blocks};
         for (uint64_t w : head) key = mix64(key ^ w);
         for (size_t i = 0; i < n; i++) {
             const vx_batch_view& b = batches[i];
@@ -462,8 +471,9 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     if (dmax >= 2) {
         VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_retry<T VX_COMMA TILE>, 256, kBlock, P, dchunks, L[1], res_ptrs_h[1], K[1],
                   tiles_h[1], ddef, ddrift);
-        VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T VX_COMMA TILE>, t1, TILE, P, K[1], tiles_h[1], res_ptrs_h[1],
-                  dchunks, L[1]);
+        uint32_t g1 = probe_group(t1, h->tune.probe_blocks_bricks);
+        VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T VX_COMMA TILE>, (t1 + g1 - 1) / g1, TILE, P, K[1], tiles_h[1],
+                  res_ptrs_h[1], dchunks, L[1], t1, g1);
         for (uint32_t hh = 2; hh <= dmax; hh++) {
             unsigned th = (unsigned)ntiles[hh];
             // the commit of height hh-1 rides along in extra blocks (none in a dry run)
@@ -471,8 +481,9 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
             VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T VX_COMMA TILE>, th + tc, TILE, P, dchunks, L[hh],
                       doff + (size_t)(hh - 1) * n, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh],
                       K[hh], tiles_h[hh], th, K[hh - 1], tiles_h[hh - 1], (uint64_t)ntiles[hh - 1], O.at[hh]);
-            VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T VX_COMMA TILE>, th, TILE, P, K[hh], tiles_h[hh], res_ptrs_h[hh],
-                      dchunks, L[hh]);
+            uint32_t gh = probe_group(th, h->tune.probe_blocks);
+            VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T VX_COMMA TILE>, (th + gh - 1) / gh, TILE, P, K[hh], tiles_h[hh],
+                      res_ptrs_h[hh], dchunks, L[hh], th, gh);
         }
         if (!opt.dry) {
             unsigned td = (unsigned)ntiles[dmax];
@@ -848,7 +859,7 @@ int vx_interner_stats(vx_interner* h, uint64_t* out, uint32_t n) {
     if (rc) return rc;
     static const int map[VX_STAT_COUNT] = {ST_HIT, ST_MISS, ST_DUP, ST_LEAF_MISS, ST_COLLAPSE,
                                            ST_RECYCLED, ST_H2, ST_H3, ST_H4, ST_H5, ST_CAND,
-                                           ST_PROBES};
+                                           ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST};
     for (uint32_t i = 0; i < n && i < (uint32_t)VX_STAT_COUNT; i++) {
         uint64_t sum = 0;
         for (int sidx = 0; sidx < kStatStripes; sidx++) sum += c.st[sidx][map[i]];

@@ -27,7 +27,7 @@ namespace vx {
 
 enum Stat {
     ST_HIT = 0, ST_MISS, ST_DUP, ST_LEAF_MISS, ST_COLLAPSE, ST_RECYCLED, ST_H2, ST_H3, ST_H4, ST_H5,
-    ST_CAND, ST_PROBES
+    ST_CAND, ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST
 };
 
 __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
@@ -833,7 +833,7 @@ __device__ inline int match_slot(const Pool<T>& P, uint32_t level, uint64_t e, u
 template <typename T>
 __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, const uint64_t key[9],
                                   uint64_t h, uint64_t e0, uint64_t* result, uint64_t* tslot,
-                                  uint32_t* probes) {
+                                  uint32_t* probes, uint32_t* claims) {
     uint32_t tag = slot_tag(h);
     uint64_t s = h & P.tmask;
     uint64_t first_free = ~0ull;
@@ -848,6 +848,7 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
             bool won = __hip_atomic_compare_exchange_strong(
                 &P.tab[target], &seen, slot_make(tag, true, level, (uint32_t)k), __ATOMIC_RELAXED,
                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            *claims += won ? 1u : 0x10000u;  // low half: won, high half: lost
             if (won) {
                 // the caller allocates the owner's node; the slot is kept when the
                 // allocation fails too (the abort path restores it), and a dry run
@@ -898,79 +899,137 @@ __device__ inline int wave_merge_keys(uint32_t rounds, bool valid, const uint64_
     return leader;
 }
 
+// Probe of one level.  Block b takes the candidates of tiles [b*group, b*group+group)
+// (group <= 64), packed densely over its lanes: sparse tiles fill whole wavefronts,
+// and equal keys meet in the same wavefront more often.  Per pass over TILE of
+// them: merge equal keys in each wavefront, look the leaders' keys up / insert
+// them (get_or_create_branch, interner/mod.rs:716-829), then allocate the owners'
+// nodes with one counter update for the whole block (a counter every block of a
+// launch updates is a serial point), and publish the results.
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
-                                                 uint64_t* res, const ChunkDev* chunks, LevelInfo L) {
-    uint32_t n = tile_cnt[blockIdx.x];
-    if ((threadIdx.x & ~63u) >= n) return;  // whole wavefront without candidates
-    int lane = (int)__lane_id();
-    bool valid = threadIdx.x < n;
-    uint64_t k = (uint64_t)blockIdx.x * TILE + threadIdx.x;
-    // The key, its hash and its level slot (lanes past the tile's count load the
-    // tile's first candidate, which exists).
-    const uint64_t kl = valid ? k : (uint64_t)blockIdx.x * TILE;
-    uint64_t key[9];
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
+                                                 uint64_t* res, const ChunkDev* chunks, LevelInfo L,
+                                                 uint32_t ntiles, uint32_t group) {
+    __shared__ uint32_t s_pre[65];  // s_pre[t]: candidates of the group's tiles before tile t
+    __shared__ uint32_t s_own, s_base;
+    const uint32_t t0 = blockIdx.x * group;
+    const uint32_t tn = min(group, ntiles - t0);
+    if (threadIdx.x < 64) {
+        uint32_t x = threadIdx.x < tn ? tile_cnt[t0 + threadIdx.x] : 0u;
 #pragma unroll
-    for (int i = 0; i < 9; i++) key[i] = K.key[9ull * kl + i];
-    const uint64_t h = K.hash[kl];
-    const uint64_t slot = K.slot[kl];
-    int leader = wave_merge_keys(P.dedup_rounds, valid, key, h);
-    // the table slot the key hashes to, read by leaders only (duplicates of a hot key
-    // would all read one line), together with the tree's status
-    uint64_t e0 = valid && leader == lane ? P.tab[h & P.tmask] : kSlotNone;
-    uint32_t tree = kNoIdx;
-    if (L.h == 1 || P.per_tree) {
-        const GroupPos G = group_pos(L, slot);
-        uint32_t c = G.chunk_base + (uint32_t)((slot - G.base) >> (3 * (G.depth - L.h)));
-        const ChunkDev* Cp = chunks + c;
-        const int32_t cstatus = Cp->status;
-        tree = Cp->orig;
-        // a brick of a tree that hit the leaf-brick panic later in the same pass
-        bool drop = valid && L.h == 1 && cstatus < 0;
-        if (drop) {
-            res[slot] = kInvalid;
-            K.id[k] = kInvalid;
-            K.tslot[k] = ~0ull;
-            valid = false;
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t y = __shfl_up(x, d);
+            if ((int)threadIdx.x >= d) x += y;
         }
-        if (__ballot(drop)) {  // rare: merge again without the dropped lanes
+        s_pre[threadIdx.x + 1] = x;
+        if (threadIdx.x == 0) {
+            s_pre[0] = 0;
+            s_own = 0;
+        }
+    }
+    __syncthreads();
+    const uint32_t N = s_pre[tn];
+    const uint32_t wbase = threadIdx.x & ~63u;
+    const int lane = (int)__lane_id();
+    // A wavefront with no candidate in any pass ends here (whole wavefronts only: a
+    // barrier waits for the wavefronts of the block that have not ended).
+    if (wbase >= N) return;
+    for (uint32_t j0 = 0; j0 < N; j0 += TILE) {  // uniform over the remaining wavefronts
+        const bool wlive = j0 + wbase < N;  // the wavefront has candidates
+        const uint32_t j = j0 + threadIdx.x;
+        bool valid = j < N;
+        uint64_t k = 0, slot = 0, result = kInvalid, tslot = ~0ull;
+        uint64_t key8 = 0;
+        int kind = -1, leader = lane;
+        uint32_t tree = kNoIdx;
+        if (wlive) {
+            const uint32_t jj = valid ? j : j0 + wbase;  // the wavefront's first candidate exists
+            uint32_t lo = 0, hi = tn;  // s_pre[lo] <= jj < s_pre[hi]: jj lies in tile lo
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= jj) lo = mid;
+                else hi = mid;
+            }
+            k = (uint64_t)(t0 + lo) * TILE + (jj - s_pre[lo]);
+            uint64_t key[9];
+#pragma unroll
+            for (int i = 0; i < 9; i++) key[i] = K.key[9ull * k + i];
+            key8 = key[8];
+            const uint64_t h = K.hash[k];
+            slot = K.slot[k];
             leader = wave_merge_keys(P.dedup_rounds, valid, key, h);
-            e0 = kSlotNone;
+            // the table slot the key hashes to, read by leaders only (duplicates of a
+            // hot key would all read one line), together with the tree's status
+            uint64_t e0 = valid && leader == lane ? P.tab[h & P.tmask] : kSlotNone;
+            if (L.h == 1 || P.per_tree) {
+                const GroupPos G = group_pos(L, slot);
+                uint32_t c = G.chunk_base + (uint32_t)((slot - G.base) >> (3 * (G.depth - L.h)));
+                const ChunkDev* Cp = chunks + c;
+                const int32_t cstatus = Cp->status;
+                tree = Cp->orig;
+                // a brick of a tree that hit the leaf-brick panic later in the same pass
+                bool drop = valid && L.h == 1 && cstatus < 0;
+                if (drop) {
+                    res[slot] = kInvalid;
+                    K.id[k] = kInvalid;
+                    K.tslot[k] = ~0ull;
+                    valid = false;
+                }
+                if (__ballot(drop)) {  // rare: merge again without the dropped lanes
+                    leader = wave_merge_keys(P.dedup_rounds, valid, key, h);
+                    e0 = kSlotNone;
+                }
+            }
+            uint32_t probes = 0, claims = 0;
+            if (valid && leader == lane)
+                kind = probe_table(P, L.h, k, key, h, e0, &result, &tslot, &probes, &claims);
+            wave_add64(stat(P.ctl, ST_PROBES), probes);
+            wave_add64(stat(P.ctl, ST_CLAIMS), (claims & 0xFFFFu) + (claims >> 16));
+            wave_add64(stat(P.ctl, ST_CLAIMS_LOST), claims >> 16);
         }
-    }
-    uint64_t result = kInvalid, tslot = ~0ull;
-    int kind = -1;
-    uint32_t probes = 0;
-    if (valid && leader == lane) kind = probe_table(P, L.h, k, key, h, e0, &result, &tslot, &probes);
-    // the owners' nodes: one counter update per wavefront
-    if (P.dry) {
-        // a placeholder id (the table slot's previous value is restored after the run)
-        uint32_t serial = wave_append(&P.ctl->dry_owners, kind == 2);
-        if (kind == 2) result = make_branch(P.capacity + 65536u + serial, 0, (uint32_t)key[8]);
-    } else {
-        uint32_t ka = wave_append(&P.ctl->alloc_count, kind == 2);
+        // owners' allocation ranks: wavefront offsets in LDS, one counter update per block
+        const uint64_t ob = __ballot(kind == 2);
+        uint32_t woff = 0;
+        if (ob && lane == 0) woff = atomicAdd(&s_own, (uint32_t)__popcll(ob));
+        woff = __shfl(woff, 0);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // s_own is only read here; cleared for the next pass before the barrier
+            // that pass's wavefront offsets come after
+            const uint32_t own = s_own;
+            if (own) s_base = atomicAdd(P.dry ? &P.ctl->dry_owners : &P.ctl->alloc_count, own);
+            s_own = 0;
+        }
+        __syncthreads();  // s_base is next written after the next pass's first barrier
         if (kind == 2) {
-            bool ok;
-            uint32_t idx = alloc_at(P, ka, &ok);
-            result = ok ? make_branch(idx, P.gens[idx], (uint32_t)key[8]) : kInvalid;
+            const uint32_t ka = s_base + woff + (uint32_t)__popcll(ob & ((1ull << lane) - 1ull));
+            if (P.dry) {
+                // a placeholder id (the table slot's previous value is restored after the run)
+                result = make_branch(P.capacity + 65536u + ka, 0, (uint32_t)key8);
+            } else {
+                bool ok;
+                uint32_t idx = alloc_at(P, ka, &ok);
+                result = ok ? make_branch(idx, P.gens[idx], (uint32_t)key8) : kInvalid;
+            }
+            if (P.per_tree && tree != kNoIdx) atomicAdd(&P.per_tree[tree], 1u);
+        }
+        if (wlive) {
+            uint64_t lres = shfl64(result, leader);
+            int lkind = __shfl(kind, leader);
+            if (valid && leader != lane) {
+                result = lres;
+                kind = lkind == 0 ? 0 : 1;  // a follower of an owner or of a duplicate is a duplicate
+            }
+            if (valid) {
+                K.id[k] = result;
+                K.tslot[k] = tslot;
+                res[slot] = result;
+            }
+            wave_count(stat(P.ctl, ST_HIT), valid && kind == 0);
+            wave_count(stat(P.ctl, ST_DUP), valid && kind == 1);
+            wave_count(stat(P.ctl, ST_MISS), valid && kind == 2);
         }
     }
-    if (kind == 2 && P.per_tree && tree != kNoIdx) atomicAdd(&P.per_tree[tree], 1u);
-    uint64_t lres = shfl64(result, leader);
-    int lkind = __shfl(kind, leader);
-    if (valid && leader != lane) {
-        result = lres;
-        kind = lkind == 0 ? 0 : 1;  // a follower of an owner or of a duplicate is a duplicate
-    }
-    if (valid) {
-        K.id[k] = result;
-        K.tslot[k] = tslot;
-        res[slot] = result;
-    }
-    wave_count(stat(P.ctl, ST_HIT), valid && kind == 0);
-    wave_count(stat(P.ctl, ST_DUP), valid && kind == 1);
-    wave_count(stat(P.ctl, ST_MISS), valid && kind == 2);
-    wave_add64(stat(P.ctl, ST_PROBES), probes);
 }
 
 // Owners write their node (children, LOD value, types/mask), publish the table
