@@ -131,8 +131,9 @@ __device__ inline void agg_init(uint32_t* akey, uint32_t* acnt) {
 
 __device__ inline bool agg_add(uint32_t* akey, uint32_t* acnt, uint32_t idx, uint32_t n) {
     uint32_t h = (idx * 2654435761u) >> 23;
+    // a few slots only: a full table (many distinct children) falls back quickly
 #pragma unroll 1
-    for (int p = 0; p < 16; p++) {
+    for (int p = 0; p < 4; p++) {
         uint32_t s = (h + p) & (kAggSlots - 1);
         uint32_t k = atomicCAS(&akey[s], kNoIdx, idx);
         if (k == kNoIdx || k == idx) {
@@ -1399,12 +1400,24 @@ __device__ inline void leaf_cleanup_body(const Pool<T>& P) {
 // decrement; a node that reaches zero drops its pattern, is recycled
 // (mod.rs:566-625), and decrements its children, queueing those that reach zero.
 // Round 0 holds the replaced roots (decremented here); later rounds hold nodes
-// whose count already reached zero.
+// whose count already reached zero.  Each pass of a block stages its pushes (freed
+// indices onto the free stack, children that reached zero onto the next queue) in
+// LDS and reserves room for them with one update of each shared counter: those
+// counters are serial points that every block of the launch updates.
+constexpr uint32_t kGcStage = 2048;  // children staged per pass (8 per lane of a 256-lane block)
+
 template <typename T>
-__global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uint32_t* next,
-                           bool decrement_self, uint64_t front_cap, bool leaf_cleanup) {
+__global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uint32_t* next,
+                                                  bool decrement_self, uint64_t front_cap, bool leaf_cleanup) {
     __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
+    __shared__ uint32_t s_next[kGcStage];
+    __shared__ uint32_t s_nnext, s_nfree, s_bnext, s_bfree;
+    __shared__ unsigned long long s_leaves, s_branches;
     agg_init(akey, acnt);
+    if (threadIdx.x == 0) {
+        s_nnext = s_nfree = 0;
+        s_leaves = s_branches = 0;
+    }
     __syncthreads();
     // new leaves nothing references (their trees failed) are released in the first
     // round; no node that this GC visits references them
@@ -1412,7 +1425,9 @@ __global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uin
     uint64_t n = P.ctl->gc_count[round];
     if (n > front_cap) n = front_cap;
     uint32_t* next_count = &P.ctl->gc_count[round + 1];
-    VX_LOOP(i, n) {
+    const int lane = (int)__lane_id();
+    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < n; b0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = b0 + threadIdx.x;  // the loop is uniform over the block
         uint32_t idx = kNoIdx;
         if (i < n) {
             uint32_t v = front[i];
@@ -1450,11 +1465,7 @@ __global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uin
                     if (ch[j] == kEmpty) continue;
                     uint32_t ci = bid_index(ch[j]);
                     if (agg_add(akey, acnt, ci, 1u)) continue;
-                    if (atomicSub(&P.refs[ci], 1u) == 1u) {
-                        uint32_t pos = atomicAdd(next_count, 1u);
-                        if (pos < front_cap) next[pos] = ci;
-                        else atomicOr(&P.ctl->err, kErrPoison);
-                    }
+                    if (atomicSub(&P.refs[ci], 1u) == 1u) s_next[atomicAdd(&s_nnext, 1u)] = ci;
                 }
             }
             P.values[idx] = 0;
@@ -1462,31 +1473,56 @@ __global__ void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uin
             uint32_t g = P.gens[idx] + 1u;
             P.gens[idx] = (uint16_t)(g >= kMaxGeneration ? 0u : g);
         }
-        uint32_t fpos = wave_append(&P.ctl->free_top, freed);
+        const uint64_t fb = __ballot(freed);
+        const uint64_t bl = __ballot(freed && is_leaf), bb = __ballot(freed && !is_leaf);
+        uint32_t woff = 0;
+        if (lane == 0) {
+            if (fb) woff = atomicAdd(&s_nfree, (uint32_t)__popcll(fb));
+            if (bl) atomicAdd(&s_leaves, (unsigned long long)__popcll(bl));
+            if (bb) atomicAdd(&s_branches, (unsigned long long)__popcll(bb));
+        }
+        woff = __shfl(woff, 0);
+        wave_count(stat(P.ctl, ST_RECYCLED), freed);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (s_nfree) s_bfree = atomicAdd(&P.ctl->free_top, s_nfree);
+            if (s_nnext) s_bnext = atomicAdd(next_count, s_nnext);
+        }
+        __syncthreads();
         if (freed) {
-            if (fpos < P.capacity) P.free_stack[fpos] = idx;
+            uint32_t pos = s_bfree + woff + (uint32_t)__popcll(fb & ((1ull << lane) - 1ull));
+            if (pos < P.capacity) P.free_stack[pos] = idx;
             else atomicOr(&P.ctl->err, kErrPoison);
         }
-        wave_count(stat(P.ctl, ST_RECYCLED), freed);
-        uint64_t bl = __ballot(freed && is_leaf), bb = __ballot(freed && !is_leaf);
-        if (__lane_id() == 0) {
-            if (bl) atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)(-(long long)__popcll(bl)));
-            if (bb) {
-                atomicAdd((unsigned long long*)&P.ctl->live_branches, (unsigned long long)(-(long long)__popcll(bb)));
-                atomicAdd((unsigned long long*)&P.ctl->tombstones, (unsigned long long)__popcll(bb));
-            }
+        for (uint32_t k = threadIdx.x; k < s_nnext; k += blockDim.x) {
+            uint32_t pos = s_bnext + k;
+            if (pos < front_cap) next[pos] = s_next[k];
+            else atomicOr(&P.ctl->err, kErrPoison);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_nfree = s_nnext = 0;
+        __syncthreads();
+    }
+    // the block's combined decrements; children reaching zero are queued the same way
+    for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
+        uint32_t key = akey[i];
+        if (key != kNoIdx && acnt[i] && atomicSub(&P.refs[key], acnt[i]) == acnt[i])
+            s_next[atomicAdd(&s_nnext, 1u)] = key;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_nnext) s_bnext = atomicAdd(next_count, s_nnext);
+        if (s_leaves) atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)(-(long long)s_leaves));
+        if (s_branches) {
+            atomicAdd((unsigned long long*)&P.ctl->live_branches, (unsigned long long)(-(long long)s_branches));
+            atomicAdd((unsigned long long*)&P.ctl->tombstones, (unsigned long long)s_branches);
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
-        bool push = false;
-        uint32_t key = akey[i];
-        if (key != kNoIdx && acnt[i]) push = atomicSub(&P.refs[key], acnt[i]) == acnt[i];
-        uint32_t pos = wave_append(next_count, push);
-        if (push) {
-            if (pos < front_cap) next[pos] = key;
-            else atomicOr(&P.ctl->err, kErrPoison);
-        }
+    for (uint32_t k = threadIdx.x; k < s_nnext; k += blockDim.x) {
+        uint32_t pos = s_bnext + k;
+        if (pos < front_cap) next[pos] = s_next[k];
+        else atomicOr(&P.ctl->err, kErrPoison);
     }
 }
 
