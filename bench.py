@@ -73,12 +73,12 @@ def kernel_bytes(name, launches, st, counts):
     cand_up = counts["cand_upper"] * steps
     slots_up = counts["upper_slots"] * steps
     s = counts["value_size"]
-    d = counts["depth"]
     if name == "k_brick_build":
-        # every brick slot: its set mask (2 B of the mask pair) and the slot write (8);
-        # touched bricks: 8 values; non-uniform bricks: old-tree path (8 B per level),
-        # base node children (64) and their values (8s), candidate key + hash + slot (88)
-        return slots1 * (2 + 8) + touched * 8 * s + cand1 * (8 * (d - 1) + 64 + 8 * s + 88)
+        # every brick slot: its set mask (2 B of the mask pair), its share of the recorded
+        # height-2 node (1) and the slot write (8); touched bricks: 8 values; non-uniform
+        # bricks: the brick-level node (8), its children (64) and their values (8s),
+        # candidate key + hash + slot (88)
+        return slots1 * (2 + 1 + 8) + touched * 8 * s + cand1 * (8 + 64 + 8 * s + 88)
     if name in ("k_probe[bricks]", "k_probe[upper]"):
         c = cand1 if name == "k_probe[bricks]" else cand_up
         # key + hash + slot read (88), one table slot (8), the matched node's key (66),
@@ -365,10 +365,8 @@ def _committed_traffic(kernel, grid):
     if grid != 64 or not os.path.exists(path):
         return None
     d = json.load(open(path))
-    for k, v in d["kernels"].items():
-        if k.startswith(kernel + "<"):
-            return int(v["hbm_bytes_scaled"] / max(v["dispatches"], 1))
-    return None
+    v = d["kernels"].get(kernel)
+    return int(v["hbm_bytes_scaled"] / max(v["dispatches"], 1)) if v else None
 
 
 def _timed_steps(f, views_cycle, steps, warmup, it, torch):
