@@ -458,11 +458,11 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     P.undo_cap = (uint32_t)undo_cap;
     // ---- launch sequence
     auto launch_all = [&]() -> int {
-    // chunk set-up, plus the old-tree descent over the level-2 slots in extra blocks
+    // chunk set-up, plus the old-tree descent over the level-3 slots in extra blocks
     unsigned np = grid_for(n);
-    unsigned nd = L[2].nslots ? grid_for(L[2].nslots) : 0u;
+    unsigned nd = L[3].nslots ? grid_for(L[3].nslots) : 0u;
     VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, np + nd, kBlock, P, dchunks, (uint32_t)n, roots, pre_status, np,
-              L[2], doff, O);
+              L[3], doff, O);
     unsigned t1 = (unsigned)ntiles[1];
     if (dmax >= 2)
         VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE>, t1, TILE, P, dchunks, L[1], res_ptrs_h[1], K[1],

@@ -131,9 +131,9 @@ __device__ inline void agg_init(uint32_t* akey, uint32_t* acnt) {
 
 __device__ inline bool agg_add(uint32_t* akey, uint32_t* acnt, uint32_t idx, uint32_t n) {
     uint32_t h = (idx * 2654435761u) >> 23;
-    // a few slots only: a full table (many distinct children) falls back quickly
+    // two slots only: a full table (many distinct children) falls back quickly
 #pragma unroll 1
-    for (int p = 0; p < 4; p++) {
+    for (int p = 0; p < 2; p++) {
         uint32_t s = (h + p) & (kAggSlots - 1);
         uint32_t k = atomicCAS(&akey[s], kNoIdx, idx);
         if (k == kNoIdx || k == idx) {
@@ -337,49 +337,60 @@ __global__ void k_end_alloc(Pool<T> P, bool rollback, uint32_t* counts) {
     }
 }
 
-// The previous tree's nodes on the path to one height-2 position (level-2 slot
-// `slot`), for the equivalent-node lookups of the brick and level passes
-// (voxtree.rs:930-952): the node at height h is recorded by the lane that owns
-// the first height-2 position under it.  Below a leaf every height records that
-// leaf (it covers the position); below an empty child, empty.  Trees with a fill
-// start from the fill leaf instead and are skipped.
+// The previous tree's nodes on the path to one height-3 position (level-3 slot
+// `slot`) and the 8 height-2 nodes below it, for the equivalent-node lookups of the
+// brick and level passes (voxtree.rs:930-952): the node at height h >= 3 is
+// recorded by the lane that owns the first height-3 position under it.  Below a
+// leaf every height records that leaf (it covers the position); below an empty
+// child, empty.  Trees with a fill start from the fill leaf instead and are
+// skipped; a depth-2 tree's height-2 node is its root (recorded by k_prepare).
 template <typename T>
-__device__ inline void descend_body(const Pool<T>& P, const ChunkDev* chunks, const LevelInfo& L2,
+__device__ inline void descend_body(const Pool<T>& P, const ChunkDev* chunks, const LevelInfo& L3,
                                     const uint64_t* doff, uint32_t n, const OldNodes& O,
                                     const uint64_t* roots, uint64_t slot) {
-    const GroupPos G = group_pos(L2, slot);
+    const GroupPos G = group_pos(L3, slot);
     uint64_t local = slot - G.base;
     uint32_t D = G.depth;
-    uint32_t sh = 3 * (D - 2);
+    uint32_t sh = 3 * (D - 3);
     uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
     uint32_t q = (uint32_t)(local & ((1ull << sh) - 1));
     const ChunkDev& C = chunks[c];
     if (C.has_fill) return;
     uint64_t cur = roots[C.orig];
-    for (int hh = (int)D; hh >= 2; hh--) {
+    for (int hh = (int)D; hh >= 3; hh--) {
         if (hh < (int)D && cur != kEmpty && !bid_is_leaf(cur)) {
             if (bid_index(cur) >= P.capacity) {
                 atomicOr(&P.ctl->err, kErrPoison);
                 cur = kEmpty;
             } else {
-                cur = P.children[8ull * bid_index(cur) + ((q >> (3 * (hh - 2))) & 7u)];
+                cur = P.children[8ull * bid_index(cur) + ((q >> (3 * (hh - 3))) & 7u)];
             }
         }
-        uint32_t sub = 3 * (uint32_t)(hh - 2);
+        uint32_t sub = 3 * (uint32_t)(hh - 3);
         if ((q & ((1u << sub) - 1u)) == 0) O.at[hh][doff[(size_t)hh * n + c] + (q >> sub)] = cur;
     }
+    uint64_t below[8];
+    const bool branch = cur != kEmpty && !bid_is_leaf(cur);
+    if (branch && bid_index(cur) >= P.capacity) atomicOr(&P.ctl->err, kErrPoison);
+    const bool ok = branch && bid_index(cur) < P.capacity;
+    const uint64_t* src = P.children + 8ull * (ok ? bid_index(cur) : 0u);
+#pragma unroll
+    for (int i = 0; i < 8; i++) below[i] = ok ? src[i] : (branch ? kEmpty : cur);
+    uint64_t* dst = O.at[2] + doff[(size_t)2 * n + c] + 8ull * q;
+#pragma unroll
+    for (int i = 0; i < 8; i++) dst[i] = below[i];
 }
 
 // Read roots, mark the fill leaf as needed (Phase 0, voxtree.rs:742-754).  Blocks
-// nprep.. run the old-tree descent over the level-2 slots instead.
+// nprep.. run the old-tree descent over the level-3 slots instead.
 template <typename T>
 __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_t* roots,
-                          const int32_t* pre_status, uint32_t nprep, LevelInfo L2, const uint64_t* doff,
+                          const int32_t* pre_status, uint32_t nprep, LevelInfo L3, const uint64_t* doff,
                           OldNodes O) {
     if (blockIdx.x >= nprep) {
         uint64_t stride = (uint64_t)(gridDim.x - nprep) * blockDim.x;
-        for (uint64_t s = (uint64_t)(blockIdx.x - nprep) * blockDim.x + threadIdx.x; s < L2.nslots; s += stride)
-            descend_body(P, chunks, L2, doff, n, O, roots, s);
+        for (uint64_t s = (uint64_t)(blockIdx.x - nprep) * blockDim.x + threadIdx.x; s < L3.nslots; s += stride)
+            descend_body(P, chunks, L3, doff, n, O, roots, s);
         return;
     }
     // the sequence's control-block reset rides in block 0 (nothing else here reads it)
@@ -390,6 +401,8 @@ __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_
             C.root_in = roots[C.orig];
             C.init = C.root_in;
             C.status = C.status0;
+            // a depth-2 tree's height-2 node is its root (deeper trees: the descent)
+            if (C.depth == 2 && C.has_patches && !C.has_fill) O.at[2][doff[(size_t)2 * n + c]] = C.root_in;
             if (P.per_tree) P.per_tree[C.orig] = 0;
             if (pre_status && pre_status[C.orig] < 0 && C.status >= 0) C.status = pre_status[C.orig];
             if (C.status >= 0 && C.has_fill) {
@@ -1461,12 +1474,24 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
 #pragma unroll
                 for (int j = 0; j < 8; j++) src[j] = kEmpty;
                 P.tms[idx] = 0;
+                // children: combined in LDS where the block's table has room, the rest
+                // decremented with 8 independent atomics (issued together, then read)
+                uint32_t gmask = 0;
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (ch[j] != kEmpty && !agg_add(akey, acnt, bid_index(ch[j]), 1u)) gmask |= 1u << j;
+                // Straight-line, so the 8 atomics are in flight together: a slot with
+                // nothing to decrement adds 0 to this node's own count (it is being
+                // recycled by this lane; nothing else touches it).
+                uint32_t old[8];
+#pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    if (ch[j] == kEmpty) continue;
-                    uint32_t ci = bid_index(ch[j]);
-                    if (agg_add(akey, acnt, ci, 1u)) continue;
-                    if (atomicSub(&P.refs[ci], 1u) == 1u) s_next[atomicAdd(&s_nnext, 1u)] = ci;
+                    const bool g = (gmask >> j) & 1u;
+                    old[j] = atomicSub(&P.refs[g ? bid_index(ch[j]) : idx], g ? 1u : 0u);
                 }
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (((gmask >> j) & 1u) && old[j] == 1u) s_next[atomicAdd(&s_nnext, 1u)] = bid_index(ch[j]);
             }
             P.values[idx] = 0;
             P.refs[idx] = 0;
