@@ -385,6 +385,21 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
         L[hh].slot_base[L[hh].ngroups] = base;
         L[hh].nslots = base;
     }
+    // each group's first slot one height below and one above (a group of height hh is
+    // a run of trees of depth >= hh; runs keep their order at every height)
+    for (uint32_t hh = 1; hh <= dmax; hh++) {
+        for (size_t g = 0; g < lruns[hh].size(); g++) {
+            const Run& r = lruns[hh][g];
+            L[hh].prev_base[g] = ~0ull;
+            L[hh].next_base[g] = ~0ull;
+            if (hh >= 2)
+                for (size_t g2 = 0; g2 < lruns[hh - 1].size(); g2++)
+                    if (lruns[hh - 1][g2].start == r.start) L[hh].prev_base[g] = L[hh - 1].slot_base[g2];
+            if (hh + 1 <= dmax)
+                for (size_t g2 = 0; g2 < lruns[hh + 1].size(); g2++)
+                    if (lruns[hh + 1][g2].start == r.start) L[hh].next_base[g] = L[hh + 1].slot_base[g2];
+        }
+    }
     // ---- workspace (a captured call owns its records)
     Grow& g_chunks = plan ? plan->chunks : h->chunks;
     Grow& g_off = plan ? plan->slot_off : h->slot_off;
@@ -466,7 +481,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     unsigned t1 = (unsigned)ntiles[1];
     if (dmax >= 2)
         VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE>, t1, TILE, P, dchunks, L[1], res_ptrs_h[1], K[1],
-                  tiles_h[1], ddef, ddrift, O.at[2], doff + (size_t)2 * n);
+                  tiles_h[1], ddef, ddrift, O.at[2]);
     VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc<T>, 1, 1024, P, dchunks, (uint32_t)n, nvalues);
     if (dmax >= 2) {
         VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_retry<T VX_COMMA TILE>, 256, kBlock, P, dchunks, L[1], res_ptrs_h[1], K[1],
@@ -479,7 +494,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
             // the commit of height hh-1 rides along in extra blocks (none in a dry run)
             unsigned tc = opt.dry ? 0u : std::min<unsigned>((unsigned)ntiles[hh - 1], h->tune.commit_grid);
             VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T VX_COMMA TILE>, th + tc, TILE, P, dchunks, L[hh],
-                      doff + (size_t)(hh - 1) * n, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh],
+                      res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh],
                       K[hh], tiles_h[hh], th, K[hh - 1], tiles_h[hh - 1], (uint64_t)ntiles[hh - 1], O.at[hh]);
             uint32_t gh = probe_group(th, h->tune.probe_blocks);
             VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T VX_COMMA TILE>, (th + gh - 1) / gh, TILE, P, K[hh], tiles_h[hh],

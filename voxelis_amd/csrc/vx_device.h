@@ -109,10 +109,14 @@ struct Ctl {
     uint32_t gc_count[kLevels + 2];
     // statistics, striped over 64 cache lines by block index (summed when read)
     uint64_t st[kStatStripes][16];
+    // u8 interners: the leaf id of every value at the start of the current apply
+    // (written by k_prepare, read by the brick pass; leaves created later in the
+    // apply are not in it)
+    uint64_t leaf_ids8[256];
 };
 
 // Per-tree record of one apply launch sequence (trees sorted by depth on the host).
-struct ChunkDev {
+struct ChunkDev {  // 8-byte aligned; depth .. skip form one dword (read as such)
     const uint8_t* masks;
     const void* values;
     uint64_t fill;
@@ -126,11 +130,16 @@ struct ChunkDev {
     uint8_t skip;  // not applied at all (an edit-list tree without edits)
     int32_t status0;  // status before the launch (a replayed launch starts from it)
 };
+static_assert(offsetof(ChunkDev, depth) % 4 == 0 && offsetof(ChunkDev, has_fill) == offsetof(ChunkDev, depth) + 1 &&
+                  offsetof(ChunkDev, has_patches) == offsetof(ChunkDev, depth) + 2,
+              "the flag bytes are read as one dword");
 
 // Slot space of one level: groups of trees of equal depth, each tree owning
 // 8^(depth-h) consecutive slots at height h (h = 1: bricks, h = depth: root).
 struct LevelInfo {
     uint64_t slot_base[kLevels + 1];
+    uint64_t prev_base[kLevels];  // the same group's first slot at height h-1
+    uint64_t next_base[kLevels];  // ... at height h+1 (~0 when its depth is h)
     uint32_t chunk_base[kLevels];
     uint32_t depth[kLevels];
     uint32_t ngroups;
@@ -153,6 +162,8 @@ struct Cands {
     uint64_t* tslot;  // owner: its table slot; ~0 otherwise
     uint64_t* slot;   // the level slot this candidate resolves
 };
+
+__host__ __device__ inline uint64_t& key_word(const Cands& K, uint64_t k, int i) { return K.key[9ull * k + i]; }
 
 // A queued reference-count correction (applied once the tree's status is final).
 struct Drift {

@@ -251,19 +251,55 @@ __device__ inline void ref_delta_logged(const Pool<T>& P, uint32_t chunk, uint32
 // uniform index, so the selection costs scalar loads and per-lane selects only.
 struct GroupPos {
     uint64_t base;
+    uint64_t prev;  // the group's first slot one height below
+    uint64_t next;  // ... one height above (~0 when the group's depth is this height)
     uint32_t depth;
     uint32_t chunk_base;
 };
 
 __device__ inline GroupPos group_pos(const LevelInfo& L, uint64_t slot) {
-    GroupPos r{L.slot_base[0], L.depth[0], L.chunk_base[0]};
+    GroupPos r{L.slot_base[0], L.prev_base[0], L.next_base[0], L.depth[0], L.chunk_base[0]};
     for (uint32_t k = 1; k < L.ngroups; k++) {
         bool in = slot >= L.slot_base[k];
         r.base = in ? L.slot_base[k] : r.base;
+        r.prev = in ? L.prev_base[k] : r.prev;
+        r.next = in ? L.next_base[k] : r.next;
         r.depth = in ? L.depth[k] : r.depth;
         r.chunk_base = in ? L.chunk_base[k] : r.chunk_base;
     }
     return r;
+}
+
+// The fields of a chunk record that the build passes read first.  A wavefront whose
+// lanes all lie in one chunk (every wavefront of a tree of depth >= 3 at the brick
+// level) reads them with scalar loads: vector-memory address processing is what
+// bounds these passes, and a wave-wide load of one address costs as much of it as
+// any other.
+struct ChunkHead {
+    const uint8_t* masks;
+    const void* values;
+    uint64_t root_in;
+    uint64_t init;
+    int32_t status;
+    bool has_fill;
+    bool has_patches;
+};
+
+typedef const __attribute__((address_space(4))) ChunkDev ConstChunk;  // read-only view: scalar loads
+
+// (A record's status may be set negative by another wavefront of the same pass; a
+// reader that misses it is covered by the probe's own status check.)
+__device__ inline ChunkHead chunk_head(const ChunkDev* chunks, uint32_t c) {
+    const uint32_t c0 = __builtin_amdgcn_readfirstlane(c);
+    if (__ballot(c != c0) == 0) {
+        ConstChunk& r = *(ConstChunk*)(chunks + c0);
+        // the four flag bytes (depth, has_fill, has_patches, skip) as one dword
+        const uint32_t flags = *(const __attribute__((address_space(4))) uint32_t*)&r.depth;
+        return ChunkHead{r.masks, r.values, r.root_in, r.init, r.status, ((flags >> 8) & 0xFFu) != 0,
+                         ((flags >> 16) & 0xFFu) != 0};
+    }
+    const ChunkDev& r = chunks[c];
+    return ChunkHead{r.masks, r.values, r.root_in, r.init, r.status, r.has_fill != 0, r.has_patches != 0};
 }
 
 // ------------------------------------------------------------------ launch setup
@@ -395,6 +431,11 @@ __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_
     }
     // the sequence's control-block reset rides in block 0 (nothing else here reads it)
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl_begin(P.ctl);
+    if (sizeof(T) == 1 && blockIdx.x == 0)
+        for (uint32_t v = threadIdx.x; v < 256; v += blockDim.x) {
+            uint32_t li = P.leaf_map[v];
+            P.ctl->leaf_ids8[v] = li ? make_leaf(li, P.gens[li]) : kEmpty;
+        }
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (uint64_t)nprep * blockDim.x) {
         {
             ChunkDev& C = chunks[c];
@@ -482,10 +523,9 @@ __device__ inline void emit_candidate(const Cands& K, uint32_t* s_cnt, bool pred
     uint32_t pos = wave_append(s_cnt, pred);
     if (pred) {
         uint64_t k = (uint64_t)blk * TILE + pos;
-        uint64_t* dst = K.key + 9ull * k;
 #pragma unroll
-        for (int i = 0; i < 8; i++) dst[i] = ch[i];
-        dst[8] = tm;
+        for (int i = 0; i < 8; i++) key_word(K, k, i) = ch[i];
+        key_word(K, k, 8) = tm;
         K.hash[k] = branch_hash(ch, tm);
         K.slot[k] = slot;
     }
@@ -622,14 +662,14 @@ template <typename T, int TILE>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
                                                        uint32_t* deferred, Drift* drift,
-                                                       const uint64_t* old2, const uint64_t* off2) {
+                                                       const uint64_t* old2) {
     __shared__ uint32_t s_cnt;
     __shared__ uint32_t s_st[8];
     // u8: the leaf id of every value, from LDS (the value map does not change here)
     __shared__ uint64_t s_leaf[sizeof(T) == 1 ? 256 : 1];
-    uint32_t lm[sizeof(T) == 1 ? (256 + TILE - 1) / TILE : 1];
+    uint64_t lm[sizeof(T) == 1 ? (256 + TILE - 1) / TILE : 1];
     if (sizeof(T) == 1)
-        for (uint32_t i = 0, v = threadIdx.x; v < 256; i++, v += TILE) lm[i] = P.leaf_map[v];
+        for (uint32_t i = 0, v = threadIdx.x; v < 256; i++, v += TILE) lm[i] = P.ctl->leaf_ids8[v];
     // This lane's brick: chunk fields, mask, values and the previous tree's node at
     // height 2 above it, in two rounds of independent loads without branches (lanes
     // past the end load the last slot), issued before the block's barrier so their
@@ -644,20 +684,20 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     const uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
     const uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
     const ChunkDev* Cp = chunks + c;
-    const uint8_t* cmask = Cp->masks;
-    const T* cval = (const T*)Cp->values;
-    const int32_t cstatus = Cp->status;
-    const bool cfill = Cp->has_fill != 0;
-    const uint64_t croot = Cp->root_in;
-    const uint64_t o2 = off2[c];  // a depth-1 tree has no height-2 slot: not used then
+    const ChunkHead H = chunk_head(chunks, c);
+    const uint8_t* cmask = H.masks;
+    const T* cval = (const T*)H.values;
+    const int32_t cstatus = H.status;
+    const bool cfill = H.has_fill;
+    const uint64_t croot = H.root_in;
     uint32_t m = cmask[2ull * p];
     uint32_t v[8];
     load8<T>(cval + 8ull * p, v);
-    const uint64_t up = old2[D >= 2 ? o2 + (p >> 3) : 0];  // unused for a tree with a fill
+    // the height-2 slot above this brick follows from the slot (a depth-1 tree has none)
+    const uint64_t up = old2[D >= 2 ? G.next + (local >> 3) : 0];  // unused for a tree with a fill
     if (!live || cstatus < 0) m = 0;
     if (sizeof(T) == 1)
-        for (uint32_t i = 0, vv = threadIdx.x; vv < 256; i++, vv += TILE)
-            s_leaf[vv] = lm[i] ? make_leaf(lm[i], P.gens[lm[i]]) : kEmpty;
+        for (uint32_t i = 0, vv = threadIdx.x; vv < 256; i++, vv += TILE) s_leaf[vv] = lm[i];
     const uint64_t* leaf_ids = sizeof(T) == 1 ? s_leaf : nullptr;
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_cnt = 0;
@@ -766,9 +806,8 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
                     uint64_t tile = slot / TILE;
                     uint32_t pos = atomicAdd(&tile_cnt[tile], 1u);
                     uint64_t k = tile * TILE + pos;
-                    uint64_t* dst = K.key + 9ull * k;
-                    for (int j = 0; j < 8; j++) dst[j] = ch[j];
-                    dst[8] = tm;
+                    for (int j = 0; j < 8; j++) key_word(K, k, j) = ch[j];
+                    key_word(K, k, 8) = tm;
                     K.hash[k] = branch_hash(ch, tm);
                     K.slot[k] = slot;
                     atomicAdd((unsigned long long*)stat(P.ctl, ST_CAND), 1ull);
@@ -806,10 +845,9 @@ __device__ inline bool key_matches_node(const Pool<T>& P, uint32_t idx, const ui
 }
 
 __device__ inline bool key_matches_cand(const Cands& K, uint32_t k2, const uint64_t key[9]) {
-    const uint64_t* c = K.key + 9ull * k2;
 #pragma unroll
     for (int i = 0; i < 9; i++)
-        if (c[i] != key[i]) return false;
+        if (key_word(K, k2, i) != key[i]) return false;
     return true;
 }
 
@@ -967,7 +1005,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
             k = (uint64_t)(t0 + lo) * TILE + (jj - s_pre[lo]);
             uint64_t key[9];
 #pragma unroll
-            for (int i = 0; i < 9; i++) key[i] = K.key[9ull * k + i];
+            for (int i = 0; i < 9; i++) key[i] = key_word(K, k, i);
             key8 = key[8];
             const uint64_t h = K.hash[k];
             slot = K.slot[k];
@@ -1067,7 +1105,7 @@ __device__ inline void commit_body(const Pool<T>& P, const Cands& K, const uint3
         uint32_t idx = bid_index(id);
         uint64_t key[9];
 #pragma unroll
-        for (int i = 0; i < 9; i++) key[i] = K.key[9ull * k + i];
+        for (int i = 0; i < 9; i++) key[i] = key_word(K, k, i);
         uint32_t vals[8];
         uint64_t* dst = P.children + 8ull * idx;
 #pragma unroll
@@ -1108,7 +1146,7 @@ __device__ inline uint64_t resolve(uint64_t r, const uint64_t* cand_id_prev) {
 // fill them with the split leaf), collapse 8 equal leaves, else emit a key.
 template <typename T, int TILE>
 __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks, const LevelInfo& L,
-                                        const uint64_t* slot_off_prev, const uint64_t* res_prev,
+                                        const uint64_t* res_prev,
                                         const uint64_t* cand_id_prev, uint64_t* res, const Cands& K,
                                         uint32_t* tile_cnt, uint32_t blk, const uint64_t* old_h) {
     __shared__ uint32_t s_cnt;
@@ -1135,7 +1173,6 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
         const bool cpatch = Cp->has_patches != 0;
         const bool cfill = Cp->has_fill != 0;
         const uint64_t cinit = Cp->init;
-        const uint64_t soff = slot_off_prev[c];
         const uint64_t recorded = old_h[sl];
         // equivalent node (voxtree.rs:930-952): the descent pass's node at this slot;
         // a tree with a fill descends from its fill leaf
@@ -1147,7 +1184,7 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
         }
         uint64_t ch[8], ec[8];
         {
-            const uint64_t* src = res_prev + soff + 8ull * q;
+            const uint64_t* src = res_prev + G.prev + (local << 3);  // the 8 child slots
             const bool eqb = eq != kEmpty && !bid_is_leaf(eq) && bid_index(eq) < P.capacity;
             const uint64_t* ecp = P.children + 8ull * (eqb ? bid_index(eq) : 0u);
 #pragma unroll
@@ -1256,12 +1293,12 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
 // the probe of h follows this launch.
 template <typename T, int TILE>
 __global__ __launch_bounds__(TILE) void k_level_build(
-    Pool<T> P, const ChunkDev* chunks, LevelInfo L, const uint64_t* slot_off_prev,
+    Pool<T> P, const ChunkDev* chunks, LevelInfo L,
     const uint64_t* res_prev, const uint64_t* cand_id_prev, uint64_t* res, Cands K,
     uint32_t* tile_cnt, uint32_t nbuild, Cands Kc, const uint32_t* tile_cnt_c, uint64_t ntiles_c,
     const uint64_t* old_h) {
     if (blockIdx.x < nbuild)
-        level_build_body<T, TILE>(P, chunks, L, slot_off_prev, res_prev, cand_id_prev, res, K, tile_cnt,
+        level_build_body<T, TILE>(P, chunks, L, res_prev, cand_id_prev, res, K, tile_cnt,
                                   blockIdx.x, old_h);
     else
         commit_body<T, TILE>(P, Kc, tile_cnt_c, ntiles_c, blockIdx.x - nbuild, gridDim.x - nbuild);
@@ -1300,7 +1337,7 @@ __device__ inline void rollback_body(const Pool<T>& P, const ChunkDev* chunks, c
         if (P.dry || id == kInvalid) continue;
         owned++;
         for (int i = 0; i < 8; i++) {
-            uint64_t c = K.key[9ull * k + i];
+            uint64_t c = key_word(K, k, i);
             if (c != kEmpty) atomicSub(&P.refs[bid_index(c)], 1u);
         }
     }
