@@ -213,6 +213,21 @@ int vx_trees_get_on(vx_interner* h, void* hip_stream, size_t n_trees, const uint
                     const uint8_t* depths, size_t nq, const uint32_t* chunk_of_query,
                     const int32_t* xyz, void* out_values, uint8_t* out_present, uint8_t* out_stop);
 
+/* ---- single-voxel write: VoxOpsWrite::set (voxtree.rs:162-260) ---- */
+
+/* set(xyz[i], values[i]) on tree i for i in [0, n), all trees in one launch
+ * sequence (a tree appears once per call; several sets on one tree take one call
+ * each, in order).  A non-default value follows set_at_depth_iterative
+ * (voxtree.rs:393-561), the default value remove_at_depth (voxtree.rs:563-705):
+ * leaves are split along the path, 8 equal leaves collapse, emptied branches are
+ * removed.  depths: host, n (1..7); roots, xyz (3n int32), values (n of the value
+ * type) and status: device.  status[i]: 1 = changed (set returned true), 0 =
+ * unchanged (false), < 0 = the reference's panic as a vx_status (VX_E_RANGE for a
+ * coordinate outside the tree; VX_E_OOM rolls the whole call back).
+ * Asynchronous. */
+int vx_trees_set(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* depths, const int32_t* xyz,
+                 const void* values, int32_t* status);
+
 /* ---- whole-tree ops: VoxOpsBulkWrite (voxtree.rs:263-292) ---- */
 
 /* fill(value) for value != 0, clear() for value == 0, on trees[i] (device

@@ -352,6 +352,16 @@ class Forest:
             return self.status.cpu().numpy()
         return None
 
+    def set_voxels(self, xyz_dev, values_dev, sync=True):
+        """vx_trees_set: VoxOpsWrite::set of one voxel per tree (tree t: xyz_dev[t], values_dev[t];
+        int32 [n,3] and value dtype [n] on the device); returns the per-tree status."""
+        check(lib().vx_trees_set(self.it.h, self.n, self.roots.data_ptr(), self.depths_host.ctypes.data,
+                                 xyz_dev.data_ptr(), values_dev.data_ptr(), self.status.data_ptr()), "set")
+        if sync:
+            self.it.sync()
+            return self.status.cpu().numpy()
+        return None
+
     def get(self, chunk_of, xyz, sync=True):
         nq = len(chunk_of)
         dev = self.it.device

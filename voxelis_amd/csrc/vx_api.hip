@@ -86,12 +86,14 @@ struct LevelBufs {
 enum KernelId : uint32_t {
     KID_BEGIN = 0, KID_PREPARE, KID_BRICK_SCAN, KID_LEAF_ALLOC, KID_BRICK_BUILD, KID_PROBE_BRICKS,
     KID_COMMIT_BRICKS, KID_LEVEL_BUILD, KID_PROBE_UPPER, KID_COMMIT_UPPER, KID_FINALIZE,
-    KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_ROLLBACK, KID_TO_VEC, KID_COUNT
+    KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_ROLLBACK, KID_TO_VEC,
+    KID_SET_DESCEND, KID_SET_LEVEL, KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_begin", "k_prepare", "k_brick_retry", "k_leaf_alloc", "k_brick_build", "k_probe[bricks]",
     "k_commit[bricks]", "k_level_build", "k_probe[upper]", "k_commit[upper]", "k_finish",
-    "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill", "k_rollback", "k_to_vec"};
+    "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill", "k_rollback", "k_to_vec",
+    "k_set_descend", "k_set_level"};
 
 struct Prof {
     bool on = false;
@@ -141,6 +143,7 @@ struct vx_interner {
     int poisoned = 0;
     // workspace
     Grow chunks, slot_off, ptr_tables, gc_front[2], stage, scratch, deferred, drift, undo;
+    Grow set_rec, set_meta;  // single-voxel set: the recorded path of every tree
     // edit-list path: zero-kept dense Batch buffers, sort buffers, tree records
     Grow edit_ws, edit_keys[2], edit_order[2], edit_tmp, edit_trees, edit_first, edit_status;
     LevelBufs lv[kLevels + 1];
@@ -270,6 +273,9 @@ struct ApplyOpts {
     bool dry = false;
     uint32_t alloc_limit = 0xFFFFFFFFu;
     uint32_t* counts = nullptr;  // device: new branches, new leaves, then per tree
+    // single-voxel set mode (vx_trees_set): one voxel per tree, device xyz[3n] and values[n]
+    const int32_t* set_xyz = nullptr;
+    const void* set_values = nullptr;
 };
 
 template <typename T, int TILE>
@@ -280,10 +286,12 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     P.dry = opt.dry ? 1u : 0u;
     P.alloc_limit = opt.alloc_limit;
     P.per_tree = opt.counts ? opt.counts + 2 : nullptr;
+    // single-voxel set mode: one slot per tree at every height (the voxel's path)
+    const bool setm = opt.set_xyz != nullptr;
     // ---- a repeated call replays its captured launch sequence
     Plan* plan = nullptr;
     uint64_t key = 0;
-    if (h->graphs && !h->prof.on && !pre_status) {
+    if (h->graphs && !h->prof.on && !pre_status && !setm) {
         key = mix64(0x5650u ^ (uint64_t)n ^ ((uint64_t)sizeof(T) << 40));
         const uint64_t head[] = {(uint64_t)roots, (uint64_t)status, (uint64_t)opt.dry,
                                  (uint64_t)opt.alloc_limit, (uint64_t)opt.counts, (uint64_t)TILE,
@@ -316,6 +324,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     // spaces per height
     auto status_of = [&](const vx_batch_view& b) -> int32_t {
         if (b.depth > kMaxDepth) return VX_E_DEPTH;
+        if (setm) return b.depth >= 1 ? 0 : VX_E_DEPTH;
         // voxtree.rs:792/913: max_depth - 1 underflows for a patched batch of depth < 2
         if (b.has_patches && b.depth < 2) return VX_E_DEPTH;
         if (b.has_patches && (!b.masks || !b.values)) return VX_E_INVALID_ARG;
@@ -377,8 +386,8 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
             const Run& r = lruns[hh][g];
             L[hh].slot_base[g] = base;
             L[hh].chunk_base[g] = r.start;
-            L[hh].depth[g] = r.depth;
-            uint64_t per = slots_at(r.depth, hh);
+            L[hh].depth[g] = setm ? hh : r.depth;  // set mode: slot = tree, shift 0
+            uint64_t per = setm ? 1 : slots_at(r.depth, hh);
             for (uint32_t k = 0; k < r.count; k++) hoff[(size_t)hh * n + r.start + k] = base + k * per;
             base += per * r.count;
         }
@@ -459,7 +468,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     uint64_t undo_cap = n;
     LevelTiles LT;
     std::memset(&LT, 0, sizeof(LT));
-    LT.dmax = dmax >= 2 ? dmax : 0;
+    LT.dmax = (dmax >= 2 || (setm && dmax >= 1)) ? dmax : 0;
     for (uint32_t hh = 1; hh <= LT.dmax; hh++) {
         undo_cap += L[hh].nslots;
         LT.cnt[hh] = tiles_h[hh];
@@ -472,7 +481,46 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     P.undo = (Drift*)h->undo.ptr;
     P.undo_cap = (uint32_t)undo_cap;
     // ---- launch sequence
+    // set mode: the recorded paths
+    uint64_t* drec = nullptr;
+    uint32_t* dmeta = nullptr;
+    if (setm) {
+        if (!h->set_rec.ensure(std::max<size_t>(n, 1) * kSetRec * 8) || !h->set_meta.ensure(std::max<size_t>(n, 1) * 4))
+            return VX_E_HIP;
+        drec = (uint64_t*)h->set_rec.ptr;
+        dmeta = (uint32_t*)h->set_meta.ptr;
+    }
+    auto launch_set = [&]() -> int {
+    const int32_t* sx = opt.set_xyz;
+    const T* sv = (const T*)opt.set_values;
+    VX_LAUNCH(h, KID_SET_DESCEND, k_set_descend<T>, grid_for(n), kBlock, P, dchunks, (uint32_t)n, roots, sx, sv,
+              drec, dmeta);
+    VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc<T>, 1, 1024, P, dchunks, (uint32_t)n, nvalues);
+    const unsigned wpb = TILE / 64;
+    for (uint32_t hh = 1; hh <= dmax; hh++) {
+        unsigned th = (unsigned)ntiles[hh];
+        // the commit of height hh-1 rides along in extra blocks (none in a dry run)
+        unsigned tc = (opt.dry || hh == 1) ? 0u : std::min<unsigned>((unsigned)ntiles[hh - 1], h->tune.commit_grid);
+        VX_LAUNCH(h, KID_SET_LEVEL, k_set_level<T VX_COMMA TILE>, th + tc, TILE, P, dchunks, L[hh], doff,
+                  (uint32_t)n, sx, sv, drec, dmeta, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh], K[hh],
+                  tiles_h[hh], th, K[hh - 1], tiles_h[hh - 1], (uint64_t)ntiles[hh - 1]);
+        uint32_t gh = probe_group(th, h->tune.probe_blocks);
+        VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T VX_COMMA TILE>, (th + gh - 1) / gh, TILE, P, K[hh], tiles_h[hh],
+                  res_ptrs_h[hh], dchunks, L[hh], th, gh);
+    }
+    (void)wpb;
+    if (!opt.dry && dmax >= 1) {
+        unsigned td = (unsigned)ntiles[dmax];
+        VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T VX_COMMA TILE>, std::min<unsigned>(td, h->tune.commit_grid),
+                  TILE, P, K[dmax], tiles_h[dmax], (uint64_t)td);
+    }
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+    };
     auto launch_all = [&]() -> int {
+    if (setm) {
+        int rc = launch_set();
+        if (rc) return rc;
+    } else {
     // chunk set-up, plus the old-tree descent over the level-3 slots in extra blocks
     unsigned np = grid_for(n);
     unsigned nd = L[3].nslots ? grid_for(L[3].nslots) : 0u;
@@ -505,6 +553,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
             VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T VX_COMMA TILE>, std::min<unsigned>(td, h->tune.commit_grid),
                       TILE, P, K[dmax], tiles_h[dmax], (uint64_t)td);
         }
+    }
     }
     uint32_t* gf0 = (uint32_t*)h->gc_front[0].ptr;
     uint32_t* gf1 = (uint32_t*)h->gc_front[1].ptr;
@@ -968,6 +1017,19 @@ int vx_trees_apply_edits(vx_interner* h, size_t n, uint64_t* roots, const uint8_
 }
 
 namespace {
+int vx_trees_set(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* depths, const int32_t* xyz,
+                 const void* values, int32_t* status) {
+    if (!h || (n && (!roots || !depths || !xyz || !values || !status))) return VX_E_INVALID_ARG;
+    if (n == 0) return VX_OK;
+    std::vector<vx_batch_view> views(n);
+    for (size_t i = 0; i < n; i++) views[i] = vx_batch_view{nullptr, nullptr, 0, depths[i], 0, 1, 0};
+    ApplyOpts o;
+    o.set_xyz = xyz;
+    o.set_values = values;
+    return h->vsize == 1 ? apply_impl<uint8_t>(h, n, roots, views.data(), status, nullptr, o)
+                         : apply_impl<uint16_t>(h, n, roots, views.data(), status, nullptr, o);
+}
+
 int opts_of(const vx_interner* h, const vx_apply_opts* o, ApplyOpts* out) {
     *out = ApplyOpts();
     if (!o) return VX_OK;

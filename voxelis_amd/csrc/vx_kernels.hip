@@ -1588,6 +1588,231 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
     }
 }
 
+// ------------------------------------------------------------------ single-voxel set
+
+// VoxOpsWrite::set (voxtree.rs:162-260) on many trees at once, one voxel per tree:
+// set_at_depth_iterative (voxtree.rs:393-561) for a non-default value,
+// remove_at_depth (voxtree.rs:563-705) for the default one.  The descent records
+// the path exactly as those functions see it; the new nodes are then built one
+// height per launch (k_set_level) and interned by the shared probe and commit, so
+// trees of one call share nodes like any hash-consed apply.  The reference-count
+// result equals the reference's: its increments and decrements along one path
+// cancel except for +1 per non-empty child of each new node and +1 on the new root
+// (no count passes through zero inside a set: a hit never contains a node created
+// lower in the same path).
+
+__device__ inline uint32_t child_slot(int32_t x, int32_t y, int32_t z, uint32_t d, uint32_t D) {
+    const uint32_t sh = D - d - 1;
+    return (((uint32_t)x >> sh) & 1u) | ((((uint32_t)y >> sh) & 1u) << 1) | ((((uint32_t)z >> sh) & 1u) << 2);
+}
+
+// Per tree: rec[c][0][d] = the branch at depth d on the path (empty when there is
+// none), rec[c][1][d] = the leaf being split above depth d; meta[c] = depth at which
+// a remove meets a leaf (the depth itself when the voxel's own leaf is removed).
+constexpr uint32_t kSetRec = 2 * kLevels;
+
+template <typename T>
+__global__ void k_set_descend(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_t* roots, const int32_t* xyz,
+                              const T* vals, uint64_t* rec, uint32_t* meta) {
+    // the sequence's control-block reset rides in block 0 (nothing else here reads it)
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl_begin(P.ctl);
+    VX_LOOP(c, n) {
+        if (c < n) {
+            ChunkDev& C = chunks[c];
+            C.root_in = roots[C.orig];
+            C.init = C.root_in;
+            C.status = C.status0;
+            if (C.status < 0) continue;
+            const uint32_t D = C.depth;
+            const int32_t x = xyz[3ull * C.orig], y = xyz[3ull * C.orig + 1], z = xyz[3ull * C.orig + 2];
+            const uint32_t v = (uint32_t)vals[C.orig];
+            const int32_t lim = 1 << D;
+            // assert!(position in 0..2^D) (voxtree.rs:164-166)
+            if (x < 0 || y < 0 || z < 0 || x >= lim || y >= lim || z >= lim) {
+                C.status = -4;
+                continue;
+            }
+            uint64_t* rn = rec + (uint64_t)c * kSetRec;
+            uint64_t* rl = rn + kLevels;
+            uint64_t cur = C.root_in;
+            bool unchanged = cur == kEmpty && v == 0;  // an empty tree and the default value
+            if (!unchanged && v != 0) {
+                // voxtree.rs:411-470: branches (empty included) are followed, a leaf is
+                // split from there down; the same value already present changes nothing
+                uint64_t leaf = kEmpty;
+                for (uint32_t d = 0; d < D && !unchanged; d++) {
+                    if (bid_index(cur) >= P.capacity) {  // a corrupt id: poisoned, read as empty
+                        atomicOr(&P.ctl->err, kErrPoison);
+                        cur = kEmpty;
+                    }
+                    if (!bid_is_leaf(cur)) {
+                        rn[d] = cur;
+                        rl[d] = leaf;
+                        if (cur != kEmpty) cur = P.children[8ull * bid_index(cur) + child_slot(x, y, z, d, D)];
+                    } else {
+                        unchanged = (uint32_t)P.values[bid_index(cur)] == v;
+                        leaf = cur;
+                        cur = kEmpty;
+                        rn[d] = kEmpty;
+                        rl[d] = leaf;
+                    }
+                }
+                if (bid_index(cur) >= P.capacity) {
+                    atomicOr(&P.ctl->err, kErrPoison);
+                    cur = kEmpty;
+                }
+                if (!unchanged && bid_is_leaf(cur) && (uint32_t)P.values[bid_index(cur)] == v) unchanged = true;
+                if (!unchanged && P.leaf_map[v] == 0) P.leaf_need[v] = 1;
+            } else if (!unchanged) {
+                // voxtree.rs:581-660: present children are followed down to a leaf (split
+                // from there, or the voxel itself at depth D); a missing child changes nothing
+                uint32_t dl = D;
+                for (uint32_t d = 0; d < D; d++) {
+                    if (bid_is_leaf(cur)) {
+                        dl = d;
+                        break;
+                    }
+                    const uint32_t ci = child_slot(x, y, z, d, D);
+                    if (!((bid_mask(cur) >> ci) & 1u)) {
+                        unchanged = true;
+                        break;
+                    }
+                    if (bid_index(cur) >= P.capacity) {
+                        atomicOr(&P.ctl->err, kErrPoison);
+                        unchanged = true;
+                        break;
+                    }
+                    rn[d] = cur;
+                    cur = P.children[8ull * bid_index(cur) + ci];
+                }
+                for (uint32_t d = dl; d < D; d++) rl[d] = cur;  // the leaf being split
+                meta[c] = dl;
+            }
+            if (unchanged) C.skip = 1;
+        }
+    }
+}
+
+// One height of the set / remove paths (one slot per tree): the node at depth
+// D - h from the result one height below, as voxtree.rs:476-555 (set) and
+// voxtree.rs:597-705 (remove) build it.  Blocks nbuild.. commit height h-1.
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_set_level(
+    Pool<T> P, const ChunkDev* chunks, LevelInfo L, const uint64_t* doff, uint32_t n, const int32_t* xyz,
+    const T* vals, const uint64_t* rec, const uint32_t* meta, const uint64_t* res_prev,
+    const uint64_t* cand_id_prev, uint64_t* res, Cands K, uint32_t* tile_cnt, uint32_t nbuild, Cands Kc,
+    const uint32_t* tile_cnt_c, uint64_t ntiles_c) {
+    if (blockIdx.x >= nbuild) {
+        commit_body<T, TILE>(P, Kc, tile_cnt_c, ntiles_c, blockIdx.x - nbuild, gridDim.x - nbuild);
+        return;
+    }
+    __shared__ uint32_t s_cnt;
+    __shared__ uint32_t s_st[8];
+    if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    const uint64_t slot = (uint64_t)blockIdx.x * TILE + threadIdx.x;
+    const bool valid = slot < L.nslots;
+    uint64_t out = kInvalid;
+    bool cand = false, collapsed = false;
+    uint64_t ch[8];
+    uint32_t tm = 0;
+    if (valid) {
+        const GroupPos G = group_pos(L, slot);
+        const uint32_t c = G.chunk_base + (uint32_t)(slot - G.base);  // one slot per tree
+        const ChunkDev& C = chunks[c];
+        if (C.status >= 0 && !C.skip) {
+            const uint32_t D = C.depth;
+            const uint32_t d = D - L.h;
+            const int32_t x = xyz[3ull * C.orig], y = xyz[3ull * C.orig + 1], z = xyz[3ull * C.orig + 2];
+            const uint32_t v = (uint32_t)vals[C.orig];
+            const uint32_t ci = child_slot(x, y, z, d, D), bit = 1u << ci;
+            const uint64_t* rn = rec + (uint64_t)c * kSetRec;
+            const uint64_t* rl = rn + kLevels;
+            // the result one height below: the voxel's leaf (or empty) at the bottom
+            uint64_t cur;
+            if (L.h == 1) {
+                if (v != 0) {
+                    // creation failed: the apply aborts; a dry run counts a placeholder
+                    uint32_t li = P.leaf_map[v];
+                    cur = li ? make_leaf(li, P.gens[li]) : (P.dry ? make_leaf(P.capacity + v, 0) : kInvalid);
+                } else {
+                    cur = kEmpty;
+                }
+            } else {
+                uint64_t r = res_prev[doff[(uint64_t)(L.h - 1) * n + c]];
+                cur = res_is_pending(r) ? cand_id_prev[(uint32_t)r] : r;
+            }
+            if (cur != kInvalid && v != 0) {
+                const uint64_t parent = rn[d], leaf = rl[d];
+                if (parent != kEmpty) {
+                    // the parent's children with one replaced; 8 equal leaves collapse
+                    uint32_t types = bid_types(parent) | ((bid_is_leaf(cur) ? 1u : 0u) << ci);
+                    bool all = true;
+#pragma unroll
+                    for (int i = 0; i < 8; i++) {
+                        ch[i] = i == (int)ci ? cur : P.children[8ull * bid_index(parent) + i];
+                        all = all && ch[i] == cur;
+                    }
+                    if (types == 0xFFu && all) {
+                        out = cur;
+                        collapsed = true;
+                    } else {
+                        tm = (types << 8) | ((bid_mask(parent) | bit) & 0xFFu);
+                        cand = true;
+                    }
+                } else if (leaf == kEmpty) {
+#pragma unroll
+                    for (int i = 0; i < 8; i++) ch[i] = i == (int)ci ? cur : kEmpty;
+                    tm = (((bid_is_leaf(cur) ? 1u : 0u) << ci) << 8) | bit;
+                    cand = true;
+                } else {
+                    // a split leaf: its 7 other slots keep it (inc_ref_by(leaf, 7))
+#pragma unroll
+                    for (int i = 0; i < 8; i++) ch[i] = i == (int)ci ? cur : leaf;
+                    const uint32_t types = (~bit | ((bid_is_leaf(cur) ? 1u : 0u) << ci)) & 0xFFu;
+                    tm = (types << 8) | 0xFFu;
+                    cand = true;
+                }
+            } else if (cur != kInvalid) {
+                if (d >= meta[c]) {
+                    // remove_at_depth_leaf: the split leaf in 7 slots, the result in one
+                    const uint64_t N = rl[d];
+#pragma unroll
+                    for (int i = 0; i < 8; i++) ch[i] = i == (int)ci ? cur : N;
+                    const uint32_t types = (~bit | ((bid_is_leaf(cur) ? 1u : 0u) << ci)) & 0xFFu;
+                    const uint32_t mask = (~bit | ((cur != kEmpty ? 1u : 0u) << ci)) & 0xFFu;
+                    tm = (types << 8) | mask;
+                    cand = true;
+                } else {
+                    // remove_at_depth_branch: an emptied child leaves the mask; no
+                    // child left removes the branch
+                    const uint64_t node = rn[d];
+                    const uint32_t mask = cur == kEmpty ? (bid_mask(node) & ~bit) : bid_mask(node);
+                    if (mask == 0) {
+                        out = kEmpty;
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 8; i++) ch[i] = i == (int)ci ? cur : P.children[8ull * bid_index(node) + i];
+                        tm = ((bid_types(node) & ~bit) << 8) | mask;
+                        cand = true;
+                    }
+                }
+            }
+            if (cand) out = res_pending(0xFFFFFFFFu);  // replaced by the probe
+        }
+    }
+    if (valid) res[slot] = out;
+    emit_candidate<TILE>(K, &s_cnt, cand, ch, tm, slot, blockIdx.x);
+    tile_count(s_st, 0, cand);
+    tile_count(s_st, 1, collapsed);
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_cnt;
+    const int ids[2] = {ST_CAND, ST_COLLAPSE};
+    if (threadIdx.x < 2 && s_st[threadIdx.x])
+        atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
+}
+
 // ------------------------------------------------------------------ fill / clear
 
 // VoxOpsBulkWrite::fill/clear (voxtree.rs:263-292): queue the old root for
