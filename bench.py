@@ -50,9 +50,11 @@ def parse():
     ap.add_argument("--requests", type=int, default=1500, help="serve: requests per rate")
     ap.add_argument("--train-requests", type=int, default=1500,
                     help="serve: requests logged (with exact dry-run counts) to train the predictor")
-    ap.add_argument("--config", default="terrain", choices=["terrain", "plain", "uneven", "gets", "serve", "calib", "tune"],
+    ap.add_argument("--config", default="terrain",
+                    choices=["terrain", "plain", "uneven", "gets", "set", "serve", "calib", "tune"],
                     help="terrain: the headline (64x64 Perlin-dune trees); plain: one 32^3 tree; "
-                         "uneven: u16 trees of depth 5/6/7, dense or sparse; gets: batched reads")
+                         "uneven: u16 trees of depth 5/6/7, dense or sparse; gets: batched reads; "
+                         "set: single-voxel set/remove, one voxel per tree per call")
     ap.add_argument("--trees", type=int, default=4096)
     ap.add_argument("--queries", type=int, default=1 << 24)
     ap.add_argument("--calib-copy", action="store_true",
@@ -113,6 +115,8 @@ def main():
         return bench_uneven(a)
     if a.config == "gets":
         return bench_gets(a)
+    if a.config == "set":
+        return bench_set(a)
     if a.config == "serve":
         return bench_serve(a)
     if a.config == "calib":
@@ -592,6 +596,75 @@ def bench_gets(a):
                      "kernel_gets_per_s": Q / (ms / n / 1e3)}
     print(json.dumps({"metric": "batched get/s", "config": {"workload": f"{Q} uniform random voxel reads "
                       f"over the 64x64 terrain world (4096 trees of 32^3)"}, "results": res}))
+
+
+def bench_set(a):
+    """Single-voxel VoxOpsWrite::set over the terrain world: every call sets (or, one time
+    in four, removes) one uniform random voxel in each of the 4096 trees, values 1..3."""
+    import torch
+    from voxelis_amd import api
+    from voxelis_amd import workloads as W
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    grid = a.grid
+    T = grid * grid
+    mA, vA, edits, _ = W.terrain_batches(grid=grid, seed=a.seed)
+    it = api.VoxInterner(1, a.budget_mib << 20, a.load_factor)
+    f = api.Forest(it, [5] * T)
+    st = f.apply([api.to_device(mA[i]) for i in range(T)], [api.to_device(vA[i]) for i in range(T)])
+    assert (st == 1).all()
+    rng = np.random.default_rng(a.seed)
+    K = a.warmup + a.steps
+    xyz = rng.integers(0, 32, size=(K, T, 3)).astype(np.int32)
+    vals = rng.integers(1, 4, size=(K, T)).astype(np.uint8)
+    vals[rng.random((K, T)) < 0.25] = 0
+    xyz_t = [torch.from_numpy(xyz[k].reshape(-1)).cuda() for k in range(K)]
+    val_t = [torch.from_numpy(vals[k]).cuda() for k in range(K)]
+    for k in range(a.warmup):
+        f.set_voxels(xyz_t[k], val_t[k], sync=False)
+    it.sync()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    changed = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j in range(a.steps):
+        evs[j][0].record()
+        f.set_voxels(xyz_t[a.warmup + j], val_t[a.warmup + j], sync=False)
+        evs[j][1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    it.sync()
+    lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+    changed = int((f.status.cpu().numpy() == 1).sum())
+    # the oracle (one thread) on the same trees and voxels, for about cpu_seconds
+    it_o = O.Interner(1, a.budget_mib << 20)
+    trees = []
+    for i in range(T):
+        t = O.Tree(5)
+        b = O.Batch(5)
+        b.load_dense(mA[i], vA[i])
+        t.apply_batch(it_o, b)
+        trees.append(t)
+    done, tsum = 0, 0.0
+    for k in range(K):  # one C call per round of T sets (no per-set call overhead)
+        c0 = time.perf_counter()
+        O.set_round(it_o, trees, xyz[k], vals[k])
+        tsum += time.perf_counter() - c0
+        done += T
+        if tsum >= a.cpu_seconds:
+            break
+    res = {"sets_per_s": T * a.steps / wall, "ms_per_call": 1e3 * wall / a.steps,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "max_ms": float(lat.max()), "changed_in_last_call": changed,
+           "cpu_baseline": {"value": done / tsum, "unit": "sets/s", "cores": 1, "kind": "port",
+                            "sample": f"oracle VoxOpsWrite::set, {done} sets on the same trees and voxels "
+                                      f"(rounds of {T} in one C call), {tsum:.2f} s"}}
+    print(f"[bench] set: {res['sets_per_s'] / 1e6:.1f} M sets/s, {res['ms_per_call']:.3f} ms per call of {T} "
+          f"sets (p50 {res['p50_ms']:.3f}, p99 {res['p99_ms']:.3f}); oracle {done / tsum / 1e6:.2f} M sets/s",
+          file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "single-voxel sets/s", "config": {
+        "workload": f"one uniform random voxel set (values 1..3, a quarter removes) in each of the {T} "
+                    f"terrain trees (32^3, u8) per call"}, "results": res}))
 
 
 # Published single-thread apply_batch / get rates of the reference (Apple M3 Max,

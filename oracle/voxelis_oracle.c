@@ -900,6 +900,20 @@ int vxo_tree_set(vxo_interner* o, uint64_t* root, uint8_t depth, int32_t x, int3
     return 0;
 }
 
+/* One set on each of n trees (tree i: roots[i], depths[i], voxel xyz[3i..3i+2], value
+ * values[i]), in tree order: the per-voxel CPU baseline without a call per set.
+ * status[i]: 1 changed, 0 unchanged, < 0 the panic's code (that tree is left as it was
+ * at the panic; the others go on). */
+int vxo_trees_set_round(vxo_interner* o, size_t n, uint64_t* roots, const uint8_t* depths, const int32_t* xyz,
+                        const uint64_t* values, int* status) {
+    for (size_t i = 0; i < n; i++) {
+        int ch = 0;
+        int rc = vxo_tree_set(o, &roots[i], depths[i], xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], values[i], &ch);
+        status[i] = rc < 0 ? rc : ch;
+    }
+    return 0;
+}
+
 /* voxtree.rs:263-292 */
 int vxo_tree_clear(vxo_interner* o, uint64_t* root) {
     API_TRY

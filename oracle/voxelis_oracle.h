@@ -126,6 +126,8 @@ int vxo_tree_get_many(const vxo_interner* o, uint64_t root, uint8_t depth, size_
 int vxo_tree_set(vxo_interner* o, uint64_t* root, uint8_t depth, int32_t x, int32_t y,
                  int32_t z, uint64_t v, int* changed);
 int vxo_tree_fill(vxo_interner* o, uint64_t* root, uint8_t depth, uint64_t v);
+int vxo_trees_set_round(vxo_interner* o, size_t n, uint64_t* roots, const uint8_t* depths, const int32_t* xyz,
+                        const uint64_t* values, int* status);
 int vxo_tree_clear(vxo_interner* o, uint64_t* root);
 int vxo_tree_apply_batch(vxo_interner* o, uint64_t* root, uint8_t depth, const vxo_batch* b,
                          int* changed);

@@ -84,6 +84,7 @@ def lib():
         "vxo_tree_get": (C.c_int, [vp, u64, u8, i32, i32, i32, P64, C.POINTER(C.c_int)]),
         "vxo_tree_get_many": (C.c_int, [vp, u64, u8, sz, vp, vp, vp]),
         "vxo_tree_set": (C.c_int, [vp, P64, u8, i32, i32, i32, u64, C.POINTER(C.c_int)]),
+        "vxo_trees_set_round": (C.c_int, [vp, sz, vp, vp, vp, vp, vp]),
         "vxo_tree_fill": (C.c_int, [vp, P64, u8, u64]),
         "vxo_tree_clear": (C.c_int, [vp, P64]),
         "vxo_tree_apply_batch": (C.c_int, [vp, P64, u8, vp, C.POINTER(C.c_int)]),
@@ -168,6 +169,24 @@ class Batch:
         n = self.bricks()
         p = C.cast(lib().vxo_batch_values(self.h), C.POINTER(C.c_uint64 * (8 * n)))
         return np.frombuffer(p.contents, dtype=np.uint64).reshape(n, 8).copy()
+
+
+def set_round(it, trees, xyz, values):
+    """One VoxOpsWrite::set on each tree (tree i: xyz[i], values[i]) inside one C call;
+    returns the per-tree status (1 changed, 0 unchanged, < 0 the panic's code)."""
+    n = len(trees)
+    roots = np.array([t.root for t in trees], dtype=np.uint64)
+    depths = np.array([t.depth for t in trees], dtype=np.uint8)
+    xyz = np.ascontiguousarray(xyz, dtype=np.int32).reshape(-1)
+    vals = np.ascontiguousarray(values, dtype=np.uint64)
+    st = np.zeros(n, dtype=np.int32)
+    lib().vxo_trees_set_round(it.h, n, roots.ctypes.data, depths.ctypes.data, xyz.ctypes.data, vals.ctypes.data,
+                              st.ctypes.data)
+    for i, t in enumerate(trees):
+        if st[i] == 1:
+            t.dirty = True
+        t.root = int(roots[i])
+    return st
 
 
 class Interner:
