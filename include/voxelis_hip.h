@@ -245,6 +245,26 @@ int vx_trees_fill(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* val
 int vx_trees_to_vec(vx_interner* h, size_t n, const uint64_t* roots, const uint8_t* depths, uint32_t lod,
                     uint64_t out_stride, void* out);
 
+/* ---- VTM snapshot of a resident world: VoxModel::serialize / deserialize
+ * (world/voxmodel.rs:177-408, world/voxchunk.rs:381-440, io/varint.rs) ---- */
+
+/* The data section of a .vtm file (what export_model_to_vtm hashes and writes after
+ * its header, io/export.rs:90-151) for the trees roots[0..n) (device) at the chunk
+ * positions positions[3n] (host): the live patterns renumbered 1.. (leaves, then
+ * branches, each in node-index order) as varint records, then one chunk record per
+ * tree.  *size receives the byte count; out (host, cap bytes) receives the data when
+ * it is large enough (out = NULL: the size only; cap too small: VX_E_RANGE).
+ * Synchronizes. */
+int vx_model_serialize(vx_interner* h, size_t n, const uint64_t* roots, const int32_t* positions, void* out,
+                       size_t cap, size_t* size);
+/* A data section rebuilt into a fresh interner (interner/mod.rs:930-993: nodes at
+ * their file ids; reference counts = parent references + roots): the chunks' roots
+ * (device, ncap) and positions (host, 3 * ncap); *n = the chunk count (VX_E_RANGE
+ * when above ncap).  VX_E_INVALID_ARG for a malformed section or an interner that
+ * holds nodes already.  Synchronizes. */
+int vx_model_deserialize(vx_interner* h, const void* data, size_t size, size_t ncap, uint64_t* roots,
+                         int32_t* positions, size_t* n);
+
 /* ---- verification: canonical structural hash (no reference counterpart) ---- */
 
 /* 128-bit Merkle hash of each tree's DAG, independent of node ids: two trees

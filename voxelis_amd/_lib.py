@@ -68,6 +68,8 @@ SIGNATURES = {
     "vx_trees_get_on": (_int, [_vp, _vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _vp]),
     "vx_trees_fill": (_int, [_vp, _sz, _vp, _vp]),
     "vx_trees_set": (_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "vx_model_serialize": (_int, [_vp, _sz, _vp, _vp, _vp, _sz, C.POINTER(C.c_size_t)]),
+    "vx_model_deserialize": (_int, [_vp, _vp, _sz, _sz, _vp, _vp, C.POINTER(C.c_size_t)]),
     "vx_trees_canonical_hash": (_int, [_vp, _sz, _vp, _vp]),
     "vx_trees_to_vec": (_int, [_vp, _sz, _vp, _vp, _u32, _u64, _vp]),
     "vx_interner_profile": (_int, [_vp, _int]),

@@ -362,6 +362,36 @@ class Forest:
             return self.status.cpu().numpy()
         return None
 
+    def serialize(self, positions):
+        """vx_model_serialize: the VTM data section of these trees at chunk positions
+        (int32 [n, 3]), as bytes."""
+        pos = np.ascontiguousarray(positions, dtype=np.int32).reshape(-1)
+        assert len(pos) == 3 * self.n
+        size = C.c_size_t(0)
+        check(lib().vx_model_serialize(self.it.h, self.n, self.roots.data_ptr(), pos.ctypes.data, None, 0,
+                                       C.byref(size)), "serialize")
+        buf = (C.c_uint8 * max(size.value, 1))()
+        check(lib().vx_model_serialize(self.it.h, self.n, self.roots.data_ptr(), pos.ctypes.data, buf, size.value,
+                                       C.byref(size)), "serialize")
+        return bytes(buf[:size.value])
+
+    @classmethod
+    def deserialize(cls, interner, data, depth):
+        """vx_model_deserialize into a fresh interner: (forest of depth `depth`, positions [n, 3])."""
+        raw = (C.c_uint8 * max(len(data), 1)).from_buffer_copy(data if data else b"\0")
+        n = C.c_size_t(0)
+        # a first call learns the chunk count (it builds nothing unless there are no chunks)
+        rc = lib().vx_model_deserialize(interner.h, raw, len(data), 0, None, None, C.byref(n))
+        if rc == 0:
+            return cls(interner, []), np.zeros((0, 3), dtype=np.int32)
+        if rc != -4:
+            check(rc, "deserialize")
+        f = cls(interner, [depth] * n.value)
+        pos = np.zeros(3 * max(n.value, 1), dtype=np.int32)
+        check(lib().vx_model_deserialize(interner.h, raw, len(data), n.value, f.roots.data_ptr(), pos.ctypes.data,
+                                         C.byref(n)), "deserialize")
+        return f, pos[:3 * n.value].reshape(-1, 3)
+
     def get(self, chunk_of, xyz, sync=True):
         nq = len(chunk_of)
         dev = self.it.device

@@ -1252,3 +1252,5 @@ int vx_memcpy_d2h(void* dst, const void* src, size_t bytes) {
 }
 
 }  // extern "C"
+
+#include "vx_vtm.hip"
