@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--train-requests", type=int, default=1500,
                     help="serve: requests logged (with exact dry-run counts) to train the predictor")
     ap.add_argument("--config", default="terrain",
-                    choices=["terrain", "plain", "uneven", "gets", "set", "serve", "calib", "tune"],
+                    choices=["terrain", "plain", "uneven", "gets", "set", "vtm", "serve", "calib", "tune"],
                     help="terrain: the headline (64x64 Perlin-dune trees); plain: one 32^3 tree; "
                          "uneven: u16 trees of depth 5/6/7, dense or sparse; gets: batched reads; "
                          "set: single-voxel set/remove, one voxel per tree per call")
@@ -117,6 +117,8 @@ def main():
         return bench_gets(a)
     if a.config == "set":
         return bench_set(a)
+    if a.config == "vtm":
+        return bench_vtm(a)
     if a.config == "serve":
         return bench_serve(a)
     if a.config == "calib":
@@ -665,6 +667,71 @@ def bench_set(a):
     print(json.dumps({"metric": "single-voxel sets/s", "config": {
         "workload": f"one uniform random voxel set (values 1..3, a quarter removes) in each of the {T} "
                     f"terrain trees (32^3, u8) per call"}, "results": res}))
+
+
+def bench_vtm(a):
+    """VTM snapshot (VoxModel::serialize / deserialize) of two resident worlds: the 64x64
+    terrain (u8, few distinct nodes) and 1024 u16 trees of depth 5/6 with random values
+    (almost every brick distinct).  Export = the data section on the host; import = a
+    fresh interner rebuilt from it."""
+    import torch
+    from voxelis_amd import api
+    from voxelis_amd import workloads as W
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    sys.path.insert(0, ROOT)
+    from oracle import vtm_oracle as V
+    res = {}
+    worlds = []
+    mA, vA, _, _ = W.terrain_batches(grid=a.grid, seed=a.seed)
+    worlds.append(("terrain_4096x32_u8", 1, [5] * len(mA), list(mA), list(vA)))
+    spec = W.uneven_spec(1024, depths=(5, 6), seed=a.seed)
+    ub = W.uneven_batches(spec)
+    worlds.append(("uneven_1024_u16", 2, [d for d, _, _, _ in spec], [m for m, _, _ in ub], [v for _, v, _ in ub]))
+    for name, vs, depths, ms, vals in worlds:
+        it = api.VoxInterner(vs, a.budget_mib << 20, a.load_factor)
+        order = np.argsort(depths, kind="stable")
+        # one VTM model has one depth: the deepest trees form the model
+        d = max(depths)
+        sel = [int(i) for i in order if depths[i] == d]
+        f = api.Forest(it, [d] * len(sel))
+        st = f.apply([api.to_device(ms[i]) for i in sel], [api.to_device(vals[i]) for i in sel])
+        assert (st == 1).all()
+        pos = np.array([[i % 64, 0, i // 64] for i in range(len(sel))], dtype=np.int32)
+        leaves, branches = it.counts()
+        f.serialize(pos)  # warm
+        t0 = time.perf_counter()
+        for _ in range(3):
+            data = f.serialize(pos)
+        t_exp = (time.perf_counter() - t0) / 3
+        t_imp = 0.0
+        for _ in range(3):
+            it2 = api.VoxInterner(vs, a.budget_mib << 20, a.load_factor)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            f2, _ = api.Forest.deserialize(it2, data, d)
+            t_imp += time.perf_counter() - t0
+            assert f2.n == f.n
+            del f2, it2
+        t_imp /= 3
+        nodes = leaves + branches - 1
+        # the oracle's restatement (pure Python) on the parsed section, rebuilt once
+        c0 = time.perf_counter()
+        parsed = V.deserialize(data, vs)
+        t_parse = time.perf_counter() - c0
+        res[name] = {"trees": f.n, "depth": d, "nodes": nodes, "bytes": len(data),
+                     "export_s": t_exp, "export_nodes_per_s": nodes / t_exp, "export_MB_per_s": len(data) / t_exp / 1e6,
+                     "import_s": t_imp, "import_nodes_per_s": nodes / t_imp, "import_MB_per_s": len(data) / t_imp / 1e6,
+                     "cpu_baseline": {"value": nodes / t_parse, "unit": "nodes/s parsed", "cores": 1, "kind": "port",
+                                      "sample": f"oracle VoxModel::deserialize parse (pure Python) of the same "
+                                                f"{len(data)} bytes, {t_parse:.2f} s"}}
+        assert len(parsed[0]) + len(parsed[1]) == nodes
+        print(f"[bench] vtm {name}: {nodes} nodes, {len(data) / 1e6:.1f} MB; export {1e3 * t_exp:.1f} ms "
+              f"({nodes / t_exp / 1e6:.1f} M nodes/s), import {1e3 * t_imp:.1f} ms ({nodes / t_imp / 1e6:.1f} M nodes/s); "
+              f"oracle parse {nodes / t_parse / 1e6:.2f} M nodes/s", file=sys.stderr, flush=True)
+        del f, it
+    print(json.dumps({"metric": "VTM export / import", "config": {"workload": "terrain 64x64 (u8) and 1024 "
+                      "uneven u16 trees (the deepest of depth 5/6)"}, "results": res}))
 
 
 # Published single-thread apply_batch / get rates of the reference (Apple M3 Max,

@@ -367,29 +367,34 @@ class Forest:
         (int32 [n, 3]), as bytes."""
         pos = np.ascontiguousarray(positions, dtype=np.int32).reshape(-1)
         assert len(pos) == 3 * self.n
+        # an upper bound of the section from the pattern counts (5-byte varints), so one
+        # call writes it
+        leaves, branches = self.it.counts()
+        s = self.it.value_size
+        cap = 12 + leaves * (5 + s) + branches * (5 + 1 + 40 + s) + self.n * (24 + 5)
+        buf = (C.c_uint8 * cap)()
         size = C.c_size_t(0)
-        check(lib().vx_model_serialize(self.it.h, self.n, self.roots.data_ptr(), pos.ctypes.data, None, 0,
-                                       C.byref(size)), "serialize")
-        buf = (C.c_uint8 * max(size.value, 1))()
-        check(lib().vx_model_serialize(self.it.h, self.n, self.roots.data_ptr(), pos.ctypes.data, buf, size.value,
-                                       C.byref(size)), "serialize")
-        return bytes(buf[:size.value])
+        rc = lib().vx_model_serialize(self.it.h, self.n, self.roots.data_ptr(), pos.ctypes.data, buf, cap,
+                                      C.byref(size))
+        if rc == -4:  # not expected: the bound holds for every live set
+            buf = (C.c_uint8 * size.value)()
+            rc = lib().vx_model_serialize(self.it.h, self.n, self.roots.data_ptr(), pos.ctypes.data, buf,
+                                          size.value, C.byref(size))
+        check(rc, "serialize")
+        return bytes(memoryview(buf)[:size.value])
 
     @classmethod
     def deserialize(cls, interner, data, depth):
         """vx_model_deserialize into a fresh interner: (forest of depth `depth`, positions [n, 3])."""
         raw = (C.c_uint8 * max(len(data), 1)).from_buffer_copy(data if data else b"\0")
         n = C.c_size_t(0)
-        # a first call learns the chunk count (it builds nothing unless there are no chunks)
-        rc = lib().vx_model_deserialize(interner.h, raw, len(data), 0, None, None, C.byref(n))
-        if rc == 0:
-            return cls(interner, []), np.zeros((0, 3), dtype=np.int32)
-        if rc != -4:
-            check(rc, "deserialize")
-        f = cls(interner, [depth] * n.value)
-        pos = np.zeros(3 * max(n.value, 1), dtype=np.int32)
-        check(lib().vx_model_deserialize(interner.h, raw, len(data), n.value, f.roots.data_ptr(), pos.ctypes.data,
+        ncap = len(data) // 25 + 1  # a chunk record takes at least 25 bytes
+        roots = torch.zeros(ncap, dtype=torch.int64, device=interner.device)
+        pos = np.zeros(3 * ncap, dtype=np.int32)
+        check(lib().vx_model_deserialize(interner.h, raw, len(data), ncap, roots.data_ptr(), pos.ctypes.data,
                                          C.byref(n)), "deserialize")
+        f = cls(interner, [depth] * n.value)
+        f.roots.copy_(roots[:n.value])
         return f, pos[:3 * n.value].reshape(-1, 3)
 
     def get(self, chunk_of, xyz, sync=True):
