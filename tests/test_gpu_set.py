@@ -137,3 +137,32 @@ def test_collapse_and_remove_to_empty(vx):
         assert_same(it, f, it_o, t_o, full_reads=False, what="removing")
     assert f.roots_host()[0] == 0 and t_o[0].root == 0
     assert it.counts() == it_o.counts()
+
+
+def test_set_beyond_capacity_rolls_back(vx):
+    """A set call that needs more nodes than the interner holds is rolled back as a whole
+    (VX_E_OOM on every tree, nothing changed); a smaller call then matches the oracle."""
+    torch, api, W = vx
+    n = 12
+    xyz = np.stack([np.arange(n) * 2, np.arange(n), 31 - np.arange(n)], axis=1)
+    vals = np.full(n, 7)
+    # the nodes the call creates (shared bricks and paths counted once), from the oracle
+    probe = O.Interner(1, 64 * MB)
+    for t in range(n):
+        O.Tree(5).set(probe, *(int(c) for c in xyz[t]), 7)
+    leaves, branches = probe.counts()
+    cap = 1 + leaves + (branches - 1) - 1  # the EMPTY branch, and one node short
+    it = api.VoxInterner(1, cap * (78 + 1))
+    assert it.capacity() == cap
+    it_o = O.Interner(1, cap * (78 + 1))
+    f = api.Forest(it, [5] * n)
+    trees_o = [O.Tree(5) for _ in range(n)]
+    before = (it.counts(), it.alive_nodes(), f.roots_host())
+    st = f.set_voxels(torch.from_numpy(xyz.astype(np.int32).reshape(-1)).cuda(), api.to_device(vals.astype(np.uint8)))
+    assert (st == -2).all(), st
+    assert (it.counts(), it.alive_nodes(), f.roots_host()) == before
+    # the first half fits
+    h = n // 2
+    fh = api.Forest(it, [5] * h)
+    _round(torch, api, fh, it_o, trees_o[:h], xyz[:h], vals[:h], 1)
+    assert_same(it, fh, it_o, trees_o[:h], what="after rollback")
