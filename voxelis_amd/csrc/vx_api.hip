@@ -79,7 +79,7 @@ struct Grow {
 };
 
 struct LevelBufs {
-    Grow res, key, hash, id, tslot, slot, tiles, old;
+    Grow res, key, hash, id, tslot, slot, tiles, old, own;
 };
 
 // Kernel ids of the per-kernel timer (vx_interner_profile_*).
@@ -281,6 +281,7 @@ struct ApplyOpts {
 template <typename T, int TILE>
 int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
                 int32_t* status, const int32_t* pre_status, const ApplyOpts& opt) {
+    constexpr int kBR = brick_rounds<TILE>();
     hipStream_t s = h->stream;
     Pool<T> P = pool_of<T>(h);
     P.dry = opt.dry ? 1u : 0u;
@@ -425,10 +426,12 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     std::memset(&O, 0, sizeof(O));
     for (uint32_t hh = 1; hh <= dmax; hh++) {
         ntiles[hh] = std::max<uint64_t>((L[hh].nslots + TILE - 1) / TILE, 1);
+        // the brick pass takes R tiles per block and writes every tile's count
+        if (hh == 1 && !setm) ntiles[hh] = (ntiles[hh] + kBR - 1) / kBR * kBR;
         uint64_t ns = ntiles[hh] * TILE;
         LevelBufs& B = h->lv[hh];
         if (!B.res.ensure(ns * 8) || !B.key.ensure(ns * 72) || !B.hash.ensure(ns * 8) ||
-            !B.id.ensure(ns * 8) || !B.tslot.ensure(ns * 8) || !B.slot.ensure(ns * 8) ||
+            !B.id.ensure(ns * 8) || !B.tslot.ensure(ns * 8) || !B.slot.ensure(ns * 8) || !B.own.ensure(ns * 4) ||
             !B.tiles.ensure(ntiles[hh] * 4) || (hh >= 2 && !B.old.ensure(ns * 8)))
             return VX_E_HIP;
         if (hh >= 2) O.at[hh] = (uint64_t*)B.old.ptr;
@@ -438,6 +441,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
         K[hh].id = (uint64_t*)B.id.ptr;
         K[hh].tslot = (uint64_t*)B.tslot.ptr;
         K[hh].slot = (uint64_t*)B.slot.ptr;
+        K[hh].own = (uint32_t*)B.own.ptr;
         res_ptrs_h[hh] = (uint64_t*)B.res.ptr;
         cid_ptrs_h[hh] = (uint64_t*)B.id.ptr;
     }
@@ -461,8 +465,8 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     pinned_release(h);
     uint32_t nvalues = 1u << (8 * sizeof(T));
     uint64_t nb = std::max<uint64_t>(dmax >= 2 ? L[1].nslots : 1, 1);
-    if (!h->deferred.ensure(nb * 4) || !h->drift.ensure(nb * sizeof(Drift))) return VX_E_HIP;
-    uint32_t* ddef = (uint32_t*)h->deferred.ptr;
+    if (!h->deferred.ensure(nb * 8) || !h->drift.ensure(nb * sizeof(Drift))) return VX_E_HIP;
+    uint64_t* ddef = (uint64_t*)h->deferred.ptr;
     Drift* ddrift = (Drift*)h->drift.ptr;
     // undo log: one entry per deferred brick, collapsed upper node or fill leaf at most
     uint64_t undo_cap = n;
@@ -503,7 +507,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
         unsigned tc = (opt.dry || hh == 1) ? 0u : std::min<unsigned>((unsigned)ntiles[hh - 1], h->tune.commit_grid);
         VX_LAUNCH(h, KID_SET_LEVEL, k_set_level<T VX_COMMA TILE>, th + tc, TILE, P, dchunks, L[hh], doff,
                   (uint32_t)n, sx, sv, drec, dmeta, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh], K[hh],
-                  tiles_h[hh], th, K[hh - 1], tiles_h[hh - 1], (uint64_t)ntiles[hh - 1]);
+                  tiles_h[hh], th, K[hh - 1]);
         uint32_t gh = probe_group(th, h->tune.probe_blocks);
         VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T VX_COMMA TILE>, (th + gh - 1) / gh, TILE, P, K[hh], tiles_h[hh],
                   res_ptrs_h[hh], dchunks, L[hh], th, gh);
@@ -512,7 +516,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     if (!opt.dry && dmax >= 1) {
         unsigned td = (unsigned)ntiles[dmax];
         VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T VX_COMMA TILE>, std::min<unsigned>(td, h->tune.commit_grid),
-                  TILE, P, K[dmax], tiles_h[dmax], (uint64_t)td);
+                  TILE, P, K[dmax], dmax);
     }
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
     };
@@ -528,7 +532,8 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
               L[3], doff, O);
     unsigned t1 = (unsigned)ntiles[1];
     if (dmax >= 2)
-        VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE>, t1, TILE, P, dchunks, L[1], res_ptrs_h[1], K[1],
+        VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE VX_COMMA kBR>, t1 / kBR, TILE, P, dchunks, L[1],
+                  res_ptrs_h[1], K[1],
                   tiles_h[1], ddef, ddrift, O.at[2]);
     VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc<T>, 1, 1024, P, dchunks, (uint32_t)n, nvalues);
     if (dmax >= 2) {
@@ -543,7 +548,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
             unsigned tc = opt.dry ? 0u : std::min<unsigned>((unsigned)ntiles[hh - 1], h->tune.commit_grid);
             VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T VX_COMMA TILE>, th + tc, TILE, P, dchunks, L[hh],
                       res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh],
-                      K[hh], tiles_h[hh], th, K[hh - 1], tiles_h[hh - 1], (uint64_t)ntiles[hh - 1], O.at[hh]);
+                      K[hh], tiles_h[hh], th, K[hh - 1], O.at[hh]);
             uint32_t gh = probe_group(th, h->tune.probe_blocks);
             VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T VX_COMMA TILE>, (th + gh - 1) / gh, TILE, P, K[hh], tiles_h[hh],
                       res_ptrs_h[hh], dchunks, L[hh], th, gh);
@@ -551,7 +556,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
         if (!opt.dry) {
             unsigned td = (unsigned)ntiles[dmax];
             VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T VX_COMMA TILE>, std::min<unsigned>(td, h->tune.commit_grid),
-                      TILE, P, K[dmax], tiles_h[dmax], (uint64_t)td);
+                      TILE, P, K[dmax], dmax);
         }
     }
     }
@@ -868,6 +873,7 @@ int vx_interner_destroy(vx_interner* h) {
         b.id.release();
         b.tslot.release();
         b.slot.release();
+        b.own.release();
         b.tiles.release();
         b.old.release();
     }

@@ -107,6 +107,7 @@ struct Ctl {
     uint32_t pad1;
     uint32_t ncand[kLevels + 1];
     uint32_t gc_count[kLevels + 2];
+    uint32_t own_lo[kLevels + 1];  // alloc_count when each level's probe starts
     // statistics, striped over 64 cache lines by block index (summed when read)
     uint64_t st[kStatStripes][16];
     // u8 interners: the leaf id of every value at the start of the current apply
@@ -161,6 +162,7 @@ struct Cands {
     uint64_t* id;     // owner: the new node's id
     uint64_t* tslot;  // owner: its table slot; ~0 otherwise
     uint64_t* slot;   // the level slot this candidate resolves
+    uint32_t* own;    // owners by allocation rank minus Ctl::own_lo of the level
 };
 
 __host__ __device__ inline uint64_t& key_word(const Cands& K, uint64_t k, int i) { return K.key[9ull * k + i]; }

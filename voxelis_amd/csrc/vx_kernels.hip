@@ -16,6 +16,8 @@
 // (dec_ref_recursive, interner/mod.rs:419-534).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "vx_device.h"
 
 namespace vx {
@@ -656,72 +658,133 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
     return 2;
 }
 
-// Register budget for 8 waves per SIMD (fits without spilling): the pass is bound by
-// the latency of its old-tree descent, so more waves in flight hide more of it.
-template <typename T, int TILE>
+// Two phases per block of R*TILE consecutive brick slots (R per lane, slot =
+// block start + j*TILE + lane in round j).  Phase 1 issues every round's loads at
+// once (edit mask, the 8 values and the old height-2 node above the brick) and
+// settles the bricks that need no old tree: untouched ones and all-same ones whose
+// leaf exists.  The rest (mixed bricks; all-same ones whose leaf is still missing)
+// are queued in LDS with their loaded data and take phase 2, one per lane, in
+// rounds of TILE: the old-tree descent and brick_work.  Round r of phase 2 emits
+// into candidate tile blk*R + r; a deferred brick keeps that tile (its round has a
+// free place for it) for the retry.  The register budget is 8 waves per SIMD: the
+// pass is bound by memory latency, so more loads in flight hide more of it.
+// Rounds of the brick pass per block: 1024 slots for TILE <= 256, 1024 or more above.
+template <int TILE>
+constexpr int brick_rounds() { return TILE >= 512 ? 2 : 4; }
+
+template <typename T, int TILE, int R>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
-                                                       uint32_t* deferred, Drift* drift,
+                                                       uint64_t* deferred, Drift* drift,
                                                        const uint64_t* old2) {
-    __shared__ uint32_t s_cnt;
+    typedef typename std::conditional<sizeof(T) == 1, uint2, uint4>::type Vox8;  // a brick's 8 values
+    static_assert(sizeof(Vox8) == 8 * sizeof(T), "u8 or u16 voxels");
+    static_assert(R * TILE <= 65536, "queued brick index is 16 bits");
+    __shared__ uint32_t s_nq;
+    __shared__ uint32_t s_cnt[R];
     __shared__ uint32_t s_st[8];
     // u8: the leaf id of every value, from LDS (the value map does not change here)
     __shared__ uint64_t s_leaf[sizeof(T) == 1 ? 256 : 1];
+    __shared__ uint64_t s_up[R * TILE];
+    __shared__ Vox8 s_v[R * TILE];
+    __shared__ uint16_t s_idx[R * TILE];
+    __shared__ uint8_t s_m[R * TILE];
     uint64_t lm[sizeof(T) == 1 ? (256 + TILE - 1) / TILE : 1];
     if (sizeof(T) == 1)
         for (uint32_t i = 0, v = threadIdx.x; v < 256; i++, v += TILE) lm[i] = P.ctl->leaf_ids8[v];
-    // This lane's brick: chunk fields, mask, values and the previous tree's node at
-    // height 2 above it, in two rounds of independent loads without branches (lanes
-    // past the end load the last slot), issued before the block's barrier so their
-    // latency overlaps the table's.
-    uint64_t slot = (uint64_t)blockIdx.x * TILE + threadIdx.x;
-    const bool live = slot < L.nslots;
-    const uint64_t sl = live ? slot : L.nslots - 1;
-    const GroupPos G = group_pos(L, sl);
-    const uint64_t local = sl - G.base;
-    const uint32_t D = G.depth;
-    const uint32_t sh = 3 * (D - 1);
-    const uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
-    const uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
-    const ChunkDev* Cp = chunks + c;
-    const ChunkHead H = chunk_head(chunks, c);
-    const uint8_t* cmask = H.masks;
-    const T* cval = (const T*)H.values;
-    const int32_t cstatus = H.status;
-    const bool cfill = H.has_fill;
-    const uint64_t croot = H.root_in;
-    uint32_t m = cmask[2ull * p];
-    uint32_t v[8];
-    load8<T>(cval + 8ull * p, v);
-    // the height-2 slot above this brick follows from the slot (a depth-1 tree has none)
-    const uint64_t up = old2[D >= 2 ? G.next + (local >> 3) : 0];  // unused for a tree with a fill
-    if (!live || cstatus < 0) m = 0;
+    const uint64_t b0 = (uint64_t)blockIdx.x * (R * TILE);
+    // phase 1 loads, without branches (lanes past the end load the last slot)
+    uint32_t m[R];
+    Vox8 vw[R];
+    uint64_t up[R];
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+        const uint64_t slot = b0 + (uint64_t)j * TILE + threadIdx.x;
+        const bool live = slot < L.nslots;
+        const uint64_t sl = live ? slot : L.nslots - 1;
+        const GroupPos G = group_pos(L, sl);
+        const uint64_t local = sl - G.base;
+        const uint32_t D = G.depth;
+        const uint32_t sh = 3 * (D - 1);
+        const uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
+        const uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
+        const ChunkHead H = chunk_head(chunks, c);
+        m[j] = H.masks[2ull * p];
+        vw[j] = *(const Vox8*)((const T*)H.values + 8ull * p);
+        // the height-2 slot above this brick follows from the slot (a depth-1 tree has
+        // none; a tree with a fill does not use it)
+        up[j] = old2[D >= 2 ? G.next + (local >> 3) : 0];
+        if (!live || H.status < 0) m[j] = 0;
+    }
     if (sizeof(T) == 1)
         for (uint32_t i = 0, vv = threadIdx.x; vv < 256; i++, vv += TILE) s_leaf[vv] = lm[i];
     const uint64_t* leaf_ids = sizeof(T) == 1 ? s_leaf : nullptr;
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
-    if (threadIdx.x == 0) s_cnt = 0;
+    if (threadIdx.x < R) s_cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_nq = 0;
     __syncthreads();
-    uint64_t out = kInvalid;
-    bool cand = false, defer = false;
-    int flags = 0;
-    uint64_t ch[8];
-    uint32_t tm = 0;
+    uint32_t ncollapse = 0;
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+        const uint64_t slot = b0 + (uint64_t)j * TILE + threadIdx.x;
+        uint32_t v[8];
+        load8<T>((const T*)&vw[j], v);
+        bool all_same = m[j] == 0xFFu;
+#pragma unroll
+        for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
+        const uint64_t lid = all_same ? leaf_lookup(P, leaf_ids, v[0]) : kEmpty;
+        const bool queue = m[j] != 0 && lid == kEmpty;
+        const uint32_t q = wave_append(&s_nq, queue);
+        if (queue) {
+            s_idx[q] = (uint16_t)(j * TILE + threadIdx.x);
+            s_m[q] = (uint8_t)m[j];
+            s_v[q] = vw[j];
+            s_up[q] = up[j];
+        } else if (slot < L.nslots) {
+            res1[slot] = m[j] != 0 ? lid : kInvalid;
+            ncollapse += m[j] != 0;
+        }
+    }
     {
-        const ChunkDev& C = *Cp;
-        if (m != 0) {
-            bool all_same = m == 0xFFu;
+        const uint32_t t = wave_sum(ncollapse);
+        if (__lane_id() == 0 && t) atomicAdd(&s_st[1], t);
+    }
+    __syncthreads();
+    const uint32_t nq = s_nq;
+    for (int r = 0; r < R; r++) {
+        if ((uint32_t)r * TILE >= nq) break;  // uniform
+        const uint32_t q = (uint32_t)r * TILE + threadIdx.x;
+        uint64_t out = kInvalid, slot = 0;
+        bool cand = false, defer = false;
+        int flags = 0;
+        uint64_t ch[8];
+        uint32_t tm = 0;
+        if (q < nq) {
+            slot = b0 + s_idx[q];
+            const uint32_t mm = s_m[q];
+            const Vox8 w = s_v[q];
+            const uint64_t upq = s_up[q];
+            uint32_t v[8];
+            load8<T>((const T*)&w, v);
+            const GroupPos G = group_pos(L, slot);
+            const uint64_t local = slot - G.base;
+            const uint32_t D = G.depth;
+            const uint32_t sh = 3 * (D - 1);
+            const uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
+            const uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
+            const ChunkDev& C = chunks[c];
+            bool all_same = mm == 0xFFu;
 #pragma unroll
             for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
             uint64_t base = kEmpty;
             bool h1 = false;
             if (!all_same) {
-                if (cfill) {
+                if (C.has_fill) {
                     base = kFillMark;
                 } else {
                     // the equivalent node at brick level: a leaf above it splits;
                     // a leaf at brick level is the reference's panic
-                    uint64_t cur = D >= 2 ? up : croot;
+                    uint64_t cur = D >= 2 ? upq : C.root_in;
                     if (D >= 2 && cur != kEmpty && !bid_is_leaf(cur)) {
                         if (bid_index(cur) >= P.capacity) {
                             atomicOr(&P.ctl->err, kErrPoison);
@@ -742,44 +805,49 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
             } else {
                 uint32_t didx;
                 int32_t dd;
-                int r = brick_work(P, C, m, v, base, &out, ch, &tm, &didx, &dd, &flags, leaf_ids);
+                int rc = brick_work(P, C, mm, v, base, &out, ch, &tm, &didx, &dd, &flags, leaf_ids);
                 if (didx != kNoIdx && !P.dry) {
                     uint32_t pos = atomicAdd(&P.ctl->n_drift, 1u);
                     drift[pos] = Drift{c, didx, dd, (uint32_t)(flags & 8 ? ST_H3 : ST_H4)};
                 }
-                if (r == 2) {
+                if (rc == 2) {
                     cand = true;
                     out = res_pending(0xFFFFFFFFu);  // replaced by the probe
-                } else if (r == 3) {
+                } else if (rc == 3) {
                     defer = true;
                     out = base;  // kept for the retry
                 }
             }
+            res1[slot] = out;
         }
+        const uint32_t tile = blockIdx.x * R + (uint32_t)r;
+        emit_candidate<TILE>(K, &s_cnt[r], cand, ch, tm, slot, tile);
+        const uint32_t dpos = wave_append(&P.ctl->n_deferred, defer);
+        if (defer) deferred[dpos] = ((uint64_t)tile << 32) | (uint32_t)slot;
+        tile_count(s_st, 0, cand);
+        tile_count(s_st, 1, (flags & 1) != 0);
+        tile_count(s_st, 2, (flags & 4) != 0);
     }
-    if (live) res1[slot] = out;
-    emit_candidate<TILE>(K, &s_cnt, cand, ch, tm, slot, blockIdx.x);
-    uint32_t dpos = wave_append(&P.ctl->n_deferred, defer);
-    if (defer) deferred[dpos] = (uint32_t)slot;
-    tile_count(s_st, 0, cand);
-    tile_count(s_st, 1, (flags & 1) != 0);
-    tile_count(s_st, 2, (flags & 4) != 0);
     __syncthreads();
-    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_cnt;
+    if (threadIdx.x < R) tile_cnt[blockIdx.x * R + threadIdx.x] = s_cnt[threadIdx.x];
     const int ids[3] = {ST_CAND, ST_COLLAPSE, ST_H2};
     if (threadIdx.x < 3 && s_st[threadIdx.x])
         atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
 }
 
 // Bricks that waited for new leaves, and the queued reference-count corrections
-// of trees that did not fail (their status is final now).
+// of trees that did not fail (their status is final now).  A deferred entry is
+// (candidate tile << 32) | slot: the tile of the brick pass's round that queued it.
 template <typename T, int TILE>
 __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, uint64_t* res1, Cands K,
-                              uint32_t* tile_cnt, const uint32_t* deferred, const Drift* drift) {
+                              uint32_t* tile_cnt, const uint64_t* deferred, const Drift* drift) {
+    mark_owner_base(P, 1);
     uint32_t nd = P.ctl->n_deferred;
     VX_LOOP(i, nd) {
         if (i < nd) {
-            uint64_t slot = deferred[i];
+            const uint64_t e = deferred[i];
+            const uint64_t slot = (uint32_t)e;
+            const uint64_t tile = e >> 32;
             const GroupPos G = group_pos(L, slot);
             uint64_t local = slot - G.base;
             uint32_t D = G.depth;
@@ -803,7 +871,6 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
                 }
                 if (flags & 1) atomicAdd((unsigned long long*)stat(P.ctl, ST_COLLAPSE), 1ull);
                 if (r == 2) {
-                    uint64_t tile = slot / TILE;
                     uint32_t pos = atomicAdd(&tile_cnt[tile], 1u);
                     uint64_t k = tile * TILE + pos;
                     for (int j = 0; j < 8; j++) key_word(K, k, j) = ch[j];
@@ -1062,6 +1129,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                 bool ok;
                 uint32_t idx = alloc_at(P, ka, &ok);
                 result = ok ? make_branch(idx, P.gens[idx], (uint32_t)key8) : kInvalid;
+                K.own[ka - P.ctl->own_lo[L.h]] = (uint32_t)k;  // the commit's list, by rank
             }
             if (P.per_tree && tree != kNoIdx) atomicAdd(&P.per_tree[tree], 1u);
         }
@@ -1085,40 +1153,40 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
 }
 
 // Owners write their node (children, LOD value, types/mask), publish the table
-// slot, and take one reference on each non-empty child.
-// Block `blk` of `nblk` (a grid-stride loop over the tiles).
-template <typename T, int TILE>
-__device__ inline void commit_body(const Pool<T>& P, const Cands& K, const uint32_t* tile_cnt, uint64_t ntiles,
-                                   uint32_t blk, uint32_t nblk) {
+// slot, and take one reference on each non-empty child.  The owners of height h
+// are the allocation ranks [own_lo[h], alloc_count) of the probe of h, listed by
+// rank in K.own (nothing allocates between that probe and this commit).
+// Block `blk` of `nblk` (a grid-stride loop over the owners).
+template <typename T>
+__device__ inline void commit_body(const Pool<T>& P, const Cands& K, uint32_t h, uint32_t blk, uint32_t nblk) {
     __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
     agg_init(akey, acnt);
     __syncthreads();
     uint32_t owned = 0;
-    for (uint64_t t = blk; t < ntiles; t += nblk) {
-        uint32_t n = tile_cnt[t];
-        if (threadIdx.x >= n) continue;
-        uint64_t k = t * TILE + threadIdx.x;
-        uint64_t ts = K.tslot[k];
-        uint64_t id = K.id[k];
-        if (ts == ~0ull || id == kInvalid) continue;
+    const uint32_t n = P.ctl->alloc_count - P.ctl->own_lo[h];
+    for (uint32_t i = blk * blockDim.x + threadIdx.x; i < n; i += nblk * blockDim.x) {
+        const uint64_t k = K.own[i];
+        const uint64_t ts = K.tslot[k];
+        const uint64_t id = K.id[k];
+        if (ts == ~0ull || id == kInvalid) continue;  // its allocation failed
         owned++;
         uint32_t idx = bid_index(id);
         uint64_t key[9];
 #pragma unroll
-        for (int i = 0; i < 9; i++) key[i] = key_word(K, k, i);
+        for (int j = 0; j < 9; j++) key[j] = key_word(K, k, j);
         uint32_t vals[8];
         uint64_t* dst = P.children + 8ull * idx;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            dst[i] = key[i];
-            vals[i] = (uint32_t)P.values[bid_index(key[i])];
+        for (int j = 0; j < 8; j++) {
+            dst[j] = key[j];
+            vals[j] = (uint32_t)P.values[bid_index(key[j])];
         }
         P.tms[idx] = (uint16_t)key[8];
         P.values[idx] = (T)calc_average(vals);
         P.tab[tslot_index(ts)] = slot_make(slot_tag(K.hash[k]), false, 0, idx);
-        for (int i = 0; i < 8; i++) {
-            if (key[i] == kEmpty) continue;
-            uint32_t ci = bid_index(key[i]);
+        for (int j = 0; j < 8; j++) {
+            if (key[j] == kEmpty) continue;
+            uint32_t ci = bid_index(key[j]);
             if (!agg_add(akey, acnt, ci, 1u)) atomicAdd(&P.refs[ci], 1u);
         }
     }
@@ -1129,14 +1197,16 @@ __device__ inline void commit_body(const Pool<T>& P, const Cands& K, const uint3
 }
 
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) void k_commit(Pool<T> P, Cands K, const uint32_t* tile_cnt,
-                                                  uint64_t ntiles) {
-    commit_body<T, TILE>(P, K, tile_cnt, ntiles, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(TILE) void k_commit(Pool<T> P, Cands K, uint32_t h) {
+    commit_body<T>(P, K, h, blockIdx.x, gridDim.x);
 }
 
-// ------------------------------------------------------------------ Phase 2
+// The first allocation rank of the probe that follows (the owners' list base).
+template <typename T>
+__device__ inline void mark_owner_base(const Pool<T>& P, uint32_t h) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.ctl->own_lo[h] = P.ctl->alloc_count;
+}
 
-// A pending result points at the candidate of the previous height that owns its key.
 __device__ inline uint64_t resolve(uint64_t r, const uint64_t* cand_id_prev) {
     return res_is_pending(r) ? cand_id_prev[(uint32_t)r] : r;
 }
@@ -1295,13 +1365,14 @@ template <typename T, int TILE>
 __global__ __launch_bounds__(TILE) void k_level_build(
     Pool<T> P, const ChunkDev* chunks, LevelInfo L,
     const uint64_t* res_prev, const uint64_t* cand_id_prev, uint64_t* res, Cands K,
-    uint32_t* tile_cnt, uint32_t nbuild, Cands Kc, const uint32_t* tile_cnt_c, uint64_t ntiles_c,
-    const uint64_t* old_h) {
-    if (blockIdx.x < nbuild)
+    uint32_t* tile_cnt, uint32_t nbuild, Cands Kc, const uint64_t* old_h) {
+    if (blockIdx.x < nbuild) {
+        mark_owner_base(P, L.h);
         level_build_body<T, TILE>(P, chunks, L, res_prev, cand_id_prev, res, K, tile_cnt,
                                   blockIdx.x, old_h);
-    else
-        commit_body<T, TILE>(P, Kc, tile_cnt_c, ntiles_c, blockIdx.x - nbuild, gridDim.x - nbuild);
+    } else {
+        commit_body<T>(P, Kc, L.h - 1, blockIdx.x - nbuild, gridDim.x - nbuild);
+    }
 }
 
 // Tiles of every level of one apply, flattened (levels 1..dmax).
@@ -1700,12 +1771,12 @@ template <typename T, int TILE>
 __global__ __launch_bounds__(TILE) void k_set_level(
     Pool<T> P, const ChunkDev* chunks, LevelInfo L, const uint64_t* doff, uint32_t n, const int32_t* xyz,
     const T* vals, const uint64_t* rec, const uint32_t* meta, const uint64_t* res_prev,
-    const uint64_t* cand_id_prev, uint64_t* res, Cands K, uint32_t* tile_cnt, uint32_t nbuild, Cands Kc,
-    const uint32_t* tile_cnt_c, uint64_t ntiles_c) {
+    const uint64_t* cand_id_prev, uint64_t* res, Cands K, uint32_t* tile_cnt, uint32_t nbuild, Cands Kc) {
     if (blockIdx.x >= nbuild) {
-        commit_body<T, TILE>(P, Kc, tile_cnt_c, ntiles_c, blockIdx.x - nbuild, gridDim.x - nbuild);
+        commit_body<T>(P, Kc, L.h - 1, blockIdx.x - nbuild, gridDim.x - nbuild);
         return;
     }
+    mark_owner_base(P, L.h);
     __shared__ uint32_t s_cnt;
     __shared__ uint32_t s_st[8];
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
