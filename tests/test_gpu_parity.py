@@ -377,9 +377,10 @@ def test_launch_shapes_match_oracle(vx, tile, dedup, grid_blocks, pb):
     vB[vA == 2] = 1
     it = api.VoxInterner(1, 64 * MB)
     it.set_tuning(tile=tile, dedup_rounds=dedup, commit_grid=grid_blocks, gc_grid=grid_blocks,
-                  probe_blocks_bricks=pb, probe_blocks=pb)
+                  probe_blocks_bricks=pb, probe_blocks=pb, brick_merge_rounds=dedup)
     assert it.get_tuning() == {"tile": tile, "dedup_rounds": dedup, "commit_grid": grid_blocks,
-                               "gc_grid": grid_blocks, "probe_blocks_bricks": pb, "probe_blocks": pb}
+                               "gc_grid": grid_blocks, "probe_blocks_bricks": pb, "probe_blocks": pb,
+                               "brick_merge_rounds": dedup}
     it_o = O.Interner(1, 64 * MB)
     f = api.Forest(it, [5] * n)
     trees_o = [O.Tree(5) for _ in range(n)]

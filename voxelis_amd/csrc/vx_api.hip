@@ -157,7 +157,7 @@ struct vx_interner {
     int ring_cur = 0;
     void* h_pinned = nullptr;  // the slot of the current call
     Prof prof;
-    vx_tuning tune = {256, 8, 1024, 1024, 32768, 512};
+    vx_tuning tune = {256, 8, 1024, 1024, 32768, 512, 4};
     // captured launch sequences of repeated calls (vx_interner_set_graphs)
     bool graphs = false;
     hipStream_t cap_stream = nullptr;
@@ -191,6 +191,7 @@ Pool<T> pool_of(vx_interner* h) {
     P.undo = nullptr;
     P.undo_cap = 0;
     P.dedup_rounds = h->tune.dedup_rounds;
+    P.brick_merge_rounds = h->tune.brick_merge_rounds;
     return P;
 }
 
@@ -298,7 +299,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
                                  (uint64_t)opt.alloc_limit, (uint64_t)opt.counts, (uint64_t)TILE,
                                  (uint64_t)h->tune.dedup_rounds, (uint64_t)h->tune.commit_grid,
                                  (uint64_t)h->tune.gc_grid, (uint64_t)h->tune.probe_blocks_bricks,
-                                 (uint64_t)h->tune.probe_blocks};
+                                 (uint64_t)h->tune.probe_blocks, (uint64_t)h->tune.brick_merge_rounds};
         for (uint64_t w : head) key = mix64(key ^ w);
         for (size_t i = 0; i < n; i++) {
             const vx_batch_view& b = batches[i];
@@ -1127,7 +1128,7 @@ int vx_trees_canonical_hash(vx_interner* h, size_t n, const uint64_t* roots, uin
 int vx_interner_set_tuning(vx_interner* h, const vx_tuning* t) {
     if (!h || !t) return VX_E_INVALID_ARG;
     if (t->tile != 128 && t->tile != 256 && t->tile != 512) return VX_E_INVALID_ARG;
-    if (t->dedup_rounds > 16 || t->commit_grid == 0 || t->commit_grid > (1u << 20) || t->gc_grid == 0 ||
+    if (t->dedup_rounds > 16 || t->brick_merge_rounds > 16 || t->commit_grid == 0 || t->commit_grid > (1u << 20) || t->gc_grid == 0 ||
         t->gc_grid > (1u << 20))
         return VX_E_INVALID_ARG;
     (void)hipStreamSynchronize(h->stream);

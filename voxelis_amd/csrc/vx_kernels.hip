@@ -520,8 +520,8 @@ __global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t n
 // Compaction of a tile's candidate keys into the tile's region of K (order of
 // discovery).  Called by every thread of the block.
 template <int TILE>
-__device__ inline void emit_candidate(const Cands& K, uint32_t* s_cnt, bool pred, const uint64_t ch[8],
-                                      uint32_t tm, uint64_t slot, uint32_t blk) {
+__device__ inline uint32_t emit_candidate(const Cands& K, uint32_t* s_cnt, bool pred, const uint64_t ch[8],
+                                          uint32_t tm, uint64_t slot, uint32_t blk) {
     uint32_t pos = wave_append(s_cnt, pred);
     if (pred) {
         uint64_t k = (uint64_t)blk * TILE + pos;
@@ -531,6 +531,7 @@ __device__ inline void emit_candidate(const Cands& K, uint32_t* s_cnt, bool pred
         K.hash[k] = branch_hash(ch, tm);
         K.slot[k] = slot;
     }
+    return pos;
 }
 
 // Tile statistics in LDS, flushed with one striped atomic per counter per block.
@@ -540,6 +541,30 @@ struct TileStats {
 __device__ inline void tile_count(uint32_t* s, int id, bool pred) {
     uint64_t b = __ballot(pred);
     if (b && (int)__lane_id() == __ffsll((unsigned long long)b) - 1) atomicAdd(&s[id], (uint32_t)__popcll(b));
+}
+
+// Wavefront-level merge of equal keys over `rounds` rounds: each round takes the
+// lowest open lane's key as the reference and closes every open lane holding it.
+// Returns the lane's leader (the lowest lane holding its key, or itself when its
+// key was not reached).
+__device__ inline int wave_merge_keys(uint32_t rounds, bool valid, const uint64_t key[9], uint64_t h) {
+    int lane = (int)__lane_id();
+    int leader = lane;
+    bool open = valid;
+    for (uint32_t r = 0; r < rounds; r++) {
+        uint64_t act = __ballot(open);
+        if (!act) break;
+        int L = __ffsll((unsigned long long)act) - 1;  // wave-uniform: a scalar broadcast
+        bool same = open && h == readlane64(h, L);
+#pragma unroll
+        for (int w = 0; w < 9; w++) same = same && key[w] == readlane64(key[w], L);
+        if (same) {
+            leader = L;
+            open = false;
+        }
+        if (lane == L) open = false;
+    }
+    return leader;
 }
 
 // Per brick (voxtree.rs:778-896): descend the old tree to the brick (792-822), then
@@ -758,7 +783,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
         bool cand = false, defer = false;
         int flags = 0;
         uint64_t ch[8];
-        uint32_t tm = 0;
+        uint32_t tm = 0, c = 0;
         if (q < nq) {
             slot = b0 + s_idx[q];
             const uint32_t mm = s_m[q];
@@ -770,7 +795,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
             const uint64_t local = slot - G.base;
             const uint32_t D = G.depth;
             const uint32_t sh = 3 * (D - 1);
-            const uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
+            c = G.chunk_base + (uint32_t)(local >> sh);
             const uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
             const ChunkDev& C = chunks[c];
             bool all_same = mm == 0xFFu;
@@ -818,20 +843,35 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                     out = base;  // kept for the retry
                 }
             }
-            res1[slot] = out;
         }
+        // Equal keys of one tree in the wavefront share one candidate: the lowest lane
+        // holding the key emits it; the others point at it (a pending result, resolved
+        // through the candidate's id once the probe has run).  The chunk is folded into
+        // the compared hash (an odd multiplier keeps distinct chunks apart), so a
+        // candidate the probe drops with its failed tree has no follower in another.
+        uint64_t key[9];
+#pragma unroll
+        for (int i = 0; i < 8; i++) key[i] = ch[i];
+        key[8] = tm;
+        const uint64_t hk = cand ? branch_hash(ch, tm) + (uint64_t)c * 0x9E3779B97F4A7C15ull : 0ull;
+        const int leader = wave_merge_keys(P.brick_merge_rounds, cand, key, hk);
+        const bool lead = cand && leader == (int)__lane_id();
         const uint32_t tile = blockIdx.x * R + (uint32_t)r;
-        emit_candidate<TILE>(K, &s_cnt[r], cand, ch, tm, slot, tile);
+        const uint32_t pos = emit_candidate<TILE>(K, &s_cnt[r], lead, ch, tm, slot, tile);
+        const uint32_t kl = (uint32_t)((uint64_t)tile * TILE) + __shfl(pos, leader);
+        if (cand && !lead) out = res_pending(kl);
+        if (q < nq) res1[slot] = out;
         const uint32_t dpos = wave_append(&P.ctl->n_deferred, defer);
         if (defer) deferred[dpos] = ((uint64_t)tile << 32) | (uint32_t)slot;
         tile_count(s_st, 0, cand);
         tile_count(s_st, 1, (flags & 1) != 0);
         tile_count(s_st, 2, (flags & 4) != 0);
+        tile_count(s_st, 3, cand && !lead);
     }
     __syncthreads();
     if (threadIdx.x < R) tile_cnt[blockIdx.x * R + threadIdx.x] = s_cnt[threadIdx.x];
-    const int ids[3] = {ST_CAND, ST_COLLAPSE, ST_H2};
-    if (threadIdx.x < 3 && s_st[threadIdx.x])
+    const int ids[4] = {ST_CAND, ST_COLLAPSE, ST_H2, ST_DUP};
+    if (threadIdx.x < 4 && s_st[threadIdx.x])
         atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
 }
 
@@ -992,30 +1032,6 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
     atomicOr(&P.ctl->err, kErrTableFull | kErrPoison);
     *result = kInvalid;
     return -1;
-}
-
-// Wavefront-level merge of equal keys over `rounds` rounds: each round takes the
-// lowest open lane's key as the reference and closes every open lane holding it.
-// Returns the lane's leader (the lowest lane holding its key, or itself when its
-// key was not reached).
-__device__ inline int wave_merge_keys(uint32_t rounds, bool valid, const uint64_t key[9], uint64_t h) {
-    int lane = (int)__lane_id();
-    int leader = lane;
-    bool open = valid;
-    for (uint32_t r = 0; r < rounds; r++) {
-        uint64_t act = __ballot(open);
-        if (!act) break;
-        int L = __ffsll((unsigned long long)act) - 1;  // wave-uniform: a scalar broadcast
-        bool same = open && h == readlane64(h, L);
-#pragma unroll
-        for (int w = 0; w < 9; w++) same = same && key[w] == readlane64(key[w], L);
-        if (same) {
-            leader = L;
-            open = false;
-        }
-        if (lane == L) open = false;
-    }
-    return leader;
 }
 
 // Probe of one level.  Block b takes the candidates of tiles [b*group, b*group+group)
@@ -1207,8 +1223,12 @@ __device__ inline void mark_owner_base(const Pool<T>& P, uint32_t h) {
     if (blockIdx.x == 0 && threadIdx.x == 0) P.ctl->own_lo[h] = P.ctl->alloc_count;
 }
 
+// A pending result names a candidate of the level; that candidate's id is final, or
+// (a brick that shared an equal key's candidate) pending on the owner's, which is.
 __device__ inline uint64_t resolve(uint64_t r, const uint64_t* cand_id_prev) {
-    return res_is_pending(r) ? cand_id_prev[(uint32_t)r] : r;
+    if (res_is_pending(r)) r = cand_id_prev[(uint32_t)r];
+    if (res_is_pending(r)) r = cand_id_prev[(uint32_t)r];
+    return r;
 }
 
 // One node at height h >= 2 (voxtree.rs:913-1106): gather the touched children
@@ -1267,7 +1287,7 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             touched |= (ch[i] != kInvalid ? 1u : 0u) << i;
-            if (res_is_pending(ch[i])) ch[i] = cand_id_prev[(uint32_t)ch[i]];
+            ch[i] = resolve(ch[i], cand_id_prev);
         }
         uint64_t out = kInvalid;
         bool cand = false, collapsed = false, h5 = false;
