@@ -72,29 +72,31 @@ def kernel_bytes(name, launches, st, counts):
     slots1 = counts["brick_slots"] * steps
     touched = counts["touched_bricks"] * steps
     cand1 = counts["cand_bricks"] * steps
+    lead1 = max(cand1 - st.get("merged", 0), 0)  # brick keys that reach the probe
     cand_up = counts["cand_upper"] * steps
     slots_up = counts["upper_slots"] * steps
     s = counts["value_size"]
     if name == "k_brick_build":
         # every brick slot: its set mask (2 B of the mask pair), its share of the recorded
         # height-2 node (1) and the slot write (8); touched bricks: 8 values; non-uniform
-        # bricks: the brick-level node (8), its children (64) and their values (8s),
-        # candidate key + hash + slot (88)
-        return slots1 * (2 + 1 + 8) + touched * 8 * s + cand1 * (8 + 64 + 8 * s + 88)
-    if name in ("k_probe[bricks]", "k_probe[upper]"):
-        c = cand1 if name == "k_probe[bricks]" else cand_up
-        # key + hash + slot read (88), one table slot (8), the matched node's key (66),
-        # result id / table slot / level slot writes (24)
-        return c * (88 + 8 + 66 + 24)
+        # bricks: the brick-level node (8), its children (64) and their values (8s);
+        # keys that reach the probe: candidate key + hash + slot (88)
+        return slots1 * (2 + 1 + 8) + touched * 8 * s + cand1 * (8 + 64 + 8 * s) + lead1 * 88
+    # a probed key: key + hash + slot read (88), one table slot (8), the matched node's
+    # key (66), result id / table slot / level slot writes (24)
+    probe_b = 88 + 8 + 66 + 24
+    if name == "k_probe[bricks]":
+        return lead1 * probe_b
+    if name == "k_probe[upper]":
+        return cand_up * probe_b
     if name == "k_level_build":
         # 8 child slots (64 B) + slot write (8) per slot; candidate key + hash + slot (88)
         return slots_up * 72 + cand_up * 88
-    if name in ("k_commit[bricks]", "k_commit[upper]"):
-        c = cand1 if name == "k_commit[bricks]" else cand_up
+    if name == "k_commit":
         owners = st.get("branch_misses", 0)
-        # every candidate: owner slot + id (16); owners: key + hash (80), node write (64 + 2 + s),
+        # owners: list entry + table slot + id (24), key + hash (80), node write (64 + 2 + s),
         # table slot (8), 8 child values (8s) and refcount read-modify-writes (8 x 8)
-        return c * 16 + owners * (80 + 66 + s + 8 + 8 * s + 64)
+        return owners * (24 + 80 + 66 + s + 8 + 8 * s + 64)
     if name == "k_gc_round":
         freed = st.get("recycled", 0)
         # per freed node: type + children read and clear (2 + 128), pattern slot read + tombstone (16),
@@ -241,8 +243,8 @@ def main():
         n_ref = 8 * st["branch_misses"] / a.steps + n_new  # child refs of new nodes + frees
         b_edit = E * (4 + s_)
         b_hc = n_cand * 88 + n_leafcand * (s_ + 24) + n_new * (64 + s_ + 8 + 4 + 2 + 16) + n_ref * 8
-        hc_kern = ("k_brick_build", "k_brick_retry", "k_probe[bricks]", "k_commit[bricks]", "k_level_build",
-                   "k_probe[upper]", "k_commit[upper]")
+        hc_kern = ("k_brick_build", "k_brick_retry", "k_probe[bricks]", "k_level_build", "k_probe[upper]",
+                   "k_commit")
         t_hc = sum(prof[k][0] for k in hc_kern if k in prof) / a.steps / 1e3
         if t_hc > 0:
             ach = (b_edit + b_hc) / t_hc / 1e9

@@ -125,6 +125,7 @@ enum {
     VX_STAT_PROBES,            /* table slots inspected */
     VX_STAT_CLAIMS,            /* atomic claims of a free table slot */
     VX_STAT_CLAIMS_LOST,       /* claims that found the slot taken meanwhile */
+    VX_STAT_MERGED,            /* brick keys that shared an equal key's candidate before the probe */
     VX_STAT_COUNT
 };
 

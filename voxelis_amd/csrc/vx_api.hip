@@ -79,19 +79,19 @@ struct Grow {
 };
 
 struct LevelBufs {
-    Grow res, key, hash, id, tslot, slot, tiles, old, own;
+    Grow res, key, hash, id, tslot, slot, tiles, old;
 };
 
 // Kernel ids of the per-kernel timer (vx_interner_profile_*).
 enum KernelId : uint32_t {
     KID_BEGIN = 0, KID_PREPARE, KID_BRICK_SCAN, KID_LEAF_ALLOC, KID_BRICK_BUILD, KID_PROBE_BRICKS,
-    KID_COMMIT_BRICKS, KID_LEVEL_BUILD, KID_PROBE_UPPER, KID_COMMIT_UPPER, KID_FINALIZE,
+    KID_COMMIT_BRICKS, KID_LEVEL_BUILD, KID_PROBE_UPPER, KID_COMMIT, KID_FINALIZE,
     KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_ROLLBACK, KID_TO_VEC,
     KID_SET_DESCEND, KID_SET_LEVEL, KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_begin", "k_prepare", "k_brick_retry", "k_leaf_alloc", "k_brick_build", "k_probe[bricks]",
-    "k_commit[bricks]", "k_level_build", "k_probe[upper]", "k_commit[upper]", "k_finish",
+    "k_commit[bricks]", "k_level_build", "k_probe[upper]", "k_commit", "k_finish",
     "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill", "k_rollback", "k_to_vec",
     "k_set_descend", "k_set_level"};
 
@@ -142,7 +142,7 @@ struct vx_interner {
     bool own_stream = false;
     int poisoned = 0;
     // workspace
-    Grow chunks, slot_off, ptr_tables, gc_front[2], stage, scratch, deferred, drift, undo;
+    Grow chunks, slot_off, ptr_tables, gc_front[2], stage, scratch, deferred, drift, undo, owners;
     Grow set_rec, set_meta;  // single-voxel set: the recorded path of every tree
     // edit-list path: zero-kept dense Batch buffers, sort buffers, tree records
     Grow edit_ws, edit_keys[2], edit_order[2], edit_tmp, edit_trees, edit_first, edit_status;
@@ -432,7 +432,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
         uint64_t ns = ntiles[hh] * TILE;
         LevelBufs& B = h->lv[hh];
         if (!B.res.ensure(ns * 8) || !B.key.ensure(ns * 72) || !B.hash.ensure(ns * 8) ||
-            !B.id.ensure(ns * 8) || !B.tslot.ensure(ns * 8) || !B.slot.ensure(ns * 8) || !B.own.ensure(ns * 4) ||
+            !B.id.ensure(ns * 8) || !B.tslot.ensure(ns * 8) || !B.slot.ensure(ns * 8) ||
             !B.tiles.ensure(ntiles[hh] * 4) || (hh >= 2 && !B.old.ensure(ns * 8)))
             return VX_E_HIP;
         if (hh >= 2) O.at[hh] = (uint64_t*)B.old.ptr;
@@ -442,10 +442,14 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
         K[hh].id = (uint64_t*)B.id.ptr;
         K[hh].tslot = (uint64_t*)B.tslot.ptr;
         K[hh].slot = (uint64_t*)B.slot.ptr;
-        K[hh].own = (uint32_t*)B.own.ptr;
         res_ptrs_h[hh] = (uint64_t*)B.res.ptr;
         cid_ptrs_h[hh] = (uint64_t*)B.id.ptr;
     }
+    // the owners of every level, by allocation rank (at most one per candidate)
+    uint64_t nown = 1;
+    for (uint32_t hh = 1; hh <= dmax; hh++) nown += ntiles[hh] * TILE;
+    if (!h->owners.ensure(nown * 8)) return VX_E_HIP;
+    P.owners = (uint64_t*)h->owners.ptr;
     void** hptrs = (void**)((char*)h->h_pinned + chunk_bytes + off_count * sizeof(uint64_t));
     for (int i = 0; i <= kLevels; i++) {
         hptrs[i] = res_ptrs_h[i];
@@ -504,21 +508,16 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     const unsigned wpb = TILE / 64;
     for (uint32_t hh = 1; hh <= dmax; hh++) {
         unsigned th = (unsigned)ntiles[hh];
-        // the commit of height hh-1 rides along in extra blocks (none in a dry run)
-        unsigned tc = (opt.dry || hh == 1) ? 0u : std::min<unsigned>((unsigned)ntiles[hh - 1], h->tune.commit_grid);
-        VX_LAUNCH(h, KID_SET_LEVEL, k_set_level<T VX_COMMA TILE>, th + tc, TILE, P, dchunks, L[hh], doff,
+        VX_LAUNCH(h, KID_SET_LEVEL, k_set_level<T VX_COMMA TILE>, th, TILE, P, dchunks, L[hh], doff,
                   (uint32_t)n, sx, sv, drec, dmeta, res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh], K[hh],
-                  tiles_h[hh], th, K[hh - 1]);
+                  tiles_h[hh]);
         uint32_t gh = probe_group(th, h->tune.probe_blocks);
         VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T VX_COMMA TILE>, (th + gh - 1) / gh, TILE, P, K[hh], tiles_h[hh],
                   res_ptrs_h[hh], dchunks, L[hh], th, gh);
     }
     (void)wpb;
-    if (!opt.dry && dmax >= 1) {
-        unsigned td = (unsigned)ntiles[dmax];
-        VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T VX_COMMA TILE>, std::min<unsigned>(td, h->tune.commit_grid),
-                  TILE, P, K[dmax], dmax);
-    }
+    if (!opt.dry && dmax >= 1)
+        VX_LAUNCH(h, KID_COMMIT, k_commit<T VX_COMMA TILE>, h->tune.commit_grid, TILE, P);
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
     };
     auto launch_all = [&]() -> int {
@@ -545,20 +544,14 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
                   res_ptrs_h[1], dchunks, L[1], t1, g1);
         for (uint32_t hh = 2; hh <= dmax; hh++) {
             unsigned th = (unsigned)ntiles[hh];
-            // the commit of height hh-1 rides along in extra blocks (none in a dry run)
-            unsigned tc = opt.dry ? 0u : std::min<unsigned>((unsigned)ntiles[hh - 1], h->tune.commit_grid);
-            VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T VX_COMMA TILE>, th + tc, TILE, P, dchunks, L[hh],
-                      res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh],
-                      K[hh], tiles_h[hh], th, K[hh - 1], O.at[hh]);
+            VX_LAUNCH(h, KID_LEVEL_BUILD, k_level_build<T VX_COMMA TILE>, th, TILE, P, dchunks, L[hh],
+                      res_ptrs_h[hh - 1], cid_ptrs_h[hh - 1], res_ptrs_h[hh], K[hh], tiles_h[hh], O.at[hh]);
             uint32_t gh = probe_group(th, h->tune.probe_blocks);
             VX_LAUNCH(h, KID_PROBE_UPPER, k_probe<T VX_COMMA TILE>, (th + gh - 1) / gh, TILE, P, K[hh], tiles_h[hh],
                       res_ptrs_h[hh], dchunks, L[hh], th, gh);
         }
-        if (!opt.dry) {
-            unsigned td = (unsigned)ntiles[dmax];
-            VX_LAUNCH(h, KID_COMMIT_UPPER, k_commit<T VX_COMMA TILE>, std::min<unsigned>(td, h->tune.commit_grid),
-                      TILE, P, K[dmax], dmax);
-        }
+        // every level's new nodes, written once no probe runs any more
+        if (!opt.dry) VX_LAUNCH(h, KID_COMMIT, k_commit<T VX_COMMA TILE>, h->tune.commit_grid, TILE, P);
     }
     }
     uint32_t* gf0 = (uint32_t*)h->gc_front[0].ptr;
@@ -862,6 +855,7 @@ int vx_interner_destroy(vx_interner* h) {
     h->deferred.release();
     h->drift.release();
     h->undo.release();
+    h->owners.release();
     for (Plan& pl : h->plans) pl.release();
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     for (Grow* g : {&h->edit_ws, &h->edit_keys[0], &h->edit_keys[1], &h->edit_order[0], &h->edit_order[1],
@@ -874,7 +868,6 @@ int vx_interner_destroy(vx_interner* h) {
         b.id.release();
         b.tslot.release();
         b.slot.release();
-        b.own.release();
         b.tiles.release();
         b.old.release();
     }
@@ -930,7 +923,7 @@ int vx_interner_stats(vx_interner* h, uint64_t* out, uint32_t n) {
     if (rc) return rc;
     static const int map[VX_STAT_COUNT] = {ST_HIT, ST_MISS, ST_DUP, ST_LEAF_MISS, ST_COLLAPSE,
                                            ST_RECYCLED, ST_H2, ST_H3, ST_H4, ST_H5, ST_CAND,
-                                           ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST};
+                                           ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST, ST_MERGED};
     for (uint32_t i = 0; i < n && i < (uint32_t)VX_STAT_COUNT; i++) {
         uint64_t sum = 0;
         for (int sidx = 0; sidx < kStatStripes; sidx++) sum += c.st[sidx][map[i]];

@@ -107,7 +107,7 @@ struct Ctl {
     uint32_t pad1;
     uint32_t ncand[kLevels + 1];
     uint32_t gc_count[kLevels + 2];
-    uint32_t own_lo[kLevels + 1];  // alloc_count when each level's probe starts
+    uint32_t own_base;  // alloc_count when the apply's first probe starts
     // statistics, striped over 64 cache lines by block index (summed when read)
     uint64_t st[kStatStripes][16];
     // u8 interners: the leaf id of every value at the start of the current apply
@@ -162,7 +162,6 @@ struct Cands {
     uint64_t* id;     // owner: the new node's id
     uint64_t* tslot;  // owner: its table slot; ~0 otherwise
     uint64_t* slot;   // the level slot this candidate resolves
-    uint32_t* own;    // owners by allocation rank minus Ctl::own_lo of the level
 };
 
 __host__ __device__ inline uint64_t& key_word(const Cands& K, uint64_t k, int i) { return K.key[9ull * k + i]; }
@@ -197,6 +196,7 @@ struct Pool {
     uint32_t* per_tree;    // optional: new branch nodes owned by each tree (caller's index)
     const Cands* levels;   // candidate arrays of every level (pending keys of earlier levels)
     Drift* undo;           // reference-count deltas applied outside commits (rolled back on abort)
+    uint64_t* owners;      // by allocation rank - Ctl::own_base: (height << 32) | candidate
     uint32_t undo_cap;
 };
 

@@ -29,7 +29,7 @@ namespace vx {
 
 enum Stat {
     ST_HIT = 0, ST_MISS, ST_DUP, ST_LEAF_MISS, ST_COLLAPSE, ST_RECYCLED, ST_H2, ST_H3, ST_H4, ST_H5,
-    ST_CAND, ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST
+    ST_CAND, ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST, ST_MERGED
 };
 
 __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
@@ -167,36 +167,27 @@ __device__ inline void load8(const T* v, uint32_t out[8]) {
 }
 
 // core/voxel.rs:96-141 calc_average: mode; ties keep the first seen unless the
-// leader is the default value and the challenger is not.
+// leader is the default value and the challenger is not.  Distinct values are taken
+// in order of first occurrence, each with its count; every index is static, so the
+// routine stays in registers.
 __device__ inline uint32_t calc_average(const uint32_t v[8]) {
-    uint32_t vals[8], cnt[8];
-    uint32_t uniq = 0;
+    uint32_t best = v[0], bc = 0;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        uint32_t c = v[k];
-        uint32_t i = 0;
-        bool found = false;
-        for (; i < uniq; i++) {
-            if (vals[i] == c) {
-                cnt[i]++;
-                found = true;
-                break;
-            }
-        }
-        if (!found) {
-            vals[uniq] = c;
-            cnt[uniq] = 1;
-            uniq++;
+        bool first = true;
+#pragma unroll
+        for (int j = 0; j < k; j++) first = first && v[j] != v[k];
+        uint32_t c = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) c += v[j] == v[k] ? 1u : 0u;
+        if (k == 0) {
+            bc = c;
+        } else if (first && (c > bc || (c == bc && best == 0 && v[k] != 0))) {
+            best = v[k];
+            bc = c;
         }
     }
-    uint32_t mi = 0, mc = cnt[0];
-    for (uint32_t i = 1; i < uniq; i++) {
-        if (cnt[i] > mc || (cnt[i] == mc && vals[mi] == 0 && vals[i] != 0)) {
-            mc = cnt[i];
-            mi = i;
-        }
-    }
-    return vals[mi];
+    return best;
 }
 
 template <typename T>
@@ -870,7 +861,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     }
     __syncthreads();
     if (threadIdx.x < R) tile_cnt[blockIdx.x * R + threadIdx.x] = s_cnt[threadIdx.x];
-    const int ids[4] = {ST_CAND, ST_COLLAPSE, ST_H2, ST_DUP};
+    const int ids[4] = {ST_CAND, ST_COLLAPSE, ST_H2, ST_MERGED};
     if (threadIdx.x < 4 && s_st[threadIdx.x])
         atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
 }
@@ -881,7 +872,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
 template <typename T, int TILE>
 __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, uint64_t* res1, Cands K,
                               uint32_t* tile_cnt, const uint64_t* deferred, const Drift* drift) {
-    mark_owner_base(P, 1);
+    mark_owner_base(P);
     uint32_t nd = P.ctl->n_deferred;
     VX_LOOP(i, nd) {
         if (i < nd) {
@@ -951,6 +942,7 @@ __device__ inline bool key_matches_node(const Pool<T>& P, uint32_t idx, const ui
     return true;
 }
 
+// The candidate was written by an earlier launch (every launch starts with coherent caches).
 __device__ inline bool key_matches_cand(const Cands& K, uint32_t k2, const uint64_t key[9]) {
 #pragma unroll
     for (int i = 0; i < 9; i++)
@@ -1034,21 +1026,23 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
     return -1;
 }
 
-// Probe of one level.  Block b takes the candidates of tiles [b*group, b*group+group)
-// (group <= 64), packed densely over its lanes: sparse tiles fill whole wavefronts,
-// and equal keys meet in the same wavefront more often.  Per pass over TILE of
-// them: merge equal keys in each wavefront, look the leaders' keys up / insert
-// them (get_or_create_branch, interner/mod.rs:716-829), then allocate the owners'
-// nodes with one counter update for the whole block (a counter every block of a
-// launch updates is a serial point), and publish the results.
+// Probe of the candidates of tiles [t0, t0+tn) of one level (tn <= 64), packed
+// densely over the block's lanes: sparse tiles fill whole wavefronts, and equal keys
+// meet in the same wavefront more often.  Per pass over TILE of them: merge equal
+// keys in each wavefront, look the leaders' keys up / insert them
+// (get_or_create_branch, interner/mod.rs:716-829), then allocate the owners' nodes
+// with one counter update for the whole block (a counter every block of a launch
+// updates is a serial point), and publish the results.  Owners are listed by
+// allocation rank for the commit, which runs after the last level's probe.  Every
+// candidate was emitted by an earlier launch: a key that another block claimed in
+// this launch is compared against that block's candidate, which is visible.  (On a
+// multi-die part, publishing a block's keys to the other dies inside one launch
+// would take a write-back of its L2.)  A wavefront with no candidate returns.
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
-                                                 uint64_t* res, const ChunkDev* chunks, LevelInfo L,
-                                                 uint32_t ntiles, uint32_t group) {
+__device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint32_t* tile_cnt, uint64_t* res,
+                                   const ChunkDev* chunks, const LevelInfo& L, uint32_t t0, uint32_t tn) {
     __shared__ uint32_t s_pre[65];  // s_pre[t]: candidates of the group's tiles before tile t
     __shared__ uint32_t s_own, s_base;
-    const uint32_t t0 = blockIdx.x * group;
-    const uint32_t tn = min(group, ntiles - t0);
     if (threadIdx.x < 64) {
         uint32_t x = threadIdx.x < tn ? tile_cnt[t0 + threadIdx.x] : 0u;
 #pragma unroll
@@ -1145,7 +1139,15 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                 bool ok;
                 uint32_t idx = alloc_at(P, ka, &ok);
                 result = ok ? make_branch(idx, P.gens[idx], (uint32_t)key8) : kInvalid;
-                K.own[ka - P.ctl->own_lo[L.h]] = (uint32_t)k;  // the commit's list, by rank
+                if (ok) {
+                    // the LOD value (core/voxel.rs:96-141) from the children's, which
+                    // earlier launches wrote: a node gets its value at its own level
+                    uint32_t vals[8];
+#pragma unroll
+                    for (int i = 0; i < 8; i++) vals[i] = (uint32_t)P.values[bid_index(key_word(K, k, i))];
+                    P.values[idx] = (T)calc_average(vals);
+                }
+                P.owners[ka - P.ctl->own_base] = ((uint64_t)L.h << 32) | k;  // the commit's list
             }
             if (P.per_tree && tree != kNoIdx) atomicAdd(&P.per_tree[tree], 1u);
         }
@@ -1168,20 +1170,31 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     }
 }
 
-// Owners write their node (children, LOD value, types/mask), publish the table
-// slot, and take one reference on each non-empty child.  The owners of height h
-// are the allocation ranks [own_lo[h], alloc_count) of the probe of h, listed by
-// rank in K.own (nothing allocates between that probe and this commit).
-// Block `blk` of `nblk` (a grid-stride loop over the owners).
-template <typename T>
-__device__ inline void commit_body(const Pool<T>& P, const Cands& K, uint32_t h, uint32_t blk, uint32_t nblk) {
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
+                                                 uint64_t* res, const ChunkDev* chunks, LevelInfo L,
+                                                 uint32_t ntiles, uint32_t group) {
+    const uint32_t t0 = blockIdx.x * group;
+    probe_tiles<T, TILE>(P, K, tile_cnt, res, chunks, L, t0, min(group, ntiles - t0));
+}
+
+// Owners write their node (children, types/mask; the probe wrote its LOD value),
+// publish the table slot, and take one reference on each non-empty child.  The owners of an apply are
+// its allocation ranks [own_base, alloc_count) after the last probe (leaves are
+// allocated before own_base), listed by rank as (height << 32) | candidate.  No
+// probe runs during the commit: until it, every new key is a pending table entry,
+// matched against its candidate.  Grid-stride over the owners.
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_commit(Pool<T> P) {
     __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
     agg_init(akey, acnt);
     __syncthreads();
     uint32_t owned = 0;
-    const uint32_t n = P.ctl->alloc_count - P.ctl->own_lo[h];
-    for (uint32_t i = blk * blockDim.x + threadIdx.x; i < n; i += nblk * blockDim.x) {
-        const uint64_t k = K.own[i];
+    const uint32_t n = P.ctl->alloc_count - P.ctl->own_base;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint64_t e = P.owners[i];
+        const Cands& K = P.levels[e >> 32];
+        const uint64_t k = (uint32_t)e;
         const uint64_t ts = K.tslot[k];
         const uint64_t id = K.id[k];
         if (ts == ~0ull || id == kInvalid) continue;  // its allocation failed
@@ -1190,15 +1203,10 @@ __device__ inline void commit_body(const Pool<T>& P, const Cands& K, uint32_t h,
         uint64_t key[9];
 #pragma unroll
         for (int j = 0; j < 9; j++) key[j] = key_word(K, k, j);
-        uint32_t vals[8];
         uint64_t* dst = P.children + 8ull * idx;
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            dst[j] = key[j];
-            vals[j] = (uint32_t)P.values[bid_index(key[j])];
-        }
+        for (int j = 0; j < 8; j++) dst[j] = key[j];
         P.tms[idx] = (uint16_t)key[8];
-        P.values[idx] = (T)calc_average(vals);
         P.tab[tslot_index(ts)] = slot_make(slot_tag(K.hash[k]), false, 0, idx);
         for (int j = 0; j < 8; j++) {
             if (key[j] == kEmpty) continue;
@@ -1212,15 +1220,11 @@ __device__ inline void commit_body(const Pool<T>& P, const Cands& K, uint32_t h,
     wave_add64(&P.ctl->live_branches, owned);
 }
 
-template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) void k_commit(Pool<T> P, Cands K, uint32_t h) {
-    commit_body<T>(P, K, h, blockIdx.x, gridDim.x);
-}
-
-// The first allocation rank of the probe that follows (the owners' list base).
+// The first allocation rank of the apply's probes (the owners' list base), set by a
+// pass that runs after the leaf allocation and before the first probe.
 template <typename T>
-__device__ inline void mark_owner_base(const Pool<T>& P, uint32_t h) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) P.ctl->own_lo[h] = P.ctl->alloc_count;
+__device__ inline void mark_owner_base(const Pool<T>& P) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.ctl->own_base = P.ctl->alloc_count;
 }
 
 // A pending result names a candidate of the level; that candidate's id is final, or
@@ -1378,21 +1382,13 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
         atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
 }
 
-// Height h >= 2, plus (blocks nbuild..) the commit of height h-1: the commit writes
-// only nodes created at h-1 and the table, which the build of h does not read, and
-// the probe of h follows this launch.
+// Height h >= 2 (its probe is the next launch).
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) void k_level_build(
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_level_build(
     Pool<T> P, const ChunkDev* chunks, LevelInfo L,
     const uint64_t* res_prev, const uint64_t* cand_id_prev, uint64_t* res, Cands K,
-    uint32_t* tile_cnt, uint32_t nbuild, Cands Kc, const uint64_t* old_h) {
-    if (blockIdx.x < nbuild) {
-        mark_owner_base(P, L.h);
-        level_build_body<T, TILE>(P, chunks, L, res_prev, cand_id_prev, res, K, tile_cnt,
-                                  blockIdx.x, old_h);
-    } else {
-        commit_body<T>(P, Kc, L.h - 1, blockIdx.x - nbuild, gridDim.x - nbuild);
-    }
+    uint32_t* tile_cnt, const uint64_t* old_h) {
+    level_build_body<T, TILE>(P, chunks, L, res_prev, cand_id_prev, res, K, tile_cnt, blockIdx.x, old_h);
 }
 
 // Tiles of every level of one apply, flattened (levels 1..dmax).
@@ -1791,12 +1787,8 @@ template <typename T, int TILE>
 __global__ __launch_bounds__(TILE) void k_set_level(
     Pool<T> P, const ChunkDev* chunks, LevelInfo L, const uint64_t* doff, uint32_t n, const int32_t* xyz,
     const T* vals, const uint64_t* rec, const uint32_t* meta, const uint64_t* res_prev,
-    const uint64_t* cand_id_prev, uint64_t* res, Cands K, uint32_t* tile_cnt, uint32_t nbuild, Cands Kc) {
-    if (blockIdx.x >= nbuild) {
-        commit_body<T>(P, Kc, L.h - 1, blockIdx.x - nbuild, gridDim.x - nbuild);
-        return;
-    }
-    mark_owner_base(P, L.h);
+    const uint64_t* cand_id_prev, uint64_t* res, Cands K, uint32_t* tile_cnt) {
+    if (L.h == 1) mark_owner_base(P);
     __shared__ uint32_t s_cnt;
     __shared__ uint32_t s_st[8];
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
