@@ -86,8 +86,10 @@ int vx_interner_set_stream(vx_interner* h, void* hip_stream);
  * up to 16 tiles into one block, keeping at least this many blocks (0: one tile
  * per block), separately for the brick level (probe_blocks_bricks) and the levels
  * above; brick_merge_rounds: wavefront key-merge rounds of the brick pass, before
- * its keys become candidates (0-16; equal keys of one tree share a candidate).
- * Defaults {256, 8, 1024, 1024, 32768, 512, 4}.  Synchronizes the interner's
+ * its keys become candidates (0-16; equal keys of one tree share a candidate);
+ * brick_rounds: brick slots per lane of the brick pass (1 or 4, 0: 4 when the
+ * pass has at least 512 blocks of them, else 1).
+ * Defaults {256, 8, 1024, 1024, 32768, 512, 4, 0}.  Synchronizes the interner's
  * stream. */
 typedef struct {
     uint32_t tile;
@@ -97,6 +99,7 @@ typedef struct {
     uint32_t probe_blocks_bricks;
     uint32_t probe_blocks;
     uint32_t brick_merge_rounds;
+    uint32_t brick_rounds;
 } vx_tuning;
 int vx_interner_set_tuning(vx_interner* h, const vx_tuning* tuning);
 int vx_interner_get_tuning(const vx_interner* h, vx_tuning* tuning);

@@ -361,12 +361,14 @@ def test_graph_replay_matches_direct_launches(vx):
     assert runs[0] == runs[1]
 
 
-@pytest.mark.parametrize("tile,dedup,grid_blocks,pb", [(128, 0, 64, 0), (512, 2, 4096, 1), (256, 8, 1024, 3)])
-def test_launch_shapes_match_oracle(vx, tile, dedup, grid_blocks, pb):
+@pytest.mark.parametrize("tile,dedup,grid_blocks,pb,br", [(128, 0, 64, 0, 4), (512, 2, 4096, 1, 1), (256, 8, 1024, 3, 4),
+                                                        (256, 4, 1024, 0, 0)])
+def test_launch_shapes_match_oracle(vx, tile, dedup, grid_blocks, pb, br):
     """Every tile size / probe schedule / grid size / probe tile grouping the
     autotuner may pick gives the oracle's trees (two applies, so the second replaces
     and reclaims the first).  probe_blocks 1 packs 16 tiles into every probe block
-    (several passes per block), 3 packs 16 at the brick level and fewer above."""
+    (several passes per block), 3 packs 16 at the brick level and fewer above; br is
+    the brick slots per lane of the brick pass (0: chosen by size)."""
     api, W = vx
     grid = 4
     n = grid * grid
@@ -377,10 +379,10 @@ def test_launch_shapes_match_oracle(vx, tile, dedup, grid_blocks, pb):
     vB[vA == 2] = 1
     it = api.VoxInterner(1, 64 * MB)
     it.set_tuning(tile=tile, dedup_rounds=dedup, commit_grid=grid_blocks, gc_grid=grid_blocks,
-                  probe_blocks_bricks=pb, probe_blocks=pb, brick_merge_rounds=dedup)
+                  probe_blocks_bricks=pb, probe_blocks=pb, brick_merge_rounds=dedup, brick_rounds=br)
     assert it.get_tuning() == {"tile": tile, "dedup_rounds": dedup, "commit_grid": grid_blocks,
                                "gc_grid": grid_blocks, "probe_blocks_bricks": pb, "probe_blocks": pb,
-                               "brick_merge_rounds": dedup}
+                               "brick_merge_rounds": dedup, "brick_rounds": br}
     it_o = O.Interner(1, 64 * MB)
     f = api.Forest(it, [5] * n)
     trees_o = [O.Tree(5) for _ in range(n)]

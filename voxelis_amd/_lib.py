@@ -36,7 +36,7 @@ APPLY_DRY_RUN = 1
 class Tuning(C.Structure):
     _fields_ = [("tile", C.c_uint32), ("dedup_rounds", C.c_uint32), ("commit_grid", C.c_uint32),
                 ("gc_grid", C.c_uint32), ("probe_blocks_bricks", C.c_uint32), ("probe_blocks", C.c_uint32),
-                ("brick_merge_rounds", C.c_uint32)]
+                ("brick_merge_rounds", C.c_uint32), ("brick_rounds", C.c_uint32)]
 
 
 class ApplyOpts(C.Structure):

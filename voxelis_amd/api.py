@@ -118,12 +118,13 @@ class VoxInterner:
         check(lib().vx_interner_sync(self.h), "sync")
 
     def set_tuning(self, tile=None, dedup_rounds=None, commit_grid=None, gc_grid=None,
-                   probe_blocks_bricks=None, probe_blocks=None, brick_merge_rounds=None):
+                   probe_blocks_bricks=None, probe_blocks=None, brick_merge_rounds=None, brick_rounds=None):
         """Launch shape of the apply (vx_tuning); None keeps a field."""
         t = self.get_tuning()
         for k, v in (("tile", tile), ("dedup_rounds", dedup_rounds), ("commit_grid", commit_grid),
                      ("gc_grid", gc_grid), ("probe_blocks_bricks", probe_blocks_bricks),
-                     ("probe_blocks", probe_blocks), ("brick_merge_rounds", brick_merge_rounds)):
+                     ("probe_blocks", probe_blocks), ("brick_merge_rounds", brick_merge_rounds),
+                     ("brick_rounds", brick_rounds)):
             if v is not None:
                 t[k] = int(v)
         tt = Tuning(*(t[f] for f, _ in Tuning._fields_))

@@ -27,11 +27,11 @@ def _time(step, steps, warmup):
 def autotune(make_case, load_factors=(0.5, 0.6, 0.7, 0.85), tiles=(128, 256, 512), dedups=(0, 2, 4, 8),
              commit_grids=(256, 1024, 4096), gc_grids=(256, 1024, 4096),
              probe_blocks_bricks=(0, 8192, 16384, 32768), probe_blocks=(0, 256, 512, 1024),
-             brick_merge_rounds=(0, 2, 4, 8), steps=6, warmup=2, log=None):
+             brick_merge_rounds=(0, 2, 4, 8), brick_rounds=(0, 1, 4), steps=6, warmup=2, log=None):
     """make_case(load_factor) -> (interner, step(k), close()); returns (best point, all points).
 
     A point is {"load_factor", "tile", "dedup_rounds", "commit_grid", "gc_grid",
-    "probe_blocks_bricks", "probe_blocks", "brick_merge_rounds", "s_per_step"}."""
+    "probe_blocks_bricks", "probe_blocks", "brick_merge_rounds", "brick_rounds", "s_per_step"}."""
     points = []
     best = None
     for lf in load_factors:
@@ -40,11 +40,11 @@ def autotune(make_case, load_factors=(0.5, 0.6, 0.7, 0.85), tiles=(128, 256, 512
             for tile in tiles:
                 for dd in dedups:
                     it.set_tuning(tile=tile, dedup_rounds=dd, commit_grid=1024, gc_grid=1024,
-                                  probe_blocks_bricks=32768, probe_blocks=512, brick_merge_rounds=4)
+                                  probe_blocks_bricks=32768, probe_blocks=512, brick_merge_rounds=4, brick_rounds=0)
                     t = _time(step, steps, warmup)
                     pt = {"load_factor": lf, "tile": tile, "dedup_rounds": dd, "commit_grid": 1024,
                           "gc_grid": 1024, "probe_blocks_bricks": 32768, "probe_blocks": 512,
-                          "brick_merge_rounds": 4, "s_per_step": t}
+                          "brick_merge_rounds": 4, "brick_rounds": 0, "s_per_step": t}
                     points.append(pt)
                     if log:
                         log(pt)
@@ -58,13 +58,13 @@ def autotune(make_case, load_factors=(0.5, 0.6, 0.7, 0.85), tiles=(128, 256, 512
     try:
         for field, choices in (("commit_grid", commit_grids), ("gc_grid", gc_grids),
                                ("probe_blocks_bricks", probe_blocks_bricks), ("probe_blocks", probe_blocks),
-                               ("brick_merge_rounds", brick_merge_rounds)):
+                               ("brick_merge_rounds", brick_merge_rounds), ("brick_rounds", brick_rounds)):
             for v in choices:
                 cand = dict(best)
                 cand[field] = v
                 it.set_tuning(**{f: cand[f] for f in ("tile", "dedup_rounds", "commit_grid", "gc_grid",
                                                        "probe_blocks_bricks", "probe_blocks",
-                                                       "brick_merge_rounds")})
+                                                       "brick_merge_rounds", "brick_rounds")})
                 cand["s_per_step"] = _time(step, steps, warmup)
                 points.append(cand)
                 if log:

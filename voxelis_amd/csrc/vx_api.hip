@@ -157,7 +157,7 @@ struct vx_interner {
     int ring_cur = 0;
     void* h_pinned = nullptr;  // the slot of the current call
     Prof prof;
-    vx_tuning tune = {256, 8, 1024, 1024, 32768, 512, 4};
+    vx_tuning tune = {256, 8, 1024, 1024, 32768, 512, 4, 0};
     // captured launch sequences of repeated calls (vx_interner_set_graphs)
     bool graphs = false;
     hipStream_t cap_stream = nullptr;
@@ -299,7 +299,8 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
                                  (uint64_t)opt.alloc_limit, (uint64_t)opt.counts, (uint64_t)TILE,
                                  (uint64_t)h->tune.dedup_rounds, (uint64_t)h->tune.commit_grid,
                                  (uint64_t)h->tune.gc_grid, (uint64_t)h->tune.probe_blocks_bricks,
-                                 (uint64_t)h->tune.probe_blocks, (uint64_t)h->tune.brick_merge_rounds};
+                                 (uint64_t)h->tune.probe_blocks, (uint64_t)h->tune.brick_merge_rounds,
+                                 (uint64_t)h->tune.brick_rounds};
         for (uint64_t w : head) key = mix64(key ^ w);
         for (size_t i = 0; i < n; i++) {
             const vx_batch_view& b = batches[i];
@@ -423,12 +424,20 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     std::memset(K, 0, sizeof(K));
     uint32_t* tiles_h[kLevels + 1] = {nullptr};
     uint64_t ntiles[kLevels + 1] = {0};
+    int br = 1;  // bricks per lane of the brick pass
     OldNodes O;
     std::memset(&O, 0, sizeof(O));
     for (uint32_t hh = 1; hh <= dmax; hh++) {
         ntiles[hh] = std::max<uint64_t>((L[hh].nslots + TILE - 1) / TILE, 1);
         // the brick pass takes R tiles per block and writes every tile's count
-        if (hh == 1 && !setm) ntiles[hh] = (ntiles[hh] + kBR - 1) / kBR * kBR;
+        if (hh == 1 && !setm) {
+            // several bricks per lane once the pass has blocks to spare for every
+            // compute unit; a small apply keeps one per lane (more blocks, shorter chains)
+            br = h->tune.brick_rounds == 1 ? 1
+                 : h->tune.brick_rounds != 0 ? kBR
+                 : L[1].nslots >= (uint64_t)kBR * TILE * 512 ? kBR : 1;
+            ntiles[hh] = (ntiles[hh] + br - 1) / br * br;
+        }
         uint64_t ns = ntiles[hh] * TILE;
         LevelBufs& B = h->lv[hh];
         if (!B.res.ensure(ns * 8) || !B.key.ensure(ns * 72) || !B.hash.ensure(ns * 8) ||
@@ -531,10 +540,12 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, np + nd, kBlock, P, dchunks, (uint32_t)n, roots, pre_status, np,
               L[3], doff, O);
     unsigned t1 = (unsigned)ntiles[1];
-    if (dmax >= 2)
+    if (dmax >= 2 && br == kBR)
         VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE VX_COMMA kBR>, t1 / kBR, TILE, P, dchunks, L[1],
-                  res_ptrs_h[1], K[1],
-                  tiles_h[1], ddef, ddrift, O.at[2]);
+                  res_ptrs_h[1], K[1], tiles_h[1], ddef, ddrift, O.at[2]);
+    else if (dmax >= 2)
+        VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE VX_COMMA 1>, t1, TILE, P, dchunks, L[1],
+                  res_ptrs_h[1], K[1], tiles_h[1], ddef, ddrift, O.at[2]);
     VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc<T>, 1, 1024, P, dchunks, (uint32_t)n, nvalues);
     if (dmax >= 2) {
         VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_retry<T VX_COMMA TILE>, 256, kBlock, P, dchunks, L[1], res_ptrs_h[1], K[1],
@@ -1121,7 +1132,8 @@ int vx_trees_canonical_hash(vx_interner* h, size_t n, const uint64_t* roots, uin
 int vx_interner_set_tuning(vx_interner* h, const vx_tuning* t) {
     if (!h || !t) return VX_E_INVALID_ARG;
     if (t->tile != 128 && t->tile != 256 && t->tile != 512) return VX_E_INVALID_ARG;
-    if (t->dedup_rounds > 16 || t->brick_merge_rounds > 16 || t->commit_grid == 0 || t->commit_grid > (1u << 20) || t->gc_grid == 0 ||
+    if (t->dedup_rounds > 16 || t->brick_merge_rounds > 16 ||
+        (t->brick_rounds != 0 && t->brick_rounds != 1 && t->brick_rounds != 4) || t->commit_grid == 0 || t->commit_grid > (1u << 20) || t->gc_grid == 0 ||
         t->gc_grid > (1u << 20))
         return VX_E_INVALID_ARG;
     (void)hipStreamSynchronize(h->stream);
