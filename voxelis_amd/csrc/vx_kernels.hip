@@ -1028,9 +1028,10 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
 
 // Probe of the candidates of tiles [t0, t0+tn) of one level (tn <= 64), packed
 // densely over the block's lanes: sparse tiles fill whole wavefronts, and equal keys
-// meet in the same wavefront more often.  Per pass over TILE of them: merge equal
-// keys in each wavefront, look the leaders' keys up / insert them
-// (get_or_create_branch, interner/mod.rs:716-829), then allocate the owners' nodes
+// meet in the same pass more often.  Per pass over TILE of them: merge equal keys of
+// the pass through an LDS hash set (dedup_rounds 0 turns the merge off), look the
+// leaders' keys up / insert them (get_or_create_branch, interner/mod.rs:716-829),
+// then allocate the owners' nodes
 // with one counter update for the whole block (a counter every block of a launch
 // updates is a serial point), and publish the results.  Owners are listed by
 // allocation rank for the commit, which runs after the last level's probe.  Every
@@ -1041,8 +1042,15 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
 template <typename T, int TILE>
 __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint32_t* tile_cnt, uint64_t* res,
                                    const ChunkDev* chunks, const LevelInfo& L, uint32_t t0, uint32_t tn) {
-    __shared__ uint32_t s_pre[65];  // s_pre[t]: candidates of the group's tiles before tile t
+    constexpr uint32_t kSlots = 2 * TILE;  // the block's key set of one pass (load <= 1/2)
+    __shared__ uint32_t s_pre[65];         // s_pre[t]: candidates of the group's tiles before tile t
     __shared__ uint32_t s_own, s_base;
+    __shared__ uint32_t s_set[kSlots];     // hash slot -> lane + 1 of the key's leader (0: free)
+    __shared__ uint64_t s_hash[TILE];      // by lane: the pass's key hash
+    __shared__ uint32_t s_cand[TILE];      // ... its candidate index
+    __shared__ uint64_t s_res[TILE];       // ... a leader's result
+    __shared__ int32_t s_kind[TILE];       // ... and kind
+    for (uint32_t i = threadIdx.x; i < kSlots; i += TILE) s_set[i] = 0;
     if (threadIdx.x < 64) {
         uint32_t x = threadIdx.x < tn ? tile_cnt[t0 + threadIdx.x] : 0u;
 #pragma unroll
@@ -1060,6 +1068,7 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
     const uint32_t N = s_pre[tn];
     const uint32_t wbase = threadIdx.x & ~63u;
     const int lane = (int)__lane_id();
+    const uint32_t me = threadIdx.x;
     // A wavefront with no candidate in any pass ends here (whole wavefronts only: a
     // barrier waits for the wavefronts of the block that have not ended).
     if (wbase >= N) return;
@@ -1067,10 +1076,10 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
         const bool wlive = j0 + wbase < N;  // the wavefront has candidates
         const uint32_t j = j0 + threadIdx.x;
         bool valid = j < N;
-        uint64_t k = 0, slot = 0, result = kInvalid, tslot = ~0ull;
-        uint64_t key8 = 0;
-        int kind = -1, leader = lane;
-        uint32_t tree = kNoIdx;
+        uint64_t k = 0, slot = 0, result = kInvalid, tslot = ~0ull, h = 0;
+        uint64_t key[9];
+        int kind = -1;
+        uint32_t leader = me, mine = kNoIdx, tree = kNoIdx;
         if (wlive) {
             const uint32_t jj = valid ? j : j0 + wbase;  // the wavefront's first candidate exists
             uint32_t lo = 0, hi = tn;  // s_pre[lo] <= jj < s_pre[hi]: jj lies in tile lo
@@ -1080,42 +1089,54 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
                 else hi = mid;
             }
             k = (uint64_t)(t0 + lo) * TILE + (jj - s_pre[lo]);
-            uint64_t key[9];
 #pragma unroll
             for (int i = 0; i < 9; i++) key[i] = key_word(K, k, i);
-            key8 = key[8];
-            const uint64_t h = K.hash[k];
+            h = K.hash[k];
             slot = K.slot[k];
-            leader = wave_merge_keys(P.dedup_rounds, valid, key, h);
-            // the table slot the key hashes to, read by leaders only (duplicates of a
-            // hot key would all read one line), together with the tree's status
-            uint64_t e0 = valid && leader == lane ? P.tab[h & P.tmask] : kSlotNone;
             if (L.h == 1 || P.per_tree) {
                 const GroupPos G = group_pos(L, slot);
                 uint32_t c = G.chunk_base + (uint32_t)((slot - G.base) >> (3 * (G.depth - L.h)));
                 const ChunkDev* Cp = chunks + c;
-                const int32_t cstatus = Cp->status;
                 tree = Cp->orig;
                 // a brick of a tree that hit the leaf-brick panic later in the same pass
-                bool drop = valid && L.h == 1 && cstatus < 0;
-                if (drop) {
+                if (valid && L.h == 1 && Cp->status < 0) {
                     res[slot] = kInvalid;
                     K.id[k] = kInvalid;
                     K.tslot[k] = ~0ull;
                     valid = false;
                 }
-                if (__ballot(drop)) {  // rare: merge again without the dropped lanes
-                    leader = wave_merge_keys(P.dedup_rounds, valid, key, h);
-                    e0 = kSlotNone;
-                }
             }
-            uint32_t probes = 0, claims = 0;
-            if (valid && leader == lane)
-                kind = probe_table(P, L.h, k, key, h, e0, &result, &tslot, &probes, &claims);
-            wave_add64(stat(P.ctl, ST_PROBES), probes);
-            wave_add64(stat(P.ctl, ST_CLAIMS), (claims & 0xFFFFu) + (claims >> 16));
-            wave_add64(stat(P.ctl, ST_CLAIMS_LOST), claims >> 16);
+            if (valid) {
+                s_hash[me] = h;
+                s_cand[me] = (uint32_t)k;
+            }
         }
+        __syncthreads();  // the pass's hashes and candidate indices are published
+        // Equal keys of the pass share one leader: the first lane to take the key's free
+        // slot of the block's LDS set (linear probing; a full compare against the slot
+        // holder's candidate, which an earlier launch wrote).
+        if (valid && P.dedup_rounds) {
+            uint32_t s = (uint32_t)h & (kSlots - 1);
+            for (uint32_t g = 0; g < kSlots; g++) {
+                const uint32_t prev = atomicCAS(&s_set[s], 0u, me + 1);
+                if (prev == 0) {
+                    mine = s;
+                    break;
+                }
+                const uint32_t o = prev - 1;
+                if (s_hash[o] == h && key_matches_cand(K, s_cand[o], key)) {
+                    leader = o;
+                    break;
+                }
+                s = (s + 1) & (kSlots - 1);
+            }
+        }
+        const bool lead = valid && leader == me;
+        uint32_t probes = 0, claims = 0;
+        if (lead) kind = probe_table(P, L.h, k, key, h, kSlotNone, &result, &tslot, &probes, &claims);
+        wave_add64(stat(P.ctl, ST_PROBES), probes);
+        wave_add64(stat(P.ctl, ST_CLAIMS), (claims & 0xFFFFu) + (claims >> 16));
+        wave_add64(stat(P.ctl, ST_CLAIMS_LOST), claims >> 16);
         // owners' allocation ranks: wavefront offsets in LDS, one counter update per block
         const uint64_t ob = __ballot(kind == 2);
         uint32_t woff = 0;
@@ -1134,11 +1155,11 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
             const uint32_t ka = s_base + woff + (uint32_t)__popcll(ob & ((1ull << lane) - 1ull));
             if (P.dry) {
                 // a placeholder id (the table slot's previous value is restored after the run)
-                result = make_branch(P.capacity + 65536u + ka, 0, (uint32_t)key8);
+                result = make_branch(P.capacity + 65536u + ka, 0, (uint32_t)key[8]);
             } else {
                 bool ok;
                 uint32_t idx = alloc_at(P, ka, &ok);
-                result = ok ? make_branch(idx, P.gens[idx], (uint32_t)key8) : kInvalid;
+                result = ok ? make_branch(idx, P.gens[idx], (uint32_t)key[8]) : kInvalid;
                 if (ok) {
                     // the LOD value (core/voxel.rs:96-141) from the children's, which
                     // earlier launches wrote: a node gets its value at its own level
@@ -1151,27 +1172,29 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
             }
             if (P.per_tree && tree != kNoIdx) atomicAdd(&P.per_tree[tree], 1u);
         }
-        if (wlive) {
-            uint64_t lres = shfl64(result, leader);
-            int lkind = __shfl(kind, leader);
-            if (valid && leader != lane) {
-                result = lres;
-                kind = lkind == 0 ? 0 : 1;  // a follower of an owner or of a duplicate is a duplicate
-            }
-            if (valid) {
-                K.id[k] = result;
-                K.tslot[k] = tslot;
-                res[slot] = result;
-            }
-            wave_count(stat(P.ctl, ST_HIT), valid && kind == 0);
-            wave_count(stat(P.ctl, ST_DUP), valid && kind == 1);
-            wave_count(stat(P.ctl, ST_MISS), valid && kind == 2);
+        if (lead) {
+            s_res[me] = result;
+            s_kind[me] = kind;
         }
+        __syncthreads();  // the leaders' results are published
+        if (valid && !lead) {
+            result = s_res[leader];
+            kind = s_kind[leader] == 0 ? 0 : 1;  // a follower of an owner or of a duplicate is a duplicate
+        }
+        if (valid) {
+            K.id[k] = result;
+            K.tslot[k] = tslot;
+            res[slot] = result;
+        }
+        if (mine != kNoIdx) s_set[mine] = 0;  // every lane's probing ended before the barrier above
+        wave_count(stat(P.ctl, ST_HIT), valid && kind == 0);
+        wave_count(stat(P.ctl, ST_DUP), valid && kind == 1);
+        wave_count(stat(P.ctl, ST_MISS), valid && kind == 2);
     }
 }
 
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
                                                  uint64_t* res, const ChunkDev* chunks, LevelInfo L,
                                                  uint32_t ntiles, uint32_t group) {
     const uint32_t t0 = blockIdx.x * group;
