@@ -191,7 +191,7 @@ struct Pool {
     uint32_t capacity;
     uint32_t dry;          // 1: count what an apply would create, change nothing
     uint32_t alloc_limit;  // allocations this apply may make (its reservation)
-    uint32_t dedup_rounds;  // wavefront key-merge rounds before the table probe
+    uint32_t dedup_rounds;  // 0: every key probes the table; else equal keys of a pass merge first
     uint32_t brick_merge_rounds;  // ... of the brick pass, before its keys become candidates
     uint32_t* per_tree;    // optional: new branch nodes owned by each tree (caller's index)
     const Cands* levels;   // candidate arrays of every level (pending keys of earlier levels)
