@@ -31,7 +31,6 @@ constexpr uint64_t kFillMark = (1ull << 63) | (0x7FFFull << 32);
 // A pending slot's payload is a candidate index of level `level` (1..7).
 constexpr uint64_t kSlotEmpty = 0ull;
 constexpr uint64_t kSlotTomb = 1ull;
-constexpr uint64_t kSlotNone = 2ull;  // never stored (every key's tag is >= 1): "not read"
 
 __host__ __device__ inline bool bid_is_leaf(uint64_t id) { return (id >> 63) != 0; }
 __host__ __device__ inline uint32_t bid_index(uint64_t id) { return (uint32_t)id; }

@@ -983,14 +983,13 @@ __device__ inline int match_slot(const Pool<T>& P, uint32_t level, uint64_t e, u
 // (A tombstone is claimed on the same terms: it cannot turn back into a tombstone.)
 template <typename T>
 __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, const uint64_t key[9],
-                                  uint64_t h, uint64_t e0, uint64_t* result, uint64_t* tslot,
+                                  uint64_t h, uint64_t* result, uint64_t* tslot,
                                   uint32_t* probes, uint32_t* claims) {
     uint32_t tag = slot_tag(h);
     uint64_t s = h & P.tmask;
     uint64_t first_free = ~0ull;
     for (uint64_t guard = 0; guard <= 4 * (P.tmask + 1); guard++) {
-        // e0: the first slot as loaded by the caller, if it did
-        uint64_t e = guard == 0 && e0 != kSlotNone ? e0 : P.tab[s];
+        uint64_t e = P.tab[s];
         (*probes)++;
         if (e == kSlotEmpty) {
             uint64_t target = first_free != ~0ull ? first_free : s;
@@ -1133,7 +1132,7 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
         }
         const bool lead = valid && leader == me;
         uint32_t probes = 0, claims = 0;
-        if (lead) kind = probe_table(P, L.h, k, key, h, kSlotNone, &result, &tslot, &probes, &claims);
+        if (lead) kind = probe_table(P, L.h, k, key, h, &result, &tslot, &probes, &claims);
         wave_add64(stat(P.ctl, ST_PROBES), probes);
         wave_add64(stat(P.ctl, ST_CLAIMS), (claims & 0xFFFFu) + (claims >> 16));
         wave_add64(stat(P.ctl, ST_CLAIMS_LOST), claims >> 16);
