@@ -439,6 +439,9 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
             ntiles[hh] = (ntiles[hh] + br - 1) / br * br;
         }
         uint64_t ns = ntiles[hh] * TILE;
+        // slot and candidate indices travel as 32-bit values (pending results, the
+        // owner list, deferred bricks)
+        if (ns > 0xFFFFFFFFull) return VX_E_INVALID_ARG;
         LevelBufs& B = h->lv[hh];
         if (!B.res.ensure(ns * 8) || !B.key.ensure(ns * 72) || !B.hash.ensure(ns * 8) ||
             !B.id.ensure(ns * 8) || !B.tslot.ensure(ns * 8) || !B.slot.ensure(ns * 8) ||

@@ -682,8 +682,9 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
 // are queued in LDS with their loaded data and take phase 2, one per lane, in
 // rounds of TILE: the old-tree descent and brick_work.  Round r of phase 2 emits
 // into candidate tile blk*R + r; a deferred brick keeps that tile (its round has a
-// free place for it) for the retry.  The register budget is 8 waves per SIMD: the
-// pass is bound by memory latency, so more loads in flight hide more of it.
+// free place for it) for the retry.  The pass is bound by memory latency, so it asks
+// for 8 waves per SIMD (the four-slot form settles at 6: its phase-1 loads and phase-2
+// keys do not fit 64 registers without spilling).
 // Rounds of the brick pass per block: 1024 slots for TILE <= 256, 1024 or more above.
 template <int TILE>
 constexpr int brick_rounds() { return TILE >= 512 ? 2 : 4; }
