@@ -1,10 +1,11 @@
 """Autotuner of the apply's launch shape: tile size (lanes per block of the build /
-probe / commit passes), wavefront key-merge rounds before the table probe (the
-probe schedule), commit and GC grid sizes, how many tiles a probe block packs
-(at the brick level and above), and the pattern table's load factor.
+probe / commit passes), the probe's key merge (dedup_rounds: 0 = off), commit and
+GC grid sizes, how many tiles a probe block packs (at the brick level and above),
+the brick pass's key-merge rounds and slots per lane, and the pattern table's load
+factor.
 
 The search is exhaustive over tile x dedup rounds for every load factor, then
-coordinate descent over the grid sizes and the probe packing from the best point.
+coordinate descent over the other fields from the best point.
 Each point times `steps` applies (after `warmup`) of the caller's workload with
 the device clock.
 """

@@ -85,13 +85,13 @@ struct LevelBufs {
 // Kernel ids of the per-kernel timer (vx_interner_profile_*).
 enum KernelId : uint32_t {
     KID_BEGIN = 0, KID_PREPARE, KID_BRICK_SCAN, KID_LEAF_ALLOC, KID_BRICK_BUILD, KID_PROBE_BRICKS,
-    KID_COMMIT_BRICKS, KID_LEVEL_BUILD, KID_PROBE_UPPER, KID_COMMIT, KID_FINALIZE,
+    KID_UNUSED, KID_LEVEL_BUILD, KID_PROBE_UPPER, KID_COMMIT, KID_FINALIZE,
     KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_ROLLBACK, KID_TO_VEC,
     KID_SET_DESCEND, KID_SET_LEVEL, KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_begin", "k_prepare", "k_brick_retry", "k_leaf_alloc", "k_brick_build", "k_probe[bricks]",
-    "k_commit[bricks]", "k_level_build", "k_probe[upper]", "k_commit", "k_finish",
+    "", "k_level_build", "k_probe[upper]", "k_commit", "k_finish",
     "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill", "k_rollback", "k_to_vec",
     "k_set_descend", "k_set_level"};
 
