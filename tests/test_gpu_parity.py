@@ -227,14 +227,17 @@ def test_terrain_small_grid_and_alternation(vx):
     assert s["h4_undercount"] == 0 and s["h5_collapse_cloned"] == 0
 
 
-def test_uneven_u16_mixed_depths_with_depth7(vx):
+@pytest.mark.parametrize("brick_rounds", [0, 1])
+def test_uneven_u16_mixed_depths_with_depth7(vx, brick_rounds):
     """Config 3 in miniature: u16, depths 5/6/7, dense set_sum-style and sparse random
-    trees in one launch; depth 7 is compared with the oracle's extended depth."""
+    trees in one launch; depth 7 is compared with the oracle's extended depth.  Its
+    635 k brick slots take four slots per lane by size (0), or one when forced."""
     api, W = vx
     spec = [(5, "dense", 0, 1), (6, "sparse", 0.01, 2), (7, "sparse", 0.001, 3), (6, "dense", 0, 4),
             (5, "sparse", 0.1, 5), (7, "dense", 0, 6), (5, "sparse", 0.001, 7), (6, "sparse", 0.1, 8)]
     batches = W.uneven_batches(spec)
     it = api.VoxInterner(2, 256 * MB)
+    it.set_tuning(brick_rounds=brick_rounds)
     it_o = O.Interner(2, 256 * MB, extended_depth=True)
     f = api.Forest(it, [d for d, _, _, _ in spec])
     trees_o = []
