@@ -682,15 +682,16 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
 // are queued in LDS with their loaded data and take phase 2, one per lane, in
 // rounds of TILE: the old-tree descent and brick_work.  Round r of phase 2 emits
 // into candidate tile blk*R + r; a deferred brick keeps that tile (its round has a
-// free place for it) for the retry.  The pass is bound by memory latency, so it asks
-// for 8 waves per SIMD (the four-slot form settles at 6: its phase-1 loads and phase-2
-// keys do not fit 64 registers without spilling).
+// free place for it) for the retry.  The pass is bound by memory latency, so it runs
+// as many waves per SIMD as fit without spilling: 8 for one slot per lane; the
+// four-slot form's phase-1 loads and phase-2 keys take more registers (6 waves for u8,
+// 5 for u16).
 // Rounds of the brick pass per block: 1024 slots for TILE <= 256, 1024 or more above.
 template <int TILE>
 constexpr int brick_rounds() { return TILE >= 512 ? 2 : 4; }
 
 template <typename T, int TILE, int R>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8 : (sizeof(T) == 1 ? 6 : 5), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
                                                        uint64_t* deferred, Drift* drift,
                                                        const uint64_t* old2) {
