@@ -68,12 +68,14 @@ def _state(it, f):
     return (it.counts(), it.alive_nodes(), f.roots_host(), f.canonical_hashes())
 
 
-def test_dry_run_counts_exactly_and_changes_nothing(vx):
+@pytest.mark.parametrize("brick_rounds", [0, 4])
+def test_dry_run_counts_exactly_and_changes_nothing(vx, brick_rounds):
     api, W = vx
     grid = 4
     n = grid * grid
     (mA, vA, hmA, hvA), (mB, vB, hmB, hvB, fills, patches) = _batches(api, W, grid)
     it = api.VoxInterner(1, 64 * MB)
+    it.set_tuning(brick_rounds=brick_rounds)  # 0: one brick slot per lane at this size
     it_o = O.Interner(1, 64 * MB)
     f = api.Forest(it, [5] * n)
     trees_o = [O.Tree(5) for _ in range(n)]
