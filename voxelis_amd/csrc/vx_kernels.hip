@@ -674,6 +674,11 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
     return 2;
 }
 
+// Brick slots per lane of the brick pass's several-slot form: 4 (2 for 512-lane
+// tiles), so a block takes 512 to 1024 slots.
+template <int TILE>
+constexpr int brick_rounds() { return TILE >= 512 ? 2 : 4; }
+
 // Two phases per block of R*TILE consecutive brick slots (R per lane, slot =
 // block start + j*TILE + lane in round j).  Phase 1 issues every round's loads at
 // once (edit mask, the 8 values and the old height-2 node above the brick) and
@@ -686,10 +691,6 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
 // as many waves per SIMD as fit without spilling: 8 for one slot per lane; the
 // four-slot form's phase-1 loads and phase-2 keys take more registers (6 waves for u8,
 // 5 for u16).
-// Rounds of the brick pass per block: 1024 slots for TILE <= 256, 1024 or more above.
-template <int TILE>
-constexpr int brick_rounds() { return TILE >= 512 ? 2 : 4; }
-
 template <typename T, int TILE, int R>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8 : (sizeof(T) == 1 ? 6 : 5), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
