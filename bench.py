@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--budget-mib", type=int, default=2048)
     ap.add_argument("--load-factor", type=float, default=0.5)
+    ap.add_argument("--brick-rounds", type=int, default=None, choices=[0, 1, 2, 4],
+                    help="uneven config: brick slots per lane of the brick pass (default: the library's)")
     ap.add_argument("--plain-trees", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -469,6 +471,8 @@ def bench_uneven(a):
     from voxelis_amd import workloads as W
     depths, mk, vA_list, vB_list, edits, E = _uneven_setup(a, api, W, torch)
     it = api.VoxInterner(2, a.budget_mib << 20, a.load_factor)
+    if a.brick_rounds is not None:
+        it.set_tuning(brick_rounds=a.brick_rounds)
     f = api.Forest(it, depths)
     views = [f.make_views(mk, vA_list), f.make_views(mk, vB_list)]
     if not a.no_profile:

@@ -88,7 +88,7 @@ int vx_interner_set_stream(vx_interner* h, void* hip_stream);
  * per block), separately for the brick level (probe_blocks_bricks) and the levels
  * above; brick_merge_rounds: wavefront key-merge rounds of the brick pass, before
  * its keys become candidates (0-16; equal keys of one tree share a candidate);
- * brick_rounds: brick slots per lane of the brick pass (1 or 4, 0: 4 when the
+ * brick_rounds: brick slots per lane of the brick pass (1, 2 or 4; 0: 4 when the
  * pass has at least 512 blocks of them, else 1).
  * Defaults {256, 8, 1024, 1024, 32768, 512, 4, 0}.  Synchronizes the interner's
  * stream. */

@@ -365,7 +365,7 @@ def test_graph_replay_matches_direct_launches(vx):
 
 
 @pytest.mark.parametrize("tile,dedup,grid_blocks,pb,br", [(128, 0, 64, 0, 4), (512, 2, 4096, 1, 1), (256, 8, 1024, 3, 4),
-                                                        (256, 4, 1024, 0, 0)])
+                                                        (256, 4, 1024, 0, 0), (256, 8, 1024, 0, 2)])
 def test_launch_shapes_match_oracle(vx, tile, dedup, grid_blocks, pb, br):
     """Every tile size / probe schedule / grid size / probe tile grouping the
     autotuner may pick gives the oracle's trees (two applies, so the second replaces

@@ -28,7 +28,7 @@ def _time(step, steps, warmup):
 def autotune(make_case, load_factors=(0.5, 0.6, 0.7, 0.85), tiles=(128, 256, 512), dedups=(0, 2, 4, 8),
              commit_grids=(256, 1024, 4096), gc_grids=(256, 1024, 4096),
              probe_blocks_bricks=(0, 8192, 16384, 32768), probe_blocks=(0, 256, 512, 1024),
-             brick_merge_rounds=(0, 2, 4, 8), brick_rounds=(0, 1, 4), steps=6, warmup=2, log=None):
+             brick_merge_rounds=(0, 2, 4, 8), brick_rounds=(0, 1, 2, 4), steps=6, warmup=2, log=None):
     """make_case(load_factor) -> (interner, step(k), close()); returns (best point, all points).
 
     A point is {"load_factor", "tile", "dedup_rounds", "commit_grid", "gc_grid",

@@ -434,6 +434,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
             // several bricks per lane once the pass has blocks to spare for every
             // compute unit; a small apply keeps one per lane (more blocks, shorter chains)
             br = h->tune.brick_rounds == 1 ? 1
+                 : h->tune.brick_rounds == 2 ? 2
                  : h->tune.brick_rounds != 0 ? kBR
                  : L[1].nslots >= (uint64_t)kBR * TILE * 512 ? kBR : 1;
             ntiles[hh] = (ntiles[hh] + br - 1) / br * br;
@@ -546,12 +547,15 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     if (dmax >= 2 && br == kBR)
         VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE VX_COMMA kBR>, t1 / kBR, TILE, P, dchunks, L[1],
                   res_ptrs_h[1], K[1], tiles_h[1], ddef, ddrift, O.at[2]);
+    else if (dmax >= 2 && br == 2)
+        VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE VX_COMMA 2>, t1 / 2, TILE, P, dchunks, L[1],
+                  res_ptrs_h[1], K[1], tiles_h[1], ddef, ddrift, O.at[2]);
     else if (dmax >= 2)
         VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE VX_COMMA 1>, t1, TILE, P, dchunks, L[1],
                   res_ptrs_h[1], K[1], tiles_h[1], ddef, ddrift, O.at[2]);
     VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc<T>, 1, 1024, P, dchunks, (uint32_t)n, nvalues);
     if (dmax >= 2) {
-        VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_retry<T VX_COMMA TILE>, 256, kBlock, P, dchunks, L[1], res_ptrs_h[1], K[1],
+        VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_retry<T VX_COMMA TILE>, 1024, kBlock, P, dchunks, L[1], res_ptrs_h[1], K[1],
                   tiles_h[1], ddef, ddrift);
         uint32_t g1 = probe_group(t1, h->tune.probe_blocks_bricks);
         VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T VX_COMMA TILE>, (t1 + g1 - 1) / g1, TILE, P, K[1], tiles_h[1],
@@ -1136,7 +1140,8 @@ int vx_interner_set_tuning(vx_interner* h, const vx_tuning* t) {
     if (!h || !t) return VX_E_INVALID_ARG;
     if (t->tile != 128 && t->tile != 256 && t->tile != 512) return VX_E_INVALID_ARG;
     if (t->dedup_rounds > 16 || t->brick_merge_rounds > 16 ||
-        (t->brick_rounds != 0 && t->brick_rounds != 1 && t->brick_rounds != 4) || t->commit_grid == 0 || t->commit_grid > (1u << 20) || t->gc_grid == 0 ||
+        (t->brick_rounds != 0 && t->brick_rounds != 1 && t->brick_rounds != 2 && t->brick_rounds != 4) ||
+        t->commit_grid == 0 || t->commit_grid > (1u << 20) || t->gc_grid == 0 ||
         t->gc_grid > (1u << 20))
         return VX_E_INVALID_ARG;
     (void)hipStreamSynchronize(h->stream);
