@@ -30,6 +30,18 @@ def test_library_exports_every_declared_symbol():
     assert _lib.lib().vx_status_string(-6) == b"apply produced the current root"
 
 
+def test_ctypes_mirrors_the_header_structs():
+    """The Python mirrors of vx_tuning and of the VX_STAT_* list follow the header."""
+    from voxelis_amd import _lib
+    text = open(os.path.join(ROOT, "include", "voxelis_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct \{([^}]*)\} vx_tuning;", text).group(1)
+    assert re.findall(r"uint32_t (\w+);", body) == [f for f, _ in _lib.Tuning._fields_]
+    assert ctypes.sizeof(_lib.Tuning) == 4 * len(_lib.Tuning._fields_)
+    stats = re.search(r"enum \{\s*(VX_STAT_\w+ = 0,.*?)VX_STAT_COUNT", text, re.S).group(1)
+    assert len(re.findall(r"VX_STAT_\w+", stats)) == len(_lib.STAT_NAMES)
+
+
 def test_morton_encoder_matches_oracle():
     from voxelis_amd.morton import encode_path
     rng = np.random.default_rng(5)
