@@ -1212,7 +1212,9 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(7, 8))) vo
 template <typename T, int TILE>
 __global__ __launch_bounds__(TILE) void k_commit(Pool<T> P) {
     __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
+    __shared__ uint32_t s_owned;
     agg_init(akey, acnt);
+    if (threadIdx.x == 0) s_owned = 0;
     __syncthreads();
     uint32_t owned = 0;
     const uint32_t n = P.ctl->alloc_count - P.ctl->own_base;
@@ -1239,10 +1241,15 @@ __global__ __launch_bounds__(TILE) void k_commit(Pool<T> P) {
             if (!agg_add(akey, acnt, ci, 1u)) atomicAdd(&P.refs[ci], 1u);
         }
     }
+    // the block's owners go to the live count with one update (a counter every block
+    // of the launch updates is a serial point)
+    const uint32_t wo = wave_sum(owned);
+    if (__lane_id() == 0 && wo) atomicAdd(&s_owned, wo);
     __syncthreads();
     for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x)
         if (akey[i] != kNoIdx && acnt[i]) atomicAdd(&P.refs[akey[i]], acnt[i]);
-    wave_add64(&P.ctl->live_branches, owned);
+    if (threadIdx.x == 0 && s_owned)
+        atomicAdd((unsigned long long*)&P.ctl->live_branches, (unsigned long long)s_owned);
 }
 
 // The first allocation rank of the apply's probes (the owners' list base), set by a
