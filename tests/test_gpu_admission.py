@@ -122,12 +122,14 @@ def _clear_all_and_compare(it, f, it_o, trees_o):
     assert f.roots_host() == [0] * f.n
 
 
-def test_node_limit_rolls_back(vx):
+@pytest.mark.parametrize("brick_rounds", [0, 4])
+def test_node_limit_rolls_back(vx, brick_rounds):
     api, W = vx
     grid = 4
     n = grid * grid
     (mA, vA, hmA, hvA), (mB, vB, hmB, hvB, fills, patches) = _batches(api, W, grid)
     it = api.VoxInterner(1, 64 * MB)
+    it.set_tuning(brick_rounds=brick_rounds)
     it_o = O.Interner(1, 64 * MB)
     f = api.Forest(it, [5] * n)
     trees_o = [O.Tree(5) for _ in range(n)]
@@ -154,7 +156,8 @@ def test_node_limit_rolls_back(vx):
     _clear_all_and_compare(it, f, it_o, trees_o)
 
 
-def test_capacity_oom_rolls_back_and_interner_stays_usable(vx):
+@pytest.mark.parametrize("brick_rounds", [0, 4])
+def test_capacity_oom_rolls_back_and_interner_stays_usable(vx, brick_rounds):
     api, W = vx
     grid = 4
     n = grid * grid
@@ -170,6 +173,7 @@ def test_capacity_oom_rolls_back_and_interner_stays_usable(vx):
     cap = alive_a + b + l - 1
     budget = cap * (78 + 1)
     it = api.VoxInterner(1, budget)
+    it.set_tuning(brick_rounds=brick_rounds)
     assert it.capacity() == cap
     it_o = O.Interner(1, budget)
     f = api.Forest(it, [5] * n)
