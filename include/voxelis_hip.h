@@ -111,6 +111,15 @@ int vx_interner_get_tuning(const vx_interner* h, vx_tuning* tuning);
  * per-kernel launches.  Up to 4 sequences are kept; any workspace growth rebuilds
  * them.  Off by default; profiling (vx_interner_profile) bypasses it. */
 int vx_interner_set_graphs(vx_interner* h, int enable);
+/* Pattern-table maintenance (no reference counterpart: the reference's HashMap
+ * removes entries outright, interner/mod.rs:518).  Frees here leave tombstones; a
+ * write call rebuilds the table by itself (clear, re-insert every live branch) once
+ * tombstones take 1/16 of the slots and live entries plus tombstones 3/4.
+ * rebuild_table forces a rebuild now and synchronizes; table_stats reports the slot
+ * count, the current tombstones and the rebuilds so far (synchronizes).  Either
+ * output pointer may be NULL. */
+int vx_interner_rebuild_table(vx_interner* h, uint64_t* rebuilds);
+int vx_interner_table_stats(vx_interner* h, uint64_t* slots, uint64_t* tombstones, uint64_t* rebuilds);
 /* Wait for all work on the interner's stream; returns the first device error. */
 int vx_interner_sync(vx_interner* h);
 
@@ -306,8 +315,10 @@ int vx_mlp_adam(void* hip_stream, float* params, const float* grad, float* m, fl
 
 /* Atomic-throughput ceiling (no reference counterpart): `ops` operations on
  * uniformly random addresses of buf (device, `slots` entries of 8 bytes for kind 0
- * and 4 bytes otherwise, slots a power of two >= 64; kind 0: 64-bit CAS, 1: 32-bit
- * add, 2: 32-bit add on 64 hot counters).  Kernel
+ * and 4 bytes otherwise, slots a power of two >= 2048; kind 0: 64-bit CAS, 1: 32-bit
+ * add, 2: 32-bit add on 64 hot counters, 3: one add per wavefront on a single counter,
+ * 4: one add per wavefront on 64 counters 128 bytes apart; kinds 3-4 count one
+ * operation per lane, so their rate is per wavefront x 64).  Kernel
  * time in *ms.  Synchronizes the stream; buf contents are overwritten. */
 int vx_bench_atomics(void* hip_stream, int kind, void* buf, uint64_t slots, uint64_t ops, float* ms);
 /* Plain copy of `bytes` (a multiple of 16, 16-byte aligned buffers) with the kernel

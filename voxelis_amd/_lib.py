@@ -56,6 +56,8 @@ SIGNATURES = {
     "vx_interner_set_stream": (_int, [_vp, _vp]),
     "vx_interner_sync": (_int, [_vp]),
     "vx_interner_set_graphs": (_int, [_vp, _int]),
+    "vx_interner_rebuild_table": (_int, [_vp, C.POINTER(_u64)]),
+    "vx_interner_table_stats": (_int, [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
     "vx_interner_set_tuning": (_int, [_vp, C.POINTER(Tuning)]),
     "vx_interner_get_tuning": (_int, [_vp, C.POINTER(Tuning)]),
     "vx_trees_apply_batches": (_int, [_vp, _sz, _vp, C.POINTER(BatchView), _vp]),

@@ -135,6 +135,18 @@ class VoxInterner:
         check(lib().vx_interner_get_tuning(self.h, C.byref(tt)), "get_tuning")
         return {f: getattr(tt, f) for f, _ in Tuning._fields_}
 
+    def rebuild_table(self):
+        """Clear the pattern table and re-insert every live branch (synchronizes)."""
+        out = C.c_uint64(0)
+        check(lib().vx_interner_rebuild_table(self.h, C.byref(out)), "rebuild_table")
+        return out.value
+
+    def table_stats(self):
+        """(slots, tombstones, rebuilds so far) of the branch pattern table."""
+        a, b, c = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        check(lib().vx_interner_table_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "table_stats")
+        return a.value, b.value, c.value
+
     def set_graphs(self, enable=True):
         """Replay repeated batched applies from captured HIP graphs."""
         check(lib().vx_interner_set_graphs(self.h, 1 if enable else 0), "set_graphs")

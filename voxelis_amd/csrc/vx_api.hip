@@ -135,6 +135,7 @@ struct vx_interner {
     uint32_t* free_stack = nullptr;
     uint64_t* tab = nullptr;
     uint32_t* leaf_map = nullptr;
+    uint64_t* leaf_ids = nullptr;  // u16: the brick pass's leaf id of every value
     uint8_t* leaf_need = nullptr;
     uint32_t* new_leaves = nullptr;
     Ctl* ctl = nullptr;
@@ -156,6 +157,12 @@ struct vx_interner {
     int ring_next = 0;
     int ring_cur = 0;
     void* h_pinned = nullptr;  // the slot of the current call
+    // the control block, copied after every write call (pinned; read by the next write
+    // call once the copy completed) to schedule pattern-table rebuilds
+    Ctl* h_ctl = nullptr;
+    hipEvent_t ctr_ev = nullptr;
+    bool ctr_pending = false;
+    uint64_t rebuilds = 0;
     Prof prof;
     vx_tuning tune = {256, 8, 1024, 1024, 32768, 512, 4, 0};
     // captured launch sequences of repeated calls (vx_interner_set_graphs)
@@ -180,6 +187,7 @@ Pool<T> pool_of(vx_interner* h) {
     P.tab = h->tab;
     P.tmask = h->tslots - 1;
     P.leaf_map = h->leaf_map;
+    P.leaf_ids = h->leaf_ids;
     P.leaf_need = h->leaf_need;
     P.new_leaves = h->new_leaves;
     P.ctl = h->ctl;
@@ -541,8 +549,9 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     // chunk set-up, plus the old-tree descent over the level-3 slots in extra blocks
     unsigned np = grid_for(n);
     unsigned nd = L[3].nslots ? grid_for(L[3].nslots) : 0u;
-    VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, np + nd, kBlock, P, dchunks, (uint32_t)n, roots, pre_status, np,
-              L[3], doff, O);
+    unsigned nl = P.leaf_ids && dmax >= 2 ? 65536u / kBlock : 0u;
+    VX_LAUNCH(h, KID_PREPARE, k_prepare<T>, np + nd + nl, kBlock, P, dchunks, (uint32_t)n, roots, pre_status, np,
+              nd, L[3], doff, O);
     unsigned t1 = (unsigned)ntiles[1];
     if (dmax >= 2 && br == kBR)
         VX_LAUNCH(h, KID_BRICK_BUILD, k_brick_build<T VX_COMMA TILE VX_COMMA kBR>, t1 / kBR, TILE, P, dchunks, L[1],
@@ -624,14 +633,54 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     return hipGraphLaunch(plan->exec, s) == hipSuccess ? VX_OK : VX_E_HIP;
 }
 
+// Tombstones in the pattern table: a counter striped over the statistics lines.
+uint64_t tombstones_of(const Ctl& c) {
+    uint64_t t = 0;
+    for (int i = 0; i < kStatStripes; i++) t += c.st[i][ST_TOMB];
+    return t;
+}
+
+// Clear the pattern table and re-insert every live branch (enqueued on the stream).
+template <typename T>
+int rebuild_table(vx_interner* h) {
+    Pool<T> P = pool_of<T>(h);
+    k_tab_clear<<<grid_for(h->tslots / 2), kBlock, 0, h->stream>>>(h->tab, h->tslots, h->ctl);
+    k_tab_reinsert<T><<<grid_for(h->capacity), kBlock, 0, h->stream>>>(P);
+    h->rebuilds++;
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
+// Before a write call: rebuild the table when the last completed counter copy shows
+// tombstones taking a sixteenth of the slots and live plus tombstones three quarters.
+template <typename T>
+int maybe_rebuild(vx_interner* h) {
+    if (!h->ctr_pending || hipEventQuery(h->ctr_ev) != hipSuccess) return VX_OK;
+    h->ctr_pending = false;
+    const uint64_t live = h->h_ctl->live_branches, tomb = tombstones_of(*h->h_ctl);
+    if (tomb >= h->tslots / 16 && live + tomb > h->tslots / 4 * 3) return rebuild_table<T>(h);
+    return VX_OK;
+}
+
+// After a write call: copy the counters for the next call's check.
+int note_counters(vx_interner* h) {
+    if (h->ctr_pending) return VX_OK;  // the previous copy has not been read yet
+    HIPCHK(hipMemcpyAsync(h->h_ctl, h->ctl, offsetof(Ctl, leaf_ids8), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipEventRecord(h->ctr_ev, h->stream));
+    h->ctr_pending = true;
+    return VX_OK;
+}
+
 template <typename T>
 int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
                int32_t* status, const int32_t* pre_status = nullptr, const ApplyOpts& opt = ApplyOpts()) {
+    int rc = maybe_rebuild<T>(h);
+    if (rc) return rc;
     switch (h->tune.tile) {
-        case 128: return apply_tiled<T, 128>(h, n, roots, batches, status, pre_status, opt);
-        case 512: return apply_tiled<T, 512>(h, n, roots, batches, status, pre_status, opt);
-        default: return apply_tiled<T, 256>(h, n, roots, batches, status, pre_status, opt);
+        case 128: rc = apply_tiled<T, 128>(h, n, roots, batches, status, pre_status, opt); break;
+        case 512: rc = apply_tiled<T, 512>(h, n, roots, batches, status, pre_status, opt); break;
+        default: rc = apply_tiled<T, 256>(h, n, roots, batches, status, pre_status, opt); break;
     }
+    return rc ? rc : note_counters(h);
 }
 
 template <typename T>
@@ -718,6 +767,8 @@ int apply_edits_impl(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* d
 
 template <typename T>
 int fill_impl(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* vals) {
+    int rc = maybe_rebuild<T>(h);
+    if (rc) return rc;
     Pool<T> P = pool_of<T>(h);
     size_t gcap = (size_t)h->capacity + n + 16;
     if (!h->gc_front[0].ensure(gcap * 4) || !h->gc_front[1].ensure(gcap * 4)) return VX_E_HIP;
@@ -737,7 +788,7 @@ int fill_impl(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* vals) {
     VX_LAUNCH(h, KID_FILL, k_fill_finish<T>, grid_for(n), kBlock, P, (uint32_t)n, roots, vals);
     VX_LAUNCH(h, KID_END_ALLOC, k_end_alloc<T>, 1, 256, P, false, (uint32_t*)nullptr);
     if (hipGetLastError() != hipSuccess) return VX_E_HIP;
-    return VX_OK;
+    return note_counters(h);
 }
 
 template <typename T>
@@ -828,7 +879,10 @@ int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_f
               dmalloc(&h->gens, cap * 2) && dmalloc(&h->free_stack, cap * 4) &&
               dmalloc(&h->tab, ts * 8) && dmalloc(&h->leaf_map, nv * 4) &&
               dmalloc(&h->leaf_need, nv) && dmalloc(&h->new_leaves, nv * 4) &&
+              (value_size != 2 || dmalloc(&h->leaf_ids, nv * 8)) &&
               dmalloc(&h->ctl, sizeof(Ctl));
+    ok = ok && hipHostMalloc((void**)&h->h_ctl, sizeof(Ctl), hipHostMallocDefault) == hipSuccess &&
+         hipEventCreateWithFlags(&h->ctr_ev, hipEventDisableTiming) == hipSuccess;
     if (!ok || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         vx_interner_destroy(h);
         return VX_E_HIP;
@@ -860,7 +914,7 @@ int vx_interner_destroy(vx_interner* h) {
     if (!h) return VX_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* ptrs[] = {h->children, h->tms, h->values, h->refs, h->gens, h->free_stack,
-                    h->tab, h->leaf_map, h->leaf_need, h->new_leaves, h->ctl};
+                    h->tab, h->leaf_map, h->leaf_need, h->new_leaves, h->ctl, h->leaf_ids};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->chunks.release();
@@ -890,6 +944,8 @@ int vx_interner_destroy(vx_interner* h) {
         b.old.release();
     }
     for (hipEvent_t e : h->prof.pool) (void)hipEventDestroy(e);
+    if (h->h_ctl) (void)hipHostFree(h->h_ctl);
+    if (h->ctr_ev) (void)hipEventDestroy(h->ctr_ev);
     for (int i = 0; i < vx_interner::kRing; i++) {
         if (h->h_pinned_ring[i]) (void)hipHostFree(h->h_pinned_ring[i]);
         if (h->ring_ev[i]) (void)hipEventDestroy(h->ring_ev[i]);
@@ -1155,6 +1211,27 @@ int vx_interner_get_tuning(const vx_interner* h, vx_tuning* t) {
     return VX_OK;
 }
 
+int vx_interner_rebuild_table(vx_interner* h, uint64_t* rebuilds) {
+    if (!h) return VX_E_INVALID_ARG;
+    if (h->poisoned) return VX_E_POISONED;
+    int rc = h->vsize == 1 ? rebuild_table<uint8_t>(h) : rebuild_table<uint16_t>(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (rebuilds) *rebuilds = h->rebuilds;
+    return VX_OK;
+}
+
+int vx_interner_table_stats(vx_interner* h, uint64_t* slots, uint64_t* tombstones, uint64_t* rebuilds) {
+    if (!h) return VX_E_INVALID_ARG;
+    Ctl c;
+    int rc = sync_read_ctl(h, &c);
+    if (rc) return rc;
+    if (slots) *slots = h->tslots;
+    if (tombstones) *tombstones = tombstones_of(c);
+    if (rebuilds) *rebuilds = h->rebuilds;
+    return VX_OK;
+}
+
 int vx_interner_set_graphs(vx_interner* h, int enable) {
     if (!h) return VX_E_INVALID_ARG;
     (void)hipStreamSynchronize(h->stream);
@@ -1195,7 +1272,7 @@ int vx_interner_profile_read(vx_interner* h, double* ms, uint64_t* launches, uin
 const char* vx_kernel_name(uint32_t id) { return id < (uint32_t)KID_COUNT ? kKernelNames[id] : ""; }
 
 int vx_bench_atomics(void* stream, int kind, void* buf, uint64_t slots, uint64_t ops, float* ms) {
-    if (!buf || !ms || kind < 0 || kind > 2 || slots < 64 || (slots & (slots - 1)) || ops == 0)
+    if (!buf || !ms || kind < 0 || kind > 4 || slots < 2048 || (slots & (slots - 1)) || ops == 0)
         return VX_E_INVALID_ARG;
     hipStream_t st = (hipStream_t)stream;
     const uint32_t per_lane = 64;

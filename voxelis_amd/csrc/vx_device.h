@@ -97,12 +97,14 @@ struct Ctl {
     uint32_t n_deferred;  // bricks waiting for leaves created later in the sequence
     uint64_t live_leaves;
     uint64_t live_branches;
-    uint64_t tombstones;
+    uint64_t pad0;
     uint32_t n_drift;  // queued reference-count corrections of the reference's quirks
     uint32_t dry_owners;      // dry run: branch keys that would be new
     uint32_t dry_new_leaves;  // dry run: leaves that would be new
     uint32_t abort_;          // this apply could not allocate: 1 capacity, 2 reservation
     uint32_t n_undo;          // entries of the undo log (reference-count deltas applied)
+    uint32_t h1_fail;         // a tree of this apply failed in the brick pass (leaf-brick panic)
+    uint32_t n_owners;        // new branch nodes of the last apply (its owner list), for the GC
     uint32_t pad1;
     uint32_t ncand[kLevels + 1];
     uint32_t gc_count[kLevels + 2];
@@ -184,6 +186,7 @@ struct Pool {
     uint64_t* tab;
     uint64_t tmask;
     uint32_t* leaf_map;  // value -> node index (0 = none)
+    uint64_t* leaf_ids;  // u16: the leaf id of every value at the start of the apply (k_prepare)
     uint8_t* leaf_need;
     uint32_t* new_leaves;
     Ctl* ctl;

@@ -29,7 +29,8 @@ namespace vx {
 
 enum Stat {
     ST_HIT = 0, ST_MISS, ST_DUP, ST_LEAF_MISS, ST_COLLAPSE, ST_RECYCLED, ST_H2, ST_H3, ST_H4, ST_H5,
-    ST_CAND, ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST, ST_MERGED
+    ST_CAND, ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST, ST_MERGED,
+    ST_TOMB  // tombstones in the pattern table (signed adds; a counter, not a statistic)
 };
 
 __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
@@ -310,6 +311,8 @@ __device__ inline void ctl_begin(Ctl* ctl) {
         ctl->dry_new_leaves = 0;
         ctl->abort_ = 0;
         ctl->n_undo = 0;
+        ctl->h1_fail = 0;
+        ctl->n_owners = 0;
         for (int i = 0; i <= kLevels; i++) ctl->ncand[i] = 0;
         for (int i = 0; i < kLevels + 2; i++) ctl->gc_count[i] = 0;
     }
@@ -352,6 +355,7 @@ __global__ void k_end_alloc(Pool<T> P, bool rollback, uint32_t* counts) {
     }
     if (threadIdx.x == 0) {
         uint32_t n = ctl->alloc_count;
+        ctl->n_owners = P.dry || n < ctl->own_base ? 0u : n - ctl->own_base;
         if (counts) {
             counts[0] = P.dry ? ctl->dry_owners : n - ctl->n_new_leaves;
             counts[1] = P.dry ? ctl->dry_new_leaves : ctl->n_new_leaves;
@@ -411,13 +415,23 @@ __device__ inline void descend_body(const Pool<T>& P, const ChunkDev* chunks, co
 }
 
 // Read roots, mark the fill leaf as needed (Phase 0, voxtree.rs:742-754).  Blocks
-// nprep.. run the old-tree descent over the level-3 slots instead.
+// nprep..nprep+ndesc run the old-tree descent over the level-3 slots instead, and the
+// blocks after them (u16) record the leaf id of every value for the brick pass.
 template <typename T>
 __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_t* roots,
-                          const int32_t* pre_status, uint32_t nprep, LevelInfo L3, const uint64_t* doff,
-                          OldNodes O) {
+                          const int32_t* pre_status, uint32_t nprep, uint32_t ndesc, LevelInfo L3,
+                          const uint64_t* doff, OldNodes O) {
+    if (blockIdx.x >= nprep + ndesc) {
+        if (P.leaf_ids)
+            for (uint32_t v = (blockIdx.x - nprep - ndesc) * blockDim.x + threadIdx.x; v < 65536u;
+                 v += (gridDim.x - nprep - ndesc) * blockDim.x) {
+                const uint32_t li = P.leaf_map[v];
+                P.leaf_ids[v] = li ? make_leaf(li, P.gens[li]) : kEmpty;
+            }
+        return;
+    }
     if (blockIdx.x >= nprep) {
-        uint64_t stride = (uint64_t)(gridDim.x - nprep) * blockDim.x;
+        uint64_t stride = (uint64_t)ndesc * blockDim.x;
         for (uint64_t s = (uint64_t)(blockIdx.x - nprep) * blockDim.x + threadIdx.x; s < L3.nslots; s += stride)
             descend_body(P, chunks, L3, doff, n, O, roots, s);
         return;
@@ -510,19 +524,64 @@ __global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t n
 // Append a branch key to this tile's region of the candidate arrays.
 // Compaction of a tile's candidate keys into the tile's region of K (order of
 // discovery).  Called by every thread of the block.
+__device__ inline void write_candidate(const Cands& K, uint64_t k, const uint64_t ch[8], uint32_t tm, uint64_t h,
+                                       uint64_t slot) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) key_word(K, k, i) = ch[i];
+    key_word(K, k, 8) = tm;
+    K.hash[k] = h;
+    K.slot[k] = slot;
+}
+
+// The candidate was written by an earlier launch (every launch starts with coherent
+// caches) or by this block before a barrier.
+__device__ inline bool key_matches_cand(const Cands& K, uint32_t k2, const uint64_t key[9]) {
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+        if (key_word(K, k2, i) != key[i]) return false;
+    return true;
+}
+
+template <int TILE>
+__device__ inline uint32_t emit_candidate(const Cands& K, uint32_t* s_cnt, bool pred, const uint64_t ch[8],
+                                          uint32_t tm, uint64_t slot, uint32_t blk, uint64_t h) {
+    uint32_t pos = wave_append(s_cnt, pred);
+    if (pred) write_candidate(K, (uint64_t)blk * TILE + pos, ch, tm, h, slot);
+    return pos;
+}
+
 template <int TILE>
 __device__ inline uint32_t emit_candidate(const Cands& K, uint32_t* s_cnt, bool pred, const uint64_t ch[8],
                                           uint32_t tm, uint64_t slot, uint32_t blk) {
-    uint32_t pos = wave_append(s_cnt, pred);
-    if (pred) {
-        uint64_t k = (uint64_t)blk * TILE + pos;
-#pragma unroll
-        for (int i = 0; i < 8; i++) key_word(K, k, i) = ch[i];
-        key_word(K, k, 8) = tm;
-        K.hash[k] = branch_hash(ch, tm);
-        K.slot[k] = slot;
+    return emit_candidate<TILE>(K, s_cnt, pred, ch, tm, slot, blk, pred ? branch_hash(ch, tm) : 0ull);
+}
+
+// Block-wide merge of equal branch keys before they become candidates: a set in LDS
+// maps a key's full 64-bit hash to the candidate of the first lane that inserted it
+// (the key's leader).  A lane that finds its hash present is a follower once a full-key
+// compare against the leader's candidate (written by this block before a barrier)
+// confirms it; the merge crosses trees, since a key's node is the same for every tree.
+// Returns the set slot (*lead when this lane inserted it), or -1 when no slot was found
+// within a bounded scan (the key is then emitted on its own).  Hash 0 marks a free slot.
+template <uint32_t N>
+__device__ inline int keyset_insert(uint64_t* s_h, uint64_t h, bool* lead) {
+    const uint64_t hk = h ? h : 1ull;
+    uint32_t s = (uint32_t)hk & (N - 1);
+    *lead = false;
+#pragma unroll 1
+    for (uint32_t g = 0; g < 32; g++) {
+        uint64_t cur = s_h[s];
+        if (cur == 0) {
+            cur = atomicCAS((unsigned long long*)&s_h[s], 0ull, (unsigned long long)hk);
+            if (cur == 0) {
+                *lead = true;
+                return (int)s;
+            }
+        }
+        if (cur == hk) return (int)s;
+        s = (s + 1) & (N - 1);
     }
-    return pos;
+    return -1;
 }
 
 // Tile statistics in LDS, flushed with one striped atomic per counter per block.
@@ -534,30 +593,6 @@ __device__ inline void tile_count(uint32_t* s, int id, bool pred) {
     if (b && (int)__lane_id() == __ffsll((unsigned long long)b) - 1) atomicAdd(&s[id], (uint32_t)__popcll(b));
 }
 
-// Wavefront-level merge of equal keys over `rounds` rounds: each round takes the
-// lowest open lane's key as the reference and closes every open lane holding it.
-// Returns the lane's leader (the lowest lane holding its key, or itself when its
-// key was not reached).
-__device__ inline int wave_merge_keys(uint32_t rounds, bool valid, const uint64_t key[9], uint64_t h) {
-    int lane = (int)__lane_id();
-    int leader = lane;
-    bool open = valid;
-    for (uint32_t r = 0; r < rounds; r++) {
-        uint64_t act = __ballot(open);
-        if (!act) break;
-        int L = __ffsll((unsigned long long)act) - 1;  // wave-uniform: a scalar broadcast
-        bool same = open && h == readlane64(h, L);
-#pragma unroll
-        for (int w = 0; w < 9; w++) same = same && key[w] == readlane64(key[w], L);
-        if (same) {
-            leader = L;
-            open = false;
-        }
-        if (lane == L) open = false;
-    }
-    return leader;
-}
-
 // Per brick (voxtree.rs:778-896): descend the old tree to the brick (792-822), then
 // the all-same brick becomes a leaf; otherwise the brick's children are built from
 // the old node, a split leaf, or nothing, and the changed slots get leaves.
@@ -565,8 +600,9 @@ __device__ inline int wave_merge_keys(uint32_t rounds, bool valid, const uint64_
 // 3 = a needed leaf does not exist yet (marked; the brick is retried later).
 // Reference-count corrections of the reference's split-leaf quirks are queued in
 // *drift_idx / *drift_delta (applied once the tree's status is final).
-// The leaf id of a value, or kEmpty when no leaf holds it.  leaf_ids (LDS, u8 only)
-// caches it for every value; otherwise the direct-mapped table is read.
+// The leaf id of a value, or kEmpty when no leaf holds it.  leaf_ids (u8: LDS, u16: the
+// table k_prepare writes) caches it for every value; otherwise the direct-mapped table is
+// read.
 template <typename T>
 __device__ inline uint64_t leaf_lookup(const Pool<T>& P, const uint64_t* leaf_ids, uint32_t v) {
     if (leaf_ids) return leaf_ids[v];
@@ -692,7 +728,7 @@ constexpr int brick_rounds() { return TILE >= 512 ? 2 : 4; }
 // four-slot form's phase-1 loads and phase-2 keys take more registers (6 waves for u8,
 // 5 for u16).
 template <typename T, int TILE, int R>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8 : (sizeof(T) == 1 ? 6 : 5), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8 : (sizeof(T) == 1 ? 5 : 4), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
                                                        uint64_t* deferred, Drift* drift,
                                                        const uint64_t* old2) {
@@ -708,6 +744,13 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8
     __shared__ Vox8 s_v[R * TILE];
     __shared__ uint16_t s_idx[R * TILE];
     __shared__ uint8_t s_m[R * TILE];
+    // the block's key set (keyset_insert): half as many slots as the block has bricks,
+    // enough for the few distinct keys of a dense region; a block of all-distinct keys
+    // fills it and emits the rest unmerged
+    constexpr uint32_t kSetB = R * TILE / 2;
+    __shared__ uint64_t s_kh[kSetB];
+    __shared__ uint32_t s_kc[kSetB];
+    for (uint32_t i = threadIdx.x; i < kSetB; i += TILE) s_kh[i] = 0;
     uint64_t lm[sizeof(T) == 1 ? (256 + TILE - 1) / TILE : 1];
     if (sizeof(T) == 1)
         for (uint32_t i = 0, v = threadIdx.x; v < 256; i++, v += TILE) lm[i] = P.ctl->leaf_ids8[v];
@@ -737,7 +780,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8
     }
     if (sizeof(T) == 1)
         for (uint32_t i = 0, vv = threadIdx.x; vv < 256; i++, vv += TILE) s_leaf[vv] = lm[i];
-    const uint64_t* leaf_ids = sizeof(T) == 1 ? s_leaf : nullptr;
+    const uint64_t* leaf_ids = sizeof(T) == 1 ? s_leaf : P.leaf_ids;
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
     if (threadIdx.x < R) s_cnt[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_nq = 0;
@@ -821,6 +864,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8
             }
             if (h1) {
                 chunks[c].status = -5;
+                P.ctl->h1_fail = 1u;
             } else {
                 uint32_t didx;
                 int32_t dd;
@@ -838,29 +882,38 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8
                 }
             }
         }
-        // Equal keys of one tree in the wavefront share one candidate: the lowest lane
-        // holding the key emits it; the others point at it (a pending result, resolved
-        // through the candidate's id once the probe has run).  The chunk is folded into
-        // the compared hash (an odd multiplier keeps distinct chunks apart), so a
-        // candidate the probe drops with its failed tree has no follower in another.
+        // Equal keys of the block (all rounds, any tree) share one candidate: the leader
+        // emits it, followers point at it (a pending result, resolved through the
+        // candidate's id once the probe has run).
         uint64_t key[9];
 #pragma unroll
         for (int i = 0; i < 8; i++) key[i] = ch[i];
         key[8] = tm;
-        const uint64_t hk = cand ? branch_hash(ch, tm) + (uint64_t)c * 0x9E3779B97F4A7C15ull : 0ull;
-        const int leader = wave_merge_keys(P.brick_merge_rounds, cand, key, hk);
-        const bool lead = cand && leader == (int)__lane_id();
+        const uint64_t hk = cand ? branch_hash(ch, tm) : 0ull;
+        bool lead = false;
+        const int ks = cand && P.brick_merge_rounds ? keyset_insert<kSetB>(s_kh, hk, &lead) : -1;
         const uint32_t tile = blockIdx.x * R + (uint32_t)r;
-        const uint32_t pos = emit_candidate<TILE>(K, &s_cnt[r], lead, ch, tm, slot, tile);
-        const uint32_t kl = (uint32_t)((uint64_t)tile * TILE) + __shfl(pos, leader);
-        if (cand && !lead) out = res_pending(kl);
+        const uint32_t pos = emit_candidate<TILE>(K, &s_cnt[r], cand && (lead || ks < 0), ch, tm, slot, tile, hk);
+        if (lead) s_kc[ks] = (uint32_t)((uint64_t)tile * TILE) + pos;
+        __syncthreads();  // the leaders' candidates and their indices are visible to the block
+        bool follow = false;
+        if (cand && ks >= 0 && !lead) {
+            const uint32_t k2 = s_kc[ks];
+            if (key_matches_cand(K, k2, key)) {
+                out = res_pending(k2);
+                follow = true;
+            } else {  // another key with the same 64-bit hash: a candidate of its own
+                const uint32_t p2 = atomicAdd(&s_cnt[r], 1u);
+                write_candidate(K, (uint64_t)tile * TILE + p2, ch, tm, hk, slot);
+            }
+        }
         if (q < nq) res1[slot] = out;
         const uint32_t dpos = wave_append(&P.ctl->n_deferred, defer);
         if (defer) deferred[dpos] = ((uint64_t)tile << 32) | (uint32_t)slot;
         tile_count(s_st, 0, cand);
         tile_count(s_st, 1, (flags & 1) != 0);
         tile_count(s_st, 2, (flags & 4) != 0);
-        tile_count(s_st, 3, cand && !lead);
+        tile_count(s_st, 3, follow);
     }
     __syncthreads();
     if (threadIdx.x < R) tile_cnt[blockIdx.x * R + threadIdx.x] = s_cnt[threadIdx.x];
@@ -945,13 +998,6 @@ __device__ inline bool key_matches_node(const Pool<T>& P, uint32_t idx, const ui
     return true;
 }
 
-// The candidate was written by an earlier launch (every launch starts with coherent caches).
-__device__ inline bool key_matches_cand(const Cands& K, uint32_t k2, const uint64_t key[9]) {
-#pragma unroll
-    for (int i = 0; i < 9; i++)
-        if (key_word(K, k2, i) != key[i]) return false;
-    return true;
-}
 
 // get_or_create_branch (interner/mod.rs:716-829) for one launch's candidates:
 // committed keys resolve to the existing node, the first inserter of a new key
@@ -990,13 +1036,19 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
                                   uint32_t* probes, uint32_t* claims) {
     uint32_t tag = slot_tag(h);
     uint64_t s = h & P.tmask;
-    uint64_t first_free = ~0ull;
+    uint64_t first_free = ~0ull;  // the first tombstone since the scan (re)started
+    uint64_t scanned = 0;         // slots scanned since then
     for (uint64_t guard = 0; guard <= 4 * (P.tmask + 1); guard++) {
         uint64_t e = P.tab[s];
         (*probes)++;
-        if (e == kSlotEmpty) {
-            uint64_t target = first_free != ~0ull ? first_free : s;
-            uint64_t expected = target == s ? kSlotEmpty : kSlotTomb;
+        const bool tomb = e == kSlotTomb;
+        if (tomb && first_free == ~0ull) first_free = s;
+        // A key that is not present is claimed at the first empty slot, or at the first
+        // tombstone before it; when a whole table's worth of slots held no empty one
+        // (tombstones pile up between rebuilds), at the first tombstone seen.
+        if (e == kSlotEmpty || (first_free != ~0ull && scanned > P.tmask)) {
+            const uint64_t target = first_free != ~0ull ? first_free : s;
+            const uint64_t expected = first_free != ~0ull ? kSlotTomb : kSlotEmpty;
             uint64_t seen = expected;
             bool won = __hip_atomic_compare_exchange_strong(
                 &P.tab[target], &seen, slot_make(tag, true, level, (uint32_t)k), __ATOMIC_RELAXED,
@@ -1013,15 +1065,15 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
             if (m >= 0) return m;
             s = (target + 1) & P.tmask;  // another key took the slot: go on behind it
             first_free = ~0ull;
+            scanned = 0;
             continue;
         }
-        if (e == kSlotTomb) {
-            if (first_free == ~0ull) first_free = s;
-        } else {
+        if (!tomb) {
             int m = match_slot(P, level, e, tag, key, result);
             if (m >= 0) return m;
         }
         s = (s + 1) & P.tmask;
+        scanned++;
     }
     atomicOr(&P.ctl->err, kErrTableFull | kErrPoison);
     *result = kInvalid;
@@ -1095,18 +1147,13 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
             for (int i = 0; i < 9; i++) key[i] = key_word(K, k, i);
             h = K.hash[k];
             slot = K.slot[k];
-            if (L.h == 1 || P.per_tree) {
+            // (a key of a tree that hit the leaf-brick panic is interned all the same:
+            // keys merge across trees, so other trees may point at its candidate; the
+            // GC releases the nodes nothing references afterwards)
+            if (P.per_tree) {
                 const GroupPos G = group_pos(L, slot);
                 uint32_t c = G.chunk_base + (uint32_t)((slot - G.base) >> (3 * (G.depth - L.h)));
-                const ChunkDev* Cp = chunks + c;
-                tree = Cp->orig;
-                // a brick of a tree that hit the leaf-brick panic later in the same pass
-                if (valid && L.h == 1 && Cp->status < 0) {
-                    res[slot] = kInvalid;
-                    K.id[k] = kInvalid;
-                    K.tslot[k] = ~0ull;
-                    valid = false;
-                }
+                tree = chunks[c].orig;
             }
             if (valid) {
                 s_hash[me] = h;
@@ -1136,6 +1183,10 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
         const bool lead = valid && leader == me;
         uint32_t probes = 0, claims = 0;
         if (lead) kind = probe_table(P, L.h, k, key, h, &result, &tslot, &probes, &claims);
+        // a claimed tombstone is a tombstone less (a dry or aborted apply restores it)
+        const uint32_t tclaim = __popcll(__ballot(kind == 2 && (tslot & kTslotTomb)));
+        if (lane == 0 && tclaim)
+            atomicAdd((unsigned long long*)stat(P.ctl, ST_TOMB), (unsigned long long)(-(long long)tclaim));
         wave_add64(stat(P.ctl, ST_PROBES), probes);
         wave_add64(stat(P.ctl, ST_CLAIMS), (claims & 0xFFFFu) + (claims >> 16));
         wave_add64(stat(P.ctl, ST_CLAIMS_LOST), claims >> 16);
@@ -1277,8 +1328,11 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
                                         uint32_t* tile_cnt, uint32_t blk, const uint64_t* old_h) {
     __shared__ uint32_t s_cnt;
     __shared__ uint32_t s_st[8];
+    __shared__ uint64_t s_kh[TILE];  // the block's key set (keyset_insert)
+    __shared__ uint32_t s_kc[TILE];
     if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_cnt = 0;
+    s_kh[threadIdx.x] = 0;
     __syncthreads();
     {
         // Loads in three rounds without branches between them (lanes past the end
@@ -1401,16 +1455,38 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
                 }
             }
         }
+        // equal keys of the block (any tree) share one candidate, as in the brick pass
+        uint64_t key[9];
+#pragma unroll
+        for (int i = 0; i < 8; i++) key[i] = ch[i];
+        key[8] = tm;
+        const uint64_t hk = cand ? branch_hash(ch, tm) : 0ull;
+        bool lead = false;
+        const int ks = cand && P.brick_merge_rounds ? keyset_insert<TILE>(s_kh, hk, &lead) : -1;
+        const uint32_t pos = emit_candidate<TILE>(K, &s_cnt, cand && (lead || ks < 0), ch, tm, slot, blk, hk);
+        if (lead) s_kc[ks] = (uint32_t)((uint64_t)blk * TILE) + pos;
+        __syncthreads();  // the leaders' candidates and their indices are visible to the block
+        bool follow = false;
+        if (cand && ks >= 0 && !lead) {
+            const uint32_t k2 = s_kc[ks];
+            if (key_matches_cand(K, k2, key)) {
+                out = res_pending(k2);
+                follow = true;
+            } else {  // another key with the same 64-bit hash: a candidate of its own
+                const uint32_t p2 = atomicAdd(&s_cnt, 1u);
+                write_candidate(K, (uint64_t)blk * TILE + p2, ch, tm, hk, slot);
+            }
+        }
         if (valid) res[slot] = out;
-        emit_candidate<TILE>(K, &s_cnt, cand, ch, tm, slot, blk);
         tile_count(s_st, 0, cand);
         tile_count(s_st, 1, collapsed);
         tile_count(s_st, 2, h5);
+        tile_count(s_st, 3, follow);
     }
     __syncthreads();
     if (threadIdx.x == 0) tile_cnt[blk] = s_cnt;
-    const int ids[3] = {ST_CAND, ST_COLLAPSE, ST_H5};
-    if (threadIdx.x < 3 && s_st[threadIdx.x])
+    const int ids[4] = {ST_CAND, ST_COLLAPSE, ST_H5, ST_MERGED};
+    if (threadIdx.x < 4 && s_st[threadIdx.x])
         atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
 }
 
@@ -1452,6 +1528,7 @@ __device__ inline void rollback_body(const Pool<T>& P, const ChunkDev* chunks, c
         uint64_t ts = K.tslot[k];
         if (ts == ~0ull) continue;
         P.tab[tslot_index(ts)] = (ts & kTslotTomb) ? kSlotTomb : kSlotEmpty;
+        wave_count(stat(P.ctl, ST_TOMB), (ts & kTslotTomb) != 0);
         uint64_t id = K.id[k];
         if (P.dry || id == kInvalid) continue;
         owned++;
@@ -1594,6 +1671,28 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
     uint64_t n = P.ctl->gc_count[round];
     if (n > front_cap) n = front_cap;
     uint32_t* next_count = &P.ctl->gc_count[round + 1];
+    // New branch nodes that nothing references go to the next round: keys of a tree that
+    // failed in the brick pass (the leaf-brick panic) are interned like every key, and
+    // the ones no other tree shares end up without a reference.  Every other new node
+    // holds one from its parent or its tree (the commit and the root swap ran).
+    if (leaf_cleanup && P.ctl->h1_fail) {
+        const uint32_t no = P.ctl->n_owners;
+        VX_LOOP(i, no) {
+            uint32_t idx = kNoIdx;
+            if (i < no) {
+                const uint64_t e = P.owners[i];
+                const Cands& K = P.levels[e >> 32];
+                const uint64_t id = K.id[(uint32_t)e];
+                if (K.tslot[(uint32_t)e] != ~0ull && id != kInvalid && P.refs[bid_index(id)] == 0u)
+                    idx = bid_index(id);
+            }
+            const uint32_t pos = wave_append(next_count, idx != kNoIdx);
+            if (idx != kNoIdx) {
+                if (pos < front_cap) next[pos] = idx;
+                else atomicOr(&P.ctl->err, kErrPoison);
+            }
+        }
+    }
     const int lane = (int)__lane_id();
     for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < n; b0 += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t i = b0 + threadIdx.x;  // the loop is uniform over the block
@@ -1696,7 +1795,7 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
         if (s_leaves) atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)(-(long long)s_leaves));
         if (s_branches) {
             atomicAdd((unsigned long long*)&P.ctl->live_branches, (unsigned long long)(-(long long)s_branches));
-            atomicAdd((unsigned long long*)&P.ctl->tombstones, (unsigned long long)s_branches);
+            atomicAdd((unsigned long long*)stat(P.ctl, ST_TOMB), (unsigned long long)s_branches);
         }
     }
     __syncthreads();
@@ -1704,6 +1803,40 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
         uint32_t pos = s_bnext + k;
         if (pos < front_cap) next[pos] = s_next[k];
         else atomicOr(&P.ctl->err, kErrPoison);
+    }
+}
+
+// ------------------------------------------------------------------ table rebuild
+
+// Frees leave tombstones in the pattern table.  A later key reuses one only when its
+// scan meets it before an empty slot, so between rebuilds the non-empty share of the
+// table grows and scans lengthen.  A rebuild clears the table and re-inserts every live
+// branch (a node with a types/mask word: the GC clears it on free, leaves have none).
+__global__ void k_tab_clear(uint64_t* tab, uint64_t nslots, Ctl* ctl) {
+    uint4* t4 = (uint4*)tab;
+    const uint64_t n4 = nslots / 2;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x)
+        t4[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (blockIdx.x == 0 && threadIdx.x < kStatStripes) ctl->st[threadIdx.x][ST_TOMB] = 0;
+}
+
+template <typename T>
+__global__ void k_tab_reinsert(Pool<T> P) {
+    const uint32_t top = P.ctl->next_index;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < top; i += gridDim.x * blockDim.x) {
+        const uint32_t tm = P.tms[i];
+        if (i == 0 || tm == 0) continue;
+        uint64_t ch[8];
+        const uint64_t* src = P.children + 8ull * i;
+#pragma unroll
+        for (int j = 0; j < 8; j++) ch[j] = src[j];
+        const uint64_t h = branch_hash(ch, tm);
+        const uint64_t want = slot_make(slot_tag(h), false, 0, i);
+        uint64_t s = h & P.tmask;
+        for (uint64_t guard = 0; guard <= P.tmask; guard++) {
+            if (atomicCAS((unsigned long long*)&P.tab[s], 0ull, (unsigned long long)want) == 0ull) break;
+            s = (s + 1) & P.tmask;
+        }
     }
 }
 

@@ -4,6 +4,9 @@
 // kind 0: 64-bit atomicCAS(expected 0) on uniformly random slots of a table
 // kind 1: 32-bit atomicAdd on uniformly random counters
 // kind 2: 32-bit atomicAdd on 64 hot counters (the contended case)
+// kind 3: one 32-bit atomicAdd per wavefront on a single counter (a serial point such
+//         as an allocation cursor that every block of a launch updates)
+// kind 4: one 32-bit atomicAdd per wavefront on 64 counters 128 bytes apart (striped)
 // Every lane issues `per_lane` operations; addresses come from a splitmix64 stream
 // of the global lane index, so runs are reproducible.
 #include <hip/hip_runtime.h>
@@ -26,9 +29,14 @@ __global__ void k_atomic_bench(void* buf, uint64_t slots, uint32_t per_lane, int
         } else if (kind == 1) {
             uint32_t* c = (uint32_t*)buf;
             acc += atomicAdd(&c[x & (slots - 1)], 1u);
-        } else {
+        } else if (kind == 2) {
             uint32_t* c = (uint32_t*)buf;
             acc += atomicAdd(&c[x & 63u], 1u);
+        } else {
+            uint32_t* c = (uint32_t*)buf;
+            uint32_t r = 0;
+            if ((threadIdx.x & 63u) == 0) r = atomicAdd(&c[kind == 3 ? 0u : 32u * (blockIdx.x & 63u)], 1u);
+            acc += __shfl(r, 0);
         }
     }
     if (acc == 0x5A5A5A5A5A5A5A5Aull) sink[0] = acc;  // keeps the returns live
