@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
 """Voxel edits/s into one shared SVO-DAG on one MI355X.
 
-Workload (the headline configuration): a 64x64 grid of 32^3 trees (MaxDepth 5,
-u8 voxels) sharing one interner; a Perlin-dune height field (seeded 2-D gradient
-noise, 4 octaves, lacunarity 2, gain 0.5, base period 128) with id 1 below the
-surface and 2 at it; one batch per tree, all trees applied in one launch
-sequence.  As in the reference's criterion harness (two prebuilt batches applied
-alternately to the same trees), every step applies the whole grid again with the
-ids swapped, so each step rebuilds all 4096 trees and reclaims the previous ones.
+Headline workload (default, --config uneven): 4096 trees sharing one u16 interner, each
+MaxDepth drawn uniformly from {5, 6, 7} (32^3 to 128^3); per tree either a dense fill
+v = ((x+y+z) mod 65535) + 1 (probability 0.5) or sparse uniform random edits at a
+density drawn from {0.1 %, 1 %, 10 %} with values uniform in [1, 65535]; one batch per
+tree, all trees in one launch sequence (replayed from a captured HIP graph).  As the
+reference's criterion harness applies two prebuilt batches alternately to the same
+trees (voxtree_bench.rs:553-579), steps alternate the values v and v+1, so every step
+rebuilds every tree and reclaims the previous one.  --config terrain is the 64x64 grid
+of 32^3 u8 Perlin-dune trees.
 
-One JSON line: whole-job edits/s of the batched form, p50/p99 per-step latency,
-the plain per-tree form on the same GPU, the dominant kernel against the HBM
-roofline, and the oracle (literal CPU restatement of the reference) timed on the
-host as the CPU baseline.
+One JSON line: whole-job edits/s of the batched form; p50/p99 per-launch latency over at
+least --lat-steps launches; the same trees sliced into launches small enough for a p99
+of at most --p99-ms, with their edits/s; the trees of depth <= 6 and of depth 7 on their
+own; the plain per-tree form on the same GPU; every kernel's algorithmic bytes against
+the HBM roofline, with the committed rocprofv3 traffic beside them; and the oracle (a
+literal CPU restatement of the reference) on a bounded sample as the CPU baseline.
 """
 import argparse
 import json
@@ -27,6 +31,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "voxel edits/s into shared DAG (batched vs plain, same GPU); p50/p99 batch latency; GB/s"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
+UNEVEN_WORKLOAD = ("uneven batched edits: 4096 u16 trees, MaxDepth uniform over {5,6,7}, dense "
+                   "((x+y+z) mod 65535)+1 with p=0.5 else sparse uniform random at 0.1/1/10 %, one batch "
+                   "per tree, all trees in one launch sequence, values alternating v and v+1 per step")
+TERRAIN_WORKLOAD = ("perlin-dune terrain, 64x64 trees of 32^3 (MaxDepth 5), u8, one batch per tree, all "
+                    "trees in one launch sequence, ids alternating 1/2 and 2/1 per step")
+HC_KERNELS = ("k_brick_build", "k_brick_retry", "k_probe[bricks]", "k_level_build", "k_probe[upper]", "k_commit")
+# rocprofv3 kernel symbol -> the library's kernel name
+TRACE_NAMES = {"k_probe_bricks": "k_probe[bricks]", "k_probe": "k_probe[upper]"}
 
 
 def parse():
@@ -34,16 +46,23 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--lat-steps", type=int, default=200,
+                    help="launches timed one by one (HIP events) for the p50/p99 latency")
+    ap.add_argument("--p99-ms", type=float, default=2.0, help="latency bound of the sliced form")
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--budget-mib", type=int, default=2048)
+    ap.add_argument("--budget-mib", type=int, default=None,
+                    help="interner budget (default 16384 for uneven, 2048 for terrain)")
     ap.add_argument("--load-factor", type=float, default=0.5)
     ap.add_argument("--brick-rounds", type=int, default=None, choices=[0, 1, 2, 4],
-                    help="uneven config: brick slots per lane of the brick pass (default: the library's)")
-    ap.add_argument("--plain-trees", type=int, default=256)
+                    help="brick slots per lane of the brick pass (default: the library's)")
+    ap.add_argument("--plain-trees", type=int, default=96)
+    ap.add_argument("--cpu-trees", type=int, default=256, help="uneven: trees in the CPU baseline's sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--quick", action="store_true",
+                    help="timed steps only (for profiler runs): no latency loop, slices, subsets, plain or CPU legs")
     ap.add_argument("--graphs", type=int, default=1,
                     help="replay the captured launch sequence of repeated applies (HIP graphs)")
     ap.add_argument("--tune-workload", default="terrain", choices=["terrain", "uneven"])
@@ -52,240 +71,513 @@ def parse():
     ap.add_argument("--requests", type=int, default=1500, help="serve: requests per rate")
     ap.add_argument("--train-requests", type=int, default=1500,
                     help="serve: requests logged (with exact dry-run counts) to train the predictor")
-    ap.add_argument("--config", default="terrain",
-                    choices=["terrain", "plain", "uneven", "gets", "set", "vtm", "serve", "calib", "tune"],
-                    help="terrain: the headline (64x64 Perlin-dune trees); plain: one 32^3 tree; "
-                         "uneven: u16 trees of depth 5/6/7, dense or sparse; gets: batched reads; "
-                         "set: single-voxel set/remove, one voxel per tree per call")
+    ap.add_argument("--config", default="uneven",
+                    choices=["uneven", "terrain", "plain", "gets", "set", "vtm", "serve", "calib", "tune"],
+                    help="uneven: the headline (4096 u16 trees of depth 5/6/7, dense or sparse); terrain: "
+                         "64x64 Perlin-dune u8 trees; plain: one 32^3 tree; gets: batched reads; set: "
+                         "single-voxel set/remove, one voxel per tree per call")
     ap.add_argument("--trees", type=int, default=4096)
     ap.add_argument("--queries", type=int, default=1 << 24)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (a known-size kernel for counter scaling)")
-    return ap.parse_args()
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarize two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; csv output, each with "
+                         "--calib-copy) into per-kernel HBM bytes, scaled by the copy kernel's readings")
+    ap.add_argument("--trace-summary", metavar="TRACE_DIR",
+                    help="summarize a rocprofv3 --kernel-trace --stats run (csv output) per kernel")
+    ap.add_argument("--out", default=None, help="summary modes: also write the JSON there")
+    a = ap.parse_args()
+    if a.budget_mib is None:
+        big = a.config == "uneven" or (a.config == "tune" and a.tune_workload == "uneven")
+        a.budget_mib = 16384 if big else 2048
+    return a
 
 
-def kernel_bytes(name, launches, st, counts):
-    """Algorithmic HBM bytes of all launches of one kernel over the timed region.
+# ---------------------------------------------------------------- algorithmic bytes
 
-    st: interner counter deltas over the timed region; counts: per-step sizes.
-    Terms are the bytes each unit of work must move at minimum (no re-reads).
+def algorithmic_bytes(U):
+    """HBM bytes each kernel must move per step at minimum in this design (no re-reads),
+    from the step's unit counts U (host-side workload sizes and the device counters).
+
+    Every term is a read or write of a structure the kernel cannot avoid: batch masks and
+    values, level results, candidate keys (72 B key + 8 B hash + 8 B slot), one table slot
+    per probing key, the matched node's key on a hit (66 B), the owner's candidate on a
+    duplicate (72 B), node writes of new nodes and the columns a free clears.  Reference
+    count updates of children are not counted (their number is not counted on the device).
     """
-    steps = counts["steps"]
-    slots1 = counts["brick_slots"] * steps
-    touched = counts["touched_bricks"] * steps
-    cand1 = counts["cand_bricks"] * steps
-    lead1 = max(cand1 - st.get("merged", 0), 0)  # brick keys that reach the probe
-    cand_up = counts["cand_upper"] * steps
-    slots_up = counts["upper_slots"] * steps
-    s = counts["value_size"]
-    if name == "k_brick_build":
-        # every brick slot: its set mask (2 B of the mask pair), its share of the recorded
-        # height-2 node (1) and the slot write (8); touched bricks: 8 values; non-uniform
-        # bricks: the brick-level node (8), its children (64) and their values (8s);
-        # keys that reach the probe: candidate key + hash + slot (88)
-        return slots1 * (2 + 1 + 8) + touched * 8 * s + cand1 * (8 + 64 + 8 * s) + lead1 * 88
-    # a probed key: key + hash + slot read (88), one table slot (8), the matched node's
-    # key (66), result id / table slot / level slot writes (24)
-    probe_b = 88 + 8 + 66 + 24
-    if name == "k_probe[bricks]":
-        return lead1 * probe_b
-    if name == "k_probe[upper]":
-        return cand_up * probe_b
-    if name == "k_level_build":
-        # 8 child slots (64 B) + slot write (8) per slot; candidate key + hash + slot (88)
-        return slots_up * 72 + cand_up * 88
-    if name == "k_commit":
-        owners = st.get("branch_misses", 0)
-        # owners: list entry + table slot + id (24), key + hash (80), node write (64 + 2 + s),
-        # table slot (8), 8 child values (8s) and refcount read-modify-writes (8 x 8)
-        return owners * (24 + 80 + 66 + s + 8 + 8 * s + 64)
-    if name == "k_gc_round":
-        freed = st.get("recycled", 0)
-        # per freed node: type + children read and clear (2 + 128), pattern slot read + tombstone (16),
-        # child refcount read-modify-writes (64), value/ref/gen/free-stack writes (s + 10)
-        return freed * (2 + 128 + 16 + 64 + s + 10)
-    return None
+    s = U["value_size"]
+    n1 = U["brick_hits"] + U["brick_misses"] + U["brick_dups"]
+    hu, mu, du = (U["branch_hits"] - U["brick_hits"], U["branch_misses"] - U["brick_misses"],
+                  U["branch_dups"] - U["brick_dups"])
+    nu = hu + mu + du
+    lead1 = max(U["brick_keys"] - U["brick_merged"], 0)
+    leadu = max((U["candidates"] - U["brick_keys"]) - (U["merged"] - U["brick_merged"]), 0)
+    probe_miss = 2 + s + 8 + 8 * s  # generation, LOD value, owner entry, the children's values
+    return {
+        # per tree record and descent: the recorded height-2 nodes (64 B per level-3 slot)
+        # and the nodes above them; u16: the leaf id of every value (map + generation + id)
+        "k_prepare": U["level3_slots"] * 72 + U["trees"] * 48 + (65536 * 14 if s == 2 else 0),
+        # per brick slot: the mask pair (2), its share of the recorded height-2 node (1), the
+        # result (8); a touched brick: its 8 values; a mixed brick: the old brick-level node
+        # (8, the leaf-brick check); a partially written one also its children (64, the
+        # unwritten slots); a key that leaves the block: the candidate (88)
+        "k_brick_build": (U["brick_slots"] * 11 + U["touched_bricks"] * 8 * s + U["mixed_bricks"] * 8
+                          + U["partial_bricks"] * 64 + lead1 * 88),
+        "k_probe[bricks]": n1 * 120 + U["brick_hits"] * 66 + U["brick_dups"] * 72 + U["brick_misses"] * probe_miss,
+        # per slot: 8 child results, the recorded old node, the result; a leader's candidate
+        "k_level_build": U["upper_slots"] * 80 + leadu * 88,
+        "k_probe[upper]": nu * 120 + hu * 66 + du * 72 + mu * probe_miss,
+        # per owner: list entry, table slot and id (24), key and hash (80), the node (64 + 2),
+        # the table slot (8)
+        "k_commit": U["branch_misses"] * 178,
+        # per freed node: frontier entry, types/mask, children read and cleared, the table
+        # slot, value, count, generation (read and written), free-stack entry
+        "k_gc_round": U["recycled"] * (4 + 2 + 128 + 8 + s + 4 + 4 + 4),
+    }
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU (this build has no multi-GPU path); got --gpus {a.gpus}",
-              file=sys.stderr)
-        sys.exit(2)
-    if a.config == "plain":
-        return bench_plain(a)
-    if a.config == "uneven":
-        return bench_uneven(a)
-    if a.config == "gets":
-        return bench_gets(a)
-    if a.config == "set":
-        return bench_set(a)
-    if a.config == "vtm":
-        return bench_vtm(a)
-    if a.config == "serve":
-        return bench_serve(a)
-    if a.config == "calib":
-        return bench_calib(a)
-    if a.config == "tune":
-        return bench_tune(a)
-    import torch
-    from voxelis_amd import api
-    from voxelis_amd import workloads as W
+def _units(wl, st, steps):
+    U = dict(wl["units"])
+    for k in ("branch_hits", "branch_misses", "branch_dups", "candidates", "merged", "brick_keys",
+              "brick_merged", "brick_hits", "brick_misses", "brick_dups", "recycled"):
+        U[k] = st.get(k, 0) / steps
+    return U
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
+
+def _committed_traffic(profile):
+    """Per-kernel HBM bytes per dispatch from a committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (copy-calibrated), or {} when the workload has none."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not profile or not os.path.exists(path):
+        return {}
+    d = json.load(open(path))
+    return {k: v["hbm_bytes_scaled"] / max(v["dispatches"], 1) for k, v in d["kernels"].items()}
+
+
+def _roofline(prof, st, steps, wl):
+    """Per-kernel table, the dominant kernel's roofline and the hash-consing passes' roofline."""
+    U = _units(wl, st, steps)
+    B = algorithmic_bytes(U)
+    traffic = _committed_traffic(wl["pmc_profile"])
+    kern = {}
+    for name, (ms, n) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+        row = {"ms_per_step": round(ms / steps, 4), "launches_per_step": n / steps}
+        if name in B and ms > 0:
+            row["algorithmic_bytes_per_step"] = int(B[name])
+            row["achieved_gbs"] = round(B[name] / (ms / steps / 1e3) / 1e9, 1)
+            row["frac"] = round(row["achieved_gbs"] / HBM_PEAK_GBS, 4)
+        if name in traffic:
+            t_step = traffic[name] * n / steps
+            row["traffic_bytes_per_step"] = int(t_step)
+            row["traffic_gbs"] = round(t_step / (ms / steps / 1e3) / 1e9, 1)
+        kern[name] = row
+    roofline = None
+    if prof:
+        name, (ms, n) = max(prof.items(), key=lambda kv: kv[1][0])
+        if name in B and ms > 0:
+            per_launch = B[name] * steps / n
+            avg_s = ms / n / 1e3
+            ach = per_launch / avg_s / 1e9
+            roofline = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": int(traffic[name]) if name in traffic else None,
+                        "avg_launch_ms": round(ms / n, 4), "algorithmic_bytes_per_launch": int(per_launch)}
+    hc = None
+    t_hc = sum(prof[k][0] for k in HC_KERNELS if k in prof) / steps / 1e3
+    if t_hc > 0:
+        b_alg = sum(B[k] for k in HC_KERNELS if k in B)
+        hc = {"kernels": [k for k in HC_KERNELS if k in prof], "kernel_s_per_step": round(t_hc, 6),
+              "algorithmic": {"bytes_per_step": int(b_alg), "achieved_gbs": round(b_alg / t_hc / 1e9, 1),
+                              "frac": round(b_alg / t_hc / 1e9 / HBM_PEAK_GBS, 4)},
+              "target_frac": 0.40}
+        if all(k in traffic for k in HC_KERNELS if k in prof):
+            b_tr = sum(traffic[k] * prof[k][1] / steps for k in HC_KERNELS if k in prof)
+            hc["traffic"] = {"bytes_per_step": int(b_tr), "achieved_gbs": round(b_tr / t_hc / 1e9, 1),
+                             "frac": round(b_tr / t_hc / 1e9 / HBM_PEAK_GBS, 4),
+                             "source": "profiles/" + wl["pmc_profile"]}
+        else:
+            hc["traffic"] = None
+    total_alg = sum(B.values())
+    return roofline, kern, hc, U, total_alg
+
+
+# ---------------------------------------------------------------- workloads
+
+def _uneven_workload(a, api, W, torch):
+    """Device buffers of the uneven workload: per tree its own masks, values and values+1."""
+    spec = W.uneven_spec(a.trees, depths=(5, 6, 7), p_dense=0.5, seed=a.seed + 2)
     t_gen = time.perf_counter()
-    grid = a.grid
-    T = grid * grid
-    mA, vA, edits, _ = W.terrain_batches(grid=grid, seed=a.seed)
+    dev = torch.device("cuda", 0)
+    from voxelis_amd.morton import apply_edits_to_dense, dense_batch_arrays
+    dense_host = {}
+    mk, vA, vB, edits = [], [], [], []
+    touched = mixed = partial = 0
+    for d, kind, dens, sd in spec:
+        if kind == "dense":
+            if d not in dense_host:
+                m, v = W.dense_to_batch(W.uneven_dense(d), d)
+                vb = np.where(v == 65535, 1, v + 1).astype(np.uint16)
+                sm = m[:, 0]
+                dense_host[d] = (api.to_device(m, dev), api.to_device(v, dev), api.to_device(vb, dev),
+                                 int(np.count_nonzero(sm)),
+                                 int(np.count_nonzero((sm != 0) & ((sm != 0xFF) | (v != v[:, :1]).any(axis=1)))),
+                                 int(np.count_nonzero((sm != 0) & (sm != 0xFF))))
+            m_t, vA_t, vB_t, tc, mx, pt = dense_host[d]
+            # every tree reads its own copy of the batch, as distinct trees would
+            m_t, vA_t, vB_t = m_t.clone(), vA_t.clone(), vB_t.clone()
+            edits.append(1 << (3 * d))
+        else:
+            xyz, vals = W.uneven_tree_edits(d, kind, dens, sd)
+            m, v = dense_batch_arrays(d, np.uint16)
+            apply_edits_to_dense(m, v, xyz, vals)
+            vb = np.where(v == 0, 0, np.where(v == 65535, 1, v + 1)).astype(np.uint16)
+            sm = m[:, 0]
+            tc = int(np.count_nonzero(sm))
+            mx = int(np.count_nonzero((sm != 0) & ((sm != 0xFF) | (v != v[:, :1]).any(axis=1))))
+            pt = int(np.count_nonzero((sm != 0) & (sm != 0xFF)))
+            m_t, vA_t, vB_t = api.to_device(m, dev), api.to_device(v, dev), api.to_device(vb, dev)
+            edits.append(int(np.unpackbits(sm).sum()))
+        touched += tc
+        mixed += mx
+        partial += pt
+        mk.append(m_t)
+        vA.append(vA_t)
+        vB.append(vB_t)
+    depths = [d for d, _, _, _ in spec]
+    t_gen = time.perf_counter() - t_gen
+    units = {"value_size": 2, "trees": len(spec),
+             "brick_slots": sum(8 ** (d - 1) for d in depths),
+             "upper_slots": sum(sum(8 ** (d - h) for h in range(2, d + 1)) for d in depths),
+             "level3_slots": sum(8 ** (d - 3) for d in depths),
+             "touched_bricks": touched, "mixed_bricks": mixed, "partial_bricks": partial}
+    print(f"[bench] uneven: {len(spec)} trees, {sum(edits)} edits per step, depths "
+          f"{ {k: depths.count(k) for k in sorted(set(depths))} } (host generation and upload {t_gen:.1f}s)",
+          file=sys.stderr, flush=True)
+    return {"name": "uneven", "workload": UNEVEN_WORKLOAD, "dtype": "u16", "vsize": 2, "depths": depths,
+            "kinds": [k for _, k, _, _ in spec], "spec": spec, "masks": mk, "vA": vA, "vB": vB,
+            "edits": np.array(edits, dtype=np.int64), "units": units,
+            "pmc_profile": "pmc_traffic_uneven_4096_u16.json" if a.trees == 4096 and a.seed == 1 else None,
+            "data": "synthetic (seeded uniform random and set_sum-style dense fills)"}
+
+
+def _uneven_host_batches(wl, idx):
+    """Host (masks, values A, values B) of trees idx (regenerated from the spec)."""
+    from voxelis_amd import workloads as W
+    out = []
+    for i, (m, v, _) in zip(idx, W.uneven_batches([wl["spec"][i] for i in idx])):
+        vb = np.where(v == 0, 0, np.where(v == 65535, 1, v.astype(np.int64) + 1)).astype(np.uint16)
+        out.append((m, v, vb))
+    return out
+
+
+def _terrain_workload(a, api, W, torch):
+    t_gen = time.perf_counter()
+    T = a.grid * a.grid
+    mA, vA, edits, _ = W.terrain_batches(grid=a.grid, seed=a.seed)
     vB = vA.copy()
     vB[vA == 1] = 2
     vB[vA == 2] = 1
-    E = int(edits.sum())
-    t_gen = time.perf_counter() - t_gen
-    print(f"[bench] workload: {T} trees, {E} edits per step (host generation {t_gen:.1f}s)",
-          file=sys.stderr, flush=True)
-
     dev = torch.device("cuda", 0)
-    mA_t = api.to_device(mA, dev)
-    vA_t = api.to_device(vA, dev)
-    vB_t = api.to_device(vB, dev)
-    it = api.VoxInterner(1, a.budget_mib << 20, a.load_factor)
-    it.set_graphs(bool(a.graphs))
-    f = api.Forest(it, [5] * T)
-    viewsA = f.make_views([mA_t[i] for i in range(T)], [vA_t[i] for i in range(T)])
-    viewsB = f.make_views([mA_t[i] for i in range(T)], [vB_t[i] for i in range(T)])
+    mA_t, vA_t, vB_t = api.to_device(mA, dev), api.to_device(vA, dev), api.to_device(vB, dev)
+    sm = mA[:, :, 0]
+    touched = int(np.count_nonzero(sm))
+    uni = int(np.count_nonzero((sm == 0xFF) & (vA == vA[:, :, :1]).all(axis=2)))
+    units = {"value_size": 1, "trees": T, "brick_slots": T * 4096, "upper_slots": T * (512 + 64 + 8 + 1),
+             "level3_slots": T * 64, "touched_bricks": touched, "mixed_bricks": touched - uni,
+             "partial_bricks": int(np.count_nonzero((sm != 0) & (sm != 0xFF)))}
+    print(f"[bench] terrain: {T} trees, {int(edits.sum())} edits per step (host generation "
+          f"{time.perf_counter() - t_gen:.1f}s)", file=sys.stderr, flush=True)
+    return {"name": "terrain", "workload": TERRAIN_WORKLOAD, "dtype": "u8", "vsize": 1, "depths": [5] * T,
+            "kinds": ["terrain"] * T, "masks": [mA_t[i] for i in range(T)], "vA": [vA_t[i] for i in range(T)],
+            "vB": [vB_t[i] for i in range(T)], "edits": edits.astype(np.int64), "units": units,
+            "host": (mA, vA, vB),
+            "pmc_profile": "pmc_traffic_terrain_4096x32_u8.json" if a.grid == 64 and a.seed == 1 else None,
+            "data": "synthetic (seeded Perlin-dune terrain)"}
 
-    for w in range(a.warmup):
-        f.apply_views(viewsA if w % 2 == 0 else viewsB)
+
+# ---------------------------------------------------------------- measurement legs
+
+def _forest(api, it, wl, idx):
+    f = api.Forest(it, [wl["depths"][i] for i in idx])
+    views = [f.make_views([wl["masks"][i] for i in idx], [wl["vA"][i] for i in idx]),
+             f.make_views([wl["masks"][i] for i in idx], [wl["vB"][i] for i in idx])]
+    return f, views
+
+
+def _interner(api, a, wl):
+    it = api.VoxInterner(wl["vsize"], a.budget_mib << 20, a.load_factor)
+    it.set_graphs(bool(a.graphs))
+    if a.brick_rounds is not None:
+        it.set_tuning(brick_rounds=a.brick_rounds)
+    return it
+
+
+def _check_status(f, what):
+    st = f.status.cpu().numpy()
+    assert (st == 1).all(), f"{what}: statuses {np.unique(st)}"
+
+
+def _latencies(f, views, n, k0, torch):
+    """n launches, each bracketed by HIP events on the current (the library's) stream."""
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(n):
+        evs[k][0].record()
+        f.apply_views(views[(k0 + k) % 2])
+        evs[k][1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return np.array([e0.elapsed_time(e1) for e0, e1 in evs]), wall
+
+
+def _sliced(api, a, wl, idx_all, torch, step_ms):
+    """The trees in S interleaved slices, one launch sequence per slice, S doubling from an
+    estimate until the p99 launch latency is within a.p99_ms (at least lat_steps launches)."""
+    T = len(idx_all)
+    S = 1
+    while S < T and step_ms / S > 0.6 * a.p99_ms:
+        S *= 2
+    best = None
+    while S <= T:
+        it = _interner(api, a, wl)
+        slices = [_forest(api, it, wl, idx_all[s::S]) for s in range(S)]
+        e_slice = [int(wl["edits"][idx_all[s::S]].sum()) for s in range(S)]
+        for w in range(3):  # fresh trees, then steady state; the last round captures the graphs
+            for f, views in slices:
+                f.apply_views(views[w % 2])
+        it.sync()
+        rounds = max(2, -(-max(a.lat_steps, 200) // S))
+        evs = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(S)]
+               for _ in range(rounds)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in range(rounds):
+            for s, (f, views) in enumerate(slices):
+                evs[r][s][0].record()
+                f.apply_views(views[(r + 3) % 2])
+                evs[r][s][1].record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        it.sync()
+        for f, _ in slices:
+            _check_status(f, f"slices of {S}")
+        lat = np.array([e0.elapsed_time(e1) for row in evs for e0, e1 in row])
+        rate = sum(e_slice) * rounds / wall
+        res = {"slices": S, "trees_per_launch": -(-T // S), "launches": int(rounds * S),
+               "edits_per_s": rate, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+               "edits_per_launch_mean": sum(e_slice) / S}
+        print(f"[bench] sliced x{S}: {rate / 1e9:.2f} G edits/s, p50 {res['p50_ms']:.3f} ms p99 "
+              f"{res['p99_ms']:.3f} ms", file=sys.stderr, flush=True)
+        it.close()
+        del slices
+        if res["p99_ms"] <= a.p99_ms:
+            best = res
+            break
+        S *= 2
+    return best
+
+
+def _subset_rate(api, a, wl, idx, steps, torch):
+    it = _interner(api, a, wl)
+    f, views = _forest(api, it, wl, idx)
+    for w in range(max(a.warmup, 3)):
+        f.apply_views(views[w % 2])
     it.sync()
-    st_w = f.status.cpu().numpy()
-    assert (st_w == 1).all(), f"warmup status {np.unique(st_w)}"
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        f.apply_views(views[(max(a.warmup, 3) + k) % 2])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    it.sync()
+    _check_status(f, "subset")
+    it.close()
+    E = int(wl["edits"][idx].sum())
+    return {"trees": len(idx), "edits_per_step": E, "edits_per_s": E * steps / wall, "ms_per_step": 1e3 * wall / steps}
+
+
+def _plain(api, a, wl, idx, torch):
+    """One apply call per tree (the plain per-tree GPU path), alternating phases."""
+    it = _interner(api, a, wl)
+    fs = [_forest(api, it, wl, [i]) for i in idx]
+    for w in range(3):
+        for f, views in fs:
+            f.apply_views(views[w % 2])
+    it.sync()
+    s0 = it.stats()
+    rounds = 2
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(rounds):
+        for f, views in fs:
+            f.apply_views(views[(r + 3) % 2])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    it.sync()
+    s1 = it.stats()
+    for f, _ in fs:
+        _check_status(f, "plain")
+    it.close()
+    E = int(wl["edits"][idx].sum())
+    st = {k: s1[k] - s0[k] for k in s1}
+    sub = dict(wl["units"])
+    frac = len(idx) / len(wl["depths"])
+    tu = [_tree_units(wl, i) for i in idx]
+    for k in ("brick_slots", "upper_slots", "level3_slots", "touched_bricks", "mixed_bricks", "partial_bricks"):
+        sub[k] = sum(u[k] for u in tu)
+    sub["trees"] = len(idx)
+    b = sum(algorithmic_bytes(_units({"units": sub}, st, rounds)).values())
+    ach = b / (wall / rounds) / 1e9
+    return {"trees": len(idx), "calls_per_step": len(idx), "edits_per_step": E, "edits_per_s": E * rounds / wall,
+            "ms_per_step": 1e3 * wall / rounds, "algorithmic_gbs": round(ach, 1),
+            "frac": round(ach / HBM_PEAK_GBS, 5), "sample_share": round(frac, 4)}
+
+
+def _tree_units(wl, i):
+    d = wl["depths"][i]
+    u = {"brick_slots": 8 ** (d - 1), "upper_slots": sum(8 ** (d - h) for h in range(2, d + 1)),
+         "level3_slots": 8 ** (d - 3) if d >= 3 else 0}
+    m = wl["masks"][i][:, 0].cpu().numpy()
+    v = wl["vA"][i].cpu().numpy().view(np.uint16 if wl["vsize"] == 2 else np.uint8)
+    u["touched_bricks"] = int(np.count_nonzero(m))
+    u["mixed_bricks"] = int(np.count_nonzero((m != 0) & ((m != 0xFF) | (v != v[:, :1]).any(axis=1))))
+    u["partial_bricks"] = int(np.count_nonzero((m != 0) & (m != 0xFF)))
+    return u
+
+
+def _cpu_baseline(a, wl):
+    """The oracle (one thread) on a bounded sample of the same trees and steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    T = len(wl["depths"])
+    if wl["name"] == "uneven":
+        idx = list(range(0, T, max(1, T // a.cpu_trees)))[: a.cpu_trees]
+        host = _uneven_host_batches(wl, idx)
+    else:
+        idx = list(range(T))
+        mA, vA, vB = wl["host"]
+        host = [(mA[i], vA[i], vB[i]) for i in idx]
+    it_o = O.Interner(wl["vsize"], a.budget_mib << 20, extended_depth=True)
+    trees, batches = [], []
+    for i, (m, va, vb) in zip(idx, host):
+        d = wl["depths"][i]
+        t = O.Tree(d)
+        ba, bb = O.Batch(d), O.Batch(d)
+        ba.load_dense(m, va)
+        bb.load_dense(m, vb)
+        t.apply_batch(it_o, ba)  # the first step (fresh trees) is not timed
+        trees.append(t)
+        batches.append((ba, bb))
+    tsum, esum, nsteps, done = 0.0, 0, 0, 0
+    while tsum < a.cpu_seconds:
+        for k, i in enumerate(idx):
+            c0 = time.perf_counter()
+            trees[k].apply_batch(it_o, batches[k][1 - nsteps % 2])
+            tsum += time.perf_counter() - c0
+            esum += int(wl["edits"][i])
+            done += 1
+            if tsum >= a.cpu_seconds and nsteps > 0:
+                break
+        nsteps += 1
+    return {"value": esum / tsum, "unit": "edits/s", "cores": 1, "kind": "port",
+            "cpu_model": _cpu_model(), "host_nproc": os.cpu_count(),
+            "sample": f"oracle apply_batch on {len(idx)} of the {T} trees (every {max(1, T // len(idx))}th, shared "
+                      f"interner), values alternating per step as on the GPU: {done} tree applies, {esum} edits, "
+                      f"{tsum:.1f} s timed on one thread"}
+
+
+def _atomics(torch):
+    import ctypes as C
+    from voxelis_amd._lib import lib
+    buf = torch.zeros(1 << 27, dtype=torch.int64, device="cuda")
+    ms = C.c_float(0)
+    out = {}
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for kind, name, ops in ((0, "cas64_random", 1 << 28), (1, "add32_random", 1 << 28),
+                            (3, "add32_one_counter_per_wavefront", 1 << 26)):
+        lib().vx_bench_atomics(stream, kind, buf.data_ptr(), 1 << 27, ops, C.byref(ms))  # warm
+        buf.zero_()
+        rc = lib().vx_bench_atomics(stream, kind, buf.data_ptr(), 1 << 27, ops, C.byref(ms))
+        if rc == 0 and ms.value > 0:
+            out[name + "_ops_per_s"] = (ops / 64 if kind == 3 else ops) / (ms.value / 1e3)
+    del buf
+    return out
+
+
+def bench_headline(a, wl_name):
+    import torch
+    from voxelis_amd import api
+    from voxelis_amd import workloads as W
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    wl = _uneven_workload(a, api, W, torch) if wl_name == "uneven" else _terrain_workload(a, api, W, torch)
+    T = len(wl["depths"])
+    E = int(wl["edits"].sum())
+    idx_all = np.arange(T)
+    it = _interner(api, a, wl)
+    f, views = _forest(api, it, wl, idx_all)
+    for w in range(a.warmup):
+        f.apply_views(views[w % 2])
+    it.sync()
+    _check_status(f, "warmup")
     print(f"[bench] warmup done ({a.warmup} steps)", file=sys.stderr, flush=True)
 
-    # ---- timed steps (no per-kernel events: value and latency percentiles)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(a.steps)]
+    # ---- the timed steps: whole-job throughput
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(a.steps):
-        views = viewsA if (a.warmup + k) % 2 == 0 else viewsB
-        evs[k][0].record()
-        f.apply_views(views)
-        evs[k][1].record()
+        f.apply_views(views[(a.warmup + k) % 2])
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    wall = time.perf_counter() - t0
     it.sync()
-    st_t = f.status.cpu().numpy()
-    assert (st_t == 1).all(), f"timed status {np.unique(st_t)}"
-    lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
-    wall = t1 - t0
+    _check_status(f, "timed")
     value = E * a.steps / wall
     leaves, branches = it.counts()
-    print(f"[bench] batched: {value / 1e9:.2f} G edits/s, {1e3 * wall / a.steps:.3f} ms/step, "
-          f"p50 {np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms; "
-          f"patterns {leaves} leaves {branches} branches", file=sys.stderr, flush=True)
+    print(f"[bench] batched: {value / 1e9:.2f} G edits/s, {1e3 * wall / a.steps:.3f} ms/step; patterns "
+          f"{leaves} leaves {branches} branches", file=sys.stderr, flush=True)
+    k0 = a.warmup + a.steps
+    lat = None
+    if not a.quick:
+        lat, lat_wall = _latencies(f, views, a.lat_steps, k0, torch)
+        k0 += a.lat_steps
+        it.sync()
+        _check_status(f, "latency")
+        print(f"[bench] latency over {a.lat_steps} launches: p50 {np.percentile(lat, 50):.3f} ms, p99 "
+              f"{np.percentile(lat, 99):.3f} ms", file=sys.stderr, flush=True)
 
-    # ---- the same steps again with per-kernel HIP events on the library's stream
-    prof = {}
-    counts_before = it.stats()
+    # ---- per-kernel HIP events on the library's stream (kernels launched one by one)
+    roofline = kern = hc = None
+    U = total_alg = None
+    st = {}
     if not a.no_profile:
         it.profile(True)
-        for k in range(2):  # creates the event pool
-            f.apply_views(viewsA if (a.warmup + a.steps + k) % 2 == 0 else viewsB)
+        for k in range(2):
+            f.apply_views(views[(k0 + k) % 2])
         it.profile_read()
-        counts_before = it.stats()
-        for k in range(a.steps):
-            f.apply_views(viewsA if (a.warmup + a.steps + k) % 2 == 0 else viewsB)
+        s0 = it.stats()
+        psteps = max(2, min(a.steps, 6))
+        for k in range(psteps):
+            f.apply_views(views[(k0 + 2 + k) % 2])
         prof = it.profile_read()
         it.profile(False)
-    counts_after = it.stats()
-    st = {k: counts_after[k] - counts_before[k] for k in counts_after}
-
-    # ---- roofline of the dominant kernel (HIP events on the library's stream)
-    roofline = None
-    kern = {}
-    if prof:
-        steps = a.steps
-        counts = {
-            "steps": steps, "value_size": 1, "depth": 5, "brick_slots": T * 4096,
-            "touched_bricks": int(np.count_nonzero(mA[:, :, 0])),
-            "cand_bricks": st["candidates"] and None,
-        }
-        # candidates per level are not split in the counters: the brick level is the
-        # touched bricks that are not uniform; the rest are upper-level keys
-        uni = int(np.count_nonzero((mA[:, :, 0] == 0xFF) & (vA == vA[:, :, :1]).all(axis=2)))
-        counts["cand_bricks"] = counts["touched_bricks"] - uni
-        counts["cand_upper"] = max(st["candidates"] // steps - counts["cand_bricks"], 0)
-        counts["upper_slots"] = T * (512 + 64 + 8 + 1)
-        dom = max(prof.items(), key=lambda kv: kv[1][0])
-        for name, (ms, n) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
-            kern[name] = {"ms_per_step": ms / steps, "launches_per_step": n / steps}
-        name, (ms, n) = dom
-        b = kernel_bytes(name, n, st, counts)
-        if b is not None and ms > 0:
-            per_launch_bytes = b / n
-            avg_s = ms / n / 1e3
-            ach = per_launch_bytes / avg_s / 1e9
-            roofline = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1),
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": _committed_traffic(name, a.grid), "avg_launch_ms": round(ms / n, 4),
-                        "algorithmic_bytes_per_launch": int(per_launch_bytes)}
-        print(f"[bench] kernels: {json.dumps(kern)}", file=sys.stderr, flush=True)
-
-    # ---- hash-consing passes against HBM, by the single algorithmic-bytes formula:
-    # B_edit = E (4 + s); B_hc = N_cand (64+16+8) + N_leafcand (s+16+8)
-    #        + N_new (64+s+8+4+2+16) + N_ref 8, over the probe / commit / build passes
-    hc = None
-    if prof:
-        s_ = 1
-        n_cand = st["candidates"] / a.steps
-        n_leafcand = E  # every set voxel looks its leaf up
-        n_new = (st["branch_misses"] + st["leaf_misses"]) / a.steps
-        n_ref = 8 * st["branch_misses"] / a.steps + n_new  # child refs of new nodes + frees
-        b_edit = E * (4 + s_)
-        b_hc = n_cand * 88 + n_leafcand * (s_ + 24) + n_new * (64 + s_ + 8 + 4 + 2 + 16) + n_ref * 8
-        hc_kern = ("k_brick_build", "k_brick_retry", "k_probe[bricks]", "k_level_build", "k_probe[upper]",
-                   "k_commit")
-        t_hc = sum(prof[k][0] for k in hc_kern if k in prof) / a.steps / 1e3
-        if t_hc > 0:
-            ach = (b_edit + b_hc) / t_hc / 1e9
-            hc = {"bytes_edit": int(b_edit), "bytes_hash_consing": int(b_hc), "kernel_s": t_hc,
-                  "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": round(ach / HBM_PEAK_GBS, 4), "kernels": list(hc_kern),
-                  "n_cand": int(n_cand), "n_new": int(n_new)}
-            print(f"[bench] hash-consing passes: {ach:.0f} GB/s = {100 * ach / HBM_PEAK_GBS:.1f} % of HBM peak "
-                  f"({(b_edit + b_hc) / 1e9:.2f} GB in {1e3 * t_hc:.3f} ms)", file=sys.stderr, flush=True)
-
-    # ---- atomic-throughput ceilings (project microbenchmark) next to the apply's rates
-    atomics = None
-    if not a.no_profile:
-        import ctypes as C
-        from voxelis_amd._lib import lib
-        slots = 1 << 27
-        buf = torch.zeros(slots, dtype=torch.int64, device=dev)
-        ms = C.c_float(0)
-        atomics = {}
-        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-        for kind, name in ((0, "cas64_random"), (1, "add32_random"), (2, "add32_64_hot")):
-            ops = 1 << 28
-            lib().vx_bench_atomics(stream, kind, buf.data_ptr(), slots, ops, C.byref(ms))  # warm
-            buf.zero_()
-            rc = lib().vx_bench_atomics(stream, kind, buf.data_ptr(), slots, ops, C.byref(ms))
-            if rc == 0 and ms.value > 0:
-                atomics[name + "_ops_per_s"] = ops / (ms.value / 1e3)
-        del buf
-        t_probe = sum(prof[k][0] for k in ("k_probe[bricks]", "k_probe[upper]") if k in prof) / a.steps / 1e3
-        if t_probe > 0:
-            atomics["apply_probe_cas_per_s"] = (st["branch_misses"] + st["branch_dups"]) / a.steps / t_probe
-        print(f"[bench] atomics: {json.dumps({k: f'{v:.3g}' for k, v in atomics.items()})}",
-              file=sys.stderr, flush=True)
-
+        s1 = it.stats()
+        k0 += 2 + psteps
+        st = {k: s1[k] - s0[k] for k in s1}
+        roofline, kern, hc, U, total_alg = _roofline(prof, st, psteps, wl)
+        print(f"[bench] kernels: {json.dumps({k: v['ms_per_step'] for k, v in kern.items()})}", file=sys.stderr,
+              flush=True)
+        if hc:
+            print(f"[bench] hash-consing passes: algorithmic {hc['algorithmic']['frac'] * 100:.1f} % of HBM peak, "
+                  f"traffic {hc['traffic']['frac'] * 100 if hc['traffic'] else float('nan'):.1f} %",
+                  file=sys.stderr, flush=True)
+    table = it.table_stats()
+    it.close()
+    del f, views
     if a.calib_copy:
-        # the reference kernel for the memory counters: exactly 1 GiB read and written
+        # the reference kernel of the memory counters: exactly 1 GiB read and 1 GiB written
         import ctypes as C
         from voxelis_amd._lib import lib
-        src = torch.ones(1 << 30, dtype=torch.uint8, device=dev)
+        src = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
         ms = C.c_float(0)
         lib().vx_bench_copy(C.c_void_p(torch.cuda.current_stream().cuda_stream), dst.data_ptr(), src.data_ptr(),
@@ -294,99 +586,146 @@ def main():
               file=sys.stderr, flush=True)
         del src, dst
 
-    # ---- plain per-tree form on the same GPU (one apply call per tree)
-    plain = None
-    if a.plain_trees > 0:
-        n_plain = min(a.plain_trees, T)
-        it2 = api.VoxInterner(1, a.budget_mib << 20, a.load_factor)
-        fs = []
-        for i in range(n_plain):
-            fi = api.Forest(it2, [5])
-            fs.append((fi, fi.make_views([mA_t[i]], [vA_t[i]]), fi.make_views([mA_t[i]], [vB_t[i]])))
-        for fi, va, vb in fs:
-            fi.apply_views(va)
-        it2.sync()
-        torch.cuda.synchronize()
-        p0 = time.perf_counter()
-        for fi, va, vb in fs:
-            fi.apply_views(vb)
-        torch.cuda.synchronize()
-        p1 = time.perf_counter()
-        it2.sync()
-        plain_edits = int(edits[:n_plain].sum())
-        plain = plain_edits / (p1 - p0)
-        print(f"[bench] plain: {plain / 1e6:.1f} M edits/s over {n_plain} trees", file=sys.stderr, flush=True)
-        it2.close()
-
-    # ---- CPU baseline: the oracle (single thread) on a bounded sample
-    cpu = None
-    if not a.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_lib as O
-        it_o = O.Interner(1, a.budget_mib << 20)
-        # the same steps as the GPU's: every tree rebuilt with the ids of the other phase,
-        # step after step, until about cpu_seconds of timed applies (at least one step)
-        trees, batches = [], []
-        for i in range(T):
-            t = O.Tree(5)
-            ba, bb = O.Batch(5), O.Batch(5)
-            ba.load_dense(mA[i], vA[i])
-            bb.load_dense(mA[i], vB[i])
-            t.apply_batch(it_o, ba)
-            trees.append(t)
-            batches.append((ba, bb))
-        tsum, esum, nsteps, done = 0.0, 0, 0, 0
-        while tsum < a.cpu_seconds:
-            for i in range(T):
-                c0 = time.perf_counter()
-                trees[i].apply_batch(it_o, batches[i][1 - nsteps % 2])
-                tsum += time.perf_counter() - c0
-                esum += int(edits[i])
-                done += 1
-                if tsum >= a.cpu_seconds and nsteps > 0:
-                    break
-            nsteps += 1
-        cpu = {"value": esum / tsum, "unit": "edits/s", "cores": 1, "kind": "port",
-               "sample": f"oracle apply_batch on the {T} terrain trees (shared interner), ids alternating "
-                         f"per step as on the GPU: {done} tree applies ({done / T:.1f} steps), "
-                         f"{esum} edits, {tsum:.1f} s timed"}
-        print(f"[bench] cpu oracle: {cpu['value'] / 1e6:.1f} M edits/s ({cpu['sample']})",
-              file=sys.stderr, flush=True)
-
     out = {
-        "metric": METRIC, "value": value, "unit": "edits/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u8",
-        "data": "synthetic (seeded Perlin-dune terrain)",
-        "config": {"workload": "perlin-dune terrain, 64x64 trees of 32^3 (MaxDepth 5), u8, "
-                               "one batch per tree, all trees in one launch sequence, ids "
-                               "alternating 1/2 and 2/1 per step",
-                   "trees": T, "edits_per_step": E, "interner_budget_mib": a.budget_mib,
-                   "load_factor": a.load_factor, "parallelism": "single GPU",
+        "metric": METRIC, "value": value, "unit": "edits/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": wl["dtype"], "data": wl["data"],
+        "config": {"workload": wl["workload"], "trees": T, "edits_per_step": E,
+                   "interner_budget_mib": a.budget_mib, "load_factor": a.load_factor, "seed": a.seed,
+                   "parallelism": "single GPU",
                    "launch": "HIP-graph replay of the captured launch sequence" if a.graphs
                    else "kernels launched one by one"},
-        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-        "plain_edits_per_s": plain,
-        "batched_over_plain": (value / plain) if plain else None,
+        "p50_ms": float(np.percentile(lat, 50)) if lat is not None else None,
+        "p99_ms": float(np.percentile(lat, 99)) if lat is not None else None,
+        "latency_launches": a.lat_steps if lat is not None else 0,
         "patterns": {"leaves": leaves, "branches": branches},
-        "roofline": roofline,
-        "kernels": kern,
-        "hash_consing_roofline": hc,
-        "atomics": atomics,
-        "cpu_baseline": cpu,
+        "pattern_table": {"slots": table[0], "tombstones": table[1], "rebuilds": table[2]},
+        "roofline": roofline, "kernels": kern, "hash_consing_roofline": hc,
+        "all_kernels_algorithmic_gbs": round(total_alg / (wall / a.steps) / 1e9, 1) if total_alg else None,
+        "counters_per_step": {k: v / max(2, min(a.steps, 6)) for k, v in st.items()} if st else None,
     }
+    if wl_name == "uneven":
+        d = np.array(wl["depths"])
+        out["config"]["trees_by_depth"] = {int(k): int((d == k).sum()) for k in (5, 6, 7)}
+        out["config"]["edits_depth_le6"] = int(wl["edits"][d <= 6].sum())
+        out["config"]["edits_depth7"] = int(wl["edits"][d == 7].sum())
+        out["config"]["p_dense"] = 0.5
+    if a.quick:
+        print(json.dumps(out))
+        return
+    step_ms = 1e3 * wall / a.steps
+    out["atomics"] = _atomics(torch)
+    sl = _sliced(api, a, wl, idx_all, torch, step_ms)
+    out["p99_bounded"] = sl
+    if wl_name == "uneven":
+        d = np.array(wl["depths"])
+        out["by_depth"] = {"depth_le6": _subset_rate(api, a, wl, idx_all[d <= 6], a.steps, torch),
+                           "depth7": _subset_rate(api, a, wl, idx_all[d == 7], a.steps, torch)}
+        print(f"[bench] depth<=6: {out['by_depth']['depth_le6']['edits_per_s'] / 1e9:.2f} G edits/s, depth 7: "
+              f"{out['by_depth']['depth7']['edits_per_s'] / 1e9:.2f} G edits/s", file=sys.stderr, flush=True)
+    plain = None
+    if a.plain_trees > 0:
+        n = min(a.plain_trees, T)
+        plain = _plain(api, a, wl, list(range(0, T, max(1, T // n)))[:n], torch)
+        print(f"[bench] plain: {plain['edits_per_s'] / 1e9:.3f} G edits/s over {plain['trees']} trees",
+              file=sys.stderr, flush=True)
+    out["plain"] = plain
+    out["plain_edits_per_s"] = plain["edits_per_s"] if plain else None
+    out["batched_over_plain"] = value / plain["edits_per_s"] if plain else None
+    out["p99_bounded_over_plain"] = sl["edits_per_s"] / plain["edits_per_s"] if (plain and sl) else None
+    cpu = None
+    if not a.no_cpu:
+        cpu = _cpu_baseline(a, wl)
+        print(f"[bench] cpu oracle: {cpu['value'] / 1e6:.1f} M edits/s ({cpu['sample']})", file=sys.stderr,
+              flush=True)
+    out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 
-def _committed_traffic(kernel, grid):
-    """HBM bytes per launch of `kernel` on the default terrain workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_terrain_4096x32_u8.json")
-    if grid != 64 or not os.path.exists(path):
-        return None
-    d = json.load(open(path))
-    v = d["kernels"].get(kernel)
-    return int(v["hbm_bytes_scaled"] / max(v["dispatches"], 1)) if v else None
+def main():
+    a = parse()
+    if a.pmc_summary:
+        return pmc_summary(a)
+    if a.trace_summary:
+        return trace_summary(a)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU (this build has no multi-GPU path); got --gpus {a.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    if a.config in ("uneven", "terrain"):
+        return bench_headline(a, a.config)
+    return {"plain": bench_plain, "gets": bench_gets, "set": bench_set, "vtm": bench_vtm,
+            "serve": bench_serve, "calib": bench_calib, "tune": bench_tune}[a.config](a)
+
+
+# ---------------------------------------------------------------- profiler summaries
+
+def _kernel_short(name):
+    import re
+    m = re.search(r"vx::(k_\w+)", name)
+    base = m.group(1) if m else name.split("(")[0].strip()
+    return TRACE_NAMES.get(base, base)
+
+
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+    rows = []
+    for p in glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True):
+        with open(p, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def pmc_summary(a):
+    """Per-kernel HBM bytes per dispatch from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE),
+    each scaled by what the counter reads for the 1 GiB copy kernel of the same run."""
+    res = {}
+    for counter, d in (("FETCH_SIZE", a.pmc_summary[0]), ("WRITE_SIZE", a.pmc_summary[1])):
+        per = {}
+        for r in _csv_rows(d, "counter_collection.csv"):
+            if r.get("Counter_Name") != counter:
+                continue
+            k = _kernel_short(r["Kernel_Name"])
+            disp = r.get("Dispatch_Id") or r.get("Correlation_Id")
+            e = per.setdefault(k, {})
+            e[disp] = e.get(disp, 0.0) + float(r["Counter_Value"])
+        res[counter] = per
+    copy_bytes = 1 << 30
+    fetch_copy = sum(res["FETCH_SIZE"].get("k_copy_bench", {}).values())
+    write_copy = sum(res["WRITE_SIZE"].get("k_copy_bench", {}).values())
+    fs = copy_bytes / 1024 / fetch_copy if fetch_copy else 1.0
+    ws = copy_bytes / 1024 / write_copy if write_copy else 1.0
+    out = {"copy_kernel_bytes": copy_bytes, "fetch_kib_for_copy": fetch_copy, "write_kib_for_copy": write_copy,
+           "fetch_scale": fs, "write_scale": ws, "kernels": {}}
+    for k in sorted(set(res["FETCH_SIZE"]) | set(res["WRITE_SIZE"])):
+        if k == "k_copy_bench":
+            continue
+        fd, wd = res["FETCH_SIZE"].get(k, {}), res["WRITE_SIZE"].get(k, {})
+        n = max(len(fd), len(wd), 1)
+        fk, wk = sum(fd.values()), sum(wd.values())
+        out["kernels"][k] = {"fetch_kib": fk, "write_kib": wk, "dispatches": n,
+                             "hbm_bytes_scaled": (fk * fs + wk * ws) * 1024}
+    s = json.dumps(out, indent=1)
+    print(s)
+    if a.out:
+        open(a.out, "w").write(s + "\n")
+
+
+def trace_summary(a):
+    """Per-kernel dispatches and durations from a rocprofv3 --kernel-trace run."""
+    per = {}
+    for r in _csv_rows(a.trace_summary, "kernel_trace.csv"):
+        k = _kernel_short(r["Kernel_Name"])
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3  # us
+        per.setdefault(k, []).append(dur)
+    out = {k: {"dispatches": len(v), "total_us": round(sum(v), 1), "avg_us": round(sum(v) / len(v), 2),
+               "median_us": round(float(np.median(v)), 2), "max_us": round(max(v), 2)}
+           for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))}
+    s = json.dumps(out, indent=1)
+    print(s)
+    if a.out:
+        open(a.out, "w").write(s + "\n")
 
 
 def _timed_steps(f, views_cycle, steps, warmup, it, torch):
@@ -462,71 +801,6 @@ def bench_plain(a):
                          "kind": "port"}}))
 
 
-def bench_uneven(a):
-    """Config 3: u16 trees of depth 5/6/7 (uniform), dense ((x+y+z) mod 65535)+1 fills with
-    p = 0.5, else sparse uniform random edits at density 0.1/1/10 %; one launch per step;
-    steps alternate the values and the values shifted by one, so every tree is rebuilt."""
-    import torch
-    from voxelis_amd import api
-    from voxelis_amd import workloads as W
-    depths, mk, vA_list, vB_list, edits, E = _uneven_setup(a, api, W, torch)
-    it = api.VoxInterner(2, a.budget_mib << 20, a.load_factor)
-    if a.brick_rounds is not None:
-        it.set_tuning(brick_rounds=a.brick_rounds)
-    f = api.Forest(it, depths)
-    views = [f.make_views(mk, vA_list), f.make_views(mk, vB_list)]
-    if not a.no_profile:
-        it.profile(True)
-    wall, lat = _timed_steps(f, views, a.steps, a.warmup, it, torch)
-    prof = it.profile_read() if not a.no_profile else {}
-    st = f.status.cpu().numpy()
-    assert (st == 1).all(), np.unique(st)
-    e57 = sum(e for e, d in zip(edits, depths) if d <= 6)
-    out = {"metric": METRIC, "value": E * a.steps / wall, "unit": "edits/s",
-           "ms_per_step": 1e3 * wall / a.steps, "p50_ms": float(np.percentile(lat, 50)),
-           "p99_ms": float(np.percentile(lat, 99)), "dtype": "u16",
-           "config": {"workload": "uneven batched edits: depth 5/6/7 trees, dense p=0.5 or sparse "
-                                  "0.1/1/10 %, u16, one launch", "trees": a.trees, "edits_per_step": E,
-                      "edits_depth_le6": int(e57), "depth7_trees": int(sum(1 for d in depths if d == 7))},
-           "counts": it.counts(),
-           "kernels": {k: {"ms_per_step": v[0] / (a.steps + a.warmup)} for k, v in prof.items()}}
-    print(json.dumps(out))
-
-
-def _uneven_setup(a, api, W, torch):
-    """Device buffers of the uneven workload: per tree masks, values, values shifted by one."""
-    spec = W.uneven_spec(a.trees, depths=(5, 6, 7), p_dense=0.5, seed=a.seed + 2)
-    t_gen = time.perf_counter()
-    dev = torch.device("cuda", 0)
-    dense_buf = {}
-    mk, vA_list, vB_list, edits = [], [], [], []
-    for d, kind, dens, sd in spec:
-        if kind == "dense":
-            if d not in dense_buf:
-                m, v = W.dense_to_batch(W.uneven_dense(d), d)
-                vb = np.where(v == 65535, 1, v + 1).astype(np.uint16)
-                dense_buf[d] = (api.to_device(m, dev), api.to_device(v, dev), api.to_device(vb, dev))
-            m_t, vA_t, vB_t = dense_buf[d]
-            edits.append(1 << (3 * d))
-        else:
-            xyz, vals = W.uneven_tree_edits(d, kind, dens, sd)
-            from voxelis_amd.morton import apply_edits_to_dense, dense_batch_arrays
-            m, v = dense_batch_arrays(d, np.uint16)
-            apply_edits_to_dense(m, v, xyz, vals)
-            vb = np.where(v == 0, 0, np.where(v == 65535, 1, v + 1)).astype(np.uint16)
-            m_t, vA_t, vB_t = api.to_device(m, dev), api.to_device(v, dev), api.to_device(vb, dev)
-            edits.append(int(np.count_nonzero(m[:, 0] != 0) and np.unpackbits(m[:, 0]).sum()))
-        mk.append(m_t)
-        vA_list.append(vA_t)
-        vB_list.append(vB_t)
-    E = int(sum(edits))
-    depths = [d for d, _, _, _ in spec]
-    t_gen = time.perf_counter() - t_gen
-    print(f"[bench] uneven: {a.trees} trees, {E} edits per step, depths "
-          f"{dict(zip(*np.unique(depths, return_counts=True)))} (gen {t_gen:.1f}s)", file=sys.stderr, flush=True)
-    return depths, mk, vA_list, vB_list, edits, E
-
-
 def bench_tune(a):
     """Autotuner sweep (tile x probe rounds x load factor, then grid sizes) on the terrain
     or the uneven workload; prints every point and the best."""
@@ -546,7 +820,8 @@ def bench_tune(a):
         mk, va, vb = [m_t[i] for i in range(T)], [vA_t[i] for i in range(T)], [vB_t[i] for i in range(T)]
         vsize, E = 1, int(edits.sum())
     else:
-        depths, mk, va, vb, _, E = _uneven_setup(a, api, W, torch)
+        wl = _uneven_workload(a, api, W, torch)
+        depths, mk, va, vb, E = wl["depths"], wl["masks"], wl["vA"], wl["vB"], int(wl["edits"].sum())
         vsize = 2
 
     def make_case(lf):

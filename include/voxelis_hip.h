@@ -108,8 +108,8 @@ int vx_interner_get_tuning(const vx_interner* h, vx_tuning* tuning);
 /* Capture the launch sequence of vx_trees_apply_batches[_ex] as a HIP graph and
  * replay it when a later call has the same arguments (same roots/status arrays,
  * same batch views, same options): repeated batched applies skip planning and
- * per-kernel launches.  Up to 4 sequences are kept; any workspace growth rebuilds
- * them.  Off by default; profiling (vx_interner_profile) bypasses it. */
+ * per-kernel launches.  Up to 256 sequences are kept (the least recently used is
+ * dropped); any workspace growth rebuilds them.  Off by default; profiling (vx_interner_profile) bypasses it. */
 int vx_interner_set_graphs(vx_interner* h, int enable);
 /* Pattern-table maintenance (no reference counterpart: the reference's HashMap
  * removes entries outright, interner/mod.rs:518).  Frees here leave tombstones; a
@@ -134,11 +134,16 @@ enum {
     VX_STAT_H3_LEAKS,
     VX_STAT_H4_UNDERCOUNT,
     VX_STAT_H5_COLLAPSE_CLONED,
-    VX_STAT_CANDIDATES,        /* branch keys probed */
+    VX_STAT_CANDIDATES,        /* branch keys built */
     VX_STAT_PROBES,            /* table slots inspected */
     VX_STAT_CLAIMS,            /* atomic claims of a free table slot */
     VX_STAT_CLAIMS_LOST,       /* claims that found the slot taken meanwhile */
-    VX_STAT_MERGED,            /* brick keys that shared an equal key's candidate before the probe */
+    VX_STAT_MERGED,            /* keys that shared an equal key's candidate before the probe */
+    VX_STAT_BRICK_KEYS,        /* the brick level's share of CANDIDATES ... */
+    VX_STAT_BRICK_MERGED,      /* ... of MERGED */
+    VX_STAT_BRICK_HITS,        /* ... of BRANCH_HITS */
+    VX_STAT_BRICK_MISSES,      /* ... of BRANCH_MISSES */
+    VX_STAT_BRICK_DUPS,        /* ... of BRANCH_DUPS */
     VX_STAT_COUNT
 };
 

@@ -17,7 +17,8 @@ STATUS_NAMES = {
 }
 STAT_NAMES = ["branch_hits", "branch_misses", "branch_dups", "leaf_misses", "collapses",
               "recycled", "h2_uncollapsed", "h3_leaks", "h4_undercount", "h5_collapse_cloned",
-              "candidates", "probes", "claims", "claims_lost", "merged"]
+              "candidates", "probes", "claims", "claims_lost", "merged", "brick_keys", "brick_merged",
+              "brick_hits", "brick_misses", "brick_dups"]
 
 # Every symbol include/voxelis_hip.h declares, with its ctypes signature.
 _u8, _u32, _u64, _sz, _vp, _int = (C.c_uint8, C.c_uint32, C.c_uint64, C.c_size_t, C.c_void_p,

@@ -168,7 +168,7 @@ struct vx_interner {
     // captured launch sequences of repeated calls (vx_interner_set_graphs)
     bool graphs = false;
     hipStream_t cap_stream = nullptr;
-    static constexpr int kPlans = 4;
+    static constexpr int kPlans = 256;
     Plan plans[kPlans];
     uint64_t plan_clock = 0;
 };
@@ -567,7 +567,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
         VX_LAUNCH(h, KID_BRICK_SCAN, k_brick_retry<T VX_COMMA TILE>, 1024, kBlock, P, dchunks, L[1], res_ptrs_h[1], K[1],
                   tiles_h[1], ddef, ddrift);
         uint32_t g1 = probe_group(t1, h->tune.probe_blocks_bricks);
-        VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe<T VX_COMMA TILE>, (t1 + g1 - 1) / g1, TILE, P, K[1], tiles_h[1],
+        VX_LAUNCH(h, KID_PROBE_BRICKS, k_probe_bricks<T VX_COMMA TILE>, (t1 + g1 - 1) / g1, TILE, P, K[1], tiles_h[1],
                   res_ptrs_h[1], dchunks, L[1], t1, g1);
         for (uint32_t hh = 2; hh <= dmax; hh++) {
             unsigned th = (unsigned)ntiles[hh];
@@ -997,7 +997,8 @@ int vx_interner_stats(vx_interner* h, uint64_t* out, uint32_t n) {
     if (rc) return rc;
     static const int map[VX_STAT_COUNT] = {ST_HIT, ST_MISS, ST_DUP, ST_LEAF_MISS, ST_COLLAPSE,
                                            ST_RECYCLED, ST_H2, ST_H3, ST_H4, ST_H5, ST_CAND,
-                                           ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST, ST_MERGED};
+                                           ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST, ST_MERGED, ST_CAND1,
+                                           ST_MERGED1, ST_HIT1, ST_MISS1, ST_DUP1};
     for (uint32_t i = 0; i < n && i < (uint32_t)VX_STAT_COUNT; i++) {
         uint64_t sum = 0;
         for (int sidx = 0; sidx < kStatStripes; sidx++) sum += c.st[sidx][map[i]];

@@ -110,7 +110,7 @@ struct Ctl {
     uint32_t gc_count[kLevels + 2];
     uint32_t own_base;  // alloc_count when the apply's first probe starts
     // statistics, striped over 64 cache lines by block index (summed when read)
-    uint64_t st[kStatStripes][16];
+    uint64_t st[kStatStripes][32];
     // u8 interners: the leaf id of every value at the start of the current apply
     // (written by k_prepare, read by the brick pass; leaves created later in the
     // apply are not in it)

@@ -30,7 +30,9 @@ namespace vx {
 enum Stat {
     ST_HIT = 0, ST_MISS, ST_DUP, ST_LEAF_MISS, ST_COLLAPSE, ST_RECYCLED, ST_H2, ST_H3, ST_H4, ST_H5,
     ST_CAND, ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST, ST_MERGED,
-    ST_TOMB  // tombstones in the pattern table (signed adds; a counter, not a statistic)
+    ST_TOMB,  // tombstones in the pattern table (signed adds; a counter, not a statistic)
+    // the brick level's share of ST_CAND, ST_MERGED, ST_HIT, ST_MISS, ST_DUP
+    ST_CAND1, ST_MERGED1, ST_HIT1, ST_MISS1, ST_DUP1
 };
 
 __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
@@ -917,9 +919,10 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8
     }
     __syncthreads();
     if (threadIdx.x < R) tile_cnt[blockIdx.x * R + threadIdx.x] = s_cnt[threadIdx.x];
-    const int ids[4] = {ST_CAND, ST_COLLAPSE, ST_H2, ST_MERGED};
-    if (threadIdx.x < 4 && s_st[threadIdx.x])
-        atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
+    const int ids[6] = {ST_CAND, ST_COLLAPSE, ST_H2, ST_MERGED, ST_CAND1, ST_MERGED1};
+    const int src[6] = {0, 1, 2, 3, 0, 3};
+    if (threadIdx.x < 6 && s_st[src[threadIdx.x]])
+        atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[src[threadIdx.x]]);
 }
 
 // Bricks that waited for new leaves, and the queued reference-count corrections
@@ -965,6 +968,7 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
                     K.hash[k] = branch_hash(ch, tm);
                     K.slot[k] = slot;
                     atomicAdd((unsigned long long*)stat(P.ctl, ST_CAND), 1ull);
+                    atomicAdd((unsigned long long*)stat(P.ctl, ST_CAND1), 1ull);
                     out = res_pending(0xFFFFFFFFu);
                 } else if (r == 3) {
                     atomicOr(&P.ctl->abort_, 1u);  // the leaf could not be created
@@ -1243,6 +1247,11 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
         wave_count(stat(P.ctl, ST_HIT), valid && kind == 0);
         wave_count(stat(P.ctl, ST_DUP), valid && kind == 1);
         wave_count(stat(P.ctl, ST_MISS), valid && kind == 2);
+        if (L.h == 1) {  // uniform
+            wave_count(stat(P.ctl, ST_HIT1), valid && kind == 0);
+            wave_count(stat(P.ctl, ST_DUP1), valid && kind == 1);
+            wave_count(stat(P.ctl, ST_MISS1), valid && kind == 2);
+        }
     }
 }
 
@@ -1250,6 +1259,16 @@ template <typename T, int TILE>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
                                                  uint64_t* res, const ChunkDev* chunks, LevelInfo L,
                                                  uint32_t ntiles, uint32_t group) {
+    const uint32_t t0 = blockIdx.x * group;
+    probe_tiles<T, TILE>(P, K, tile_cnt, res, chunks, L, t0, min(group, ntiles - t0));
+}
+
+// The same pass for the brick level, under its own name so that kernel traces tell the
+// two apart.
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_probe_bricks(
+    Pool<T> P, Cands K, const uint32_t* tile_cnt, uint64_t* res, const ChunkDev* chunks, LevelInfo L,
+    uint32_t ntiles, uint32_t group) {
     const uint32_t t0 = blockIdx.x * group;
     probe_tiles<T, TILE>(P, K, tile_cnt, res, chunks, L, t0, min(group, ntiles - t0));
 }
