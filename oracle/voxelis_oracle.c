@@ -1291,3 +1291,43 @@ int vxo_tree_canonical_hash(vxo_interner* o, uint64_t root, uint8_t depth, uint6
     canon(o, root, out);
     return 0;
 }
+
+/* Every live pattern of the interner, independent of node ids: leaf values, and the
+ * canonical hash pair of every branch (the EMPTY branch excluded).  Test infrastructure:
+ * oracles run in separate processes on disjoint trees give the pattern counts of one
+ * shared interner as the union of these sets (equal structure <=> equal pattern). */
+int vxo_live_patterns(vxo_interner* o, uint64_t* leaf_values, uint64_t* branch_canon, size_t cap,
+                      size_t* n_leaves, size_t* n_branches) {
+    if (!o->memo) {
+        o->memo = (uint64_t(*)[2])calloc(o->capacity, sizeof(uint64_t[2]));
+        o->memo_stamp = (uint32_t*)calloc(o->capacity, sizeof(uint32_t));
+    }
+    o->stamp++;
+    if (o->stamp == 0) {
+        memset(o->memo_stamp, 0, o->capacity * sizeof(uint32_t));
+        o->stamp = 1;
+    }
+    size_t nl = 0, nb = 0;
+    for (int leaf = 0; leaf < 2; leaf++) {
+        const pmap* m = &o->maps[leaf];
+        for (size_t k = 0; k < m->cap; k++) {
+            uint64_t id = m->id[k];
+            if (id == 0 || id == INVALID_ID) continue;
+            if (leaf) {
+                if (nl < cap) leaf_values[nl] = o->values[bid_index(id)];
+                nl++;
+            } else {
+                uint64_t h[2];
+                canon(o, id, h);
+                if (nb < cap) {
+                    branch_canon[2 * nb] = h[0];
+                    branch_canon[2 * nb + 1] = h[1];
+                }
+                nb++;
+            }
+        }
+    }
+    *n_leaves = nl;
+    *n_branches = nb;
+    return (nl > cap || nb > cap) ? -1 : 0;
+}

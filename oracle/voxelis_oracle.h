@@ -133,6 +133,9 @@ int vxo_tree_apply_batch(vxo_interner* o, uint64_t* root, uint8_t depth, const v
                          int* changed);
 int vxo_tree_to_vec(const vxo_interner* o, uint64_t root, uint8_t depth, uint64_t* out);
 int vxo_tree_canonical_hash(vxo_interner* o, uint64_t root, uint8_t depth, uint64_t out[2]);
+/* live patterns: leaf values and branch canonical hash pairs (test infrastructure) */
+int vxo_live_patterns(vxo_interner* o, uint64_t* leaf_values, uint64_t* branch_canon, size_t cap,
+                      size_t* n_leaves, size_t* n_branches);
 
 #ifdef __cplusplus
 }

@@ -93,6 +93,7 @@ def lib():
         "vxo_snapshot_free": (None, [vp]),
         "vxo_tree_to_vec": (C.c_int, [vp, u64, u8, vp]),
         "vxo_tree_canonical_hash": (C.c_int, [vp, u64, u8, P64]),
+        "vxo_live_patterns": (C.c_int, [vp, vp, vp, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -232,6 +233,16 @@ class Interner:
 
     def alive_nodes(self):
         return lib().vxo_alive_nodes(self.h)
+
+    def live_patterns(self):
+        """(leaf values, branch canonical hash pairs) of every live pattern (EMPTY excluded)."""
+        nl, nb = C.c_size_t(0), C.c_size_t(0)
+        lib().vxo_live_patterns(self.h, None, None, 0, C.byref(nl), C.byref(nb))
+        cap = max(nl.value, nb.value, 1)
+        lv = np.zeros(cap, dtype=np.uint64)
+        bc = np.zeros((cap, 2), dtype=np.uint64)
+        _check(lib().vxo_live_patterns(self.h, lv.ctypes.data, bc.ctypes.data, cap, C.byref(nl), C.byref(nb)))
+        return lv[: nl.value], bc[: nb.value]
 
     def get_or_create_leaf(self, v):
         out = C.c_uint64(0)
