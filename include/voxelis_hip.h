@@ -55,9 +55,12 @@ typedef struct {
 /* ---- interner: interner/mod.rs ---- */
 
 /* VoxInterner::<T>::with_memory_budget (interner/mod.rs:45-155).
- * value_size is sizeof(T): 1 (u8/i8) or 2 (u16/i16).  capacity =
- * budget_bytes / (78 + value_size), exactly the reference's node_size.
- * load_factor (0 = default 0.5) sizes the branch pattern table. */
+ * value_size is sizeof(T): 1 (u8/i8), 2 (u16/i16) or 4 (i32/u32, core/voxel.rs:93; the
+ * type of the reference's tests and benchmarks).  Values cross the boundary as their bit
+ * patterns (the path compares them for equality only; 0 is T::default()).  capacity =
+ * budget_bytes / (78 + value_size), exactly the reference's node_size.  load_factor
+ * (0 = default 0.5) sizes the branch pattern table; with value_size 4 the leaf table is
+ * hashed as well (2^32 values are not mapped directly) and has as many slots. */
 int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_factor,
                        vx_interner** out);
 int vx_interner_destroy(vx_interner* h);

@@ -66,6 +66,10 @@ def live_nodes(roots, children):
     return leaves, branches
 
 
+# values are big-endian T::to_be_bytes (u8, u16, i32/u32 as their bit pattern)
+_FMT = {1: ">B", 2: ">H", 4: ">I"}
+
+
 def serialize(leaves, branches, roots, positions, children, value, value_size):
     """voxmodel.rs:177-283: ids 1.. for leaves then branches, each sorted by node index."""
     leaves = sorted(leaves, key=_index)
@@ -78,7 +82,7 @@ def serialize(leaves, branches, roots, positions, children, value, value_size):
     for b in branches:
         id_map[_index(b)] = nxt
         nxt += 1
-    fmt = ">B" if value_size == 1 else ">H"
+    fmt = _FMT[value_size]
     out = bytearray(struct.pack(">I", len(leaves)))
     for b in leaves:
         out += varint_encode(id_map[_index(b)]) + struct.pack(fmt, value(b))
@@ -98,8 +102,8 @@ def serialize(leaves, branches, roots, positions, children, value, value_size):
 def deserialize(data, value_size):
     """voxmodel.rs:285-408 parsing: ({leaf id: value}, {branch id: (mask, [8 child ids], lod)},
     [(position, root id)]); ids are checked to run 1.. as the importer asserts."""
-    size = 1 if value_size == 1 else 2
-    fmt = ">B" if size == 1 else ">H"
+    size = value_size
+    fmt = _FMT[value_size]
     at = 0
     (nl,) = struct.unpack_from(">I", data, at)
     at += 4

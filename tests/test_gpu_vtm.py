@@ -38,6 +38,8 @@ def _world(api, W, vsize, grid=3, seed=5):
     m, v, _, _ = W.terrain_batches(grid=grid, seed=seed)
     if vsize == 2:
         v = v.astype(np.uint16) * 257
+    elif vsize == 4:  # i32 patterns over the whole 32-bit range
+        v = v.astype(np.uint32) * np.uint32(2654435761)
     it = api.VoxInterner(vsize, 64 * MB)
     it_o = O.Interner(vsize, 64 * MB)
     f = api.Forest(it, [5] * n)
@@ -58,7 +60,7 @@ def _world(api, W, vsize, grid=3, seed=5):
     return it, f, it_o, trees, pos
 
 
-@pytest.mark.parametrize("vsize", [1, 2])
+@pytest.mark.parametrize("vsize", [1, 2, 4])
 def test_gpu_export_rebuilds_in_oracle(vx, vsize):
     torch, api, vtm, W = vx
     it, f, it_o, trees, pos = _world(api, W, vsize)
@@ -77,7 +79,7 @@ def test_gpu_export_rebuilds_in_oracle(vx, vsize):
         assert t2.canonical_hash(it2) == trees[i].canonical_hash(it_o) == g[i], i
 
 
-@pytest.mark.parametrize("vsize", [1, 2])
+@pytest.mark.parametrize("vsize", [1, 2, 4])
 def test_oracle_export_imports_on_gpu(vx, vsize):
     torch, api, vtm, W = vx
     _, _, it_o, trees, pos = _world(api, W, vsize, seed=6)

@@ -33,6 +33,8 @@ def _oracle_world(trees, seed, vsize=1):
         m, v = W.dense_to_batch(W.terrain_dense(W.terrain_heights(grid=2, chunk=32, seed=seed), t % 2, t // 2 % 2), 5)
         if vsize == 2:
             v = v.astype(np.uint16) * 300
+        elif vsize == 4:
+            v = v.astype(np.uint32) * np.uint32(2654435761)
         oracle_apply_dense(it, tr, m, v)
         ts.append(tr)
     return it, ts
@@ -61,7 +63,7 @@ def test_plain_checker_section_length():
 
 
 def test_roundtrip_rebuilds_every_tree():
-    for vsize in (1, 2):
+    for vsize in (1, 2, 4):
         it, trees = _oracle_world(5, seed=3, vsize=vsize)
         data, leaves, branches, pos = _section(it, trees, vsize)
         parsed = V.deserialize(data, vsize)

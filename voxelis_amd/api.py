@@ -19,7 +19,7 @@ from ._lib import APPLY_DRY_RUN, STAT_NAMES, ApplyOpts, BatchView, Tuning, VxErr
 
 EMPTY = 0
 INVALID = (1 << 64) - 1
-_DTYPES = {1: (np.uint8, torch.uint8), 2: (np.uint16, torch.uint16)}
+_DTYPES = {1: (np.uint8, torch.uint8), 2: (np.uint16, torch.uint16), 4: (np.uint32, torch.int32)}  # 4: i32/u32 bit patterns
 
 
 def _dev():

@@ -21,6 +21,12 @@
 
 using namespace vx;
 
+// One call per voxel width: u8 (1), u16 (2) and four-byte voxels (i32/u32, kept as their
+// bit pattern: the path only compares values for equality).
+#define VX_WIDTH_DISPATCH(h, fn, ...)                                                 \
+    ((h)->vsize == 1 ? fn<uint8_t>(__VA_ARGS__)                                      \
+                     : (h)->vsize == 2 ? fn<uint16_t>(__VA_ARGS__) : fn<uint32_t>(__VA_ARGS__))
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -136,6 +142,9 @@ struct vx_interner {
     uint64_t* tab = nullptr;
     uint32_t* leaf_map = nullptr;
     uint64_t* leaf_ids = nullptr;  // u16: the brick pass's leaf id of every value
+    uint64_t* ltab = nullptr;      // four-byte voxels: the hashed leaf table
+    uint64_t lslots = 0;
+    uint32_t* need_list = nullptr;
     uint8_t* leaf_need = nullptr;
     uint32_t* new_leaves = nullptr;
     Ctl* ctl = nullptr;
@@ -188,6 +197,10 @@ Pool<T> pool_of(vx_interner* h) {
     P.tmask = h->tslots - 1;
     P.leaf_map = h->leaf_map;
     P.leaf_ids = h->leaf_ids;
+    P.ltab = h->ltab;
+    P.lmask = h->lslots ? h->lslots - 1 : 0;
+    P.need_list = h->need_list;
+    P.need_cap = (uint32_t)std::min<uint64_t>(h->lslots, 0xFFFFFFFFull);
     P.leaf_need = h->leaf_need;
     P.new_leaves = h->new_leaves;
     P.ctl = h->ctl;
@@ -365,7 +378,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
         std::memset(&C, 0, sizeof(C));
         C.masks = b.masks;
         C.values = b.values;
-        C.fill = b.fill_value & (sizeof(T) == 1 ? 0xFFull : 0xFFFFull);
+        C.fill = b.fill_value & (sizeof(T) == 1 ? 0xFFull : sizeof(T) == 2 ? 0xFFFFull : 0xFFFFFFFFull);
         C.orig = perm[c];
         C.depth = b.depth;
         C.has_fill = b.has_fill ? 1 : 0;
@@ -489,7 +502,7 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     HIPCHK(hipMemcpyAsync(g_tables.ptr, hptrs, tab_bytes, hipMemcpyHostToDevice, s));
     P.levels = (const Cands*)((void**)g_tables.ptr + 2 * (kLevels + 1));
     pinned_release(h);
-    uint32_t nvalues = 1u << (8 * sizeof(T));
+    uint32_t nvalues = sizeof(T) <= 2 ? 1u << (8 * sizeof(T)) : 0u;  // four-byte voxels: the request list
     uint64_t nb = std::max<uint64_t>(dmax >= 2 ? L[1].nslots : 1, 1);
     if (!h->deferred.ensure(nb * 8) || !h->drift.ensure(nb * sizeof(Drift))) return VX_E_HIP;
     uint64_t* ddef = (uint64_t*)h->deferred.ptr;
@@ -507,6 +520,9 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     for (uint32_t hh = LT.dmax + 1; hh <= (uint32_t)kLevels; hh++) LT.start[hh + 1] = LT.start[hh];
     if (undo_cap > 0xFFFFFFFFull) return VX_E_INVALID_ARG;
     if (opt.dry && (uint64_t)h->capacity + 65536 + (undo_cap - n) > 0xFFFFFFFFull) return VX_E_INVALID_ARG;
+    // a dry run's missing leaf is capacity + its key (the value, or the request's slot)
+    if (opt.dry && (uint64_t)h->capacity + (sizeof(T) <= 2 ? 65536ull : h->lslots) > 0xFFFFFFFFull)
+        return VX_E_INVALID_ARG;
     if (!h->undo.ensure(std::max<uint64_t>(undo_cap, 1) * sizeof(Drift))) return VX_E_HIP;
     P.undo = (Drift*)h->undo.ptr;
     P.undo_cap = (uint32_t)undo_cap;
@@ -634,9 +650,9 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
 }
 
 // Tombstones in the pattern table: a counter striped over the statistics lines.
-uint64_t tombstones_of(const Ctl& c) {
+uint64_t tombstones_of(const Ctl& c, int which = ST_TOMB) {
     uint64_t t = 0;
-    for (int i = 0; i < kStatStripes; i++) t += c.st[i][ST_TOMB];
+    for (int i = 0; i < kStatStripes; i++) t += c.st[i][which];
     return t;
 }
 
@@ -646,6 +662,10 @@ int rebuild_table(vx_interner* h) {
     Pool<T> P = pool_of<T>(h);
     k_tab_clear<<<grid_for(h->tslots / 2), kBlock, 0, h->stream>>>(h->tab, h->tslots, h->ctl);
     k_tab_reinsert<T><<<grid_for(h->capacity), kBlock, 0, h->stream>>>(P);
+    if (h->ltab) {
+        k_ltab_clear<<<grid_for(h->lslots / 2), kBlock, 0, h->stream>>>(h->ltab, h->lslots, h->ctl);
+        k_ltab_reinsert<T><<<grid_for(h->capacity), kBlock, 0, h->stream>>>(P);
+    }
     h->rebuilds++;
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
 }
@@ -658,6 +678,10 @@ int maybe_rebuild(vx_interner* h) {
     h->ctr_pending = false;
     const uint64_t live = h->h_ctl->live_branches, tomb = tombstones_of(*h->h_ctl);
     if (tomb >= h->tslots / 16 && live + tomb > h->tslots / 4 * 3) return rebuild_table<T>(h);
+    if (h->ltab) {
+        const uint64_t ll = h->h_ctl->live_leaves, lt = tombstones_of(*h->h_ctl, ST_LTOMB);
+        if (lt >= h->lslots / 16 && ll + lt > h->lslots / 4 * 3) return rebuild_table<T>(h);
+    }
     return VX_OK;
 }
 
@@ -774,7 +798,7 @@ int fill_impl(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* vals) {
     if (!h->gc_front[0].ensure(gcap * 4) || !h->gc_front[1].ensure(gcap * 4)) return VX_E_HIP;
     uint32_t* gf0 = (uint32_t*)h->gc_front[0].ptr;
     uint32_t* gf1 = (uint32_t*)h->gc_front[1].ptr;
-    uint32_t nvalues = 1u << (8 * sizeof(T));
+    uint32_t nvalues = sizeof(T) <= 2 ? 1u << (8 * sizeof(T)) : 0u;
     VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
     VX_LAUNCH(h, KID_FILL, k_fill_prepare<T>, grid_for(n), kBlock, P, (uint32_t)n, roots, vals, gf0);
     for (uint32_t r = 0; r < (uint32_t)kLevels + 1; r++) {
@@ -860,7 +884,7 @@ int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_f
                        vx_interner** out) {
     if (!out) return VX_E_INVALID_ARG;
     *out = nullptr;
-    if (value_size != 1 && value_size != 2) return VX_E_UNSUPPORTED;
+    if (value_size != 1 && value_size != 2 && value_size != 4) return VX_E_UNSUPPORTED;
     uint64_t node_size = 4 + 2 + 64 + value_size + 8;  // interner/mod.rs:158-164
     uint64_t cap = budget_bytes / node_size;
     if (cap == 0 || cap > 0xFFFFFFFEull) return VX_E_BUDGET;
@@ -873,12 +897,18 @@ int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_f
     uint64_t ts = 1024;
     while (ts < want) ts <<= 1;
     h->tslots = ts;
-    uint64_t nv = 1ull << (8 * value_size);
+    // u8/u16: a direct map of every value; four-byte voxels: a hashed leaf table as large as
+    // the branch table, and every leaf an apply creates listed once
+    const bool direct = value_size <= 2;
+    uint64_t nv = direct ? 1ull << (8 * value_size) : 1;
+    h->lslots = direct ? 0 : ts;
+    uint64_t nnew = direct ? nv : std::min<uint64_t>(cap, ts);
     bool ok = dmalloc(&h->children, cap * 64) && dmalloc(&h->tms, cap * 2) &&
               dmalloc(&h->values, cap * value_size) && dmalloc(&h->refs, cap * 4) &&
               dmalloc(&h->gens, cap * 2) && dmalloc(&h->free_stack, cap * 4) &&
               dmalloc(&h->tab, ts * 8) && dmalloc(&h->leaf_map, nv * 4) &&
-              dmalloc(&h->leaf_need, nv) && dmalloc(&h->new_leaves, nv * 4) &&
+              dmalloc(&h->leaf_need, nv) && dmalloc(&h->new_leaves, nnew * 4) &&
+              (direct || (dmalloc(&h->ltab, h->lslots * 8) && dmalloc(&h->need_list, h->lslots * 4))) &&
               (value_size != 2 || dmalloc(&h->leaf_ids, nv * 8)) &&
               dmalloc(&h->ctl, sizeof(Ctl));
     ok = ok && hipHostMalloc((void**)&h->h_ctl, sizeof(Ctl), hipHostMallocDefault) == hipSuccess &&
@@ -895,7 +925,8 @@ int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_f
              hipMemset(h->values, 0, cap * value_size) == hipSuccess &&
              hipMemset(h->refs, 0, cap * 4) == hipSuccess && hipMemset(h->gens, 0, cap * 2) == hipSuccess &&
              hipMemset(h->tab, 0, ts * 8) == hipSuccess && hipMemset(h->leaf_map, 0, nv * 4) == hipSuccess &&
-             hipMemset(h->leaf_need, 0, nv) == hipSuccess;
+             hipMemset(h->leaf_need, 0, nv) == hipSuccess &&
+             (direct || hipMemset(h->ltab, 0, h->lslots * 8) == hipSuccess);
     Ctl c;
     std::memset(&c, 0, sizeof(c));
     c.next_index = 1;
@@ -914,7 +945,8 @@ int vx_interner_destroy(vx_interner* h) {
     if (!h) return VX_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* ptrs[] = {h->children, h->tms, h->values, h->refs, h->gens, h->free_stack,
-                    h->tab, h->leaf_map, h->leaf_need, h->new_leaves, h->ctl, h->leaf_ids};
+                    h->tab, h->leaf_map, h->leaf_need, h->new_leaves, h->ctl, h->leaf_ids,
+                    h->ltab, h->need_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->chunks.release();
@@ -1038,8 +1070,7 @@ int vx_trees_apply_batches(vx_interner* h, size_t n, uint64_t* roots, const vx_b
     if (n > 0x7FFFFFFFull) return VX_E_INVALID_ARG;
     for (size_t i = 0; i < n; i++)
         if (batches[i].reserved != 0) return VX_E_INVALID_ARG;
-    return h->vsize == 1 ? apply_impl<uint8_t>(h, n, roots, batches, status)
-                         : apply_impl<uint16_t>(h, n, roots, batches, status);
+    return VX_WIDTH_DISPATCH(h, apply_impl, h, n, roots, batches, status);
 }
 
 int vx_tree_apply_batch_host(vx_interner* h, uint64_t* root, const vx_batch_view* batch,
@@ -1086,9 +1117,7 @@ int vx_trees_apply_edits(vx_interner* h, size_t n, uint64_t* roots, const uint8_
     if (n == 0) return VX_OK;
     if (n > 0x7FFFFFFFull) return VX_E_INVALID_ARG;
     if (edit_offsets[n] > 0 && (!xyz || !values)) return VX_E_INVALID_ARG;
-    return h->vsize == 1
-               ? apply_edits_impl<uint8_t>(h, n, roots, depths, edit_offsets, xyz, values, status)
-               : apply_edits_impl<uint16_t>(h, n, roots, depths, edit_offsets, xyz, values, status);
+    return VX_WIDTH_DISPATCH(h, apply_edits_impl, h, n, roots, depths, edit_offsets, xyz, values, status);
 }
 
 namespace {
@@ -1101,8 +1130,7 @@ int vx_trees_set(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* depth
     ApplyOpts o;
     o.set_xyz = xyz;
     o.set_values = values;
-    return h->vsize == 1 ? apply_impl<uint8_t>(h, n, roots, views.data(), status, nullptr, o)
-                         : apply_impl<uint16_t>(h, n, roots, views.data(), status, nullptr, o);
+    return VX_WIDTH_DISPATCH(h, apply_impl, h, n, roots, views.data(), status, nullptr, o);
 }
 
 int opts_of(const vx_interner* h, const vx_apply_opts* o, ApplyOpts* out) {
@@ -1128,8 +1156,7 @@ int vx_trees_apply_batches_ex(vx_interner* h, size_t n, uint64_t* roots, const v
     ApplyOpts o;
     int rc = opts_of(h, opts, &o);
     if (rc) return rc;
-    return h->vsize == 1 ? apply_impl<uint8_t>(h, n, roots, batches, status, nullptr, o)
-                         : apply_impl<uint16_t>(h, n, roots, batches, status, nullptr, o);
+    return VX_WIDTH_DISPATCH(h, apply_impl, h, n, roots, batches, status, nullptr, o);
 }
 
 int vx_trees_apply_edits_ex(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* depths,
@@ -1143,9 +1170,7 @@ int vx_trees_apply_edits_ex(vx_interner* h, size_t n, uint64_t* roots, const uin
     ApplyOpts o;
     int rc = opts_of(h, opts, &o);
     if (rc) return rc;
-    return h->vsize == 1
-               ? apply_edits_impl<uint8_t>(h, n, roots, depths, edit_offsets, xyz, values, status, o)
-               : apply_edits_impl<uint16_t>(h, n, roots, depths, edit_offsets, xyz, values, status, o);
+    return VX_WIDTH_DISPATCH(h, apply_edits_impl, h, n, roots, depths, edit_offsets, xyz, values, status, o);
 }
 
 int vx_trees_get(vx_interner* h, size_t ntrees, const uint64_t* roots, const uint8_t* depths,
@@ -1153,9 +1178,7 @@ int vx_trees_get(vx_interner* h, size_t ntrees, const uint64_t* roots, const uin
                  uint8_t* present) {
     if (!h || (nq && (!roots || !depths || !chunk_of || !xyz || !out))) return VX_E_INVALID_ARG;
     if (nq == 0) return VX_OK;
-    return h->vsize == 1
-               ? get_impl<uint8_t>(h, nq, ntrees, roots, depths, chunk_of, xyz, out, present)
-               : get_impl<uint16_t>(h, nq, ntrees, roots, depths, chunk_of, xyz, out, present);
+    return VX_WIDTH_DISPATCH(h, get_impl, h, nq, ntrees, roots, depths, chunk_of, xyz, out, present);
 }
 
 int vx_trees_get_on(vx_interner* h, void* stream, size_t ntrees, const uint64_t* roots,
@@ -1164,17 +1187,14 @@ int vx_trees_get_on(vx_interner* h, void* stream, size_t ntrees, const uint64_t*
     if (!h || (nq && (!roots || !depths || !chunk_of || !xyz || !out))) return VX_E_INVALID_ARG;
     if (nq == 0) return VX_OK;
     hipStream_t st = (hipStream_t)stream;
-    return h->vsize == 1
-               ? get_on_impl<uint8_t>(h, st, nq, ntrees, roots, depths, chunk_of, xyz, out, present, stop)
-               : get_on_impl<uint16_t>(h, st, nq, ntrees, roots, depths, chunk_of, xyz, out, present, stop);
+    return VX_WIDTH_DISPATCH(h, get_on_impl, h, st, nq, ntrees, roots, depths, chunk_of, xyz, out, present, stop);
 }
 
 int vx_trees_fill(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* values) {
     if (!h || (n && (!roots || !values))) return VX_E_INVALID_ARG;
     if (h->poisoned) return VX_E_POISONED;
     if (n == 0) return VX_OK;
-    return h->vsize == 1 ? fill_impl<uint8_t>(h, n, roots, values)
-                         : fill_impl<uint16_t>(h, n, roots, values);
+    return VX_WIDTH_DISPATCH(h, fill_impl, h, n, roots, values);
 }
 
 int vx_trees_to_vec(vx_interner* h, size_t n, const uint64_t* roots, const uint8_t* depths, uint32_t lod,
@@ -1182,15 +1202,13 @@ int vx_trees_to_vec(vx_interner* h, size_t n, const uint64_t* roots, const uint8
     if (!h || (n && (!roots || !depths || !out || out_stride == 0))) return VX_E_INVALID_ARG;
     if (n == 0) return VX_OK;
     if (lod > (uint32_t)kMaxDepth || (uint64_t)n * out_stride / out_stride != (uint64_t)n) return VX_E_INVALID_ARG;
-    return h->vsize == 1 ? to_vec_impl<uint8_t>(h, n, roots, depths, lod, out_stride, out)
-                         : to_vec_impl<uint16_t>(h, n, roots, depths, lod, out_stride, out);
+    return VX_WIDTH_DISPATCH(h, to_vec_impl, h, n, roots, depths, lod, out_stride, out);
 }
 
 int vx_trees_canonical_hash(vx_interner* h, size_t n, const uint64_t* roots, uint64_t* out) {
     if (!h || (n && (!roots || !out))) return VX_E_INVALID_ARG;
     if (n == 0) return VX_OK;
-    return h->vsize == 1 ? canon_impl<uint8_t>(h, n, roots, out)
-                         : canon_impl<uint16_t>(h, n, roots, out);
+    return VX_WIDTH_DISPATCH(h, canon_impl, h, n, roots, out);
 }
 
 int vx_interner_set_tuning(vx_interner* h, const vx_tuning* t) {
@@ -1215,7 +1233,7 @@ int vx_interner_get_tuning(const vx_interner* h, vx_tuning* t) {
 int vx_interner_rebuild_table(vx_interner* h, uint64_t* rebuilds) {
     if (!h) return VX_E_INVALID_ARG;
     if (h->poisoned) return VX_E_POISONED;
-    int rc = h->vsize == 1 ? rebuild_table<uint8_t>(h) : rebuild_table<uint16_t>(h);
+    int rc = VX_WIDTH_DISPATCH(h, rebuild_table, h);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
     if (rebuilds) *rebuilds = h->rebuilds;

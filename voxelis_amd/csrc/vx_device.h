@@ -104,6 +104,7 @@ struct Ctl {
     uint32_t abort_;          // this apply could not allocate: 1 capacity, 2 reservation
     uint32_t n_undo;          // entries of the undo log (reference-count deltas applied)
     uint32_t h1_fail;         // a tree of this apply failed in the brick pass (leaf-brick panic)
+    uint32_t n_need;          // i32: values requested by this apply (need_list entries)
     uint32_t n_owners;        // new branch nodes of the last apply (its owner list), for the GC
     uint32_t pad1;
     uint32_t ncand[kLevels + 1];
@@ -185,7 +186,11 @@ struct Pool {
     uint32_t* free_stack;
     uint64_t* tab;
     uint64_t tmask;
-    uint32_t* leaf_map;  // value -> node index (0 = none)
+    uint32_t* leaf_map;  // u8/u16: value -> node index (0 = none)
+    uint64_t* ltab;      // i32: open-addressing leaf table (value << 32 | index)
+    uint64_t lmask;      // its slot count - 1
+    uint32_t* need_list; // i32: values requested by this apply (k_leaf_alloc creates them)
+    uint32_t need_cap;
     uint64_t* leaf_ids;  // u16: the leaf id of every value at the start of the apply (k_prepare)
     uint8_t* leaf_need;
     uint32_t* new_leaves;

@@ -32,7 +32,8 @@ enum Stat {
     ST_CAND, ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST, ST_MERGED,
     ST_TOMB,  // tombstones in the pattern table (signed adds; a counter, not a statistic)
     // the brick level's share of ST_CAND, ST_MERGED, ST_HIT, ST_MISS, ST_DUP
-    ST_CAND1, ST_MERGED1, ST_HIT1, ST_MISS1, ST_DUP1
+    ST_CAND1, ST_MERGED1, ST_HIT1, ST_MISS1, ST_DUP1,
+    ST_LTOMB  // tombstones in the hashed leaf table of four-byte voxels
 };
 
 __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
@@ -149,6 +150,10 @@ __device__ inline bool agg_add(uint32_t* akey, uint32_t* acnt, uint32_t idx, uin
     return false;
 }
 
+struct Vox8x4 {  // 8 four-byte voxels
+    uint4 lo, hi;
+};
+
 template <typename T>
 __device__ inline void load8(const T* v, uint32_t out[8]) {
     if constexpr (sizeof(T) == 1) {
@@ -164,8 +169,9 @@ __device__ inline void load8(const T* v, uint32_t out[8]) {
             out[2 * i + 1] = a[i] >> 16;
         }
     } else {
-#pragma unroll
-        for (int i = 0; i < 8; i++) out[i] = (uint32_t)v[i];
+        const uint4 a = *(const uint4*)v, b = *(const uint4*)(v + 4);
+        out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
+        out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
     }
 }
 
@@ -193,9 +199,123 @@ __device__ inline uint32_t calc_average(const uint32_t v[8]) {
     return best;
 }
 
+// ------------------------------------------------------------------ leaf table
+
+// value -> leaf node index (get_or_create_leaf's map, interner/mod.rs:627-710).  u8/u16:
+// direct-mapped arrays (leaf_map; leaf_need marks values to create).  i32: 2^32 values
+// cannot be mapped directly, so an open-addressing table (ltab) of 8-byte slots
+// value << 32 | index: 0 empty, kLeafTomb a removed entry (value 0 never has a leaf), and
+// index kLeafNeed a value requested by this apply and not created yet; the claim that
+// inserts a request lists the value once (need_list) for k_leaf_alloc.  A value is the
+// whole key, so a slot compare is the full-key compare.
+constexpr uint64_t kLeafTomb = 1ull;
+constexpr uint32_t kLeafNeed = 0xFFFFFFFFu;
+
+__device__ inline uint64_t leaf_hash(uint32_t v) { return mix64((uint64_t)v * 0x9E3779B97F4A7C15ull + 0x4C454146ull); }
+
+// i32: the slot holding value v, or ~0 when absent.
+template <typename T>
+__device__ inline uint64_t ltab_find(const Pool<T>& P, uint32_t v) {
+    uint64_t s = leaf_hash(v) & P.lmask;
+    for (uint64_t g = 0; g <= P.lmask; g++) {
+        const uint64_t e = P.ltab[s];
+        if (e == 0) return ~0ull;
+        if ((uint32_t)(e >> 32) == v && e != kLeafTomb) return s;
+        s = (s + 1) & P.lmask;
+    }
+    return ~0ull;
+}
+
+// The live leaf index of value v (0: none, or requested and not created yet).
+template <typename T>
+__device__ inline uint32_t leaf_index(const Pool<T>& P, uint32_t v) {
+    if constexpr (sizeof(T) <= 2) {
+        return P.leaf_map[v];
+    } else {
+        const uint64_t s = ltab_find(P, v);
+        if (s == ~0ull) return 0;
+        const uint32_t i = (uint32_t)P.ltab[s];
+        return i == kLeafNeed ? 0u : i;
+    }
+}
+
+// Ask k_leaf_alloc to create the leaf of value v; returns the key of its dry-run
+// placeholder id (capacity + key): the value itself for u8/u16, the request's slot for
+// i32 (one per value, the same for every requester).
+template <typename T>
+__device__ inline uint32_t leaf_request(const Pool<T>& P, uint32_t v) {
+    if constexpr (sizeof(T) <= 2) {
+        P.leaf_need[v] = 1;
+        return v;
+    } else {
+        uint64_t s = leaf_hash(v) & P.lmask;
+        uint64_t first_tomb = ~0ull;
+        for (uint64_t g = 0; g <= 4 * (P.lmask + 1); g++) {
+            const uint64_t e = P.ltab[s];
+            if (e != 0 && e != kLeafTomb && (uint32_t)(e >> 32) == v) return (uint32_t)s;
+            if (e == kLeafTomb && first_tomb == ~0ull) first_tomb = s;
+            if (e == 0 || (first_tomb != ~0ull && g > P.lmask)) {
+                const uint64_t target = first_tomb != ~0ull ? first_tomb : s;
+                const uint64_t expected = first_tomb != ~0ull ? kLeafTomb : 0ull;
+                const uint64_t want = ((uint64_t)v << 32) | kLeafNeed;
+                const uint64_t seen = atomicCAS((unsigned long long*)&P.ltab[target], (unsigned long long)expected,
+                                                (unsigned long long)want);
+                if (seen == expected) {
+                    const uint32_t pos = atomicAdd(&P.ctl->n_need, 1u);
+                    if (pos < P.need_cap) P.need_list[pos] = v;
+                    else atomicOr(&P.ctl->err, kErrTableFull | kErrPoison);
+                    return (uint32_t)target;
+                }
+                if ((uint32_t)(seen >> 32) == v && seen != kLeafTomb) return (uint32_t)target;
+                s = (target + 1) & P.lmask;  // another value took the slot: go on behind it
+                first_tomb = ~0ull;
+                continue;
+            }
+            s = (s + 1) & P.lmask;
+        }
+        atomicOr(&P.ctl->err, kErrTableFull | kErrPoison);
+        return 0;
+    }
+}
+
+// Publish the leaf index of value v (k_leaf_alloc, snapshot import).
+template <typename T>
+__device__ inline void leaf_publish(const Pool<T>& P, uint32_t v, uint32_t idx) {
+    if constexpr (sizeof(T) <= 2) {
+        P.leaf_map[v] = idx;
+    } else {
+        const uint64_t want = ((uint64_t)v << 32) | idx;
+        uint64_t s = leaf_hash(v) & P.lmask;
+        for (uint64_t g = 0; g <= P.lmask; g++) {
+            const uint64_t e = P.ltab[s];
+            if (e != 0 && e != kLeafTomb && (uint32_t)(e >> 32) == v) {  // its request
+                P.ltab[s] = want;
+                return;
+            }
+            if (e == 0 && atomicCAS((unsigned long long*)&P.ltab[s], 0ull, (unsigned long long)want) == 0ull) return;
+            s = (s + 1) & P.lmask;
+        }
+        atomicOr(&P.ctl->err, kErrTableFull | kErrPoison);
+    }
+}
+
+// Remove value v's leaf (freed, or its creation rolled back).
+template <typename T>
+__device__ inline void leaf_remove(const Pool<T>& P, uint32_t v) {
+    if constexpr (sizeof(T) <= 2) {
+        P.leaf_map[v] = 0;
+    } else {
+        const uint64_t s = ltab_find(P, v);
+        if (s != ~0ull) {
+            P.ltab[s] = kLeafTomb;
+            atomicAdd((unsigned long long*)stat(P.ctl, ST_LTOMB), 1ull);
+        }
+    }
+}
+
 template <typename T>
 __device__ inline uint64_t leaf_id_of(const Pool<T>& P, uint32_t v) {
-    uint32_t idx = P.leaf_map[v];
+    uint32_t idx = leaf_index(P, v);
     return make_leaf(idx, P.gens[idx]);
 }
 
@@ -324,6 +444,19 @@ __global__ void k_begin(Ctl* ctl) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl_begin(ctl);
 }
 
+// i32: requests still open at the end of an apply (a dry run's, whose placeholders were
+// counted; a leaf whose creation failed) leave the table.  One block.
+template <typename T>
+__device__ inline void drop_requests(const Pool<T>& P) {
+    if constexpr (sizeof(T) == 4) {
+        const uint32_t nn = min(P.ctl->n_need, P.need_cap);
+        for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x) {
+            const uint64_t sl = ltab_find(P, P.need_list[i]);
+            if (sl != ~0ull && (uint32_t)P.ltab[sl] == kLeafNeed) P.ltab[sl] = kLeafTomb;
+        }
+    }
+}
+
 // Close the allocation window of a launch sequence.  An aborted apply
 // (rollback = true) gives every allocation back instead: the allocator cursors
 // stay, reference counts of the taken indices go back to zero and new leaves
@@ -345,7 +478,8 @@ __global__ void k_end_alloc(Pool<T> P, bool rollback, uint32_t* counts) {
         }
         uint32_t nl = ctl->n_new_leaves;
         for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x)
-            P.leaf_map[(uint32_t)P.values[P.new_leaves[i]]] = 0;
+            leaf_remove(P, (uint32_t)P.values[P.new_leaves[i]]);
+        drop_requests(P);
         __syncthreads();
         if (threadIdx.x == 0) {
             ctl->live_leaves -= nl;
@@ -355,6 +489,7 @@ __global__ void k_end_alloc(Pool<T> P, bool rollback, uint32_t* counts) {
         }
         return;
     }
+    drop_requests(P);
     if (threadIdx.x == 0) {
         uint32_t n = ctl->alloc_count;
         ctl->n_owners = P.dry || n < ctl->own_base ? 0u : n - ctl->own_base;
@@ -457,7 +592,7 @@ __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_
             if (pre_status && pre_status[C.orig] < 0 && C.status >= 0) C.status = pre_status[C.orig];
             if (C.status >= 0 && C.has_fill) {
                 uint32_t v = (uint32_t)C.fill;
-                if (P.leaf_map[v] == 0) P.leaf_need[v] = 1;
+                if (leaf_index(P, v) == 0) leaf_request(P, v);
             }
         }
     }
@@ -468,34 +603,55 @@ __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_
 // One block: create the leaves marked as needed (interner/mod.rs:669-708), then
 // fix each tree's Phase-1 start node.  The fill leaf of a batch that also has
 // patches keeps its reference (H3, voxtree.rs:742-754).
+// Create the leaves requested so far (one block; a dry run only counts them).  Returns
+// the number created (or counted).
 template <typename T>
-__global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t nvalues) {
+__device__ inline uint32_t create_requested_leaves(const Pool<T>& P, uint32_t nvalues) {
     __shared__ uint32_t created;
     if (threadIdx.x == 0) created = 0;
     __syncthreads();
-    for (uint32_t v = threadIdx.x; v < nvalues; v += blockDim.x) {
-        if (P.leaf_need[v]) {
+    const uint32_t nreq = sizeof(T) == 4 ? min(P.ctl->n_need, P.need_cap) : nvalues;
+    for (uint32_t r = threadIdx.x; r < nreq; r += blockDim.x) {
+        uint32_t v = r;
+        uint64_t sl = 0;
+        if constexpr (sizeof(T) == 4) {
+            v = P.need_list[r];
+            sl = ltab_find(P, v);
+            if (sl == ~0ull || (uint32_t)P.ltab[sl] != kLeafNeed) continue;
+        } else {
+            if (!P.leaf_need[v]) continue;
             P.leaf_need[v] = 0;
-            if (P.leaf_map[v] == 0) {
-                if (P.dry) {
-                    atomicAdd(&created, 1u);
-                    continue;
-                }
-                bool ok;
-                uint32_t idx = alloc_node(P, &ok);
-                if (ok) {
-                    P.values[idx] = (T)v;
-                    P.tms[idx] = 0;
-                    P.refs[idx] = 0;
-                    P.leaf_map[v] = idx;
-                    uint32_t k = atomicAdd(&P.ctl->n_new_leaves, 1u);
-                    P.new_leaves[k] = idx;
-                    atomicAdd(&created, 1u);
-                }
-            }
+            if (P.leaf_map[v] != 0) continue;
+        }
+        if (P.dry) {  // counted; an i32 request stays until the end of the apply (its placeholder)
+            atomicAdd(&created, 1u);
+            continue;
+        }
+        bool ok;
+        uint32_t idx = alloc_node(P, &ok);
+        if (ok) {
+            P.values[idx] = (T)v;
+            P.tms[idx] = 0;
+            P.refs[idx] = 0;
+            if constexpr (sizeof(T) == 4) P.ltab[sl] = ((uint64_t)v << 32) | idx;
+            else P.leaf_map[v] = idx;
+            uint32_t k = atomicAdd(&P.ctl->n_new_leaves, 1u);
+            P.new_leaves[k] = idx;
+            atomicAdd(&created, 1u);
+        } else if constexpr (sizeof(T) == 4) {
+            P.ltab[sl] = kLeafTomb;
         }
     }
     __syncthreads();
+    return created;
+}
+
+// One block: create the leaves marked as needed (interner/mod.rs:669-708), then
+// fix each tree's Phase-1 start node.  The fill leaf of a batch that also has
+// patches keeps its reference (H3, voxtree.rs:742-754).
+template <typename T>
+__global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t nvalues) {
+    const uint32_t created = create_requested_leaves(P, nvalues);
     if (threadIdx.x == 0) {
         if (P.dry) {
             P.ctl->dry_new_leaves += created;
@@ -507,9 +663,9 @@ __global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t n
     for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
         ChunkDev& C = chunks[c];
         if (C.status < 0 || !C.has_fill) continue;
-        uint32_t idx = P.leaf_map[(uint32_t)C.fill];
+        uint32_t idx = leaf_index(P, (uint32_t)C.fill);
         if (idx == 0 && P.dry) {
-            C.init = make_leaf(P.capacity + (uint32_t)C.fill, 0);
+            C.init = make_leaf(P.capacity + leaf_request(P, (uint32_t)C.fill), 0);
             continue;
         }
         if (idx == 0) continue;  // allocation failed (OOM flagged)
@@ -608,7 +764,7 @@ __device__ inline void tile_count(uint32_t* s, int id, bool pred) {
 template <typename T>
 __device__ inline uint64_t leaf_lookup(const Pool<T>& P, const uint64_t* leaf_ids, uint32_t v) {
     if (leaf_ids) return leaf_ids[v];
-    uint32_t li = P.leaf_map[v];
+    uint32_t li = leaf_index(P, v);
     return li ? make_leaf(li, P.gens[li]) : kEmpty;
 }
 
@@ -616,7 +772,7 @@ template <typename T>
 __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m, const uint32_t v[8],
                                  uint64_t base, uint64_t* out, uint64_t ch[8], uint32_t* tm,
                                  uint32_t* drift_idx, int32_t* drift_delta, int* flags,
-                                 const uint64_t* leaf_ids) {
+                                 const uint64_t* leaf_ids, bool request = true) {
     *drift_idx = kNoIdx;
     bool all_same = m == 0xFFu;
 #pragma unroll
@@ -624,9 +780,9 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
     if (all_same) {
         uint64_t lid = leaf_lookup(P, leaf_ids, v[0]);
         if (lid == kEmpty) {
-            P.leaf_need[v[0]] = 1;
+            const uint32_t key = request ? leaf_request(P, v[0]) : 0u;
             if (!P.dry) return 3;
-            *out = make_leaf(P.capacity + v[0], 0);  // dry run: a placeholder for the new leaf
+            *out = make_leaf(P.capacity + key, 0);  // dry run: a placeholder for the new leaf
         } else {
             *out = lid;
         }
@@ -637,11 +793,11 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
     bool missing = false;
     bool fill_split = base == kFillMark;
     if (fill_split) {
-        uint32_t fi = P.leaf_map[(uint32_t)C.fill];
+        uint32_t fi = leaf_index(P, (uint32_t)C.fill);
         if (fi == 0) {
-            P.leaf_need[(uint32_t)C.fill] = 1;
+            const uint32_t key = request ? leaf_request(P, (uint32_t)C.fill) : 0u;
             if (P.dry) {
-                base = make_leaf(P.capacity + (uint32_t)C.fill, 0);
+                base = make_leaf(P.capacity + key, 0);
             } else {
                 missing = true;
                 base = make_leaf(0, 0);
@@ -675,12 +831,12 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
             if (same) continue;
         }
         if (lid == kEmpty) {
-            P.leaf_need[v[i]] = 1;
+            const uint32_t key = request ? leaf_request(P, v[i]) : 0u;
             if (!P.dry) {
                 missing = true;
                 continue;
             }
-            ch[i] = make_leaf(P.capacity + v[i], 0);
+            ch[i] = make_leaf(P.capacity + key, 0);
         } else {
             ch[i] = lid;
         }
@@ -730,12 +886,14 @@ constexpr int brick_rounds() { return TILE >= 512 ? 2 : 4; }
 // four-slot form's phase-1 loads and phase-2 keys take more registers (6 waves for u8,
 // 5 for u16).
 template <typename T, int TILE, int R>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8 : (sizeof(T) == 1 ? 5 : 4), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8 : (sizeof(T) == 1 ? 5 : sizeof(T) == 2 ? 4 : 3), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
                                                        uint64_t* deferred, Drift* drift,
                                                        const uint64_t* old2) {
-    typedef typename std::conditional<sizeof(T) == 1, uint2, uint4>::type Vox8;  // a brick's 8 values
-    static_assert(sizeof(Vox8) == 8 * sizeof(T), "u8 or u16 voxels");
+    // a brick's 8 values
+    typedef typename std::conditional<sizeof(T) == 1, uint2, typename std::conditional<sizeof(T) == 2, uint4,
+                                                                                          Vox8x4>::type>::type Vox8;
+    static_assert(sizeof(Vox8) == 8 * sizeof(T), "8 voxels");
     static_assert(R * TILE <= 65536, "queued brick index is 16 bits");
     __shared__ uint32_t s_nq;
     __shared__ uint32_t s_cnt[R];
@@ -954,7 +1112,10 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
                 uint32_t tm = 0, didx;
                 int32_t dd;
                 int flags = 0;
-                int r = brick_work(P, C, m, v, res1[slot], &out, ch, &tm, &didx, &dd, &flags, (const uint64_t*)nullptr);
+                // no new requests here: k_leaf_alloc has run, a leaf still missing could
+                // not be allocated and the apply aborts
+                int r = brick_work(P, C, m, v, res1[slot], &out, ch, &tm, &didx, &dd, &flags, (const uint64_t*)nullptr,
+                                   false);
                 if (didx != kNoIdx) {
                     ref_delta_logged(P, c, didx, dd, flags & 8 ? ST_H3 : ST_H4);
                     atomicAdd((unsigned long long*)stat(P.ctl, flags & 8 ? ST_H3 : ST_H4), 1ull);
@@ -1646,7 +1807,7 @@ __device__ inline void leaf_cleanup_body(const Pool<T>& P) {
             uint32_t idx = P.new_leaves[i];
             if (P.refs[idx] == 0) {
                 freed = true;
-                P.leaf_map[(uint32_t)P.values[idx]] = 0;
+                leaf_remove(P, (uint32_t)P.values[idx]);
                 P.values[idx] = 0;
                 uint32_t g = P.gens[idx] + 1u;
                 P.gens[idx] = (uint16_t)(g >= kMaxGeneration ? 0u : g);
@@ -1726,7 +1887,7 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
             uint32_t tm = P.tms[idx];
             is_leaf = tm == 0;
             if (is_leaf) {
-                P.leaf_map[(uint32_t)P.values[idx]] = 0;
+                leaf_remove(P, (uint32_t)P.values[idx]);
             } else {
                 uint64_t ch[8];
                 uint64_t* src = P.children + 8ull * idx;
@@ -1859,6 +2020,23 @@ __global__ void k_tab_reinsert(Pool<T> P) {
     }
 }
 
+// The hashed leaf table of four-byte voxels, rebuilt the same way: every live leaf (a
+// node without a types/mask word that holds a reference) is inserted again.
+__global__ void k_ltab_clear(uint64_t* ltab, uint64_t nslots, Ctl* ctl) {
+    uint4* t4 = (uint4*)ltab;
+    const uint64_t n4 = nslots / 2;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x)
+        t4[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (blockIdx.x == 0 && threadIdx.x < kStatStripes) ctl->st[threadIdx.x][ST_LTOMB] = 0;
+}
+
+template <typename T>
+__global__ void k_ltab_reinsert(Pool<T> P) {
+    const uint32_t top = P.ctl->next_index;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < top; i += gridDim.x * blockDim.x)
+        if (i != 0 && P.tms[i] == 0 && P.refs[i] != 0) leaf_publish(P, (uint32_t)P.values[i], i);
+}
+
 // ------------------------------------------------------------------ single-voxel set
 
 // VoxOpsWrite::set (voxtree.rs:162-260) on many trees at once, one voxel per tree:
@@ -1933,7 +2111,7 @@ __global__ void k_set_descend(Pool<T> P, ChunkDev* chunks, uint32_t n, const uin
                     cur = kEmpty;
                 }
                 if (!unchanged && bid_is_leaf(cur) && (uint32_t)P.values[bid_index(cur)] == v) unchanged = true;
-                if (!unchanged && P.leaf_map[v] == 0) P.leaf_need[v] = 1;
+                if (!unchanged && leaf_index(P, v) == 0) leaf_request(P, v);
             } else if (!unchanged) {
                 // voxtree.rs:581-660: present children are followed down to a leaf (split
                 // from there, or the voxel itself at depth D); a missing child changes nothing
@@ -2001,7 +2179,7 @@ __global__ __launch_bounds__(TILE) void k_set_level(
             if (L.h == 1) {
                 if (v != 0) {
                     // creation failed: the apply aborts; a dry run counts a placeholder
-                    uint32_t li = P.leaf_map[v];
+                    uint32_t li = leaf_index(P, v);
                     cur = li ? make_leaf(li, P.gens[li]) : (P.dry ? make_leaf(P.capacity + v, 0) : kInvalid);
                 } else {
                     cur = kEmpty;
@@ -2087,7 +2265,7 @@ __global__ __launch_bounds__(TILE) void k_set_level(
 template <typename T>
 __global__ void k_fill_prepare(Pool<T> P, uint32_t n, const uint64_t* roots, const uint64_t* vals,
                                uint32_t* gc_front) {
-    const uint32_t vmask = sizeof(T) == 1 ? 0xFFu : 0xFFFFu;
+    const uint32_t vmask = sizeof(T) == 1 ? 0xFFu : sizeof(T) == 2 ? 0xFFFFu : 0xFFFFFFFFu;
     VX_LOOP(c, n) {
         bool push = false;
         uint64_t r = kEmpty;
@@ -2095,7 +2273,7 @@ __global__ void k_fill_prepare(Pool<T> P, uint32_t n, const uint64_t* roots, con
             r = roots[c];
             push = r != kEmpty;
             uint32_t v = (uint32_t)vals[c] & vmask;
-            if (v != 0) P.leaf_need[v] = 1;
+            if (v != 0 && leaf_index(P, v) == 0) leaf_request(P, v);
         }
         uint32_t k = wave_append(&P.ctl->gc_count[0], push);
         if (push) gc_front[k] = bid_index(r);
@@ -2104,14 +2282,14 @@ __global__ void k_fill_prepare(Pool<T> P, uint32_t n, const uint64_t* roots, con
 
 template <typename T>
 __global__ void k_fill_finish(Pool<T> P, uint32_t n, uint64_t* roots, const uint64_t* vals) {
-    const uint32_t vmask = sizeof(T) == 1 ? 0xFFu : 0xFFFFu;
+    const uint32_t vmask = sizeof(T) == 1 ? 0xFFu : sizeof(T) == 2 ? 0xFFFFu : 0xFFFFFFFFu;
     VX_LOOP(c, n) {
         if (c < n) {
             uint32_t v = (uint32_t)vals[c] & vmask;
             if (v == 0) {
                 roots[c] = kEmpty;
             } else {
-                uint32_t idx = P.leaf_map[v];
+                uint32_t idx = leaf_index(P, v);
                 if (idx != 0) {
                     atomicAdd(&P.refs[idx], 1u);
                     roots[c] = make_leaf(idx, P.gens[idx]);
@@ -2123,30 +2301,10 @@ __global__ void k_fill_finish(Pool<T> P, uint32_t n, uint64_t* roots, const uint
 
 template <typename T>
 __global__ void k_leaf_alloc_only(Pool<T> P, uint32_t nvalues) {
-    __shared__ uint32_t created;
-    if (threadIdx.x == 0) created = 0;
-    __syncthreads();
-    for (uint32_t v = threadIdx.x; v < nvalues; v += blockDim.x) {
-        if (P.leaf_need[v]) {
-            P.leaf_need[v] = 0;
-            if (P.leaf_map[v] == 0) {
-                bool ok;
-                uint32_t idx = alloc_node(P, &ok);
-                if (ok) {
-                    P.values[idx] = (T)v;
-                    P.tms[idx] = 0;
-                    P.refs[idx] = 0;
-                    P.leaf_map[v] = idx;
-                    atomicAdd(&created, 1u);
-                } else {
-                    // fill released the old roots first: there is no state to return to
-                    atomicOr(&P.ctl->err, kErrOom | kErrPoison);
-                }
-            }
-        }
-    }
-    __syncthreads();
+    const uint32_t created = create_requested_leaves(P, nvalues);
     if (threadIdx.x == 0) {
+        // fill released the old roots first: there is no state to return to
+        if (P.ctl->abort_) atomicOr(&P.ctl->err, kErrOom | kErrPoison);
         atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)created);
         atomicAdd((unsigned long long*)stat(P.ctl, ST_LEAF_MISS), (unsigned long long)created);
     }

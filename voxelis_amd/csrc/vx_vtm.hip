@@ -144,12 +144,8 @@ __global__ void k_vtm_sizes(Pool<T> P, uint64_t N, const uint32_t* newid, uint64
 
 template <typename T>
 __device__ inline void put_be(uint8_t* dst, uint32_t v) {
-    if (sizeof(T) == 1) {
-        dst[0] = (uint8_t)v;
-    } else {
-        dst[0] = (uint8_t)(v >> 8);
-        dst[1] = (uint8_t)v;
-    }
+#pragma unroll
+    for (int k = 0; k < (int)sizeof(T); k++) dst[k] = (uint8_t)(v >> (8 * ((int)sizeof(T) - 1 - k)));
 }
 
 // Leaves at 4 + off_leaf[i], branches at 8 + (leaf bytes) + off_branch[i].
@@ -196,7 +192,7 @@ __global__ void k_vtm_link(Pool<T> P, uint64_t N) {
         if (i >= 1 && i < N) {
             const uint32_t tm = P.tms[i];
             if (tm == 0) {
-                P.leaf_map[(uint32_t)P.values[i]] = (uint32_t)i;
+                leaf_publish(P, (uint32_t)P.values[i], (uint32_t)i);
             } else {
                 uint64_t ch[8];
                 for (int j = 0; j < 8; j++) {
@@ -458,16 +454,14 @@ int vx_model_serialize(vx_interner* h, size_t n, const uint64_t* roots, const in
                        size_t cap, size_t* size) {
     if (!h || !size || (n && (!roots || !positions))) return VX_E_INVALID_ARG;
     if (h->poisoned) return VX_E_POISONED;
-    return h->vsize == 1 ? serialize_impl<uint8_t>(h, n, roots, positions, out, cap, size)
-                         : serialize_impl<uint16_t>(h, n, roots, positions, out, cap, size);
+    return VX_WIDTH_DISPATCH(h, serialize_impl, h, n, roots, positions, out, cap, size);
 }
 
 int vx_model_deserialize(vx_interner* h, const void* data, size_t size, size_t ncap, uint64_t* roots,
                          int32_t* positions, size_t* n) {
     if (!h || !data || !n || (ncap && (!roots || !positions))) return VX_E_INVALID_ARG;
     if (h->poisoned) return VX_E_POISONED;
-    return h->vsize == 1 ? deserialize_impl<uint8_t>(h, (const uint8_t*)data, size, ncap, roots, positions, n)
-                         : deserialize_impl<uint16_t>(h, (const uint8_t*)data, size, ncap, roots, positions, n);
+    return VX_WIDTH_DISPATCH(h, deserialize_impl, h, (const uint8_t*)data, size, ncap, roots, positions, n);
 }
 
 }  // extern "C"
