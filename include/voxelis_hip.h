@@ -33,7 +33,8 @@ typedef enum {
     VX_E_BUDGET = -8,      /* interner/mod.rs:63-68 budget too small / too large */
     VX_E_UNSUPPORTED = -9, /* voxel width not built into this library */
     VX_E_TABLE_FULL = -10, /* pattern table exhausted (load factor too high) */
-    VX_E_POISONED = -11    /* a previous device-side failure left the interner unusable */
+    VX_E_POISONED = -11,   /* a previous device-side failure left the interner unusable */
+    VX_E_SCRATCH = -12     /* larger than the fixed scratch of a no-allocation entry point */
 } vx_status;
 
 typedef struct vx_interner vx_interner;
@@ -282,6 +283,16 @@ int vx_trees_to_vec(vx_interner* h, size_t n, const uint64_t* roots, const uint8
  * Synchronizes. */
 int vx_model_serialize(vx_interner* h, size_t n, const uint64_t* roots, const int32_t* positions, void* out,
                        size_t cap, size_t* size);
+/* vx_model_serialize without any device allocation: no counterpart in the reference;
+ * for callers that cannot allocate (a serving loop that checkpoints its world, code
+ * that must not call hipMalloc).  The same bytes as vx_model_serialize, with every
+ * scratch array (ranks, record sizes and offsets, the staged section) carved from one
+ * fixed 64 MiB device array per GPU that the library holds until the process ends;
+ * a world whose export needs more (about 70 bytes per live node and 36 per chunk, at
+ * most) returns VX_E_SCRATCH and changes nothing.  Calls on one GPU share that array:
+ * calls that overlap on one GPU (two interners on two threads) race.  Synchronizes. */
+int vx_model_serialize_noalloc(vx_interner* h, size_t n, const uint64_t* roots, const int32_t* positions,
+                               void* out, size_t cap, size_t* size);
 /* A data section rebuilt into a fresh interner (interner/mod.rs:930-993: nodes at
  * their file ids; reference counts = parent references + roots): the chunks' roots
  * (device, ncap) and positions (host, 3 * ncap); *n = the chunk count (VX_E_RANGE

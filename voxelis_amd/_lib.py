@@ -13,7 +13,7 @@ VX_OK = 0
 STATUS_NAMES = {
     0: "OK", -1: "INVALID_ARG", -2: "OOM", -3: "DEPTH", -4: "RANGE", -5: "LEAF_BRICK",
     -6: "SAME_ROOT", -7: "HIP", -8: "BUDGET", -9: "UNSUPPORTED", -10: "TABLE_FULL",
-    -11: "POISONED",
+    -11: "POISONED", -12: "SCRATCH",
 }
 STAT_NAMES = ["branch_hits", "branch_misses", "branch_dups", "leaf_misses", "collapses",
               "recycled", "h2_uncollapsed", "h3_leaks", "h4_undercount", "h5_collapse_cloned",
@@ -73,6 +73,7 @@ SIGNATURES = {
     "vx_trees_fill": (_int, [_vp, _sz, _vp, _vp]),
     "vx_trees_set": (_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp]),
     "vx_model_serialize": (_int, [_vp, _sz, _vp, _vp, _vp, _sz, C.POINTER(C.c_size_t)]),
+    "vx_model_serialize_noalloc": (_int, [_vp, _sz, _vp, _vp, _vp, _sz, C.POINTER(C.c_size_t)]),
     "vx_model_deserialize": (_int, [_vp, _vp, _sz, _sz, _vp, _vp, C.POINTER(C.c_size_t)]),
     "vx_trees_canonical_hash": (_int, [_vp, _sz, _vp, _vp]),
     "vx_trees_to_vec": (_int, [_vp, _sz, _vp, _vp, _u32, _u64, _vp]),

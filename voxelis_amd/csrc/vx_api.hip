@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
+#include <mutex>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -876,6 +877,7 @@ const char* vx_status_string(int st) {
         case VX_E_UNSUPPORTED: return "unsupported voxel width";
         case VX_E_TABLE_FULL: return "pattern table full";
         case VX_E_POISONED: return "interner poisoned by an earlier device failure";
+        case VX_E_SCRATCH: return "exceeds the fixed scratch of a no-allocation entry point";
         default: return "unknown";
     }
 }

@@ -75,18 +75,48 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_add(uint64_t* data, uint64_
         if (base + i < n) data[base + i] += add;
 }
 
+// Device scratch of one export: carved from a fixed region (the no-allocation entry
+// point) or, without a region, hipMalloc'd and freed at the end of the call.
+struct Arena {
+    uint8_t* base = nullptr;  // nullptr: allocate
+    size_t cap = 0, used = 0;
+    void* held[16] = {nullptr};
+    int nheld = 0;
+    bool take(size_t bytes, void** out) {
+        *out = nullptr;
+        if (!base) {
+            if (nheld == 16 || hipMalloc(out, bytes ? bytes : 1) != hipSuccess) return false;
+            held[nheld++] = *out;
+            return true;
+        }
+        size_t at = (used + 255) & ~(size_t)255;
+        if (at > cap || bytes > cap - at) return false;
+        *out = base + at;
+        used = at + bytes;
+        return true;
+    }
+    bool fits(size_t bytes) const {
+        const size_t at = (used + 255) & ~(size_t)255;
+        return !base || (at <= cap && bytes <= cap - at);
+    }
+    void release() {
+        for (int i = 0; i < nheld; i++) (void)hipFree(held[i]);
+        nheld = 0;
+    }
+};
+
 // In place; data holds n + 1 elements, the last one zero, so data[n] ends as the total.
-inline int scan_u64(uint64_t* data, uint64_t n1, hipStream_t s) {
+// The tile sums come from the arena.
+inline int scan_u64(uint64_t* data, uint64_t n1, hipStream_t s, Arena* a) {
     const uint64_t tiles = (n1 + kScanTile - 1) / kScanTile;
     uint64_t* sums = nullptr;
-    if (hipMallocAsync((void**)&sums, (tiles + 1) * 8, s) != hipSuccess) return VX_E_HIP;
+    if (!a->take((tiles + 1) * 8, (void**)&sums)) return a->base ? VX_E_SCRATCH : VX_E_HIP;
     hipLaunchKernelGGL(k_scan_tile, dim3((unsigned)tiles), dim3(kScanBlock), 0, s, data, n1, sums);
     int rc = VX_OK;
     if (tiles > 1) {
-        rc = scan_u64(sums, tiles, s);
+        rc = scan_u64(sums, tiles, s, a);
         if (rc == VX_OK) hipLaunchKernelGGL(k_scan_add, dim3((unsigned)tiles), dim3(kScanBlock), 0, s, data, n1, sums);
     }
-    (void)hipFreeAsync(sums, s);
     if (hipGetLastError() != hipSuccess) return VX_E_HIP;
     return rc;
 }
@@ -212,6 +242,36 @@ __global__ void k_vtm_link(Pool<T> P, uint64_t N) {
     }
 }
 
+// The chunk records (voxchunk.rs:381-440) on the device: the chunk count, then per
+// chunk the magic, its position (3 x i32 big-endian) and the varint id of its root.
+__global__ void k_vtm_tail_sizes(const uint32_t* rid, uint64_t n, uint64_t* tsz) {
+    VX_LOOP(t, n + 1) {
+        if (t <= n) tsz[t] = t < n ? 24 + varint_len(rid[t]) : 0;
+    }
+}
+
+__device__ inline void put_be32(uint8_t* d, uint32_t x) {
+    d[0] = (uint8_t)(x >> 24);
+    d[1] = (uint8_t)(x >> 16);
+    d[2] = (uint8_t)(x >> 8);
+    d[3] = (uint8_t)x;
+}
+
+__global__ void k_vtm_tail_write(const uint32_t* rid, const int32_t* pos, uint64_t n, const uint64_t* toff,
+                                 uint8_t* tail) {
+    const uint8_t magic[12] = {'V', 'o', 'x', 'T', 'r', 'e', 'e', 'C', 'h', 'u', 'n', 'k'};
+    VX_LOOP(t, n + 1) {
+        if (t == n) put_be32(tail, (uint32_t)n);
+        if (t < n) {
+            uint8_t* d = tail + 4 + toff[t];
+#pragma unroll
+            for (int k = 0; k < 12; k++) d[k] = magic[k];
+            for (int k = 0; k < 3; k++) put_be32(d + 12 + 4 * k, (uint32_t)pos[3 * t + k]);
+            put_varint(d + 24, rid[t]);
+        }
+    }
+}
+
 __global__ void k_vtm_roots(uint32_t* refs, const uint64_t* roots, uint64_t n) {
     VX_LOOP(t, n) {
         if (t < n && roots[t] != kEmpty) atomicAdd(&refs[bid_index(roots[t])], 1u);
@@ -230,10 +290,6 @@ __global__ void k_vtm_ctl(Ctl* ctl, uint32_t N, uint64_t leaves, uint64_t branch
 namespace {
 
 using namespace vx;
-
-inline void put_u32_be(std::vector<uint8_t>& v, uint32_t x) {
-    for (int s = 24; s >= 0; s -= 8) v.push_back((uint8_t)(x >> s));
-}
 
 struct Reader {
     const uint8_t* p;
@@ -279,92 +335,114 @@ struct Reader {
 
 const uint8_t kVtcMagic[12] = {'V', 'o', 'x', 'T', 'r', 'e', 'e', 'C', 'h', 'u', 'n', 'k'};
 
+// The export (VoxModel::serialize, voxmodel.rs:177-283): live nodes ranked by two scans,
+// record sizes, two more scans for their offsets, one pass writing every record; the
+// chunk records likewise.  All of it is written on the device into one buffer and copied
+// to out once; the two counts are filled in on the host.  Scratch comes from `a`.
 template <typename T>
-int serialize_impl(vx_interner* h, size_t n, const uint64_t* roots, const int32_t* positions, void* out,
-                   size_t cap, size_t* size) {
+int serialize_core(vx_interner* h, size_t n, const uint64_t* roots, const int32_t* positions, void* out,
+                   size_t cap, size_t* size, Arena* a) {
     hipStream_t s = h->stream;
     Ctl c;
     int rc = sync_read_ctl(h, &c);
     if (rc) return rc;
     const uint64_t N = c.next_index;
     Pool<T> P = pool_of<T>(h);
-    uint64_t *rl = nullptr, *rb = nullptr;
+    const int short_of = a->base ? VX_E_SCRATCH : VX_E_HIP;
+    uint64_t *rl = nullptr, *rb = nullptr, *tsz = nullptr;
     uint32_t *newid = nullptr, *rid = nullptr;
+    int32_t* dpos = nullptr;
     uint8_t* dout = nullptr;
     const size_t words = (size_t)N + 1;
-    auto cleanup = [&]() {
-        (void)hipStreamSynchronize(s);
-        (void)hipFree(rl);
-        (void)hipFree(rb);
-        (void)hipFree(newid);
-        (void)hipFree(rid);
-        (void)hipFree(dout);
-    };
-    if (hipMalloc(&rl, words * 8) != hipSuccess || hipMalloc(&rb, words * 8) != hipSuccess ||
-        hipMalloc(&newid, words * 4) != hipSuccess || hipMalloc(&rid, std::max<size_t>(n, 1) * 4) != hipSuccess) {
-        cleanup();
-        return VX_E_HIP;
-    }
+    if (!a->take(words * 8, (void**)&rl) || !a->take(words * 8, (void**)&rb) ||
+        !a->take(words * 4, (void**)&newid) || !a->take(std::max<size_t>(n, 1) * 4, (void**)&rid) ||
+        !a->take(std::max<size_t>(n, 1) * 12, (void**)&dpos) || !a->take((n + 1) * 8, (void**)&tsz))
+        return short_of;
     const unsigned g = grid_for(words);
     hipLaunchKernelGGL(k_vtm_count<T>, dim3(g), dim3(kBlock), 0, s, P, N, rl, rb);
-    if ((rc = scan_u64(rl, words, s)) || (rc = scan_u64(rb, words, s))) {
-        cleanup();
-        return rc;
-    }
+    if ((rc = scan_u64(rl, words, s, a)) || (rc = scan_u64(rb, words, s, a))) return rc;
     hipLaunchKernelGGL(k_vtm_ids<T>, dim3(g), dim3(kBlock), 0, s, P, N, rl, rb, newid);
     uint64_t L = 0, B = 0;
-    (void)hipMemcpyAsync(&L, rl + N, 8, hipMemcpyDeviceToHost, s);
-    (void)hipMemcpyAsync(&B, rb + N, 8, hipMemcpyDeviceToHost, s);
+    HIPCHK(hipMemcpyAsync(&L, rl + N, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&B, rb + N, 8, hipMemcpyDeviceToHost, s));
     // the rank arrays become the record sizes, then their offsets
     hipLaunchKernelGGL(k_vtm_sizes<T>, dim3(g), dim3(kBlock), 0, s, P, N, newid, rl, rb);
-    if ((rc = scan_u64(rl, words, s)) || (rc = scan_u64(rb, words, s))) {
-        cleanup();
-        return rc;
+    if ((rc = scan_u64(rl, words, s, a)) || (rc = scan_u64(rb, words, s, a))) return rc;
+    uint64_t LB = 0, BB = 0, TB = 0;
+    HIPCHK(hipMemcpyAsync(&LB, rl + N, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&BB, rb + N, 8, hipMemcpyDeviceToHost, s));
+    if (n) {
+        HIPCHK(hipMemcpyAsync(dpos, positions, n * 12, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_vtm_root_ids, dim3(grid_for(n)), dim3(kBlock), 0, s, roots, (uint64_t)n, newid, N, rid);
     }
-    uint64_t LB = 0, BB = 0;
-    (void)hipMemcpyAsync(&LB, rl + N, 8, hipMemcpyDeviceToHost, s);
-    (void)hipMemcpyAsync(&BB, rb + N, 8, hipMemcpyDeviceToHost, s);
-    if (n) hipLaunchKernelGGL(k_vtm_root_ids, dim3(grid_for(n)), dim3(kBlock), 0, s, roots, (uint64_t)n, newid, N, rid);
-    std::vector<uint32_t> root_ids(n);
-    if (n) (void)hipMemcpyAsync(root_ids.data(), rid, n * 4, hipMemcpyDeviceToHost, s);
-    if (hipStreamSynchronize(s) != hipSuccess || L + B >= 0xFFFFFFFFull) {
-        cleanup();
-        return hipGetLastError() != hipSuccess ? VX_E_HIP : VX_E_RANGE;
-    }
-    // the chunk records: magic, position (3 x i32 BE), varint root id
-    std::vector<uint8_t> tail;
-    put_u32_be(tail, (uint32_t)n);
-    for (size_t t = 0; t < n; t++) {
-        tail.insert(tail.end(), kVtcMagic, kVtcMagic + 12);
-        for (int k = 0; k < 3; k++) put_u32_be(tail, (uint32_t)positions[3 * t + k]);
-        uint8_t vb[10];
-        uint32_t k = put_varint(vb, root_ids[t]);
-        tail.insert(tail.end(), vb, vb + k);
-    }
-    const size_t total = 8 + LB + BB + tail.size();
+    hipLaunchKernelGGL(k_vtm_tail_sizes, dim3(grid_for(n + 1)), dim3(kBlock), 0, s, rid, (uint64_t)n, tsz);
+    if ((rc = scan_u64(tsz, n + 1, s, a))) return rc;
+    HIPCHK(hipMemcpyAsync(&TB, tsz + n, 8, hipMemcpyDeviceToHost, s));
+    if (hipStreamSynchronize(s) != hipSuccess) return VX_E_HIP;
+    if (L + B >= 0xFFFFFFFFull) return VX_E_RANGE;
+    const size_t total = 8 + LB + BB + 4 + TB;
     *size = total;
-    if (!out || cap < total) {
-        cleanup();
-        return out ? VX_E_RANGE : VX_OK;
-    }
-    if (hipMalloc(&dout, 8 + LB + BB) != hipSuccess) {
-        cleanup();
-        return VX_E_HIP;
-    }
+    if (!a->fits(total)) return VX_E_SCRATCH;  // a size query tells it as well
+    if (!out || cap < total) return out ? VX_E_RANGE : VX_OK;
+    if (!a->take(total, (void**)&dout)) return short_of;
     hipLaunchKernelGGL(k_vtm_write<T>, dim3(g), dim3(kBlock), 0, s, P, N, newid, rl, rb, dout);
+    hipLaunchKernelGGL(k_vtm_tail_write, dim3(grid_for(n + 1)), dim3(kBlock), 0, s, rid, dpos, (uint64_t)n, tsz,
+                       dout + 8 + LB + BB);
     uint8_t* o = (uint8_t*)out;
-    (void)hipMemcpyAsync(o, dout, 8 + LB + BB, hipMemcpyDeviceToHost, s);
-    rc = hipStreamSynchronize(s) == hipSuccess ? VX_OK : VX_E_HIP;
+    HIPCHK(hipMemcpyAsync(o, dout, total, hipMemcpyDeviceToHost, s));
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) return VX_E_HIP;
     // the two counts (the branch count leaves out the EMPTY branch, voxmodel.rs:237)
-    std::vector<uint8_t> hdr;
-    put_u32_be(hdr, (uint32_t)L);
-    std::memcpy(o, hdr.data(), 4);
-    hdr.clear();
-    put_u32_be(hdr, (uint32_t)B);
-    std::memcpy(o + 4 + LB, hdr.data(), 4);
-    std::memcpy(o + 8 + LB + BB, tail.data(), tail.size());
-    cleanup();
+    for (int k = 0; k < 4; k++) {
+        o[k] = (uint8_t)(L >> (24 - 8 * k));
+        o[4 + LB + k] = (uint8_t)(B >> (24 - 8 * k));
+    }
+    return VX_OK;
+}
+
+template <typename T>
+int serialize_impl(vx_interner* h, size_t n, const uint64_t* roots, const int32_t* positions, void* out,
+                   size_t cap, size_t* size) {
+    Arena a;  // hipMalloc'd scratch
+    int rc = serialize_core<T>(h, n, roots, positions, out, cap, size, &a);
+    (void)hipStreamSynchronize(h->stream);
+    a.release();
     return rc;
+}
+
+// ---- export without allocation
+
+// The scratch of vx_model_serialize_noalloc: one file-scope array per GPU, held until the
+// process ends.  Host code never names it directly: its address on each device comes
+// from hipGetSymbolAddress, once per device.
+constexpr size_t kExportScratchBytes = 64ull << 20;
+static __device__ __align__(256) uint8_t scratch[kExportScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// This device's address of `scratch` (the current device); nullptr on failure.
+void* scratch_of_current_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+template <typename T>
+int serialize_noalloc_impl(vx_interner* h, size_t n, const uint64_t* roots, const int32_t* positions, void* out,
+                           size_t cap, size_t* size) {
+    void* base = scratch_of_current_device();
+    if (!base) return VX_E_HIP;
+    Arena a;
+    a.base = (uint8_t*)base;
+    a.cap = kExportScratchBytes;
+    return serialize_core<T>(h, n, roots, positions, out, cap, size, &a);
 }
 
 template <typename T>
@@ -455,6 +533,13 @@ int vx_model_serialize(vx_interner* h, size_t n, const uint64_t* roots, const in
     if (!h || !size || (n && (!roots || !positions))) return VX_E_INVALID_ARG;
     if (h->poisoned) return VX_E_POISONED;
     return VX_WIDTH_DISPATCH(h, serialize_impl, h, n, roots, positions, out, cap, size);
+}
+
+int vx_model_serialize_noalloc(vx_interner* h, size_t n, const uint64_t* roots, const int32_t* positions,
+                               void* out, size_t cap, size_t* size) {
+    if (!h || !size || (n && (!roots || !positions))) return VX_E_INVALID_ARG;
+    if (h->poisoned) return VX_E_POISONED;
+    return VX_WIDTH_DISPATCH(h, serialize_noalloc_impl, h, n, roots, positions, out, cap, size);
 }
 
 int vx_model_deserialize(vx_interner* h, const void* data, size_t size, size_t ncap, uint64_t* roots,
