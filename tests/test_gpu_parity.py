@@ -421,3 +421,29 @@ def test_to_vec_with_lod_matches_oracle(vx, lod):
             exp = t.to_vec(it_o, lod)
             assert np.array_equal(got[i].reshape(-1).astype(np.uint64), exp), (vsize, i, lod)
         it.close()
+
+
+def test_edit_lists_many_rounds_per_sort_block(vx):
+    """Edit lists large enough that every block of the device radix sort walks several
+    rounds of its range (1.9 M edits over 64 trees), with many duplicate voxels whose
+    last write must win (the sort is stable, core/batch.rs:145-175)."""
+    api, W = vx
+    rng = np.random.default_rng(314)
+    depths = [5] * 48 + [4] * 16
+    it = api.VoxInterner(2, 512 * MB)
+    it_o = O.Interner(2, 512 * MB)
+    f = api.Forest(it, depths)
+    trees_o = [O.Tree(d) for d in depths]
+    xyzs, vals, offs = [], [], [0]
+    for d in depths:
+        k = 30000
+        xyzs.append(rng.integers(0, 1 << d, size=(k, 3)).astype(np.int32))
+        vals.append(rng.integers(0, 50, size=k).astype(np.uint16))  # 0: a clear, ignored by the apply
+        offs.append(offs[-1] + k)
+    st = f.apply_edits(offs, api.to_device(np.concatenate(xyzs)), api.to_device(np.concatenate(vals)))
+    assert (st == 1).all()
+    for t, xyz, v in zip(trees_o, xyzs, vals):
+        b = O.Batch(t.depth)
+        b.set_many(xyz, v.astype(np.uint64))
+        assert t.apply_batch(it_o, b)
+    assert_same(it, f, it_o, trees_o, full_reads=True, what="large edit lists")

@@ -750,9 +750,14 @@ int apply_edits_impl(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* d
     EditTree* dtrees = (EditTree*)h->edit_trees.ptr;
     uint64_t* dfirst = (uint64_t*)h->edit_first.ptr;
     int32_t* dpre = (int32_t*)h->edit_status.ptr;
-    HIPCHK(hipMemcpyAsync(dtrees, trees.data(), n * sizeof(EditTree), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(dfirst, first.data(), n * 8, hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));  // the host vectors above go out of scope
+    // the tree records travel through a pinned staging slot, reused only once these
+    // copies completed (no wait for earlier work on the stream)
+    if (!pinned_ensure(h, n * (sizeof(EditTree) + 8))) return VX_E_HIP;
+    std::memcpy(h->h_pinned, trees.data(), n * sizeof(EditTree));
+    std::memcpy((char*)h->h_pinned + n * sizeof(EditTree), first.data(), n * 8);
+    HIPCHK(hipMemcpyAsync(dtrees, h->h_pinned, n * sizeof(EditTree), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dfirst, (char*)h->h_pinned + n * sizeof(EditTree), n * 8, hipMemcpyHostToDevice, s));
+    pinned_release(h);
     uint64_t* k0 = (uint64_t*)h->edit_keys[0].ptr;
     uint64_t* k1 = (uint64_t*)h->edit_keys[1].ptr;
     uint32_t* o0 = (uint32_t*)h->edit_order[0].ptr;
@@ -762,11 +767,20 @@ int apply_edits_impl(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* d
     unsigned end_bit = kPathBits + tree_bits;
     if (E > 0) {
         k_edit_keys<<<grid_for(E), kBlock, 0, s>>>(E, dfirst, (uint32_t)n, dtrees, xyz, k0, o0);
-        size_t tmp_bytes = 0;
-        HIPCHK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, k0, k1, o0, o1, (size_t)E, 0, end_bit, s));
-        if (!h->edit_tmp.ensure(tmp_bytes + 16)) return VX_E_HIP;
-        HIPCHK(rocprim::radix_sort_pairs(h->edit_tmp.ptr, tmp_bytes, k0, k1, o0, o1, (size_t)E, 0, end_bit,
-                                         s));
+        // stable LSD radix sort, 8 bits per pass, ping-ponging between the two buffers
+        const unsigned G = (unsigned)std::min<uint64_t>(2048, (E + kSortBlock - 1) / kSortBlock);
+        const uint64_t per = (E + G - 1) / G;
+        if (!h->edit_tmp.ensure((size_t)256 * G * 4)) return VX_E_HIP;
+        uint32_t* hist = (uint32_t*)h->edit_tmp.ptr;
+        for (unsigned shift = 0; shift < end_bit; shift += 8) {
+            k_sort_hist<<<G, kSortBlock, 0, s>>>(k0, E, per, shift, hist);
+            k_sort_scan<<<1, kSortScanBlock, 0, s>>>(hist, 256u * G);
+            k_sort_scatter<<<G, kSortBlock, 0, s>>>(k0, o0, k1, o1, E, per, shift, hist);
+            std::swap(k0, k1);
+            std::swap(o0, o1);
+        }
+        std::swap(k0, k1);  // the sorted pairs are in (k1, o1)
+        std::swap(o0, o1);
         k_edit_scatter<T><<<grid_for(E), kBlock, 0, s>>>(E, (uint32_t)n, dtrees, k1, o1, (const T*)values, mws,
                                                          vws, false);
     }
@@ -783,10 +797,12 @@ int apply_edits_impl(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* d
         v.values = vws + trees[t].vbase;
     }
     int rc = apply_impl<T>(h, n, roots, views.data(), status, dpre, opt);
-    if (rc) return rc;
+    // the touched bricks are cleared again on every path, a failed apply included, so
+    // the workspace stays zero for the next call
     if (E > 0)
         k_edit_scatter<T><<<grid_for(E), kBlock, 0, s>>>(E, (uint32_t)n, dtrees, k1, o1, (const T*)values, mws,
                                                          vws, true);
+    if (rc) return rc;
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
 }
 

@@ -2,7 +2,8 @@
 //
 // Input: per tree a contiguous range of edits (x, y, z, value) in submission
 // order.  Edits are radix-sorted by (tree, Morton path) with the submission index
-// as payload (a stable sort, so the last write to a voxel ends each run), and the
+// as payload (the project's stable LSD sort below, so the last write to a voxel ends
+// each run), and the
 // run ends are scattered into zeroed dense Batch buffers exactly as Batch::just_set
 // records them (core/batch.rs:145-175): a non-default value sets the slot's set
 // bit, the default value sets its clear bit, the value is stored either way.
@@ -10,7 +11,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "vx_device.h"
 
@@ -95,6 +95,104 @@ __global__ void k_edit_scatter(uint64_t n, uint32_t ntrees, const EditTree* tree
         uint64_t byte = tr.mbase + 2 * p + (v != 0 ? 0 : 1);
         uint32_t* word = (uint32_t*)(masks_ws + (byte & ~3ull));
         atomicOr(word, (1u << slot) << (8 * (byte & 3)));
+    }
+}
+
+// ---- stable LSD radix sort of (u64 key, u32 value) pairs, 8 key bits per pass
+//
+// A pass over G blocks, each owning one contiguous range of the input: (1) a digit
+// histogram per block; (2) one exclusive scan of the histograms in digit-major order,
+// so that every block's keys of a digit land after those of the blocks before it;
+// (3) a scatter in which each block walks its range in order, 256 keys per round:
+// equal digits are ranked within a wavefront by ballots over the digit's bits, across
+// the block's four wavefronts and across rounds through LDS counters.  Order among equal
+// digits is the input order, so the sort is stable (core/batch.rs:145-175: the last
+// write to a voxel wins).
+constexpr unsigned kSortBlock = 256;
+constexpr unsigned kSortScanBlock = 1024;
+
+__global__ __launch_bounds__(kSortBlock) void k_sort_hist(const uint64_t* keys, uint64_t n, uint64_t per,
+                                                          unsigned shift, uint32_t* hist) {
+    __shared__ uint32_t c[256];
+    c[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t b0 = (uint64_t)blockIdx.x * per;
+    const uint64_t b1 = min(n, b0 + per);
+    for (uint64_t i = b0 + threadIdx.x; i < b1; i += kSortBlock) atomicAdd(&c[(keys[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = c[threadIdx.x];
+}
+
+// Exclusive scan of m counts in place, one block: each thread scans a contiguous segment.
+__global__ __launch_bounds__(kSortScanBlock) void k_sort_scan(uint32_t* hist, uint32_t m) {
+    __shared__ uint32_t part[kSortScanBlock];
+    const uint32_t seg = (m + kSortScanBlock - 1) / kSortScanBlock;
+    const uint32_t a = min(m, threadIdx.x * seg), b = min(m, a + seg);
+    uint32_t sum = 0;
+    for (uint32_t i = a; i < b; i++) sum += hist[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < kSortScanBlock; off <<= 1) {
+        const uint32_t t = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    for (uint32_t i = a; i < b; i++) {
+        const uint32_t v = hist[i];
+        hist[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
+                                                             uint32_t* vout, uint64_t n, uint64_t per, unsigned shift,
+                                                             const uint32_t* offs) {
+    __shared__ uint32_t base[256];             // next output position of each digit (this block)
+    __shared__ uint32_t wcnt[kSortBlock / 64][256];  // per wavefront: its digit counts, then offsets
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+    base[tid] = offs[(uint64_t)tid * gridDim.x + blockIdx.x];
+    const uint64_t b0 = (uint64_t)blockIdx.x * per;
+    const uint64_t b1 = min(n, b0 + per);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint64_t r0 = b0; r0 < b1; r0 += kSortBlock) {  // uniform over the block
+        const uint64_t i = r0 + tid;
+        const bool valid = i < b1;
+        const uint64_t key = valid ? kin[i] : 0ull;
+        const uint32_t val = valid ? vin[i] : 0u;
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+#pragma unroll
+        for (uint32_t w = 0; w < kSortBlock / 64; w++) wcnt[w][tid] = 0;
+        // lanes of this wavefront holding the same digit
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const bool on = (d >> bit) & 1u;
+            const uint64_t bal = __ballot(valid && on);
+            peers &= on ? bal : ~bal;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        __syncthreads();  // counters cleared (and the previous round's reads done)
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {  // thread = digit: offsets of the wavefronts in order, then the block's next base
+            uint32_t run = base[tid];
+#pragma unroll
+            for (uint32_t w = 0; w < kSortBlock / 64; w++) {
+                const uint32_t t = wcnt[w][tid];
+                wcnt[w][tid] = run;
+                run += t;
+            }
+            base[tid] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t dst = wcnt[wave][d] + rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
     }
 }
 
