@@ -141,6 +141,7 @@ struct vx_interner {
     uint16_t* gens = nullptr;
     uint32_t* free_stack = nullptr;
     uint64_t* tab = nullptr;
+    uint64_t* single = nullptr;  // u8/u16: the one-leaf key index (8 slots x every value)
     uint32_t* leaf_map = nullptr;
     uint64_t* leaf_ids = nullptr;  // u16: the brick pass's leaf id of every value
     uint64_t* ltab = nullptr;      // four-byte voxels: the hashed leaf table
@@ -195,6 +196,7 @@ Pool<T> pool_of(vx_interner* h) {
     P.gens = h->gens;
     P.free_stack = h->free_stack;
     P.tab = h->tab;
+    P.single = h->single;
     P.tmask = h->tslots - 1;
     P.leaf_map = h->leaf_map;
     P.leaf_ids = h->leaf_ids;
@@ -927,6 +929,7 @@ int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_f
               dmalloc(&h->tab, ts * 8) && dmalloc(&h->leaf_map, nv * 4) &&
               dmalloc(&h->leaf_need, nv) && dmalloc(&h->new_leaves, nnew * 4) &&
               (direct || (dmalloc(&h->ltab, h->lslots * 8) && dmalloc(&h->need_list, h->lslots * 4))) &&
+              (!direct || dmalloc(&h->single, 8 * nv * 8)) &&
               (value_size != 2 || dmalloc(&h->leaf_ids, nv * 8)) &&
               dmalloc(&h->ctl, sizeof(Ctl));
     ok = ok && hipHostMalloc((void**)&h->h_ctl, sizeof(Ctl), hipHostMallocDefault) == hipSuccess &&
@@ -944,7 +947,8 @@ int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_f
              hipMemset(h->refs, 0, cap * 4) == hipSuccess && hipMemset(h->gens, 0, cap * 2) == hipSuccess &&
              hipMemset(h->tab, 0, ts * 8) == hipSuccess && hipMemset(h->leaf_map, 0, nv * 4) == hipSuccess &&
              hipMemset(h->leaf_need, 0, nv) == hipSuccess &&
-             (direct || hipMemset(h->ltab, 0, h->lslots * 8) == hipSuccess);
+             (direct || hipMemset(h->ltab, 0, h->lslots * 8) == hipSuccess) &&
+             (!direct || hipMemset(h->single, 0, 8 * nv * 8) == hipSuccess);
     Ctl c;
     std::memset(&c, 0, sizeof(c));
     c.next_index = 1;
@@ -964,7 +968,7 @@ int vx_interner_destroy(vx_interner* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* ptrs[] = {h->children, h->tms, h->values, h->refs, h->gens, h->free_stack,
                     h->tab, h->leaf_map, h->leaf_need, h->new_leaves, h->ctl, h->leaf_ids,
-                    h->ltab, h->need_list};
+                    h->ltab, h->need_list, h->single};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->chunks.release();

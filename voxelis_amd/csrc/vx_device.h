@@ -186,6 +186,7 @@ struct Pool {
     uint32_t* free_stack;
     uint64_t* tab;
     uint64_t tmask;
+    uint64_t* single;  // u8/u16: the direct-mapped index of one-leaf keys (nullptr: none)
     uint32_t* leaf_map;  // u8/u16: value -> node index (0 = none)
     uint64_t* ltab;      // i32: open-addressing leaf table (value << 32 | index)
     uint64_t lmask;      // its slot count - 1
@@ -207,9 +208,21 @@ struct Pool {
     uint32_t undo_cap;
 };
 
-// An owner's table slot with bit 63 set when the slot held a tombstone before.
+// An owner's table slot with bit 63 set when the slot held a tombstone before, or bit 62
+// set when the key lives in the one-leaf index instead (the entry's index).
 constexpr uint64_t kTslotTomb = 1ull << 63;
-__host__ __device__ inline uint64_t tslot_index(uint64_t ts) { return ts & ~kTslotTomb; }
+constexpr uint64_t kTslotSingle = 1ull << 62;
+__host__ __device__ inline uint64_t tslot_index(uint64_t ts) { return ts & ~(kTslotTomb | kTslotSingle); }
+
+// One-leaf keys: a branch whose only child is a leaf (mask and types one and the same
+// bit) -- a brick holding one voxel, the most common key of sparse content.
+__host__ __device__ inline bool tm_is_single(uint32_t tm) {
+    const uint32_t m = tm & 0xFFu, t = (tm >> 8) & 0xFFu;
+    return m != 0 && (m & (m - 1u)) == 0 && t == m;
+}
+// Index entries: 0 none, the node's BlockId (bit 63 clear), or a claim of the current
+// launch (bit 63 set, level << 32 | candidate).
+constexpr uint64_t kSinglePend = 1ull << 63;
 
 
 }  // namespace vx

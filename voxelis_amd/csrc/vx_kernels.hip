@@ -320,28 +320,28 @@ __device__ inline uint64_t leaf_id_of(const Pool<T>& P, uint32_t v) {
 }
 
 template <typename T>
-__device__ inline uint32_t alloc_at(const Pool<T>& P, uint32_t k, bool* ok);
+__device__ inline uint32_t alloc_at(const Pool<T>& P, uint32_t k, bool* ok, uint32_t ft0, uint32_t next0);
 
 template <typename T>
 __device__ inline uint32_t alloc_node(const Pool<T>& P, bool* ok) {
-    return alloc_at(P, atomicAdd(&P.ctl->alloc_count, 1u), ok);
+    return alloc_at(P, atomicAdd(&P.ctl->alloc_count, 1u), ok, P.ctl->free_top0, P.ctl->next0);
 }
 
 // The index of the launch sequence's k-th allocation: the free stack first, then
-// fresh indices.
+// fresh indices.  ft0 / next0: the allocation window's snapshot (free_top0, next0), which
+// no kernel of the sequence changes before k_end_alloc.
 template <typename T>
-__device__ inline uint32_t alloc_at(const Pool<T>& P, uint32_t k, bool* ok) {
+__device__ inline uint32_t alloc_at(const Pool<T>& P, uint32_t k, bool* ok, uint32_t ft0, uint32_t next0) {
     if (k >= P.alloc_limit) {
         atomicOr(&P.ctl->abort_, 2u);  // beyond the caller's reservation
         *ok = false;
         return 0;
     }
-    uint32_t ft0 = P.ctl->free_top0;
     if (k < ft0) {
         *ok = true;
         return P.free_stack[ft0 - 1 - k];
     }
-    uint64_t b = (uint64_t)P.ctl->next0 + (uint64_t)(k - ft0);
+    uint64_t b = (uint64_t)next0 + (uint64_t)(k - ft0);
     if (b >= P.capacity) {
         atomicOr(&P.ctl->abort_, 1u);  // interner capacity (interner/macros.rs:37-38)
         *ok = false;
@@ -1245,6 +1245,83 @@ __device__ inline int probe_table(const Pool<T>& P, uint32_t level, uint64_t k, 
     return -1;
 }
 
+// u8/u16 interners keep one-leaf keys in a direct-mapped index, single[(slot << 8s) |
+// value of the leaf], instead of the hashed table: the pair is the whole key (a live
+// leaf is the one leaf of its value), so a lookup is one read of a small, cached array.
+// A dry run's placeholder leaf (index capacity + value) carries its value in its index.
+template <typename T>
+__device__ inline uint64_t single_index(const Pool<T>& P, uint32_t tm, const uint64_t* ch) {
+    const uint32_t slot = (uint32_t)__ffs(tm & 0xFFu) - 1u;
+    const uint32_t li = bid_index(ch[slot]);
+    const uint32_t v = li >= P.capacity ? li - P.capacity : (uint32_t)P.values[li];
+    return ((uint64_t)slot << (8 * sizeof(T))) | v;
+}
+
+// get_or_create_branch for a one-leaf key: a committed entry is a hit, a claim of this
+// launch at the same level a duplicate (at an earlier level -- a dry run keeps those --
+// its owner's id is final), an empty entry is claimed with atomicCAS.
+template <typename T>
+__device__ inline int probe_single(const Pool<T>& P, uint32_t level, uint64_t k, const uint64_t key[9],
+                                   uint64_t* result, uint64_t* tslot, uint32_t* probes, uint32_t* claims) {
+    const uint64_t ix = single_index(P, (uint32_t)key[8], key);
+    uint64_t e = P.single[ix];
+    (*probes)++;
+    if (e == 0) {
+        uint64_t seen = 0;
+        const uint64_t want = kSinglePend | ((uint64_t)level << 32) | (uint32_t)k;
+        const bool won = __hip_atomic_compare_exchange_strong(&P.single[ix], &seen, want, __ATOMIC_RELAXED,
+                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *claims += won ? 1u : 0x10000u;
+        if (won) {
+            *tslot = kTslotSingle | ix;
+            return 2;
+        }
+        e = seen;
+    }
+    if (e & kSinglePend) {
+        const uint32_t lv = (uint32_t)(e >> 32) & 7u, c = (uint32_t)e;
+        *result = lv == level ? res_pending(c) : P.levels[lv].id[c];
+        return lv == level ? 1 : 0;
+    }
+    *result = e;
+    return 0;
+}
+
+// The probe's statistics: each wavefront adds its sums to the block's LDS counters as
+// it ends; the last one to end adds them to the global counters.
+constexpr uint32_t kProbeStats = 7;
+
+template <typename T>
+__device__ inline void probe_stats_flush(const Pool<T>& P, uint32_t h, uint32_t* s_stat, uint32_t* s_done,
+                                         uint32_t probes, uint32_t claims, uint32_t lost, uint32_t tomb,
+                                         uint32_t hit, uint32_t dup, uint32_t miss) {
+    const uint32_t v[kProbeStats] = {wave_sum(probes), wave_sum(claims), wave_sum(lost), wave_sum(tomb),
+                                     wave_sum(hit), wave_sum(dup), wave_sum(miss)};
+    uint32_t last = 0;
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (uint32_t i = 0; i < kProbeStats; i++)
+            if (v[i]) atomicAdd(&s_stat[i], v[i]);
+        __threadfence_block();
+        last = atomicAdd(s_done, 1u) == blockDim.x / 64 - 1 ? 1u : 0u;
+    }
+    last = __shfl(last, 0);
+    if (!last) return;
+    __threadfence_block();
+    const uint32_t lane = __lane_id();
+    if (lane < kProbeStats) {
+        const uint32_t x = atomicAdd(&s_stat[lane], 0u);  // read through the LDS atomic unit
+        const int ids[kProbeStats] = {ST_PROBES, ST_CLAIMS, ST_CLAIMS_LOST, ST_TOMB, ST_HIT, ST_DUP, ST_MISS};
+        const int ids1[kProbeStats] = {-1, -1, -1, -1, ST_HIT1, ST_DUP1, ST_MISS1};
+        if (x) {
+            // a claimed tombstone is subtracted
+            const unsigned long long add = lane == 3 ? (unsigned long long)(-(long long)x) : (unsigned long long)x;
+            atomicAdd((unsigned long long*)stat(P.ctl, ids[lane]), add);
+            if (h == 1 && ids1[lane] >= 0) atomicAdd((unsigned long long*)stat(P.ctl, ids1[lane]), add);
+        }
+    }
+}
+
 // Probe of the candidates of tiles [t0, t0+tn) of one level (tn <= 64), packed
 // densely over the block's lanes: sparse tiles fill whole wavefronts, and equal keys
 // meet in the same pass more often.  Per pass over TILE of them: merge equal keys of
@@ -1269,7 +1346,17 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
     __shared__ uint32_t s_cand[TILE];      // ... its candidate index
     __shared__ uint64_t s_res[TILE];       // ... a leader's result
     __shared__ int32_t s_kind[TILE];       // ... and kind
+    // statistics of the block, flushed once by the last wavefront to end (a global atomic
+    // per wavefront and pass would be waited for with every later load: stores and
+    // atomics share the load counter on this part)
+    __shared__ uint32_t s_stat[kProbeStats];
+    __shared__ uint32_t s_done;
     for (uint32_t i = threadIdx.x; i < kSlots; i += TILE) s_set[i] = 0;
+    if (threadIdx.x < kProbeStats) s_stat[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_done = 0;
+    uint32_t n_probes = 0, n_claims = 0, n_lost = 0, n_tomb = 0, n_hit = 0, n_dup = 0, n_miss = 0;
+    // the allocation window of the sequence (constant during this launch)
+    const uint32_t ft0 = P.ctl->free_top0, next0 = P.ctl->next0, own_base = P.ctl->own_base;
     if (threadIdx.x < 64) {
         uint32_t x = threadIdx.x < tn ? tile_cnt[t0 + threadIdx.x] : 0u;
 #pragma unroll
@@ -1290,7 +1377,10 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
     const uint32_t me = threadIdx.x;
     // A wavefront with no candidate in any pass ends here (whole wavefronts only: a
     // barrier waits for the wavefronts of the block that have not ended).
-    if (wbase >= N) return;
+    if (wbase >= N) {
+        probe_stats_flush(P, L.h, s_stat, &s_done, 0, 0, 0, 0, 0, 0, 0);
+        return;
+    }
     for (uint32_t j0 = 0; j0 < N; j0 += TILE) {  // uniform over the remaining wavefronts
         const bool wlive = j0 + wbase < N;  // the wavefront has candidates
         const uint32_t j = j0 + threadIdx.x;
@@ -1347,14 +1437,25 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
         }
         const bool lead = valid && leader == me;
         uint32_t probes = 0, claims = 0;
-        if (lead) kind = probe_table(P, L.h, k, key, h, &result, &tslot, &probes, &claims);
+        if (lead)
+            kind = P.single && tm_is_single((uint32_t)key[8])
+                       ? probe_single(P, L.h, k, key, &result, &tslot, &probes, &claims)
+                       : probe_table(P, L.h, k, key, h, &result, &tslot, &probes, &claims);
+        // An owner's LOD value (core/voxel.rs:96-141) from its children's values, which
+        // earlier launches wrote (a node gets its value at its own level): its loads are
+        // issued now and complete while the block allocates.
+        uint32_t lod = 0;
+        if (kind == 2 && !P.dry) {
+            uint32_t vals[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) vals[i] = (uint32_t)P.values[bid_index(key[i])];
+            lod = calc_average(vals);
+        }
         // a claimed tombstone is a tombstone less (a dry or aborted apply restores it)
-        const uint32_t tclaim = __popcll(__ballot(kind == 2 && (tslot & kTslotTomb)));
-        if (lane == 0 && tclaim)
-            atomicAdd((unsigned long long*)stat(P.ctl, ST_TOMB), (unsigned long long)(-(long long)tclaim));
-        wave_add64(stat(P.ctl, ST_PROBES), probes);
-        wave_add64(stat(P.ctl, ST_CLAIMS), (claims & 0xFFFFu) + (claims >> 16));
-        wave_add64(stat(P.ctl, ST_CLAIMS_LOST), claims >> 16);
+        n_tomb += kind == 2 && (tslot & kTslotTomb) ? 1u : 0u;
+        n_probes += probes;
+        n_claims += (claims & 0xFFFFu) + (claims >> 16);
+        n_lost += claims >> 16;
         // owners' allocation ranks: wavefront offsets in LDS, one counter update per block
         const uint64_t ob = __ballot(kind == 2);
         uint32_t woff = 0;
@@ -1376,17 +1477,10 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
                 result = make_branch(P.capacity + 65536u + ka, 0, (uint32_t)key[8]);
             } else {
                 bool ok;
-                uint32_t idx = alloc_at(P, ka, &ok);
+                uint32_t idx = alloc_at(P, ka, &ok, ft0, next0);
                 result = ok ? make_branch(idx, P.gens[idx], (uint32_t)key[8]) : kInvalid;
-                if (ok) {
-                    // the LOD value (core/voxel.rs:96-141) from the children's, which
-                    // earlier launches wrote: a node gets its value at its own level
-                    uint32_t vals[8];
-#pragma unroll
-                    for (int i = 0; i < 8; i++) vals[i] = (uint32_t)P.values[bid_index(key_word(K, k, i))];
-                    P.values[idx] = (T)calc_average(vals);
-                }
-                P.owners[ka - P.ctl->own_base] = ((uint64_t)L.h << 32) | k;  // the commit's list
+                if (ok) P.values[idx] = (T)lod;
+                P.owners[ka - own_base] = ((uint64_t)L.h << 32) | k;  // the commit's list
             }
             if (P.per_tree && tree != kNoIdx) atomicAdd(&P.per_tree[tree], 1u);
         }
@@ -1405,15 +1499,11 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
             res[slot] = result;
         }
         if (mine != kNoIdx) s_set[mine] = 0;  // every lane's probing ended before the barrier above
-        wave_count(stat(P.ctl, ST_HIT), valid && kind == 0);
-        wave_count(stat(P.ctl, ST_DUP), valid && kind == 1);
-        wave_count(stat(P.ctl, ST_MISS), valid && kind == 2);
-        if (L.h == 1) {  // uniform
-            wave_count(stat(P.ctl, ST_HIT1), valid && kind == 0);
-            wave_count(stat(P.ctl, ST_DUP1), valid && kind == 1);
-            wave_count(stat(P.ctl, ST_MISS1), valid && kind == 2);
-        }
+        n_hit += valid && kind == 0 ? 1u : 0u;
+        n_dup += valid && kind == 1 ? 1u : 0u;
+        n_miss += valid && kind == 2 ? 1u : 0u;
     }
+    probe_stats_flush(P, L.h, s_stat, &s_done, n_probes, n_claims, n_lost, n_tomb, n_hit, n_dup, n_miss);
 }
 
 template <typename T, int TILE>
@@ -1465,7 +1555,8 @@ __global__ __launch_bounds__(TILE) void k_commit(Pool<T> P) {
 #pragma unroll
         for (int j = 0; j < 8; j++) dst[j] = key[j];
         P.tms[idx] = (uint16_t)key[8];
-        P.tab[tslot_index(ts)] = slot_make(slot_tag(K.hash[k]), false, 0, idx);
+        if (ts & kTslotSingle) P.single[tslot_index(ts)] = id;
+        else P.tab[tslot_index(ts)] = slot_make(slot_tag(K.hash[k]), false, 0, idx);
         for (int j = 0; j < 8; j++) {
             if (key[j] == kEmpty) continue;
             uint32_t ci = bid_index(key[j]);
@@ -1707,7 +1798,8 @@ __device__ inline void rollback_body(const Pool<T>& P, const ChunkDev* chunks, c
         uint64_t k = lt * TILE + threadIdx.x;
         uint64_t ts = K.tslot[k];
         if (ts == ~0ull) continue;
-        P.tab[tslot_index(ts)] = (ts & kTslotTomb) ? kSlotTomb : kSlotEmpty;
+        if (ts & kTslotSingle) P.single[tslot_index(ts)] = 0;
+        else P.tab[tslot_index(ts)] = (ts & kTslotTomb) ? kSlotTomb : kSlotEmpty;
         wave_count(stat(P.ctl, ST_TOMB), (ts & kTslotTomb) != 0);
         uint64_t id = K.id[k];
         if (P.dry || id == kInvalid) continue;
@@ -1838,11 +1930,11 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
     __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
     __shared__ uint32_t s_next[kGcStage];
     __shared__ uint32_t s_nnext, s_nfree, s_bnext, s_bfree;
-    __shared__ unsigned long long s_leaves, s_branches;
+    __shared__ unsigned long long s_leaves, s_branches, s_tombs, s_recycled;
     agg_init(akey, acnt);
     if (threadIdx.x == 0) {
         s_nnext = s_nfree = 0;
-        s_leaves = s_branches = 0;
+        s_leaves = s_branches = s_tombs = s_recycled = 0;
     }
     __syncthreads();
     // new leaves nothing references (their trees failed) are released in the first
@@ -1882,7 +1974,7 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
             if (v != 0 && v < P.capacity) idx = v;
         }
         bool freed = decrement_self ? wave_ref_dec_zero(P.refs, idx) : idx != kNoIdx;
-        bool is_leaf = false;
+        bool is_leaf = false, tomb = false;
         if (freed) {
             uint32_t tm = P.tms[idx];
             is_leaf = tm == 0;
@@ -1893,18 +1985,25 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
                 uint64_t* src = P.children + 8ull * idx;
 #pragma unroll
                 for (int j = 0; j < 8; j++) ch[j] = src[j];
-                // remove the pattern (mod.rs:518)
-                uint64_t h = branch_hash(ch, tm);
-                uint64_t want = slot_make(slot_tag(h), false, 0, idx);
-                uint64_t s = h & P.tmask;
-                for (uint64_t guard = 0; guard <= P.tmask; guard++) {
-                    uint64_t e = P.tab[s];
-                    if (e == want) {
-                        P.tab[s] = kSlotTomb;
-                        break;
+                // remove the pattern (mod.rs:518): a one-leaf key from its index (the
+                // leaf is still live: this node's reference goes below), any other from
+                // the table, leaving a tombstone
+                if (P.single && tm_is_single(tm)) {
+                    P.single[single_index(P, tm, ch)] = 0;
+                } else {
+                    uint64_t h = branch_hash(ch, tm);
+                    uint64_t want = slot_make(slot_tag(h), false, 0, idx);
+                    uint64_t s = h & P.tmask;
+                    for (uint64_t guard = 0; guard <= P.tmask; guard++) {
+                        uint64_t e = P.tab[s];
+                        if (e == want) {
+                            P.tab[s] = kSlotTomb;
+                            tomb = true;
+                            break;
+                        }
+                        if (e == kSlotEmpty) break;
+                        s = (s + 1) & P.tmask;
                     }
-                    if (e == kSlotEmpty) break;
-                    s = (s + 1) & P.tmask;
                 }
 #pragma unroll
                 for (int j = 0; j < 8; j++) src[j] = kEmpty;
@@ -1934,15 +2033,16 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
             P.gens[idx] = (uint16_t)(g >= kMaxGeneration ? 0u : g);
         }
         const uint64_t fb = __ballot(freed);
-        const uint64_t bl = __ballot(freed && is_leaf), bb = __ballot(freed && !is_leaf);
+        const uint64_t bl = __ballot(freed && is_leaf), bb = __ballot(freed && !is_leaf), bt = __ballot(tomb);
         uint32_t woff = 0;
         if (lane == 0) {
             if (fb) woff = atomicAdd(&s_nfree, (uint32_t)__popcll(fb));
             if (bl) atomicAdd(&s_leaves, (unsigned long long)__popcll(bl));
             if (bb) atomicAdd(&s_branches, (unsigned long long)__popcll(bb));
+            if (bt) atomicAdd(&s_tombs, (unsigned long long)__popcll(bt));
+            if (fb) atomicAdd(&s_recycled, (unsigned long long)__popcll(fb));
         }
         woff = __shfl(woff, 0);
-        wave_count(stat(P.ctl, ST_RECYCLED), freed);
         __syncthreads();
         if (threadIdx.x == 0) {
             if (s_nfree) s_bfree = atomicAdd(&P.ctl->free_top, s_nfree);
@@ -1975,7 +2075,10 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
         if (s_leaves) atomicAdd((unsigned long long*)&P.ctl->live_leaves, (unsigned long long)(-(long long)s_leaves));
         if (s_branches) {
             atomicAdd((unsigned long long*)&P.ctl->live_branches, (unsigned long long)(-(long long)s_branches));
-            atomicAdd((unsigned long long*)stat(P.ctl, ST_TOMB), (unsigned long long)s_branches);
+            if (s_tombs) atomicAdd((unsigned long long*)stat(P.ctl, ST_TOMB), (unsigned long long)s_tombs);
+        }
+        if (s_recycled) {
+            atomicAdd((unsigned long long*)stat(P.ctl, ST_RECYCLED), s_recycled);
         }
     }
     __syncthreads();
@@ -2010,6 +2113,7 @@ __global__ void k_tab_reinsert(Pool<T> P) {
         const uint64_t* src = P.children + 8ull * i;
 #pragma unroll
         for (int j = 0; j < 8; j++) ch[j] = src[j];
+        if (P.single && tm_is_single(tm)) continue;  // in the one-leaf index, which has no tombstones
         const uint64_t h = branch_hash(ch, tm);
         const uint64_t want = slot_make(slot_tag(h), false, 0, i);
         uint64_t s = h & P.tmask;

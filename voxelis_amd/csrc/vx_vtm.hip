@@ -229,6 +229,10 @@ __global__ void k_vtm_link(Pool<T> P, uint64_t N) {
                     ch[j] = P.children[8ull * i + j];
                     if (ch[j] != kEmpty) atomicAdd(&P.refs[bid_index(ch[j])], 1u);
                 }
+                if (P.single && tm_is_single(tm)) {  // one-leaf keys live in their index
+                    P.single[single_index(P, tm, ch)] = make_branch((uint32_t)i, 0, tm);
+                    continue;
+                }
                 const uint64_t h = branch_hash(ch, tm);
                 uint64_t s = h & P.tmask;
                 for (uint64_t guard = 0; guard <= P.tmask; guard++) {
