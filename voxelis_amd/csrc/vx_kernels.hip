@@ -772,13 +772,13 @@ template <typename T>
 __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m, const uint32_t v[8],
                                  uint64_t base, uint64_t* out, uint64_t ch[8], uint32_t* tm,
                                  uint32_t* drift_idx, int32_t* drift_delta, int* flags,
-                                 const uint64_t* leaf_ids, bool request = true) {
+                                 const uint64_t lids[8], bool request = true) {
     *drift_idx = kNoIdx;
     bool all_same = m == 0xFFu;
 #pragma unroll
     for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
     if (all_same) {
-        uint64_t lid = leaf_lookup(P, leaf_ids, v[0]);
+        uint64_t lid = lids[0];
         if (lid == kEmpty) {
             const uint32_t key = request ? leaf_request(P, v[0]) : 0u;
             if (!P.dry) return 3;
@@ -825,7 +825,7 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
         if (!(m & (1u << i))) continue;
         // a live leaf is the one leaf of its value: the child holds v[i] exactly when it
         // is that leaf (the children of a brick are leaves)
-        uint64_t lid = leaf_lookup(P, leaf_ids, v[i]);
+        uint64_t lid = lids[i];
         if (ch[i] != kEmpty) {
             bool same = split ? split_value == v[i] : (lid != kEmpty && ch[i] == lid);
             if (same) continue;
@@ -994,25 +994,36 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8
             const uint32_t sh = 3 * (D - 1);
             c = G.chunk_base + (uint32_t)(local >> sh);
             const uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
+            // one round of independent loads: the tree's record, the leaf ids of the values
+            // and the node one hop below the recorded height-2 node (used unless the tree
+            // has a fill)
             const ChunkDev& C = chunks[c];
+            const bool c_fill = C.has_fill != 0;
+            const uint64_t c_root = C.root_in;
+            uint64_t lids[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) lids[i] = (mm >> i) & 1u ? leaf_lookup(P, leaf_ids, v[i]) : kEmpty;
+            const bool up_branch = D >= 2 && upq != kEmpty && !bid_is_leaf(upq);
+            const bool up_ok = up_branch && bid_index(upq) < P.capacity;
+            const uint64_t below = P.children[8ull * (up_ok ? bid_index(upq) : 0u) + (p & 7u)];
             bool all_same = mm == 0xFFu;
 #pragma unroll
             for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
             uint64_t base = kEmpty;
             bool h1 = false;
             if (!all_same) {
-                if (C.has_fill) {
+                if (c_fill) {
                     base = kFillMark;
                 } else {
                     // the equivalent node at brick level: a leaf above it splits;
                     // a leaf at brick level is the reference's panic
-                    uint64_t cur = D >= 2 ? upq : C.root_in;
-                    if (D >= 2 && cur != kEmpty && !bid_is_leaf(cur)) {
-                        if (bid_index(cur) >= P.capacity) {
+                    uint64_t cur = D >= 2 ? upq : c_root;
+                    if (up_branch) {
+                        if (!up_ok) {
                             atomicOr(&P.ctl->err, kErrPoison);
                             cur = kEmpty;
                         } else {
-                            cur = P.children[8ull * bid_index(cur) + (p & 7u)];
+                            cur = below;
                         }
                         // BlockId::types() on a leaf brick panics (block_id.rs:273-276)
                         h1 = cur != kEmpty && bid_is_leaf(cur);
@@ -1028,7 +1039,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8
             } else {
                 uint32_t didx;
                 int32_t dd;
-                int rc = brick_work(P, C, mm, v, base, &out, ch, &tm, &didx, &dd, &flags, leaf_ids);
+                int rc = brick_work(P, C, mm, v, base, &out, ch, &tm, &didx, &dd, &flags, lids);
                 if (didx != kNoIdx && !P.dry) {
                     uint32_t pos = atomicAdd(&P.ctl->n_drift, 1u);
                     drift[pos] = Drift{c, didx, dd, (uint32_t)(flags & 8 ? ST_H3 : ST_H4)};
@@ -1114,8 +1125,11 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
                 int flags = 0;
                 // no new requests here: k_leaf_alloc has run, a leaf still missing could
                 // not be allocated and the apply aborts
-                int r = brick_work(P, C, m, v, res1[slot], &out, ch, &tm, &didx, &dd, &flags, (const uint64_t*)nullptr,
-                                   false);
+                uint64_t lids[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    lids[i] = (m >> i) & 1u ? leaf_lookup(P, (const uint64_t*)nullptr, v[i]) : kEmpty;
+                int r = brick_work(P, C, m, v, res1[slot], &out, ch, &tm, &didx, &dd, &flags, lids, false);
                 if (didx != kNoIdx) {
                     ref_delta_logged(P, c, didx, dd, flags & 8 ? ST_H3 : ST_H4);
                     atomicAdd((unsigned long long*)stat(P.ctl, flags & 8 ? ST_H3 : ST_H4), 1ull);
@@ -1976,18 +1990,36 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
         bool freed = decrement_self ? wave_ref_dec_zero(P.refs, idx) : idx != kNoIdx;
         bool is_leaf = false, tomb = false;
         if (freed) {
-            uint32_t tm = P.tms[idx];
+            // every column of the node in one round of loads (a leaf's children are empty)
+            const uint32_t tm = P.tms[idx];
+            const uint32_t g = P.gens[idx] + 1u;
+            uint64_t ch[8];
+            uint64_t* src = P.children + 8ull * idx;
+#pragma unroll
+            for (int j = 0; j < 8; j++) ch[j] = src[j];
+            const uint32_t lval = (uint32_t)P.values[idx];
             is_leaf = tm == 0;
             if (is_leaf) {
-                leaf_remove(P, (uint32_t)P.values[idx]);
+                leaf_remove(P, lval);
             } else {
-                uint64_t ch[8];
-                uint64_t* src = P.children + 8ull * idx;
+                // children first: combined in LDS where the block's table has room, the
+                // rest decremented with 8 independent atomics (straight-line, so they are
+                // in flight together while the pattern is removed below; a slot with
+                // nothing to decrement adds 0 to this node's own count: it is being
+                // recycled by this lane and nothing else touches it)
+                uint32_t gmask = 0;
 #pragma unroll
-                for (int j = 0; j < 8; j++) ch[j] = src[j];
+                for (int j = 0; j < 8; j++)
+                    if (ch[j] != kEmpty && !agg_add(akey, acnt, bid_index(ch[j]), 1u)) gmask |= 1u << j;
+                uint32_t old[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const bool gj = (gmask >> j) & 1u;
+                    old[j] = atomicSub(&P.refs[gj ? bid_index(ch[j]) : idx], gj ? 1u : 0u);
+                }
                 // remove the pattern (mod.rs:518): a one-leaf key from its index (the
-                // leaf is still live: this node's reference goes below), any other from
-                // the table, leaving a tombstone
+                // leaf is still live: this node's reference is still being dropped), any
+                // other from the table, leaving a tombstone
                 if (P.single && tm_is_single(tm)) {
                     P.single[single_index(P, tm, ch)] = 0;
                 } else {
@@ -2008,28 +2040,12 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
 #pragma unroll
                 for (int j = 0; j < 8; j++) src[j] = kEmpty;
                 P.tms[idx] = 0;
-                // children: combined in LDS where the block's table has room, the rest
-                // decremented with 8 independent atomics (issued together, then read)
-                uint32_t gmask = 0;
-#pragma unroll
-                for (int j = 0; j < 8; j++)
-                    if (ch[j] != kEmpty && !agg_add(akey, acnt, bid_index(ch[j]), 1u)) gmask |= 1u << j;
-                // Straight-line, so the 8 atomics are in flight together: a slot with
-                // nothing to decrement adds 0 to this node's own count (it is being
-                // recycled by this lane; nothing else touches it).
-                uint32_t old[8];
-#pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const bool g = (gmask >> j) & 1u;
-                    old[j] = atomicSub(&P.refs[g ? bid_index(ch[j]) : idx], g ? 1u : 0u);
-                }
 #pragma unroll
                 for (int j = 0; j < 8; j++)
                     if (((gmask >> j) & 1u) && old[j] == 1u) s_next[atomicAdd(&s_nnext, 1u)] = bid_index(ch[j]);
             }
             P.values[idx] = 0;
             P.refs[idx] = 0;
-            uint32_t g = P.gens[idx] + 1u;
             P.gens[idx] = (uint16_t)(g >= kMaxGeneration ? 0u : g);
         }
         const uint64_t fb = __ballot(freed);
