@@ -85,6 +85,8 @@ def parse():
                          "--calib-copy) into per-kernel HBM bytes, scaled by the copy kernel's readings")
     ap.add_argument("--trace-summary", metavar="TRACE_DIR",
                     help="summarize a rocprofv3 --kernel-trace --stats run (csv output) per kernel")
+    ap.add_argument("--skip-applies", type=int, default=1,
+                    help="summary modes: applies left out at the start (the first builds from empty trees)")
     ap.add_argument("--out", default=None, help="summary modes: also write the JSON there")
     a = ap.parse_args()
     if a.budget_mib is None:
@@ -677,17 +679,27 @@ def _csv_rows(d, suffix):
     return rows
 
 
+def _steady_from(rows, skip):
+    """Dispatch id where the apply after the first `skip` applies begins (applies are
+    delimited by their k_prepare dispatches), or 0 when there are not that many."""
+    starts = sorted({int(r["Dispatch_Id"]) for r in rows if _kernel_short(r["Kernel_Name"]) == "k_prepare"})
+    return starts[skip] if len(starts) > skip else 0
+
+
 def pmc_summary(a):
     """Per-kernel HBM bytes per dispatch from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE),
-    each scaled by what the counter reads for the 1 GiB copy kernel of the same run."""
+    each scaled by what the counter reads for the 1 GiB copy kernel of the same run.  The
+    first --skip-applies applies (the build from empty trees) are left out."""
     res = {}
     for counter, d in (("FETCH_SIZE", a.pmc_summary[0]), ("WRITE_SIZE", a.pmc_summary[1])):
+        rows = [r for r in _csv_rows(d, "counter_collection.csv") if r.get("Counter_Name") == counter]
+        first = _steady_from(rows, a.skip_applies)
         per = {}
-        for r in _csv_rows(d, "counter_collection.csv"):
-            if r.get("Counter_Name") != counter:
-                continue
+        for r in rows:
             k = _kernel_short(r["Kernel_Name"])
-            disp = r.get("Dispatch_Id") or r.get("Correlation_Id")
+            disp = int(r["Dispatch_Id"])
+            if k != "k_copy_bench" and disp < first:
+                continue
             e = per.setdefault(k, {})
             e[disp] = e.get(disp, 0.0) + float(r["Counter_Value"])
         res[counter] = per
@@ -696,8 +708,10 @@ def pmc_summary(a):
     write_copy = sum(res["WRITE_SIZE"].get("k_copy_bench", {}).values())
     fs = copy_bytes / 1024 / fetch_copy if fetch_copy else 1.0
     ws = copy_bytes / 1024 / write_copy if write_copy else 1.0
+    applies = max(len(res["FETCH_SIZE"].get("k_prepare", {})), 1)
     out = {"copy_kernel_bytes": copy_bytes, "fetch_kib_for_copy": fetch_copy, "write_kib_for_copy": write_copy,
-           "fetch_scale": fs, "write_scale": ws, "kernels": {}}
+           "fetch_scale": fs, "write_scale": ws, "applies": applies, "skipped_applies": a.skip_applies,
+           "kernels": {}}
     for k in sorted(set(res["FETCH_SIZE"]) | set(res["WRITE_SIZE"])):
         if k == "k_copy_bench":
             continue
@@ -713,15 +727,23 @@ def pmc_summary(a):
 
 
 def trace_summary(a):
-    """Per-kernel dispatches and durations from a rocprofv3 --kernel-trace run."""
+    """Per-kernel dispatches and durations from a rocprofv3 --kernel-trace run, after the
+    first --skip-applies applies."""
+    rows = _csv_rows(a.trace_summary, "kernel_trace.csv")
+    first = _steady_from(rows, a.skip_applies)
     per = {}
-    for r in _csv_rows(a.trace_summary, "kernel_trace.csv"):
+    for r in rows:
+        if int(r["Dispatch_Id"]) < first:
+            continue
         k = _kernel_short(r["Kernel_Name"])
         dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3  # us
         per.setdefault(k, []).append(dur)
-    out = {k: {"dispatches": len(v), "total_us": round(sum(v), 1), "avg_us": round(sum(v) / len(v), 2),
+    applies = max(len(per.get("k_prepare", [])), 1)
+    out = {k: {"dispatches": len(v), "per_apply": round(len(v) / applies, 2), "total_us": round(sum(v), 1),
+               "us_per_apply": round(sum(v) / applies, 1), "avg_us": round(sum(v) / len(v), 2),
                "median_us": round(float(np.median(v)), 2), "max_us": round(max(v), 2)}
            for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))}
+    out = {"applies": applies, "skipped_applies": a.skip_applies, "kernels": out}
     s = json.dumps(out, indent=1)
     print(s)
     if a.out:

@@ -239,6 +239,18 @@ int vx_trees_get_on(vx_interner* h, void* hip_stream, size_t n_trees, const uint
                     const uint8_t* depths, size_t nq, const uint32_t* chunk_of_query,
                     const int32_t* xyz, void* out_values, uint8_t* out_present, uint8_t* out_stop);
 
+/* Batched VoxOpsRead::get with the queries grouped by tree, as a caller reading many
+ * voxels of each chunk has them: tree t's queries are [query_offsets[t],
+ * query_offsets[t+1]) (n_trees + 1 non-decreasing device offsets, the last one nq),
+ * voxel xyz[3q..3q+3).  For every run of queries of one tree the top three levels of
+ * the tree are walked once into LDS and each query continues from there
+ * (utils/common.rs:122-156).  Results, out_present, out_stop and VX_E_RANGE (a voxel
+ * outside its tree, a query outside the offsets) as vx_trees_get_on.  hip_stream NULL =
+ * the interner's stream.  Asynchronous. */
+int vx_trees_get_grouped(vx_interner* h, void* hip_stream, size_t n_trees, const uint64_t* roots,
+                         const uint8_t* depths, const uint64_t* query_offsets, size_t nq, const int32_t* xyz,
+                         void* out_values, uint8_t* out_present, uint8_t* out_stop);
+
 /* ---- single-voxel write: VoxOpsWrite::set (voxtree.rs:162-260) ---- */
 
 /* set(xyz[i], values[i]) on tree i for i in [0, n), all trees in one launch

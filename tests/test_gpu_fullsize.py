@@ -30,6 +30,9 @@ def _gpu_reads(f, depths, n_reads, seed0):
     chunk = np.concatenate([np.full(n_reads, i, dtype=np.uint32) for i in range(len(depths))])
     xyz = np.concatenate([OP.read_coords(d, n_reads, seed0 + i) for i, d in enumerate(depths)])
     vals, pres = f.get(chunk, xyz)
+    # the grouped form (top levels staged in LDS) reads the same
+    gv, gp = f.get_grouped(np.arange(len(depths) + 1, dtype=np.uint64) * n_reads, xyz)
+    assert np.array_equal(gp, pres) and np.array_equal(gv, vals)
     return np.where(pres == 1, vals, 0).astype(np.uint64).reshape(len(depths), n_reads)
 
 

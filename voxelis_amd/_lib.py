@@ -70,6 +70,7 @@ SIGNATURES = {
     "vx_trees_apply_edits_ex": (_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(ApplyOpts)]),
     "vx_trees_get": (_int, [_vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
     "vx_trees_get_on": (_int, [_vp, _vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "vx_trees_get_grouped": (_int, [_vp, _vp, _sz, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
     "vx_trees_fill": (_int, [_vp, _sz, _vp, _vp]),
     "vx_trees_set": (_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp]),
     "vx_model_serialize": (_int, [_vp, _sz, _vp, _vp, _vp, _sz, C.POINTER(C.c_size_t)]),

@@ -438,6 +438,33 @@ class Forest:
             sp = C.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
             check(lib().vx_trees_get_on(self.it.h, sp, *args, stop.data_ptr() if stop is not None else None), "get")
 
+    def get_grouped(self, offsets, xyz):
+        """get of queries grouped by tree: tree t's voxels are xyz[offsets[t]:offsets[t+1]]
+        (offsets: n+1 non-decreasing ints).  Returns (values u64, present u8) like get."""
+        dev = self.it.device
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        nq = int(len(xyz))
+        off_t = torch.from_numpy(off.view(np.int64)).to(dev)
+        xyz_t = torch.from_numpy(np.ascontiguousarray(xyz, dtype=np.int32).reshape(-1)).to(dev)
+        out = torch.zeros(max(nq, 1) * self.it.value_size, dtype=torch.uint8, device=dev)
+        pres = torch.zeros(max(nq, 1), dtype=torch.uint8, device=dev)
+        self.get_grouped_device(off_t, xyz_t, out, pres, nq=nq)
+        torch.cuda.current_stream().synchronize()
+        rc = lib().vx_interner_sync(self.it.h)
+        p = pres.cpu().numpy()[:nq]
+        if rc != 0 and not (rc == -4 and (p == 0xFF).any()):
+            check(rc, "get_grouped")
+        return out.cpu().numpy().view(self.it.np_dtype)[:nq].astype(np.uint64), p
+
+    def get_grouped_device(self, off_t, xyz_t, out, pres, nq=None, stream=None, stop=None):
+        """Device form of get_grouped (off_t int64 [n+1], xyz_t int32 [3 nq])."""
+        nq = xyz_t.numel() // 3 if nq is None else nq
+        sp = C.c_void_p(stream.cuda_stream) if stream is not None else None
+        check(lib().vx_trees_get_grouped(self.it.h, sp, self.n, self.roots.data_ptr(), self.depths.data_ptr(),
+                                         off_t.data_ptr(), nq, xyz_t.data_ptr(), out.data_ptr(),
+                                         pres.data_ptr() if pres is not None else None,
+                                         stop.data_ptr() if stop is not None else None), "get_grouped")
+
     def fill(self, values):
         vals = torch.from_numpy(np.asarray(values, dtype=np.uint64).view(np.int64)).to(self.it.device)
         check(lib().vx_trees_fill(self.it.h, self.n, self.roots.data_ptr(), vals.data_ptr()), "fill")

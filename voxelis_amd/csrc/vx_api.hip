@@ -31,6 +31,7 @@ using namespace vx;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kGetTile = 256;  // lanes of a grouped-get block
 constexpr uint64_t kMaxGrid = 8192;
 
 // Tiles per probe block: candidates of sparse tiles are packed into full
@@ -94,13 +95,13 @@ enum KernelId : uint32_t {
     KID_BEGIN = 0, KID_PREPARE, KID_BRICK_SCAN, KID_LEAF_ALLOC, KID_BRICK_BUILD, KID_PROBE_BRICKS,
     KID_UNUSED, KID_LEVEL_BUILD, KID_PROBE_UPPER, KID_COMMIT, KID_FINALIZE,
     KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_ROLLBACK, KID_TO_VEC,
-    KID_SET_DESCEND, KID_SET_LEVEL, KID_COUNT
+    KID_SET_DESCEND, KID_SET_LEVEL, KID_GET_GROUPED, KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_begin", "k_prepare", "k_brick_retry", "k_leaf_alloc", "k_brick_build", "k_probe[bricks]",
     "", "k_level_build", "k_probe[upper]", "k_commit", "k_finish",
     "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill", "k_rollback", "k_to_vec",
-    "k_set_descend", "k_set_level"};
+    "k_set_descend", "k_set_level", "k_get_grouped"};
 
 struct Prof {
     bool on = false;
@@ -855,6 +856,26 @@ int get_on_impl(vx_interner* h, hipStream_t st, size_t nq, size_t ntrees, const 
 }
 
 template <typename T>
+int get_grouped_impl(vx_interner* h, hipStream_t st, size_t ntrees, const uint64_t* roots, const uint8_t* depths,
+                     const uint64_t* qoff, size_t nq, const int32_t* xyz, void* out, uint8_t* present,
+                     uint8_t* stop) {
+    Pool<T> P = pool_of<T>(h);
+    // about 4096 blocks for large batches; at least 512 queries per block
+    uint64_t per = std::min<uint64_t>(4096, std::max<uint64_t>(512, (nq + 4095) / 4096));
+    per = (per + kGetTile - 1) / kGetTile * kGetTile;
+    const uint64_t blocks = (nq + per - 1) / per;
+    if (blocks > 0x7FFFFFFFull) return VX_E_INVALID_ARG;
+    if (st) {
+        k_get_grouped<T, kGetTile><<<(unsigned)blocks, kGetTile, 0, st>>>(P, nq, (uint32_t)ntrees, roots, depths,
+                                                                          qoff, xyz, (T*)out, present, stop, per);
+    } else {
+        VX_LAUNCH(h, KID_GET_GROUPED, k_get_grouped<T VX_COMMA kGetTile>, (unsigned)blocks, kGetTile, P, nq, (uint32_t)ntrees,
+                  roots, depths, qoff, xyz, (T*)out, present, stop, per);
+    }
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
+template <typename T>
 int to_vec_impl(vx_interner* h, size_t n, const uint64_t* roots, const uint8_t* depths, uint32_t lod,
                 uint64_t stride, void* out) {
     Pool<T> P = pool_of<T>(h);
@@ -1210,6 +1231,16 @@ int vx_trees_get_on(vx_interner* h, void* stream, size_t ntrees, const uint64_t*
     if (nq == 0) return VX_OK;
     hipStream_t st = (hipStream_t)stream;
     return VX_WIDTH_DISPATCH(h, get_on_impl, h, st, nq, ntrees, roots, depths, chunk_of, xyz, out, present, stop);
+}
+
+int vx_trees_get_grouped(vx_interner* h, void* stream, size_t ntrees, const uint64_t* roots,
+                         const uint8_t* depths, const uint64_t* query_offsets, size_t nq, const int32_t* xyz,
+                         void* out, uint8_t* present, uint8_t* stop) {
+    if (!h || (nq && (!roots || !depths || !query_offsets || !xyz || !out))) return VX_E_INVALID_ARG;
+    if (ntrees > 0xFFFFFFFFull) return VX_E_INVALID_ARG;
+    if (nq == 0) return VX_OK;
+    return VX_WIDTH_DISPATCH(h, get_grouped_impl, h, (hipStream_t)stream, ntrees, roots, depths, query_offsets,
+                             nq, xyz, out, present, stop);
 }
 
 int vx_trees_fill(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* values) {

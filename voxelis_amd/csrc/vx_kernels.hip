@@ -630,6 +630,11 @@ __device__ inline uint32_t create_requested_leaves(const Pool<T>& P, uint32_t nv
         bool ok;
         uint32_t idx = alloc_node(P, &ok);
         if (ok) {
+            // a recycled node keeps its old children (the free does not clear them): a
+            // leaf has none
+            uint64_t* ch = P.children + 8ull * idx;
+#pragma unroll
+            for (int j = 0; j < 8; j++) ch[j] = kEmpty;
             P.values[idx] = (T)v;
             P.tms[idx] = 0;
             P.refs[idx] = 0;
@@ -1990,33 +1995,32 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
         bool freed = decrement_self ? wave_ref_dec_zero(P.refs, idx) : idx != kNoIdx;
         bool is_leaf = false, tomb = false;
         if (freed) {
-            // every column of the node in one round of loads (a leaf's children are empty)
+            // the node's columns in one round of loads (a leaf's children are empty).  A
+            // free writes only what a later reader needs: the generation, the types/mask
+            // word (0 marks a free or leaf slot for the table rebuild), the pattern's
+            // table slot and the free stack.  The count is already 0, the value and the
+            // children are rewritten when the slot is allocated again.
             const uint32_t tm = P.tms[idx];
             const uint32_t g = P.gens[idx] + 1u;
             uint64_t ch[8];
-            uint64_t* src = P.children + 8ull * idx;
+            const uint64_t* src = P.children + 8ull * idx;
 #pragma unroll
             for (int j = 0; j < 8; j++) ch[j] = src[j];
-            const uint32_t lval = (uint32_t)P.values[idx];
             is_leaf = tm == 0;
             if (is_leaf) {
-                leaf_remove(P, lval);
+                leaf_remove(P, (uint32_t)P.values[idx]);
             } else {
                 // children first: combined in LDS where the block's table has room, the
-                // rest decremented with 8 independent atomics (straight-line, so they are
-                // in flight together while the pattern is removed below; a slot with
-                // nothing to decrement adds 0 to this node's own count: it is being
-                // recycled by this lane and nothing else touches it)
+                // rest decremented with independent atomics (issued under the lane mask,
+                // so they are in flight together while the pattern is removed below)
                 uint32_t gmask = 0;
 #pragma unroll
                 for (int j = 0; j < 8; j++)
                     if (ch[j] != kEmpty && !agg_add(akey, acnt, bid_index(ch[j]), 1u)) gmask |= 1u << j;
                 uint32_t old[8];
 #pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const bool gj = (gmask >> j) & 1u;
-                    old[j] = atomicSub(&P.refs[gj ? bid_index(ch[j]) : idx], gj ? 1u : 0u);
-                }
+                for (int j = 0; j < 8; j++)
+                    old[j] = ((gmask >> j) & 1u) ? atomicSub(&P.refs[bid_index(ch[j])], 1u) : 0u;
                 // remove the pattern (mod.rs:518): a one-leaf key from its index (the
                 // leaf is still live: this node's reference is still being dropped), any
                 // other from the table, leaving a tombstone
@@ -2037,15 +2041,11 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
                         s = (s + 1) & P.tmask;
                     }
                 }
-#pragma unroll
-                for (int j = 0; j < 8; j++) src[j] = kEmpty;
                 P.tms[idx] = 0;
 #pragma unroll
                 for (int j = 0; j < 8; j++)
                     if (((gmask >> j) & 1u) && old[j] == 1u) s_next[atomicAdd(&s_nnext, 1u)] = bid_index(ch[j]);
             }
-            P.values[idx] = 0;
-            P.refs[idx] = 0;
             P.gens[idx] = (uint16_t)(g >= kMaxGeneration ? 0u : g);
         }
         const uint64_t fb = __ballot(freed);
@@ -2432,6 +2432,42 @@ __global__ void k_leaf_alloc_only(Pool<T> P, uint32_t nvalues) {
 
 // ------------------------------------------------------------------ reads
 
+// The rest of get_at_depth (utils/common.rs:122-156) from `node` at depth d of a tree
+// of depth D: child digits from bit D-d-1 of x, y, z down to a leaf or an empty child.
+template <typename T>
+__device__ inline void get_walk(const Pool<T>& P, uint64_t node, uint32_t d, uint32_t D, uint32_t x, uint32_t y,
+                                uint32_t z, uint32_t* v_out, uint8_t* pr_out, uint8_t* stop_out) {
+    uint32_t v = 0;
+    uint8_t pr = 0;
+    bool at_leaf = false;
+    while (node != kEmpty) {
+        const uint32_t idx = bid_index(node);
+        if (idx >= P.capacity) {
+            atomicOr(&P.ctl->err, kErrPoison);
+            break;
+        }
+        if (d >= D) {
+            v = (uint32_t)P.values[idx];
+            pr = v != 0;
+            break;
+        }
+        if (!bid_is_leaf(node)) {
+            const uint32_t sh = D - d - 1;
+            const uint32_t ci = ((x >> sh) & 1u) | (((y >> sh) & 1u) << 1) | (((z >> sh) & 1u) << 2);
+            node = P.children[8ull * idx + ci];
+            d++;
+        } else {
+            v = (uint32_t)P.values[idx];
+            pr = 1;
+            at_leaf = true;
+            break;
+        }
+    }
+    *v_out = pr ? v : 0u;
+    *pr_out = pr;
+    *stop_out = (uint8_t)(d | (at_leaf ? 0x80u : 0u));
+}
+
 // VoxOpsRead::get (voxtree.rs:145-159) -> get_at_depth (utils/common.rs:122-156),
 // one lane per query.
 template <typename T>
@@ -2448,43 +2484,115 @@ __global__ void k_get(Pool<T> P, uint64_t nq, uint64_t ntrees, const uint64_t* r
             int32_t x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
             int32_t lim = 1 << D;
             uint32_t v = 0;
-            uint8_t pr = 0;
-            uint32_t d = 0;
-            bool at_leaf = false;
+            uint8_t pr = 0, sd = 0;
             if (!tree_ok || x < 0 || y < 0 || z < 0 || x >= lim || y >= lim || z >= lim) {
                 atomicOr(&P.ctl->err, kErrRange);
                 pr = 0xFF;
             } else {
-                while (node != kEmpty) {
-                    uint32_t idx = bid_index(node);
-                    if (idx >= P.capacity) {
-                        atomicOr(&P.ctl->err, kErrPoison);
-                        break;
-                    }
-                    if (d >= D) {
-                        v = (uint32_t)P.values[idx];
-                        pr = v != 0;
-                        break;
-                    }
-                    if (!bid_is_leaf(node)) {
-                        uint32_t sh = D - d - 1;
-                        uint32_t ci = (((uint32_t)x >> sh) & 1u) | ((((uint32_t)y >> sh) & 1u) << 1) |
-                                      ((((uint32_t)z >> sh) & 1u) << 2);
-                        node = P.children[8ull * idx + ci];
-                        d++;
-                    } else {
-                        v = (uint32_t)P.values[idx];
-                        pr = 1;
-                        at_leaf = true;
-                        break;
-                    }
-                }
-                if (!pr) v = 0;
+                get_walk(P, node, 0, D, (uint32_t)x, (uint32_t)y, (uint32_t)z, &v, &pr, &sd);
             }
             out[i] = (T)v;
             if (present) present[i] = pr;
-            if (stop) stop[i] = (uint8_t)(d | (at_leaf ? 0x80u : 0u));
+            if (stop) stop[i] = sd;
         }
+    }
+}
+
+// get for queries grouped by tree: tree t's queries are [qoff[t], qoff[t+1]).  A block
+// takes `per` consecutive queries.  For each tree its range meets, the block first walks
+// every path of the tree's top k levels (k = 3 for a segment of at least 256 queries,
+// 2 from 32, else none) and keeps the node each path reaches in LDS; a query then takes
+// the node of its first k child digits from there and walks the rest in HBM
+// (utils/common.rs:122-156): a depth-5 read makes three dependent loads instead of six.
+// A query outside its tree, or beyond qoff[ntrees], is VX_E_RANGE (present 0xFF).
+constexpr uint32_t kGetTop = 3;
+template <typename T, int TILE>
+__global__ __launch_bounds__(TILE) void k_get_grouped(Pool<T> P, uint64_t nq, uint32_t ntrees,
+                                                      const uint64_t* roots, const uint8_t* depths,
+                                                      const uint64_t* qoff, const int32_t* xyz, T* out,
+                                                      uint8_t* present, uint8_t* stop, uint64_t per) {
+    __shared__ uint64_t s_node[1u << (3 * kGetTop)];
+    __shared__ uint8_t s_dep[1u << (3 * kGetTop)];
+    __shared__ uint32_t s_lo;
+    const uint64_t q1 = min(nq, (uint64_t)(blockIdx.x + 1) * per);
+    uint64_t q0 = (uint64_t)blockIdx.x * per;
+    if (q0 >= q1) return;
+    // the tree holding q0, the last t with qoff[t] <= q0: a TILE-way search per round
+    {
+        uint32_t lo = 0, span = ntrees;
+        while (span > 1) {  // uniform
+            const uint32_t step = (span + TILE - 1) / TILE;
+            const uint32_t t = lo + threadIdx.x * step;
+            const int below = t < lo + span && t < ntrees && qoff[t] <= q0;
+            const uint32_t cnt = (uint32_t)__syncthreads_count(below);
+            lo += (cnt ? cnt - 1 : 0) * step;
+            span = min(step, ntrees - lo);
+        }
+        if (threadIdx.x == 0) s_lo = lo;
+        __syncthreads();
+    }
+    uint32_t t = s_lo;
+    while (q0 < q1) {  // uniform: one segment (tree) per pass
+        while (t + 1 < ntrees && qoff[t + 1] <= q0) t++;  // skip trees with no query here
+        const bool t_ok = t < ntrees;
+        const uint64_t a = t_ok ? qoff[t] : 0ull, b = t_ok ? qoff[t + 1] : 0ull;
+        const bool seg_ok = t_ok && a <= q0 && b > q0;
+        const uint64_t e = seg_ok ? min(q1, b) : q1;
+        const uint64_t root = seg_ok ? roots[t] : kEmpty;
+        const uint32_t D = seg_ok ? depths[t] : 0xFFu;
+        const bool tree_ok = seg_ok && D <= (uint32_t)kMaxDepth;
+        const uint64_t n_seg = e - q0;
+        const uint32_t k = !tree_ok ? 0u : min(D, n_seg >= 256 ? kGetTop : n_seg >= 32 ? 2u : 0u);
+        if (k) {
+            const uint32_t np = 1u << (3 * k);
+            for (uint32_t p = threadIdx.x; p < np; p += TILE) {
+                uint64_t node = root;
+                uint32_t d = 0;
+                for (uint32_t l = 0; l < k; l++) {
+                    if (node == kEmpty || bid_is_leaf(node)) break;
+                    const uint32_t idx = bid_index(node);
+                    if (idx >= P.capacity) {
+                        atomicOr(&P.ctl->err, kErrPoison);
+                        node = kEmpty;
+                        break;
+                    }
+                    node = P.children[8ull * idx + ((p >> (3 * (k - 1 - l))) & 7u)];
+                    d++;
+                }
+                s_node[p] = node;
+                s_dep[p] = (uint8_t)d;
+            }
+            __syncthreads();
+        }
+        for (uint64_t q = q0 + threadIdx.x; q < e; q += TILE) {
+            const int32_t x = xyz[3 * q], y = xyz[3 * q + 1], z = xyz[3 * q + 2];
+            const int32_t lim = tree_ok ? 1 << D : 0;
+            uint32_t v = 0;
+            uint8_t pr = 0xFF, sd = 0;
+            if (!tree_ok || x < 0 || y < 0 || z < 0 || x >= lim || y >= lim || z >= lim) {
+                atomicOr(&P.ctl->err, kErrRange);
+            } else {
+                uint64_t node = root;
+                uint32_t d = 0;
+                if (k) {
+                    uint32_t pth = 0;
+                    for (uint32_t l = 0; l < k; l++) {
+                        const uint32_t sh = D - l - 1;
+                        pth = (pth << 3) | (((uint32_t)x >> sh) & 1u) | ((((uint32_t)y >> sh) & 1u) << 1) |
+                              ((((uint32_t)z >> sh) & 1u) << 2);
+                    }
+                    node = s_node[pth];
+                    d = s_dep[pth];
+                }
+                get_walk(P, node, d, D, (uint32_t)x, (uint32_t)y, (uint32_t)z, &v, &pr, &sd);
+            }
+            out[q] = (T)v;
+            if (present) present[q] = pr;
+            if (stop) stop[q] = sd;
+        }
+        __syncthreads();  // the staged paths are read before the next segment restages them
+        q0 = e;
+        t++;
     }
 }
 
