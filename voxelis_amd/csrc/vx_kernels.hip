@@ -698,11 +698,16 @@ __device__ inline void write_candidate(const Cands& K, uint64_t k, const uint64_
 
 // The candidate was written by an earlier launch (every launch starts with coherent
 // caches) or by this block before a barrier.
+// Every word is loaded before any is compared (no early exit), so the nine loads are in
+// flight together: one memory round trip instead of up to nine dependent ones.
 __device__ inline bool key_matches_cand(const Cands& K, uint32_t k2, const uint64_t key[9]) {
+    uint64_t w[9];
 #pragma unroll
-    for (int i = 0; i < 9; i++)
-        if (key_word(K, k2, i) != key[i]) return false;
-    return true;
+    for (int i = 0; i < 9; i++) w[i] = key_word(K, k2, i);
+    bool eq = true;
+#pragma unroll
+    for (int i = 0; i < 9; i++) eq &= w[i] == key[i];
+    return eq;
 }
 
 template <int TILE>
@@ -887,11 +892,11 @@ constexpr int brick_rounds() { return TILE >= 512 ? 2 : 4; }
 // rounds of TILE: the old-tree descent and brick_work.  Round r of phase 2 emits
 // into candidate tile blk*R + r; a deferred brick keeps that tile (its round has a
 // free place for it) for the retry.  The pass is bound by memory latency, so it runs
-// as many waves per SIMD as fit without spilling: 8 for one slot per lane; the
-// four-slot form's phase-1 loads and phase-2 keys take more registers (6 waves for u8,
-// 5 for u16).
+// as many waves per SIMD as fit without spilling: 6 for one slot per lane; the
+// four-slot form's phase-1 loads and phase-2 keys take more registers (5 waves for u8,
+// 4 for u16, 3 for 4-byte values).
 template <typename T, int TILE, int R>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 8 : (sizeof(T) == 1 ? 5 : sizeof(T) == 2 ? 4 : 3), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 6 : (sizeof(T) == 1 ? 5 : sizeof(T) == 2 ? 4 : 3), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
                                                        uint64_t* deferred, Drift* drift,
                                                        const uint64_t* old2) {
@@ -1174,12 +1179,16 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
 
 template <typename T>
 __device__ inline bool key_matches_node(const Pool<T>& P, uint32_t idx, const uint64_t key[9]) {
-    if ((uint32_t)P.tms[idx] != (uint32_t)key[8]) return false;
-    const uint64_t* c = P.children + 8ull * idx;
+    // the node's 64-byte children row and its types/mask word in one round of loads
+    const uint32_t tm = P.tms[idx];
+    const ulonglong2* c = (const ulonglong2*)(P.children + 8ull * idx);
+    ulonglong2 w[4];
 #pragma unroll
-    for (int i = 0; i < 8; i++)
-        if (c[i] != key[i]) return false;
-    return true;
+    for (int i = 0; i < 4; i++) w[i] = c[i];
+    bool eq = tm == (uint32_t)key[8];
+#pragma unroll
+    for (int i = 0; i < 4; i++) eq &= w[i].x == key[2 * i] && w[i].y == key[2 * i + 1];
+    return eq;
 }
 
 
@@ -1525,8 +1534,10 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
     probe_stats_flush(P, L.h, s_stat, &s_done, n_probes, n_claims, n_lost, n_tomb, n_hit, n_dup, n_miss);
 }
 
+// The probe kernels hold a key, its hash and the scan state per lane: 5 waves per SIMD
+// is the most that fits in registers without spilling to scratch.
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_probe(Pool<T> P, Cands K, const uint32_t* tile_cnt,
                                                  uint64_t* res, const ChunkDev* chunks, LevelInfo L,
                                                  uint32_t ntiles, uint32_t group) {
     const uint32_t t0 = blockIdx.x * group;
@@ -1536,7 +1547,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(7, 8))) vo
 // The same pass for the brick level, under its own name so that kernel traces tell the
 // two apart.
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_probe_bricks(
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_probe_bricks(
     Pool<T> P, Cands K, const uint32_t* tile_cnt, uint64_t* res, const ChunkDev* chunks, LevelInfo L,
     uint32_t ntiles, uint32_t group) {
     const uint32_t t0 = blockIdx.x * group;
@@ -1780,9 +1791,10 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
         atomicAdd((unsigned long long*)stat(P.ctl, ids[threadIdx.x]), (unsigned long long)s_st[threadIdx.x]);
 }
 
-// Height h >= 2 (its probe is the next launch).
+// Height h >= 2 (its probe is the next launch).  6 waves per SIMD: the most that fit
+// without spilling.
 template <typename T, int TILE>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_level_build(
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_level_build(
     Pool<T> P, const ChunkDev* chunks, LevelInfo L,
     const uint64_t* res_prev, const uint64_t* cand_id_prev, uint64_t* res, Cands K,
     uint32_t* tile_cnt, const uint64_t* old_h) {
