@@ -53,7 +53,11 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--budget-mib", type=int, default=None,
                     help="interner budget (default 16384 for uneven, 2048 for terrain)")
-    ap.add_argument("--load-factor", type=float, default=0.5)
+    ap.add_argument("--load-factor", type=float, default=None,
+                    help="pattern table load factor (default: the autotuner's pick, else 0.5)")
+    ap.add_argument("--tuning", default="pick", choices=["pick", "default"],
+                    help="pick: the autotuner's committed pick for the workload (voxelis_amd/tuning.json); "
+                         "default: the library's built-in launch shape")
     ap.add_argument("--brick-rounds", type=int, default=None, choices=[0, 1, 2, 4],
                     help="brick slots per lane of the brick pass (default: the library's)")
     ap.add_argument("--plain-trees", type=int, default=96)
@@ -89,6 +93,14 @@ def parse():
                     help="summary modes: applies left out at the start (the first builds from empty trees)")
     ap.add_argument("--out", default=None, help="summary modes: also write the JSON there")
     a = ap.parse_args()
+    a.pick = None
+    wl_key = {"uneven": "uneven_4096_u16", "terrain": "terrain_4096x32_u8"}.get(a.config)
+    if a.tuning == "pick" and wl_key and not (a.config == "uneven" and (a.trees != 4096 or a.seed != 1)) \
+            and not (a.config == "terrain" and (a.grid != 64 or a.seed != 1)):
+        from voxelis_amd.autotune import committed_pick
+        a.pick = committed_pick(wl_key)
+    if a.load_factor is None:
+        a.load_factor = float(a.pick["load_factor"]) if a.pick else 0.5
     if a.budget_mib is None:
         big = a.config == "uneven" or (a.config == "tune" and a.tune_workload == "uneven")
         a.budget_mib = 16384 if big else 2048
@@ -309,6 +321,9 @@ def _forest(api, it, wl, idx):
 def _interner(api, a, wl):
     it = api.VoxInterner(wl["vsize"], a.budget_mib << 20, a.load_factor)
     it.set_graphs(bool(a.graphs))
+    if a.pick:
+        from voxelis_amd.autotune import apply_pick
+        apply_pick(it, a.pick)
     if a.brick_rounds is not None:
         it.set_tuning(brick_rounds=a.brick_rounds)
     return it
@@ -596,7 +611,9 @@ def bench_headline(a, wl_name):
                    "interner_budget_mib": a.budget_mib, "load_factor": a.load_factor, "seed": a.seed,
                    "parallelism": "single GPU",
                    "launch": "HIP-graph replay of the captured launch sequence" if a.graphs
-                   else "kernels launched one by one"},
+                   else "kernels launched one by one",
+                   "tuning": dict(a.pick, source="voxelis_amd/tuning.json (autotuner pick)") if a.pick
+                   else "library defaults"},
         "p50_ms": float(np.percentile(lat, 50)) if lat is not None else None,
         "p99_ms": float(np.percentile(lat, 99)) if lat is not None else None,
         "latency_launches": a.lat_steps if lat is not None else 0,
@@ -848,6 +865,7 @@ def bench_tune(a):
 
     def make_case(lf):
         it = api.VoxInterner(vsize, a.budget_mib << 20, lf)
+        it.set_graphs(bool(a.graphs))
         f = api.Forest(it, depths)
         views = [f.make_views(mk, va), f.make_views(mk, vb)]
         return it, (lambda k: f.apply_views(views[k % 2])), it.close
@@ -857,9 +875,11 @@ def bench_tune(a):
               flush=True)
 
     best, points = autotune(make_case, steps=a.steps, warmup=a.warmup, log=log)
-    print(json.dumps({"metric": "autotuned apply launch shape", "workload": a.tune_workload,
-                      "edits_per_step": E, "best": best,
-                      "best_edits_per_s": E / best["s_per_step"], "points": points}))
+    out = {"metric": "autotuned apply launch shape", "workload": a.tune_workload, "edits_per_step": E,
+           "graphs": bool(a.graphs), "best": best, "best_edits_per_s": E / best["s_per_step"], "points": points}
+    print(json.dumps(out))
+    if a.out:
+        open(a.out, "w").write(json.dumps(out, indent=1) + "\n")
 
 
 def bench_gets(a):

@@ -83,3 +83,21 @@ def test_terrain_heights_deterministic_and_in_range():
     assert np.array_equal(h1, h2)
     assert h1.min() >= 8 and h1.max() <= 23
     assert len(np.unique(h1)) > 4
+
+
+def test_committed_tuning_picks_are_valid_launch_shapes():
+    """The autotuner's committed picks (voxelis_amd/tuning.json, loaded by bench.py) hold
+    values vx_interner_set_tuning accepts, and name the sweep that chose them."""
+    import os
+    from voxelis_amd.autotune import TUNING_FIELDS, committed_pick
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for name in ("uneven_4096_u16", "terrain_4096x32_u8"):
+        p = committed_pick(name)
+        assert p is not None, name
+        assert set(TUNING_FIELDS) <= set(p)
+        assert 0.0 < p["load_factor"] < 1.0
+        assert p["tile"] in (128, 256, 512)
+        assert 0 <= p["dedup_rounds"] <= 16 and 0 <= p["brick_merge_rounds"] <= 16
+        assert p["brick_rounds"] in (0, 1, 2, 4)
+        assert 0 < p["commit_grid"] <= 1 << 20 and 0 < p["gc_grid"] <= 1 << 20
+        assert os.path.exists(os.path.join(root, p["sweep"])), p["sweep"]

@@ -9,9 +9,28 @@ coordinate descent over the other fields from the best point.
 Each point times `steps` applies (after `warmup`) of the caller's workload with
 the device clock.
 """
+import json
+import os
 import time
 
 import torch
+
+TUNING_FIELDS = ("tile", "dedup_rounds", "commit_grid", "gc_grid", "probe_blocks_bricks", "probe_blocks",
+                 "brick_merge_rounds", "brick_rounds")
+_PICKS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+
+
+def committed_pick(workload):
+    """The autotuner's pick for a named workload from the package's tuning.json (the best
+    point of a sweep on that workload: load factor and launch shape), or None."""
+    if not os.path.exists(_PICKS):
+        return None
+    return json.load(open(_PICKS)).get(workload)
+
+
+def apply_pick(interner, pick):
+    """Set a pick's launch shape on an interner (its load factor is a constructor argument)."""
+    interner.set_tuning(**{f: int(pick[f]) for f in TUNING_FIELDS if f in pick})
 
 
 def _time(step, steps, warmup):
