@@ -76,11 +76,16 @@ def parse():
     ap.add_argument("--train-requests", type=int, default=1500,
                     help="serve: requests logged (with exact dry-run counts) to train the predictor")
     ap.add_argument("--config", default="uneven",
-                    choices=["uneven", "terrain", "plain", "gets", "set", "vtm", "serve", "calib", "tune"],
+                    choices=["uneven", "terrain", "plain", "gets", "set", "vtm", "serve", "calib", "tune",
+                             "predictor"],
                     help="uneven: the headline (4096 u16 trees of depth 5/6/7, dense or sparse); terrain: "
                          "64x64 Perlin-dune u8 trees; plain: one 32^3 tree; gets: batched reads; set: "
                          "single-voxel set/remove, one voxel per tree per call")
     ap.add_argument("--trees", type=int, default=4096)
+    ap.add_argument("--pairs", type=int, default=1 << 20,
+                    help="predictor: (features, exact growth) pairs to log before training")
+    ap.add_argument("--admit-steps", type=int, default=20,
+                    help="uneven: applies run under predictor-driven admission (0: none)")
     ap.add_argument("--queries", type=int, default=1 << 24)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (a known-size kernel for counter scaling)")
@@ -102,7 +107,7 @@ def parse():
     if a.load_factor is None:
         a.load_factor = float(a.pick["load_factor"]) if a.pick else 0.5
     if a.budget_mib is None:
-        big = a.config == "uneven" or (a.config == "tune" and a.tune_workload == "uneven")
+        big = a.config in ("uneven", "predictor") or (a.config == "tune" and a.tune_workload == "uneven")
         a.budget_mib = 16384 if big else 2048
     return a
 
@@ -217,14 +222,16 @@ def _roofline(prof, st, steps, wl):
 
 # ---------------------------------------------------------------- workloads
 
-def _uneven_workload(a, api, W, torch):
-    """Device buffers of the uneven workload: per tree its own masks, values and values+1."""
+def _uneven_workload(a, api, W, torch, features=False):
+    """Device buffers of the uneven workload: per tree its own masks, values and values+1
+    (and, with features, each tree's batch features for the growth predictor)."""
+    from voxelis_amd.admission import batch_features
     spec = W.uneven_spec(a.trees, depths=(5, 6, 7), p_dense=0.5, seed=a.seed + 2)
     t_gen = time.perf_counter()
     dev = torch.device("cuda", 0)
     from voxelis_amd.morton import apply_edits_to_dense, dense_batch_arrays
     dense_host = {}
-    mk, vA, vB, edits = [], [], [], []
+    mk, vA, vB, edits, feats = [], [], [], [], []
     touched = mixed = partial = 0
     for d, kind, dens, sd in spec:
         if kind == "dense":
@@ -235,8 +242,10 @@ def _uneven_workload(a, api, W, torch):
                 dense_host[d] = (api.to_device(m, dev), api.to_device(v, dev), api.to_device(vb, dev),
                                  int(np.count_nonzero(sm)),
                                  int(np.count_nonzero((sm != 0) & ((sm != 0xFF) | (v != v[:, :1]).any(axis=1)))),
-                                 int(np.count_nonzero((sm != 0) & (sm != 0xFF))))
-            m_t, vA_t, vB_t, tc, mx, pt = dense_host[d]
+                                 int(np.count_nonzero((sm != 0) & (sm != 0xFF))),
+                                 batch_features(m, v, d, 2) if features else None)
+            m_t, vA_t, vB_t, tc, mx, pt, ft = dense_host[d]
+            feats.append(ft)
             # every tree reads its own copy of the batch, as distinct trees would
             m_t, vA_t, vB_t = m_t.clone(), vA_t.clone(), vB_t.clone()
             edits.append(1 << (3 * d))
@@ -251,6 +260,7 @@ def _uneven_workload(a, api, W, torch):
             pt = int(np.count_nonzero((sm != 0) & (sm != 0xFF)))
             m_t, vA_t, vB_t = api.to_device(m, dev), api.to_device(v, dev), api.to_device(vb, dev)
             edits.append(int(np.unpackbits(sm).sum()))
+            feats.append(batch_features(m, v, d, 2) if features else None)
         touched += tc
         mixed += mx
         partial += pt
@@ -270,6 +280,7 @@ def _uneven_workload(a, api, W, torch):
     return {"name": "uneven", "workload": UNEVEN_WORKLOAD, "dtype": "u16", "vsize": 2, "depths": depths,
             "kinds": [k for _, k, _, _ in spec], "spec": spec, "masks": mk, "vA": vA, "vB": vB,
             "edits": np.array(edits, dtype=np.int64), "units": units,
+            "features": np.stack(feats) if features else None,
             "pmc_profile": "pmc_traffic_uneven_4096_u16.json" if a.trees == 4096 and a.seed == 1 else None,
             "data": "synthetic (seeded uniform random and set_sum-style dense fills)"}
 
@@ -527,7 +538,9 @@ def bench_headline(a, wl_name):
     from voxelis_amd import api
     from voxelis_amd import workloads as W
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    wl = _uneven_workload(a, api, W, torch) if wl_name == "uneven" else _terrain_workload(a, api, W, torch)
+    admit = wl_name == "uneven" and not a.quick and a.admit_steps > 0 and os.path.exists(PREDICTOR_FILE)
+    wl = (_uneven_workload(a, api, W, torch, features=admit) if wl_name == "uneven"
+          else _terrain_workload(a, api, W, torch))
     T = len(wl["depths"])
     E = int(wl["edits"].sum())
     idx_all = np.arange(T)
@@ -652,6 +665,16 @@ def bench_headline(a, wl_name):
     out["plain_edits_per_s"] = plain["edits_per_s"] if plain else None
     out["batched_over_plain"] = value / plain["edits_per_s"] if plain else None
     out["p99_bounded_over_plain"] = sl["edits_per_s"] / plain["edits_per_s"] if (plain and sl) else None
+    if admit:
+        # admission by memory budget with the committed growth predictor
+        from voxelis_amd.predictor import GrowthMLP, Predictor
+        d = np.load(PREDICTOR_FILE)
+        pred = Predictor(GrowthMLP(params=d["params"]), d["mu"], d["sd"])
+        adm = _admitted_steps(api, a, wl, pred, float(d["margin"]), a.admit_steps, torch)
+        adm["predictor"] = "voxelis_amd/growth_batch.npz (bench.py --config predictor)"
+        out["admission"] = adm
+        print(f"[bench] admission: {adm['fallbacks']} fallback recounts in {adm['applies']} applies, reserved/used "
+              f"{adm['reserved_over_used']:.3f}", file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu:
         cpu = _cpu_baseline(a, wl)
@@ -674,7 +697,8 @@ def main():
     if a.config in ("uneven", "terrain"):
         return bench_headline(a, a.config)
     return {"plain": bench_plain, "gets": bench_gets, "set": bench_set, "vtm": bench_vtm,
-            "serve": bench_serve, "calib": bench_calib, "tune": bench_tune}[a.config](a)
+            "serve": bench_serve, "calib": bench_calib, "tune": bench_tune,
+            "predictor": bench_predictor}[a.config](a)
 
 
 # ---------------------------------------------------------------- profiler summaries
@@ -1116,6 +1140,158 @@ def bench_vtm(a):
         del f, it
     print(json.dumps({"metric": "VTM export / import", "config": {"workload": "terrain 64x64 (u8) and 1024 "
                       "uneven u16 trees (the deepest of depth 5/6)"}, "results": res}))
+
+
+PREDICTOR_FILE = os.path.join(ROOT, "voxelis_amd", "growth_batch.npz")
+
+
+def _admitted_steps(api, a, wl, pred, margin, n, torch):
+    """n applies of the whole workload (values alternating per step) under
+    predictor-driven admission; returns (admission report, edits/s)."""
+    from voxelis_amd.admission import BatchAdmission, with_state
+    it = _interner(api, a, wl)
+    idx = np.arange(len(wl["depths"]))
+    f, views = _forest(api, it, wl, idx)
+    adm = BatchAdmission(pred, margin, wl["vsize"])
+    last = np.zeros(len(idx))
+    static = wl["features"]
+    hard = float(np.expm1(static[:, 15].astype(np.float64)).sum())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(n):
+        empty = (f.roots.cpu().numpy() == 0)
+        per = adm.apply(f, views[k % 2], with_state(static, empty, last))
+        last = per
+    wall = time.perf_counter() - t0
+    _check_status(f, "admitted")
+    rep = adm.report()
+    rep["hard_bound_over_used"] = hard / max(float(np.mean(adm.used)), 1.0)
+    rep["edits_per_s"] = float(wl["edits"].sum()) * n / wall
+    rep["note"] = ("each apply synchronizes to read its counts and statuses (the admission decision "
+                   "needs them); the first apply builds the trees from empty")
+    it.close()
+    return rep
+
+
+def bench_predictor(a):
+    """The learned growth predictor for batched admission.  Logs (features, exact per-tree
+    growth) pairs with dry runs (vx_apply_opts.counts) from the plain checker tree, the
+    64x64 terrain grid and the 4096-tree uneven workload: random subsets of trees with
+    random value phases, applied for real after their dry run so the worlds evolve, and
+    reset now and then so builds from empty trees are logged too.  Trains the MLP on the
+    GPU with the library's kernels, then runs the uneven workload under predictor-driven
+    admission (node_limit from the prediction, the exact dry-run recount when the library
+    reports VX_E_BUDGET) and saves the predictor (--out, .npz)."""
+    import torch
+    from voxelis_amd import api
+    from voxelis_amd import workloads as W
+    from voxelis_amd.admission import batch_features, fit, with_state
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(a.seed + 77)
+    t_all = time.perf_counter()
+    # ---- the three worlds: per tree its depth, masks, value phases (device) and batch features
+    worlds = {}
+    wl = _uneven_workload(a, api, W, torch, features=True)
+    def shift(v):  # u16 (held as int16 bits) v -> v + 1, 65535 -> 1, 0 stays 0
+        u = v.to(torch.int32) & 0xFFFF
+        u = torch.where(u == 0, u, (u % 65535) + 1)
+        return torch.where(u > 32767, u - 65536, u).to(v.dtype)
+    v2 = [shift(v) for v in wl["vB"]]
+    worlds["uneven"] = {"vsize": 2, "depths": np.array(wl["depths"]), "masks": wl["masks"],
+                        "phases": [wl["vA"], wl["vB"], v2], "static": wl["features"], "budget": a.budget_mib}
+    mA, vA, _, _ = W.terrain_batches(grid=a.grid, seed=a.seed)
+    T = len(mA)
+    m_t, v_t = api.to_device(mA, dev), api.to_device(vA, dev)
+    tphases = []
+    for below, surface in ((1, 2), (2, 1), (1, 3), (3, 2)):
+        x = torch.where(v_t == 1, torch.full_like(v_t, below), torch.where(v_t == 2, torch.full_like(v_t, surface), v_t))
+        tphases.append([x[i] for i in range(T)])
+    worlds["terrain"] = {"vsize": 1, "depths": np.full(T, 5), "masks": [m_t[i] for i in range(T)],
+                         "phases": tphases, "budget": 2048,
+                         "static": np.stack([batch_features(mA[i], vA[i], 5, 1) for i in range(T)])}
+    mp, vp = W.dense_to_batch(W.plain_checker(5), 5)
+    P = 64
+    mp_t, vp_t = api.to_device(mp, dev), api.to_device(vp, dev)
+    pph = [[torch.where(vp_t == 1, torch.full_like(vp_t, c), torch.full_like(vp_t, d))] * P
+           for c, d in ((1, 2), (2, 1), (3, 1), (1, 4))]
+    worlds["plain"] = {"vsize": 1, "depths": np.full(P, 5), "masks": [mp_t] * P, "phases": pph, "budget": 256,
+                       "static": np.stack([batch_features(mp, vp, 5, 1)] * P)}
+    print(f"[predictor] worlds ready ({time.perf_counter() - t_all:.1f}s)", file=sys.stderr, flush=True)
+
+    def reset(w):
+        if "it" in w:
+            w["it"].close()
+        w["it"] = api.VoxInterner(w["vsize"], w["budget"] << 20, a.load_factor)
+        n = len(w["depths"])
+        w["roots"] = torch.zeros(n, dtype=torch.int64, device=dev)
+        w["last"] = np.zeros(n)
+        w["rounds"] = 0
+
+    for w in worlds.values():
+        reset(w)
+    # ---- log pairs
+    Xs, ys, gs = [], [], []
+    npairs = rnd = 0
+    t_log = time.perf_counter()
+    names, weights = ["uneven", "terrain", "plain"], [0.6, 0.35, 0.05]
+    while npairs < a.pairs:
+        name = names[int(rng.choice(3, p=weights))]
+        w = worlds[name]
+        if w["rounds"] >= 40:
+            reset(w)
+        n = len(w["depths"])
+        k = int(rng.integers(max(1, n // 64), n + 1))
+        idx = np.sort(rng.choice(n, size=k, replace=False))
+        vals = w["phases"][int(rng.integers(len(w["phases"])))]
+        f = api.Forest(w["it"], w["depths"][idx])
+        idx_t = torch.from_numpy(idx).to(dev)
+        f.roots.copy_(w["roots"][idx_t])
+        views = f.make_views([w["masks"][i] for i in idx], [vals[i] for i in idx])
+        X = with_state(w["static"][idx], (f.roots.cpu().numpy() == 0), w["last"][idx])
+        f.apply_views(views, dry_run=True, counts=True)
+        w["it"].sync()
+        nb, nl, per = f.counts_host()
+        f.apply_views(views)
+        w["it"].sync()
+        st = f.status.cpu().numpy()
+        ok = st == 1
+        Xs.append(X[ok])
+        ys.append(per[ok].astype(np.float64))
+        gs.append(np.full(int(ok.sum()), rnd))
+        w["roots"][idx_t] = f.roots
+        w["last"][idx] = np.where(ok, per, w["last"][idx])
+        w["rounds"] += 1
+        npairs += int(ok.sum())
+        rnd += 1
+        if rnd % 50 == 0:
+            print(f"[predictor] {rnd} applies, {npairs} pairs ({time.perf_counter() - t_log:.1f}s)",
+                  file=sys.stderr, flush=True)
+    for w in worlds.values():
+        w["it"].close()
+    t_log = time.perf_counter() - t_log
+    X, y, g = np.concatenate(Xs), np.concatenate(ys), np.concatenate(gs)
+    # ---- train on the GPU (vx_mlp_gradient / vx_mlp_adam) and evaluate on held-out applies
+    t0 = time.perf_counter()
+    pred, margin, rep = fit(X, y, g)
+    t_fit = time.perf_counter() - t0
+    print(f"[predictor] {len(y)} pairs from {rnd} applies ({t_log:.1f}s); trained in {t_fit:.1f}s; held-out "
+          f"MAE {rep['mae_new_branches_per_apply']:.0f} new branches per apply (mean "
+          f"{rep['mean_new_branches_per_apply']:.0f}), margin {margin:.3f}", file=sys.stderr, flush=True)
+    if a.out:
+        np.savez(a.out, params=pred.model.p.cpu().numpy(), mu=pred.mu, sd=pred.sd, margin=np.float64(margin))
+    # ---- deployed: the uneven workload's steps under predictor-driven admission
+    a.pick = None
+    adm = _admitted_steps(api, a, wl, pred, margin, max(a.admit_steps, 4), torch)
+    print(f"[predictor] admission: {adm['fallbacks']} fallbacks in {adm['applies']} applies, MAE "
+          f"{adm['mae_new_branches_per_apply']:.0f} new branches per apply, reserved/used "
+          f"{adm['reserved_over_used']:.3f} (hard bound {adm['hard_bound_over_used']:.1f})", file=sys.stderr,
+          flush=True)
+    print(json.dumps({"metric": "learned growth predictor: held-out error and admission fallback rate",
+                      "config": {"workload": "pairs logged from the plain checker tree, the 64x64 terrain grid "
+                                             "and the 4096-tree uneven workload (random tree subsets, value "
+                                             "phases); deployed in the uneven workload's admission",
+                                 "network": "FP32 MLP 16-64-64-1, trained with vx_mlp_gradient / vx_mlp_adam"},
+                      "log_s": t_log, "train_s": t_fit, "training": rep, "admission": adm}))
 
 
 # Published single-thread apply_batch / get rates of the reference (Apple M3 Max,

@@ -236,3 +236,36 @@ def test_edit_list_dry_run_matches_real(vx):
         assert np.array_equal(st, st_d)
         assert d[:2] == r[:2] and d[2].sum() == d[0] and r[2].sum() == r[0], (d[:2], r[:2])
         world.forest.roots[idx] = sub.roots
+
+
+def test_batch_admission_predicts_and_falls_back_exactly(vx):
+    """Predictor-driven admission (voxelis_amd/admission.py): an untrained network with a
+    margin far too small under-predicts, the library rolls the apply back (VX_E_BUDGET) and
+    the exact dry-run recount admits it; a generous margin admits directly.  Either way
+    the trees equal the oracle's after every apply."""
+    api, W = vx
+    from voxelis_amd.admission import BatchAdmission, batch_features, with_state
+    from voxelis_amd.predictor import GrowthMLP, Predictor
+    (mA_t, vA_t, mA, vA), (mB_t, vB_t, mB, vB, _, _) = _batches(api, W, 4)
+    n = len(mA)
+    static = np.stack([batch_features(mA[i], vA[i], 5, 1) for i in range(n)])
+    pred = Predictor(GrowthMLP(seed=3), np.zeros(16, np.float32), np.ones(16, np.float32))
+    it = api.VoxInterner(1, 64 * MB)
+    f = api.Forest(it, [5] * n)
+    it_o = O.Interner(1, 64 * MB)
+    trees_o = [O.Tree(5) for _ in range(n)]
+    views = [f.make_views(mA_t, vA_t), f.make_views(mB_t, vB_t)]
+    last = np.zeros(n)
+    for k, (margin, want_fallback) in enumerate(((1e-6, True), (1e-6, True), (1e6, False))):
+        adm = BatchAdmission(pred, margin, 1)
+        adm.value_size = 0  # no leaf allowance: the tiny margin must under-predict
+        X = with_state(static, f.roots.cpu().numpy() == 0, last)
+        per = adm.apply(f, views[k % 2], X)
+        assert adm.fallbacks == (1 if want_fallback else 0), k
+        assert (f.status.cpu().numpy() == 1).all()
+        m, v = (mA, vA) if k % 2 == 0 else (mB, vB)
+        _oracle_step(it_o, trees_o, m, v)
+        assert_same(it, f, it_o, trees_o, full_reads=(k == 2), what=f"admitted apply {k}")
+        assert per.sum() == adm.last[0]  # the per-tree counts add up to the apply's new branches
+        last = per
+    it.close()
