@@ -64,3 +64,47 @@ def test_table_churn_rebuilds(vx):
     for _ in range(20):
         _step(api, f, it_o, trees_o, rng, depth)
     assert_same(it, f, it_o, trees_o, full_reads=True, what="after rebuild and more churn")
+
+
+def test_table_rebuilds_when_applies_are_queued(vx):
+    """Applies queued back to back with no synchronization (the host far ahead of the
+    device, as a benchmark's latency loop runs): the rebuild decision is made on the
+    device in stream order, so the table keeps empty slots and scans stay short however
+    many applies are queued, and the trees still match the oracle."""
+    api = vx
+    from voxelis_amd import workloads as W
+    from gpu_helpers import oracle_apply_dense
+    depth, n_trees, K, N = 4, 8, 6, 300
+    budget = 2 * MB
+    it = api.VoxInterner(2, budget)
+    it.set_graphs(True)
+    f = api.Forest(it, [depth] * n_trees)
+    rng = np.random.default_rng(9)
+    side = 1 << depth
+    host, dev, views = [], [], []
+    for k in range(K):
+        dense = rng.integers(1, 201, size=(n_trees, side, side, side)).astype(np.uint16)
+        mv = [W.dense_to_batch(dense[t], depth) for t in range(n_trees)]
+        host.append(mv)
+        # the views hold raw device pointers: their tensors stay referenced
+        dev.append(([api.to_device(m) for m, _ in mv], [api.to_device(v) for _, v in mv]))
+        views.append(f.make_views(*dev[-1]))
+    s0 = it.stats()
+    for j in range(N):
+        f.apply_views(views[j % K])
+    it.sync()
+    assert (f.status.cpu().numpy() == 1).all()
+    s1 = it.stats()
+    slots, tomb, rebuilds = it.table_stats()
+    leaders = (s1["branch_hits"] - s0["branch_hits"]) + (s1["branch_misses"] - s0["branch_misses"])
+    probes = s1["probes"] - s0["probes"]
+    assert probes < 16 * leaders, (probes, leaders)
+    assert rebuilds >= 1 and tomb < slots, (tomb, rebuilds)
+    it_o = O.Interner(2, budget)
+    trees_o = [O.Tree(depth) for _ in range(n_trees)]
+    for j in range(N):
+        for t, (m, v) in zip(trees_o, host[j % K]):
+            ok, _ = oracle_apply_dense(it_o, t, m, v)
+            assert ok
+    assert_same(it, f, it_o, trees_o, full_reads=True, what=f"after {N} queued applies")
+    it.close()

@@ -292,7 +292,9 @@ class Forest:
         return [_u64(int(r)) for r in self.roots.cpu().tolist()]
 
     def make_views(self, masks, values, fills=None, has_patches=None):
-        """Device batch views over per-tree tensors (masks [P,2] u8, values [P,8])."""
+        """Device batch views over per-tree tensors (masks [P,2] u8, values [P,8]).  The views
+        hold the tensors' raw device pointers: the caller keeps the tensors alive while the
+        views are used."""
         views = (BatchView * self.n)()
         for i in range(self.n):
             v = views[i]

@@ -169,12 +169,6 @@ struct vx_interner {
     int ring_next = 0;
     int ring_cur = 0;
     void* h_pinned = nullptr;  // the slot of the current call
-    // the control block, copied after every write call (pinned; read by the next write
-    // call once the copy completed) to schedule pattern-table rebuilds
-    Ctl* h_ctl = nullptr;
-    hipEvent_t ctr_ev = nullptr;
-    bool ctr_pending = false;
-    uint64_t rebuilds = 0;
     Prof prof;
     vx_tuning tune = {256, 8, 1024, 1024, 32768, 512, 4, 0};
     // captured launch sequences of repeated calls (vx_interner_set_graphs)
@@ -303,6 +297,9 @@ struct ApplyOpts {
     const int32_t* set_xyz = nullptr;
     const void* set_values = nullptr;
 };
+
+template <typename T>
+int rebuild_step(vx_interner* h, bool force);
 
 template <typename T, int TILE>
 int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
@@ -562,6 +559,10 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
     };
     auto launch_all = [&]() -> int {
+    if (!opt.dry) {
+        int rc = rebuild_step<T>(h, false);
+        if (rc) return rc;
+    }
     if (setm) {
         int rc = launch_set();
         if (rc) return rc;
@@ -653,6 +654,25 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     return hipGraphLaunch(plan->exec, s) == hipSuccess ? VX_OK : VX_E_HIP;
 }
 
+// The table-rebuild step that starts every write call (enqueued on the interner's
+// stream, captured with the call's launch sequence): the device decides whether the
+// pattern table (and the i32 leaf table) is rebuilt now, and the clear and re-insert
+// kernels do it or return at once.  force: rebuild regardless.
+constexpr unsigned kRebuildGrid = 2048;
+template <typename T>
+int rebuild_step(vx_interner* h, bool force) {
+    Pool<T> P = pool_of<T>(h);
+    hipStream_t s = h->stream;
+    k_rebuild_check<<<1, 64, 0, s>>>(h->ctl, h->tslots, h->ltab ? h->lslots : 0ull, force ? 1u : 0u);
+    k_tab_clear<<<kRebuildGrid, kBlock, 0, s>>>(h->tab, h->tslots, h->ctl);
+    k_tab_reinsert<T><<<kRebuildGrid, kBlock, 0, s>>>(P);
+    if (h->ltab) {
+        k_ltab_clear<<<kRebuildGrid, kBlock, 0, s>>>(h->ltab, h->lslots, h->ctl);
+        k_ltab_reinsert<T><<<kRebuildGrid, kBlock, 0, s>>>(P);
+    }
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
+}
+
 // Tombstones in the pattern table: a counter striped over the statistics lines.
 uint64_t tombstones_of(const Ctl& c, int which = ST_TOMB) {
     uint64_t t = 0;
@@ -660,55 +680,16 @@ uint64_t tombstones_of(const Ctl& c, int which = ST_TOMB) {
     return t;
 }
 
-// Clear the pattern table and re-insert every live branch (enqueued on the stream).
-template <typename T>
-int rebuild_table(vx_interner* h) {
-    Pool<T> P = pool_of<T>(h);
-    k_tab_clear<<<grid_for(h->tslots / 2), kBlock, 0, h->stream>>>(h->tab, h->tslots, h->ctl);
-    k_tab_reinsert<T><<<grid_for(h->capacity), kBlock, 0, h->stream>>>(P);
-    if (h->ltab) {
-        k_ltab_clear<<<grid_for(h->lslots / 2), kBlock, 0, h->stream>>>(h->ltab, h->lslots, h->ctl);
-        k_ltab_reinsert<T><<<grid_for(h->capacity), kBlock, 0, h->stream>>>(P);
-    }
-    h->rebuilds++;
-    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
-}
-
-// Before a write call: rebuild the table when the last completed counter copy shows
-// tombstones taking a sixteenth of the slots and live plus tombstones three quarters.
-template <typename T>
-int maybe_rebuild(vx_interner* h) {
-    if (!h->ctr_pending || hipEventQuery(h->ctr_ev) != hipSuccess) return VX_OK;
-    h->ctr_pending = false;
-    const uint64_t live = h->h_ctl->live_branches, tomb = tombstones_of(*h->h_ctl);
-    if (tomb >= h->tslots / 16 && live + tomb > h->tslots / 4 * 3) return rebuild_table<T>(h);
-    if (h->ltab) {
-        const uint64_t ll = h->h_ctl->live_leaves, lt = tombstones_of(*h->h_ctl, ST_LTOMB);
-        if (lt >= h->lslots / 16 && ll + lt > h->lslots / 4 * 3) return rebuild_table<T>(h);
-    }
-    return VX_OK;
-}
-
-// After a write call: copy the counters for the next call's check.
-int note_counters(vx_interner* h) {
-    if (h->ctr_pending) return VX_OK;  // the previous copy has not been read yet
-    HIPCHK(hipMemcpyAsync(h->h_ctl, h->ctl, offsetof(Ctl, leaf_ids8), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipEventRecord(h->ctr_ev, h->stream));
-    h->ctr_pending = true;
-    return VX_OK;
-}
-
 template <typename T>
 int apply_impl(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* batches,
                int32_t* status, const int32_t* pre_status = nullptr, const ApplyOpts& opt = ApplyOpts()) {
-    int rc = maybe_rebuild<T>(h);
-    if (rc) return rc;
+    int rc = VX_OK;
     switch (h->tune.tile) {
         case 128: rc = apply_tiled<T, 128>(h, n, roots, batches, status, pre_status, opt); break;
         case 512: rc = apply_tiled<T, 512>(h, n, roots, batches, status, pre_status, opt); break;
         default: rc = apply_tiled<T, 256>(h, n, roots, batches, status, pre_status, opt); break;
     }
-    return rc ? rc : note_counters(h);
+    return rc;
 }
 
 template <typename T>
@@ -811,7 +792,7 @@ int apply_edits_impl(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* d
 
 template <typename T>
 int fill_impl(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* vals) {
-    int rc = maybe_rebuild<T>(h);
+    int rc = rebuild_step<T>(h, false);
     if (rc) return rc;
     Pool<T> P = pool_of<T>(h);
     size_t gcap = (size_t)h->capacity + n + 16;
@@ -831,8 +812,7 @@ int fill_impl(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* vals) {
     VX_LAUNCH(h, KID_LEAF_ALLOC, k_leaf_alloc_only<T>, 1, 1024, P, nvalues);
     VX_LAUNCH(h, KID_FILL, k_fill_finish<T>, grid_for(n), kBlock, P, (uint32_t)n, roots, vals);
     VX_LAUNCH(h, KID_END_ALLOC, k_end_alloc<T>, 1, 256, P, false, (uint32_t*)nullptr);
-    if (hipGetLastError() != hipSuccess) return VX_E_HIP;
-    return note_counters(h);
+    return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
 }
 
 template <typename T>
@@ -953,8 +933,6 @@ int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_f
               (!direct || dmalloc(&h->single, 8 * nv * 8)) &&
               (value_size != 2 || dmalloc(&h->leaf_ids, nv * 8)) &&
               dmalloc(&h->ctl, sizeof(Ctl));
-    ok = ok && hipHostMalloc((void**)&h->h_ctl, sizeof(Ctl), hipHostMallocDefault) == hipSuccess &&
-         hipEventCreateWithFlags(&h->ctr_ev, hipEventDisableTiming) == hipSuccess;
     if (!ok || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         vx_interner_destroy(h);
         return VX_E_HIP;
@@ -1019,8 +997,6 @@ int vx_interner_destroy(vx_interner* h) {
         b.old.release();
     }
     for (hipEvent_t e : h->prof.pool) (void)hipEventDestroy(e);
-    if (h->h_ctl) (void)hipHostFree(h->h_ctl);
-    if (h->ctr_ev) (void)hipEventDestroy(h->ctr_ev);
     for (int i = 0; i < vx_interner::kRing; i++) {
         if (h->h_pinned_ring[i]) (void)hipHostFree(h->h_pinned_ring[i]);
         if (h->ring_ev[i]) (void)hipEventDestroy(h->ring_ev[i]);
@@ -1286,10 +1262,12 @@ int vx_interner_get_tuning(const vx_interner* h, vx_tuning* t) {
 int vx_interner_rebuild_table(vx_interner* h, uint64_t* rebuilds) {
     if (!h) return VX_E_INVALID_ARG;
     if (h->poisoned) return VX_E_POISONED;
-    int rc = VX_WIDTH_DISPATCH(h, rebuild_table, h);
+    int rc = VX_WIDTH_DISPATCH(h, rebuild_step, h, true);
     if (rc) return rc;
-    HIPCHK(hipStreamSynchronize(h->stream));
-    if (rebuilds) *rebuilds = h->rebuilds;
+    Ctl c;
+    rc = sync_read_ctl(h, &c);
+    if (rc) return rc;
+    if (rebuilds) *rebuilds = c.rebuilds;
     return VX_OK;
 }
 
@@ -1300,7 +1278,7 @@ int vx_interner_table_stats(vx_interner* h, uint64_t* slots, uint64_t* tombstone
     if (rc) return rc;
     if (slots) *slots = h->tslots;
     if (tombstones) *tombstones = tombstones_of(c);
-    if (rebuilds) *rebuilds = h->rebuilds;
+    if (rebuilds) *rebuilds = c.rebuilds;
     return VX_OK;
 }
 

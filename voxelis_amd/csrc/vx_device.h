@@ -97,7 +97,8 @@ struct Ctl {
     uint32_t n_deferred;  // bricks waiting for leaves created later in the sequence
     uint64_t live_leaves;
     uint64_t live_branches;
-    uint64_t pad0;
+    uint32_t rebuild;   // this call rebuilds the pattern table (bit 0) and the i32 leaf table (bit 1)
+    uint32_t rebuilds;  // table rebuilds so far
     uint32_t n_drift;  // queued reference-count corrections of the reference's quirks
     uint32_t dry_owners;      // dry run: branch keys that would be new
     uint32_t dry_new_leaves;  // dry run: leaves that would be new

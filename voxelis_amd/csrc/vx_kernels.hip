@@ -2123,7 +2123,29 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
 // scan meets it before an empty slot, so between rebuilds the non-empty share of the
 // table grows and scans lengthen.  A rebuild clears the table and re-inserts every live
 // branch (a node with a types/mask word: the GC clears it on free, leaves have none).
+//
+// Every write call starts with the decision, made here on the device in stream order
+// (so it stays current however many calls the host has queued): rebuild when the
+// tombstones take a sixteenth of the slots and live keys plus tombstones three quarters,
+// in either table (force: always).  The clear and re-insert kernels that follow return at
+// once when no rebuild is due.
+__global__ void k_rebuild_check(Ctl* ctl, uint64_t tslots, uint64_t lslots, uint32_t force) {
+    if (threadIdx.x == 0) {
+        uint64_t t = 0, lt = 0;  // the counters take signed adds: the sums wrap to the count
+        for (int i = 0; i < kStatStripes; i++) {
+            t += ctl->st[i][ST_TOMB];
+            lt += ctl->st[i][ST_LTOMB];
+        }
+        const uint64_t live = ctl->live_branches, ll = ctl->live_leaves;
+        bool due = force != 0 || (t >= tslots / 16 && live + t > tslots / 4 * 3);
+        if (lslots) due = due || (lt >= lslots / 16 && ll + lt > lslots / 4 * 3);
+        ctl->rebuild = due ? (lslots ? 3u : 1u) : 0u;
+        if (due) ctl->rebuilds++;
+    }
+}
+
 __global__ void k_tab_clear(uint64_t* tab, uint64_t nslots, Ctl* ctl) {
+    if (!(ctl->rebuild & 1u)) return;
     uint4* t4 = (uint4*)tab;
     const uint64_t n4 = nslots / 2;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x)
@@ -2133,6 +2155,7 @@ __global__ void k_tab_clear(uint64_t* tab, uint64_t nslots, Ctl* ctl) {
 
 template <typename T>
 __global__ void k_tab_reinsert(Pool<T> P) {
+    if (!(P.ctl->rebuild & 1u)) return;
     const uint32_t top = P.ctl->next_index;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < top; i += gridDim.x * blockDim.x) {
         const uint32_t tm = P.tms[i];
@@ -2155,6 +2178,7 @@ __global__ void k_tab_reinsert(Pool<T> P) {
 // The hashed leaf table of four-byte voxels, rebuilt the same way: every live leaf (a
 // node without a types/mask word that holds a reference) is inserted again.
 __global__ void k_ltab_clear(uint64_t* ltab, uint64_t nslots, Ctl* ctl) {
+    if (!(ctl->rebuild & 2u)) return;
     uint4* t4 = (uint4*)ltab;
     const uint64_t n4 = nslots / 2;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x)
@@ -2164,6 +2188,7 @@ __global__ void k_ltab_clear(uint64_t* ltab, uint64_t nslots, Ctl* ctl) {
 
 template <typename T>
 __global__ void k_ltab_reinsert(Pool<T> P) {
+    if (!(P.ctl->rebuild & 2u)) return;
     const uint32_t top = P.ctl->next_index;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < top; i += gridDim.x * blockDim.x)
         if (i != 0 && P.tms[i] == 0 && P.refs[i] != 0) leaf_publish(P, (uint32_t)P.values[i], i);
