@@ -752,6 +752,18 @@ __device__ inline int keyset_insert(uint64_t* s_h, uint64_t h, bool* lead) {
     return -1;
 }
 
+// Equality of two bricks' eight values (any voxel width).
+template <typename V>
+__device__ inline bool vox8_equal(const V& a, const V& b) {
+    static_assert(sizeof(V) % 4 == 0, "whole dwords");
+    const uint32_t* x = (const uint32_t*)&a;
+    const uint32_t* y = (const uint32_t*)&b;
+    bool eq = true;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(V) / 4); i++) eq &= x[i] == y[i];
+    return eq;
+}
+
 // Tile statistics in LDS, flushed with one striped atomic per counter per block.
 struct TileStats {
     uint32_t v[8];
@@ -920,6 +932,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 6
     constexpr uint32_t kSetB = R * TILE / 2;
     __shared__ uint64_t s_kh[kSetB];
     __shared__ uint32_t s_kc[kSetB];
+    __shared__ uint16_t s_kq[kSetB];  // the leader's queue entry (its mask and values in LDS)
     for (uint32_t i = threadIdx.x; i < kSetB; i += TILE) s_kh[i] = 0;
     uint64_t lm[sizeof(T) == 1 ? (256 + TILE - 1) / TILE : 1];
     if (sizeof(T) == 1)
@@ -1075,12 +1088,22 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 6
         const int ks = cand && P.brick_merge_rounds ? keyset_insert<kSetB>(s_kh, hk, &lead) : -1;
         const uint32_t tile = blockIdx.x * R + (uint32_t)r;
         const uint32_t pos = emit_candidate<TILE>(K, &s_cnt[r], cand && (lead || ks < 0), ch, tm, slot, tile, hk);
-        if (lead) s_kc[ks] = (uint32_t)((uint64_t)tile * TILE) + pos;
+        if (lead) {
+            s_kc[ks] = (uint32_t)((uint64_t)tile * TILE) + pos;
+            s_kq[ks] = (uint16_t)q;
+        }
         __syncthreads();  // the leaders' candidates and their indices are visible to the block
         bool follow = false;
         if (cand && ks >= 0 && !lead) {
             const uint32_t k2 = s_kc[ks];
-            if (key_matches_cand(K, k2, key)) {
+            // Two completely written bricks have equal keys exactly when their values are
+            // equal: every child is then the leaf of its value (a live leaf is the one leaf
+            // of its value; a dry run's placeholder is a function of the value too) and
+            // types = mask = 0xFF, whatever the old brick was.  Such a pair is compared in
+            // LDS; any other pair compares the leader's candidate key.
+            const uint32_t lq = s_kq[ks];
+            const bool both_full = s_m[q] == 0xFFu && s_m[lq] == 0xFFu;
+            if (both_full ? vox8_equal(s_v[q], s_v[lq]) : key_matches_cand(K, k2, key)) {
                 out = res_pending(k2);
                 follow = true;
             } else {  // another key with the same 64-bit hash: a candidate of its own
