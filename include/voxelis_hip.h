@@ -118,8 +118,9 @@ int vx_interner_set_graphs(vx_interner* h, int enable);
 /* Pattern-table maintenance (no reference counterpart: the reference's HashMap
  * removes entries outright, interner/mod.rs:518).  Frees here leave tombstones; a
  * write call rebuilds the table by itself (clear, re-insert every live branch) once
- * tombstones take 1/16 of the slots and live entries plus tombstones 3/4 (decided on
- * the device at the start of the call, in stream order, so queued calls see it).
+ * tombstones take 1/16 of the slots and live entries plus tombstones 1/2 (3/4 in the
+ * i32 leaf table; decided on the device at the start of the call, in stream order, so
+ * queued calls see it).
  * rebuild_table forces a rebuild now and synchronizes; table_stats reports the slot
  * count, the current tombstones and the rebuilds so far (synchronizes).  Either
  * output pointer may be NULL. */
