@@ -658,13 +658,13 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
 // stream, captured with the call's launch sequence): the device decides whether the
 // pattern table (and the i32 leaf table) is rebuilt now, and the clear and re-insert
 // kernels do it or return at once.  force: rebuild regardless.
-constexpr unsigned kRebuildGrid = 2048;
+constexpr unsigned kRebuildGrid = 512;
 template <typename T>
 int rebuild_step(vx_interner* h, bool force) {
     Pool<T> P = pool_of<T>(h);
     hipStream_t s = h->stream;
-    k_rebuild_check<<<1, 64, 0, s>>>(h->ctl, h->tslots, h->ltab ? h->lslots : 0ull, force ? 1u : 0u);
-    k_tab_clear<<<kRebuildGrid, kBlock, 0, s>>>(h->tab, h->tslots, h->ctl);
+    k_tab_clear<<<kRebuildGrid, kBlock, 0, s>>>(h->tab, h->tslots, h->ctl, h->ltab ? h->lslots : 0ull,
+                                                force ? 1u : 0u);
     k_tab_reinsert<T><<<kRebuildGrid, kBlock, 0, s>>>(P);
     if (h->ltab) {
         k_ltab_clear<<<kRebuildGrid, kBlock, 0, s>>>(h->ltab, h->lslots, h->ctl);

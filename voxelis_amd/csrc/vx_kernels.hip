@@ -2147,41 +2147,49 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
 // table grows and scans lengthen.  A rebuild clears the table and re-inserts every live
 // branch (a node with a types/mask word: the GC clears it on free, leaves have none).
 //
-// Every write call starts with the decision, made here on the device in stream order
+// Every write call starts with the decision, made on the device in stream order
 // (so it stays current however many calls the host has queued): rebuild when the
 // tombstones take a sixteenth of the slots and live keys plus tombstones half of them
 // in the pattern table, or three quarters in the i32 leaf table (force: always).  Half
 // keeps a miss's scan to the next empty slot at about two probes; the rebuild (a clear
 // of the table and one insert per live branch) costs less than the longer scans of a
-// fuller table (measured on the 4096-tree uneven apply: 27.1 against 28.3 ms per step).  The clear and re-insert kernels that follow return at
-// once when no rebuild is due.
-__global__ void k_rebuild_check(Ctl* ctl, uint64_t tslots, uint64_t lslots, uint32_t force) {
-    if (threadIdx.x == 0) {
-        uint64_t t = 0, lt = 0;  // the counters take signed adds: the sums wrap to the count
-        for (int i = 0; i < kStatStripes; i++) {
-            t += ctl->st[i][ST_TOMB];
-            lt += ctl->st[i][ST_LTOMB];
+// fuller table (measured on the 4096-tree uneven apply: 27.1 against 28.3 ms per step).
+// Every block of the clear kernel takes the decision (block 0 records it for the kernels
+// that follow); the clear and re-insert kernels return at once when no rebuild is due.
+__global__ void k_tab_clear(uint64_t* tab, uint64_t nslots, Ctl* ctl, uint64_t lslots, uint32_t force) {
+    static_assert(kStatStripes == 64, "one statistics stripe per lane");
+    __shared__ uint32_t s_due;
+    if (threadIdx.x < 64) {  // every block takes the same decision from the same counters
+        const int lane = (int)threadIdx.x;
+        uint64_t t = ctl->st[lane][ST_TOMB], lt = ctl->st[lane][ST_LTOMB];
+        for (int off = 32; off > 0; off >>= 1) {  // signed adds: the sums wrap to the count
+            t += shfl64(t, lane ^ off);
+            lt += shfl64(lt, lane ^ off);
         }
-        const uint64_t live = ctl->live_branches, ll = ctl->live_leaves;
-        bool due = force != 0 || (t >= tslots / 16 && live + t > tslots / 2);
-        if (lslots) due = due || (lt >= lslots / 16 && ll + lt > lslots / 4 * 3);
-        ctl->rebuild = due ? (lslots ? 3u : 1u) : 0u;
-        if (due) ctl->rebuilds++;
+        if (lane == 0) {
+            const uint64_t live = ctl->live_branches, ll = ctl->live_leaves;
+            bool due = force != 0 || (t >= nslots / 16 && live + t > nslots / 2);
+            if (lslots) due = due || (lt >= lslots / 16 && ll + lt > lslots / 4 * 3);
+            s_due = due ? (lslots ? 3u : 1u) : 0u;
+            if (blockIdx.x == 0) {  // for the re-insert and the leaf-table kernels
+                ctl->rebuild = s_due;
+                if (due) ctl->rebuilds++;
+            }
+        }
     }
-}
-
-__global__ void k_tab_clear(uint64_t* tab, uint64_t nslots, Ctl* ctl) {
-    if (!(ctl->rebuild & 1u)) return;
+    __syncthreads();
+    if (!(s_due & 1u)) return;
     uint4* t4 = (uint4*)tab;
     const uint64_t n4 = nslots / 2;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x)
         t4[i] = make_uint4(0u, 0u, 0u, 0u);
-    if (blockIdx.x == 0 && threadIdx.x < kStatStripes) ctl->st[threadIdx.x][ST_TOMB] = 0;
 }
 
 template <typename T>
 __global__ void k_tab_reinsert(Pool<T> P) {
     if (!(P.ctl->rebuild & 1u)) return;
+    // the cleared table holds no tombstone (no other block reads the counter here)
+    if (blockIdx.x == 0 && threadIdx.x < kStatStripes) P.ctl->st[threadIdx.x][ST_TOMB] = 0;
     const uint32_t top = P.ctl->next_index;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < top; i += gridDim.x * blockDim.x) {
         const uint32_t tm = P.tms[i];
