@@ -149,9 +149,9 @@ def algorithmic_bytes(U):
         # per owner: list entry, table slot and id (24), key and hash (80), the node (64 + 2),
         # the table slot (8)
         "k_commit": U["branch_misses"] * 178,
-        # per freed node: frontier entry, types/mask, children read and cleared, the table
-        # slot, value, count, generation (read and written), free-stack entry
-        "k_gc_round": U["recycled"] * (4 + 2 + 128 + 8 + s + 4 + 4 + 4),
+        # per freed node: frontier entry, types/mask and generation (read and written), the
+        # children (read), the table slot (read and written), free-stack entry
+        "k_gc_round": U["recycled"] * (4 + 4 + 4 + 64 + 16 + 4),
     }
 
 
