@@ -447,3 +447,41 @@ def test_edit_lists_many_rounds_per_sort_block(vx):
         b.set_many(xyz, v.astype(np.uint64))
         assert t.apply_batch(it_o, b)
     assert_same(it, f, it_o, trees_o, full_reads=True, what="large edit lists")
+
+
+def test_unchanged_full_bricks_are_untouched(vx):
+    """A completely written brick is rebuilt from its values alone (its old children are
+    not read); when no value changed its probe finds the old node, and the level above
+    must count it as untouched, as the reference leaves an unchanged brick unpushed
+    (voxtree.rs:865-868): re-applying a full batch returns false, and a full batch that
+    changes a few bricks only changes exactly what the oracle changes."""
+    api, W = vx
+    from gpu_helpers import assert_same, oracle_apply_dense
+    depth, n = 4, 6
+    side = 1 << depth
+    rng = np.random.default_rng(31)
+    dense = rng.integers(1, 5, size=(n, side, side, side)).astype(np.uint16)
+    dense[1] = dense[0]  # two trees share every node
+    it = api.VoxInterner(2, 64 * MB)
+    f = api.Forest(it, [depth] * n)
+    it_o = O.Interner(2, 64 * MB)
+    trees_o = [O.Tree(depth) for _ in range(n)]
+
+    def step(d, want):
+        mv = [W.dense_to_batch(d[t], depth) for t in range(n)]
+        st = f.apply([api.to_device(m) for m, _ in mv], [api.to_device(v) for _, v in mv])
+        oks = [int(oracle_apply_dense(it_o, t, m, v)[0]) for t, (m, v) in zip(trees_o, mv)]
+        assert list(st) == oks == want, (list(st), oks, want)
+        assert_same(it, f, it_o, trees_o, full_reads=True, what=f"want {want}")
+        roots = f.roots_host()
+        for i, t in enumerate(trees_o):
+            assert it.get_ref(roots[i]) == it_o.get_ref(t.root), i
+
+    step(dense, [1] * n)
+    step(dense, [0] * n)                    # nothing changed: every tree returns false
+    d2 = dense.copy()
+    d2[2, 0, 0, 0] = 7                       # one brick of tree 2
+    d2[3, 4:6, 8:10, 12:14] = d2[3, 4:6, 8:10, 12:14] % 4 + 1  # one brick of tree 3, all its voxels
+    step(d2, [0, 0, 1, 1, 0, 0])
+    step(d2, [0] * n)
+    it.close()

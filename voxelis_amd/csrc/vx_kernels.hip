@@ -831,6 +831,13 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
     bool split = bid_is_leaf(base);
     uint32_t split_value = 0;
     if (split) split_value = fill_split ? (uint32_t)C.fill : (uint32_t)P.values[bid_index(base)];
+    // A completely written brick replaces every child of an old brick-level node: its key
+    // is (the leaves of its values, types = mask = 0xFF) whatever the old children were, so
+    // they are not loaded.  The old children only tell whether the brick is unchanged (every
+    // new leaf equal to the old child); such a brick is built anyway, its probe finds the
+    // old node itself, and the level above counts a child equal to the old one as
+    // untouched, as the reference leaves an unchanged brick unpushed (voxtree.rs:865-868).
+    if (!split && m == 0xFFu) base = kEmpty;
     uint32_t types = 0, mask = 0;
     if (split) {
         for (int i = 0; i < 8; i++) ch[i] = base;
@@ -1697,11 +1704,15 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
                 ec[i] = ecp[i];
             }
         }
+        // a child result equal to the old child is untouched: a completely written brick is
+        // rebuilt even when no value changed (its probe then finds the old node)
+        const bool same_old = eq != kEmpty && !bid_is_leaf(eq) && bid_index(eq) < P.capacity && ell == 0;
         uint32_t touched = 0;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            touched |= (ch[i] != kInvalid ? 1u : 0u) << i;
+            const bool t = ch[i] != kInvalid;
             ch[i] = resolve(ch[i], cand_id_prev);
+            touched |= (t && !(same_old && ch[i] == ec[i]) ? 1u : 0u) << i;
         }
         uint64_t out = kInvalid;
         bool cand = false, collapsed = false, h5 = false;
