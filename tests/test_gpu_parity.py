@@ -485,3 +485,45 @@ def test_unchanged_full_bricks_are_untouched(vx):
     step(d2, [0, 0, 1, 1, 0, 0])
     step(d2, [0] * n)
     it.close()
+
+
+def test_reapplied_all_same_brick_is_pushed(vx):
+    """The reference pushes an all-same brick whether or not it changed (voxtree.rs:887-895),
+    so re-applying a batch whose only unchanged-but-pushed bricks are all-same ones rebuilds
+    the same root and panics on assert_ne! (voxtree.rs:311, VX_E_SAME_ROOT here), while a
+    tree whose bricks are all mixed and unchanged returns false."""
+    api, W = vx
+    depth, n = 4, 3
+    side = 1 << depth
+    rng = np.random.default_rng(41)
+    dense = rng.integers(1, 5, size=(n, side, side, side)).astype(np.uint16)
+    # no accidental all-same brick in tree 0; one all-same brick in trees 1 and 2
+    for t in range(n):
+        for y in range(0, side, 2):
+            for z in range(0, side, 2):
+                for x in range(0, side, 2):
+                    b = dense[t, y:y + 2, z:z + 2, x:x + 2]
+                    if (b == b.flat[0]).all():
+                        b[0, 0, 0] = b.flat[0] % 4 + 1
+    dense[1, 0:2, 0:2, 0:2] = 5
+    dense[2, 6:8, 2:4, 10:12] = 3
+    it = api.VoxInterner(2, 64 * MB)
+    f = api.Forest(it, [depth] * n)
+    it_o = O.Interner(2, 64 * MB)
+    trees_o = [O.Tree(depth) for _ in range(n)]
+    mv = [W.dense_to_batch(dense[t], depth) for t in range(n)]
+    views = ([api.to_device(m) for m, _ in mv], [api.to_device(v) for _, v in mv])
+    for step, want in ((0, [1, 1, 1]), (1, [0, -6, -6])):
+        st = f.apply(*views)
+        got_o = []
+        for t, (m, v) in zip(trees_o, mv):
+            b = O.Batch(depth)
+            b.load_dense(m, v)
+            try:
+                got_o.append(int(t.apply_batch_atomic(it_o, b)))
+            except O.OraclePanic as e:
+                got_o.append({"SAME_ROOT": -6}[e.kind])
+        assert list(st) == got_o == want, (step, list(st), got_o)
+    from gpu_helpers import assert_same
+    assert_same(it, f, it_o, trees_o, full_reads=True, what="after the re-apply")
+    it.close()

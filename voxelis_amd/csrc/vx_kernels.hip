@@ -912,10 +912,10 @@ constexpr int brick_rounds() { return TILE >= 512 ? 2 : 4; }
 // into candidate tile blk*R + r; a deferred brick keeps that tile (its round has a
 // free place for it) for the retry.  The pass is bound by memory latency, so it runs
 // as many waves per SIMD as fit without spilling: 6 for one slot per lane; the
-// four-slot form's phase-1 loads and phase-2 keys take more registers (5 waves for u8,
-// 4 for u16, 3 for 4-byte values).
+// four-slot form's phase-1 loads and phase-2 keys take more registers and its queue more
+// LDS (5 waves for u8 and u16, 3 for 4-byte values).
 template <typename T, int TILE, int R>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 6 : (sizeof(T) == 1 ? 5 : sizeof(T) == 2 ? 4 : 3), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 6 : (sizeof(T) == 4 ? 3 : 5), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
                                                        uint64_t* deferred, Drift* drift,
                                                        const uint64_t* old2) {
@@ -933,10 +933,11 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 6
     __shared__ Vox8 s_v[R * TILE];
     __shared__ uint16_t s_idx[R * TILE];
     __shared__ uint8_t s_m[R * TILE];
-    // the block's key set (keyset_insert): half as many slots as the block has bricks,
-    // enough for the few distinct keys of a dense region; a block of all-distinct keys
-    // fills it and emits the rest unmerged
-    constexpr uint32_t kSetB = R * TILE / 2;
+    // the block's key set (keyset_insert): a quarter as many slots as the block has
+    // bricks, enough for the few distinct keys of a dense region (a block of all-distinct
+    // keys fills it and emits the rest unmerged), and small enough that five blocks of
+    // 256 lanes with u16 values share a CU's LDS
+    constexpr uint32_t kSetB = R * TILE / 4;
     __shared__ uint64_t s_kh[kSetB];
     __shared__ uint32_t s_kc[kSetB];
     __shared__ uint16_t s_kq[kSetB];  // the leader's queue entry (its mask and values in LDS)
@@ -1704,15 +1705,20 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
                 ec[i] = ecp[i];
             }
         }
-        // a child result equal to the old child is untouched: a completely written brick is
-        // rebuilt even when no value changed (its probe then finds the old node)
-        const bool same_old = eq != kEmpty && !bid_is_leaf(eq) && bid_index(eq) < P.capacity && ell == 0;
+        // A completely written brick is rebuilt even when no value changed (its probe then
+        // finds the old node); the reference does not push such a brick, so at height 2 a
+        // branch result equal to the old child is untouched.  A leaf result (an all-same
+        // brick) is pushed by the reference whether or not it changed (voxtree.rs:888-894),
+        // and above height 2 an equal result only follows from such pushes: both stay
+        // touched.
+        const bool same_old =
+            L.h == 2 && eq != kEmpty && !bid_is_leaf(eq) && bid_index(eq) < P.capacity && ell == 0;
         uint32_t touched = 0;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const bool t = ch[i] != kInvalid;
             ch[i] = resolve(ch[i], cand_id_prev);
-            touched |= (t && !(same_old && ch[i] == ec[i]) ? 1u : 0u) << i;
+            touched |= (t && !(same_old && !bid_is_leaf(ch[i]) && ch[i] == ec[i]) ? 1u : 0u) << i;
         }
         uint64_t out = kInvalid;
         bool cand = false, collapsed = false, h5 = false;
