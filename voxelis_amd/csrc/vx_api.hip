@@ -136,10 +136,9 @@ struct vx_interner {
     uint64_t capacity = 0;
     uint64_t tslots = 0;
     uint64_t* children = nullptr;
-    uint16_t* tms = nullptr;
+    uint32_t* tmg = nullptr;  // (generation << 16) | types/mask
     void* values = nullptr;
     uint32_t* refs = nullptr;
-    uint16_t* gens = nullptr;
     uint32_t* free_stack = nullptr;
     uint64_t* tab = nullptr;
     uint64_t* single = nullptr;  // u8/u16: the one-leaf key index (8 slots x every value)
@@ -185,10 +184,9 @@ template <typename T>
 Pool<T> pool_of(vx_interner* h) {
     Pool<T> P;
     P.children = h->children;
-    P.tms = h->tms;
+    P.tmg = h->tmg;
     P.values = (T*)h->values;
     P.refs = h->refs;
-    P.gens = h->gens;
     P.free_stack = h->free_stack;
     P.tab = h->tab;
     P.single = h->single;
@@ -924,9 +922,9 @@ int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_f
     uint64_t nv = direct ? 1ull << (8 * value_size) : 1;
     h->lslots = direct ? 0 : ts;
     uint64_t nnew = direct ? nv : std::min<uint64_t>(cap, ts);
-    bool ok = dmalloc(&h->children, cap * 64) && dmalloc(&h->tms, cap * 2) &&
+    bool ok = dmalloc(&h->children, cap * 64) && dmalloc(&h->tmg, cap * 4) &&
               dmalloc(&h->values, cap * value_size) && dmalloc(&h->refs, cap * 4) &&
-              dmalloc(&h->gens, cap * 2) && dmalloc(&h->free_stack, cap * 4) &&
+              dmalloc(&h->free_stack, cap * 4) &&
               dmalloc(&h->tab, ts * 8) && dmalloc(&h->leaf_map, nv * 4) &&
               dmalloc(&h->leaf_need, nv) && dmalloc(&h->new_leaves, nnew * 4) &&
               (direct || (dmalloc(&h->ltab, h->lslots * 8) && dmalloc(&h->need_list, h->lslots * 4))) &&
@@ -941,9 +939,9 @@ int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_f
     // PoolAllocatorLite::new zeroes every column (pool_allocator_lite.rs:29-81);
     // the EMPTY branch is index 0 with everything zero (interner/mod.rs:83-101).
     bool z = hipMemset(h->children, 0, cap * 64) == hipSuccess &&
-             hipMemset(h->tms, 0, cap * 2) == hipSuccess &&
+             hipMemset(h->tmg, 0, cap * 4) == hipSuccess &&
              hipMemset(h->values, 0, cap * value_size) == hipSuccess &&
-             hipMemset(h->refs, 0, cap * 4) == hipSuccess && hipMemset(h->gens, 0, cap * 2) == hipSuccess &&
+             hipMemset(h->refs, 0, cap * 4) == hipSuccess &&
              hipMemset(h->tab, 0, ts * 8) == hipSuccess && hipMemset(h->leaf_map, 0, nv * 4) == hipSuccess &&
              hipMemset(h->leaf_need, 0, nv) == hipSuccess &&
              (direct || hipMemset(h->ltab, 0, h->lslots * 8) == hipSuccess) &&
@@ -965,7 +963,7 @@ int vx_interner_create(uint32_t value_size, uint64_t budget_bytes, double load_f
 int vx_interner_destroy(vx_interner* h) {
     if (!h) return VX_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* ptrs[] = {h->children, h->tms, h->values, h->refs, h->gens, h->free_stack,
+    void* ptrs[] = {h->children, h->tmg, h->values, h->refs, h->free_stack,
                     h->tab, h->leaf_map, h->leaf_need, h->new_leaves, h->ctl, h->leaf_ids,
                     h->ltab, h->need_list, h->single};
     for (void* p : ptrs)

@@ -34,6 +34,11 @@ constexpr uint64_t kSlotTomb = 1ull;
 
 __host__ __device__ inline bool bid_is_leaf(uint64_t id) { return (id >> 63) != 0; }
 __host__ __device__ inline uint32_t bid_index(uint64_t id) { return (uint32_t)id; }
+// A node's types/mask word and its generation share one 32-bit column entry, so a free or
+// a commit touches one sector for both.
+__host__ __device__ inline uint32_t tmg_tm(uint32_t w) { return w & 0xFFFFu; }
+__host__ __device__ inline uint32_t tmg_gen(uint32_t w) { return w >> 16; }
+__host__ __device__ inline uint32_t tmg_make(uint32_t tm, uint32_t gen) { return (gen << 16) | (tm & 0xFFFFu); }
 __host__ __device__ inline uint32_t bid_gen(uint64_t id) { return (uint32_t)(id >> 32) & 0x7FFFu; }
 __host__ __device__ inline uint32_t bid_tm(uint64_t id) { return (uint32_t)(id >> 47) & 0xFFFFu; }
 __host__ __device__ inline uint32_t bid_types(uint64_t id) { return (uint32_t)(id >> 55) & 0xFFu; }
@@ -180,10 +185,9 @@ struct Drift {
 template <typename T>
 struct Pool {
     uint64_t* children;  // capacity * 8
-    uint16_t* tms;       // (types << 8) | mask of branch nodes, 0 for leaves
+    uint32_t* tmg;       // (generation << 16) | (types << 8) | mask; types/mask 0 for leaves and free slots
     T* values;
     uint32_t* refs;
-    uint16_t* gens;
     uint32_t* free_stack;
     uint64_t* tab;
     uint64_t tmask;

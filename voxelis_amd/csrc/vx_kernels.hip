@@ -316,7 +316,7 @@ __device__ inline void leaf_remove(const Pool<T>& P, uint32_t v) {
 template <typename T>
 __device__ inline uint64_t leaf_id_of(const Pool<T>& P, uint32_t v) {
     uint32_t idx = leaf_index(P, v);
-    return make_leaf(idx, P.gens[idx]);
+    return make_leaf(idx, tmg_gen(P.tmg[idx]));
 }
 
 template <typename T>
@@ -563,7 +563,7 @@ __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_
             for (uint32_t v = (blockIdx.x - nprep - ndesc) * blockDim.x + threadIdx.x; v < 65536u;
                  v += (gridDim.x - nprep - ndesc) * blockDim.x) {
                 const uint32_t li = P.leaf_map[v];
-                P.leaf_ids[v] = li ? make_leaf(li, P.gens[li]) : kEmpty;
+                P.leaf_ids[v] = li ? make_leaf(li, tmg_gen(P.tmg[li])) : kEmpty;
             }
         return;
     }
@@ -578,7 +578,7 @@ __global__ void k_prepare(Pool<T> P, ChunkDev* chunks, uint32_t n, const uint64_
     if (sizeof(T) == 1 && blockIdx.x == 0)
         for (uint32_t v = threadIdx.x; v < 256; v += blockDim.x) {
             uint32_t li = P.leaf_map[v];
-            P.ctl->leaf_ids8[v] = li ? make_leaf(li, P.gens[li]) : kEmpty;
+            P.ctl->leaf_ids8[v] = li ? make_leaf(li, tmg_gen(P.tmg[li])) : kEmpty;
         }
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (uint64_t)nprep * blockDim.x) {
         {
@@ -636,7 +636,7 @@ __device__ inline uint32_t create_requested_leaves(const Pool<T>& P, uint32_t nv
 #pragma unroll
             for (int j = 0; j < 8; j++) ch[j] = kEmpty;
             P.values[idx] = (T)v;
-            P.tms[idx] = 0;
+            P.tmg[idx] &= 0xFFFF0000u;  // a leaf: no types/mask, the generation kept
             P.refs[idx] = 0;
             if constexpr (sizeof(T) == 4) P.ltab[sl] = ((uint64_t)v << 32) | idx;
             else P.leaf_map[v] = idx;
@@ -674,7 +674,7 @@ __global__ void k_leaf_alloc(Pool<T> P, ChunkDev* chunks, uint32_t n, uint32_t n
             continue;
         }
         if (idx == 0) continue;  // allocation failed (OOM flagged)
-        C.init = make_leaf(idx, P.gens[idx]);
+        C.init = make_leaf(idx, tmg_gen(P.tmg[idx]));
         if (C.has_patches && !P.dry) {
             ref_delta_logged(P, c, idx, 1, ST_H3);
             atomicAdd((unsigned long long*)stat(P.ctl, ST_H3), 1ull);
@@ -787,7 +787,7 @@ template <typename T>
 __device__ inline uint64_t leaf_lookup(const Pool<T>& P, const uint64_t* leaf_ids, uint32_t v) {
     if (leaf_ids) return leaf_ids[v];
     uint32_t li = leaf_index(P, v);
-    return li ? make_leaf(li, P.gens[li]) : kEmpty;
+    return li ? make_leaf(li, tmg_gen(P.tmg[li])) : kEmpty;
 }
 
 template <typename T>
@@ -825,7 +825,7 @@ __device__ inline int brick_work(const Pool<T>& P, const ChunkDev& C, uint32_t m
                 base = make_leaf(0, 0);
             }
         } else {
-            base = make_leaf(fi, P.gens[fi]);
+            base = make_leaf(fi, tmg_gen(P.tmg[fi]));
         }
     }
     bool split = bid_is_leaf(base);
@@ -1209,9 +1209,11 @@ __global__ void k_brick_retry(Pool<T> P, const ChunkDev* chunks, LevelInfo L, ui
 // ------------------------------------------------------------------ interning
 
 template <typename T>
-__device__ inline bool key_matches_node(const Pool<T>& P, uint32_t idx, const uint64_t key[9]) {
-    // the node's 64-byte children row and its types/mask word in one round of loads
-    const uint32_t tm = P.tms[idx];
+__device__ inline bool key_matches_node(const Pool<T>& P, uint32_t idx, const uint64_t key[9], uint32_t* gen) {
+    // the node's 64-byte children row and its types/mask + generation word in one round
+    const uint32_t tg = P.tmg[idx];
+    const uint32_t tm = tmg_tm(tg);
+    *gen = tmg_gen(tg);
     const ulonglong2* c = (const ulonglong2*)(P.children + 8ull * idx);
     ulonglong2 w[4];
 #pragma unroll
@@ -1241,8 +1243,8 @@ __device__ inline int match_slot(const Pool<T>& P, uint32_t level, uint64_t e, u
         *result = lv == level ? res_pending(slot_payload(e)) : P.levels[lv].id[slot_payload(e)];
         return lv == level ? 1 : 0;
     }
-    const uint32_t gen = P.gens[slot_payload(e)];  // loaded with the node's key
-    if (!key_matches_node(P, slot_payload(e), key)) return -1;
+    uint32_t gen;
+    if (!key_matches_node(P, slot_payload(e), key, &gen)) return -1;
     *result = make_branch(slot_payload(e), gen, tm);
     return 0;
 }
@@ -1537,7 +1539,7 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
             } else {
                 bool ok;
                 uint32_t idx = alloc_at(P, ka, &ok, ft0, next0);
-                result = ok ? make_branch(idx, P.gens[idx], (uint32_t)key[8]) : kInvalid;
+                result = ok ? make_branch(idx, tmg_gen(P.tmg[idx]), (uint32_t)key[8]) : kInvalid;
                 if (ok) P.values[idx] = (T)lod;
                 P.owners[ka - own_base] = ((uint64_t)L.h << 32) | k;  // the commit's list
             }
@@ -1615,7 +1617,7 @@ __global__ __launch_bounds__(TILE) void k_commit(Pool<T> P) {
         uint64_t* dst = P.children + 8ull * idx;
 #pragma unroll
         for (int j = 0; j < 8; j++) dst[j] = key[j];
-        P.tms[idx] = (uint16_t)key[8];
+        P.tmg[idx] = tmg_make((uint32_t)key[8], bid_gen(id));
         if (ts & kTslotSingle) P.single[tslot_index(ts)] = id;
         else P.tab[tslot_index(ts)] = slot_make(slot_tag(K.hash[k]), false, 0, idx);
         for (int j = 0; j < 8; j++) {
@@ -1972,8 +1974,8 @@ __device__ inline void leaf_cleanup_body(const Pool<T>& P) {
                 freed = true;
                 leaf_remove(P, (uint32_t)P.values[idx]);
                 P.values[idx] = 0;
-                uint32_t g = P.gens[idx] + 1u;
-                P.gens[idx] = (uint16_t)(g >= kMaxGeneration ? 0u : g);
+                const uint32_t g = tmg_gen(P.tmg[idx]) + 1u;
+                P.tmg[idx] = tmg_make(0u, g >= kMaxGeneration ? 0u : g);
                 uint32_t pos = atomicAdd(&P.ctl->free_top, 1u);
                 if (pos < P.capacity) P.free_stack[pos] = idx;
                 else atomicOr(&P.ctl->err, kErrPoison);
@@ -2052,8 +2054,9 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
             // word (0 marks a free or leaf slot for the table rebuild), the pattern's
             // table slot and the free stack.  The count is already 0, the value and the
             // children are rewritten when the slot is allocated again.
-            const uint32_t tm = P.tms[idx];
-            const uint32_t g = P.gens[idx] + 1u;
+            const uint32_t w = P.tmg[idx];
+            const uint32_t tm = tmg_tm(w);
+            const uint32_t g = tmg_gen(w) + 1u;
             uint64_t ch[8];
             const uint64_t* src = P.children + 8ull * idx;
 #pragma unroll
@@ -2093,12 +2096,11 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
                         s = (s + 1) & P.tmask;
                     }
                 }
-                P.tms[idx] = 0;
 #pragma unroll
                 for (int j = 0; j < 8; j++)
                     if (((gmask >> j) & 1u) && old[j] == 1u) s_next[atomicAdd(&s_nnext, 1u)] = bid_index(ch[j]);
             }
-            P.gens[idx] = (uint16_t)(g >= kMaxGeneration ? 0u : g);
+            P.tmg[idx] = tmg_make(0u, g >= kMaxGeneration ? 0u : g);  // no types/mask: free
         }
         const uint64_t fb = __ballot(freed);
         const uint64_t bl = __ballot(freed && is_leaf), bb = __ballot(freed && !is_leaf), bt = __ballot(tomb);
@@ -2209,7 +2211,7 @@ __global__ void k_tab_reinsert(Pool<T> P) {
     if (blockIdx.x == 0 && threadIdx.x < kStatStripes) P.ctl->st[threadIdx.x][ST_TOMB] = 0;
     const uint32_t top = P.ctl->next_index;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < top; i += gridDim.x * blockDim.x) {
-        const uint32_t tm = P.tms[i];
+        const uint32_t tm = tmg_tm(P.tmg[i]);
         if (i == 0 || tm == 0) continue;
         uint64_t ch[8];
         const uint64_t* src = P.children + 8ull * i;
@@ -2242,7 +2244,7 @@ __global__ void k_ltab_reinsert(Pool<T> P) {
     if (!(P.ctl->rebuild & 2u)) return;
     const uint32_t top = P.ctl->next_index;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < top; i += gridDim.x * blockDim.x)
-        if (i != 0 && P.tms[i] == 0 && P.refs[i] != 0) leaf_publish(P, (uint32_t)P.values[i], i);
+        if (i != 0 && tmg_tm(P.tmg[i]) == 0 && P.refs[i] != 0) leaf_publish(P, (uint32_t)P.values[i], i);
 }
 
 // ------------------------------------------------------------------ single-voxel set
@@ -2388,7 +2390,7 @@ __global__ __launch_bounds__(TILE) void k_set_level(
                 if (v != 0) {
                     // creation failed: the apply aborts; a dry run counts a placeholder
                     uint32_t li = leaf_index(P, v);
-                    cur = li ? make_leaf(li, P.gens[li]) : (P.dry ? make_leaf(P.capacity + v, 0) : kInvalid);
+                    cur = li ? make_leaf(li, tmg_gen(P.tmg[li])) : (P.dry ? make_leaf(P.capacity + v, 0) : kInvalid);
                 } else {
                     cur = kEmpty;
                 }
@@ -2500,7 +2502,7 @@ __global__ void k_fill_finish(Pool<T> P, uint32_t n, uint64_t* roots, const uint
                 uint32_t idx = leaf_index(P, v);
                 if (idx != 0) {
                     atomicAdd(&P.refs[idx], 1u);
-                    roots[c] = make_leaf(idx, P.gens[idx]);
+                    roots[c] = make_leaf(idx, tmg_gen(P.tmg[idx]));
                 }
             }
         }

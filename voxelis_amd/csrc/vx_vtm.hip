@@ -129,7 +129,7 @@ __global__ void k_vtm_count(Pool<T> P, uint64_t N, uint64_t* leaf, uint64_t* bra
     VX_LOOP(i, N + 1) {
         if (i <= N) {
             const bool live = i > 0 && i < N && P.refs[i] > 0;
-            const bool is_leaf = live && P.tms[i] == 0;
+            const bool is_leaf = live && tmg_tm(P.tmg[i]) == 0;
             leaf[i] = is_leaf ? 1 : 0;
             branch[i] = live && !is_leaf ? 1 : 0;
         }
@@ -157,7 +157,7 @@ __global__ void k_vtm_sizes(Pool<T> P, uint64_t N, const uint32_t* newid, uint64
         if (i <= N) {
             uint64_t sl = 0, sb = 0;
             if (i < N && newid[i]) {
-                if (P.tms[i] == 0) {
+                if (tmg_tm(P.tmg[i]) == 0) {
                     sl = varint_len(newid[i]) + sizeof(T);
                 } else {
                     sb = varint_len(newid[i]) + 1 + sizeof(T);
@@ -185,14 +185,14 @@ __global__ void k_vtm_write(Pool<T> P, uint64_t N, const uint32_t* newid, const 
     const uint64_t LB = off_leaf[N];
     VX_LOOP(i, N) {
         if (i < N && newid[i]) {
-            if (P.tms[i] == 0) {
+            if (tmg_tm(P.tmg[i]) == 0) {
                 uint8_t* d = out + 4 + off_leaf[i];
                 uint32_t k = put_varint(d, newid[i]);
                 put_be<T>(d + k, (uint32_t)P.values[i]);
             } else {
                 uint8_t* d = out + 8 + LB + off_branch[i];
                 uint32_t k = put_varint(d, newid[i]);
-                d[k++] = (uint8_t)(P.tms[i] & 0xFFu);
+                d[k++] = (uint8_t)(P.tmg[i] & 0xFFu);
                 const uint64_t* ch = P.children + 8ull * i;
                 for (int j = 0; j < 8; j++)
                     if (ch[j] != kEmpty) k += put_varint(d + k, newid[bid_index(ch[j])]);
@@ -220,7 +220,7 @@ template <typename T>
 __global__ void k_vtm_link(Pool<T> P, uint64_t N) {
     VX_LOOP(i, N) {
         if (i >= 1 && i < N) {
-            const uint32_t tm = P.tms[i];
+            const uint32_t tm = tmg_tm(P.tmg[i]);
             if (tm == 0) {
                 leaf_publish(P, (uint32_t)P.values[i], (uint32_t)i);
             } else {
@@ -461,7 +461,7 @@ int deserialize_impl(vx_interner* h, const uint8_t* data, size_t size, size_t nc
     Reader r{data, size};
     const uint32_t L = r.u32();
     std::vector<T> val(1);
-    std::vector<uint16_t> tm(1, 0);
+    std::vector<uint32_t> tm(1, 0);  // generation 0
     val.reserve((size_t)L + 1);
     for (uint32_t i = 0; i < L && r.ok; i++) {
         if (r.varint() != i + 1) return VX_E_INVALID_ARG;
@@ -488,7 +488,7 @@ int deserialize_impl(vx_interner* h, const uint8_t* data, size_t size, size_t nc
             kids.push_back(id);
         }
         val.push_back((T)r.be(sizeof(T)));
-        tm.push_back((uint16_t)((types << 8) | mask));
+        tm.push_back((uint32_t)((types << 8) | mask));
     }
     const uint32_t nch = r.u32();
     if (!r.ok) return VX_E_INVALID_ARG;
@@ -513,10 +513,9 @@ int deserialize_impl(vx_interner* h, const uint8_t* data, size_t size, size_t nc
         for (int k = 0; k < 8; k++) ch[8ull * (L + 1 + j) + k] = id_of(kids[8ull * j + k]);
     // node columns, then the links on the device
     HIPCHK(hipMemcpyAsync(h->values, val.data(), N * sizeof(T), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(h->tms, tm.data(), N * 2, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(h->tmg, tm.data(), N * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(h->children, ch.data(), N * 64, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemsetAsync(h->refs, 0, N * 4, s));
-    HIPCHK(hipMemsetAsync(h->gens, 0, N * 2, s));
     Pool<T> P = pool_of<T>(h);
     hipLaunchKernelGGL(k_vtm_link<T>, dim3(grid_for(N)), dim3(kBlock), 0, s, P, N);
     if (nch) {
