@@ -140,3 +140,25 @@ def test_file_wrapper(vx, tmp_path):
     p.write_bytes(bytes(comp))
     with pytest.raises(NotImplementedError):
         vtm.import_model_from_vtm(p, 64 * MB)
+
+
+@pytest.mark.timeout(600)
+def test_export_with_more_trees_and_nodes_than_a_scan_tile(vx):
+    """2304 trees and more than 2048 nodes: every one of the export's five scans takes a
+    second level of tile sums (2048 elements per tile), the most scratch arrays an export
+    of this size allocates."""
+    torch, api, vtm, W = vx
+    it, f, it_o, trees, pos = _world(api, W, 2, grid=48, seed=9)
+    assert len(trees) > 2048 and sum(it.counts()) > 2048
+    data = f.serialize(pos)
+    parsed = V.deserialize(data, 2)
+    leaves, branches = V.live_nodes([t.root for t in trees], it_o.children)
+    assert (len(parsed[0]), len(parsed[1])) == (len(leaves), len(branches))
+    it2 = O.Interner(2, 64 * MB)
+    roots, pos2 = V.rebuild(parsed, it2.get_or_create_leaf, it2.get_or_create_branch)
+    assert np.array_equal(np.array(pos2), pos)
+    g = f.canonical_hashes()
+    for i in range(0, len(roots), 97):
+        t2 = O.Tree(5)
+        t2.root = roots[i]
+        assert t2.canonical_hash(it2) == trees[i].canonical_hash(it_o) == g[i], i

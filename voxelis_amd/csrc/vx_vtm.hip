@@ -77,15 +77,18 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_add(uint64_t* data, uint64_
 
 // Device scratch of one export: carved from a fixed region (the no-allocation entry
 // point) or, without a region, hipMalloc'd and freed at the end of the call.
+// An export takes 7 arrays plus up to 3 tile-sum arrays for each of its 5 scans (a scan
+// recurses once per factor of 2048 elements): 22 at most below 2^33 elements.
 struct Arena {
+    static constexpr int kHeld = 64;
     uint8_t* base = nullptr;  // nullptr: allocate
     size_t cap = 0, used = 0;
-    void* held[16] = {nullptr};
+    void* held[kHeld] = {nullptr};
     int nheld = 0;
     bool take(size_t bytes, void** out) {
         *out = nullptr;
         if (!base) {
-            if (nheld == 16 || hipMalloc(out, bytes ? bytes : 1) != hipSuccess) return false;
+            if (nheld == kHeld || hipMalloc(out, bytes ? bytes : 1) != hipSuccess) return false;
             held[nheld++] = *out;
             return true;
         }
