@@ -2004,6 +2004,11 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
     __shared__ uint32_t s_next[kGcStage];
     __shared__ uint32_t s_nnext, s_nfree, s_bnext, s_bfree;
     __shared__ unsigned long long s_leaves, s_branches, s_tombs, s_recycled;
+    uint64_t n = P.ctl->gc_count[round];
+    if (n > front_cap) n = front_cap;
+    // a block past the frontier has nothing to do (the rounds near the leaves hold few
+    // nodes); the first round also releases new leaves and orphans with every block
+    if (!leaf_cleanup && (uint64_t)blockIdx.x * blockDim.x >= n) return;
     agg_init(akey, acnt);
     if (threadIdx.x == 0) {
         s_nnext = s_nfree = 0;
@@ -2013,8 +2018,6 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
     // new leaves nothing references (their trees failed) are released in the first
     // round; no node that this GC visits references them
     if (leaf_cleanup) leaf_cleanup_body(P);
-    uint64_t n = P.ctl->gc_count[round];
-    if (n > front_cap) n = front_cap;
     uint32_t* next_count = &P.ctl->gc_count[round + 1];
     // New branch nodes that nothing references go to the next round: keys of a tree that
     // failed in the brick pass (the leaf-brick panic) are interned like every key, and
