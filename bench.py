@@ -77,13 +77,14 @@ def parse():
                     help="serve: requests logged (with exact dry-run counts) to train the predictor")
     ap.add_argument("--config", default="uneven",
                     choices=["uneven", "terrain", "plain", "gets", "set", "vtm", "serve", "calib", "tune",
-                             "predictor"],
+                             "predictor", "edits"],
                     help="uneven: the headline (4096 u16 trees of depth 5/6/7, dense or sparse); terrain: "
                          "64x64 Perlin-dune u8 trees; plain: one 32^3 tree; gets: batched reads; set: "
                          "single-voxel set/remove, one voxel per tree per call")
     ap.add_argument("--trees", type=int, default=4096)
     ap.add_argument("--pairs", type=int, default=1 << 20,
                     help="predictor: (features, exact growth) pairs to log before training")
+    ap.add_argument("--spheres", type=int, default=64, help="edits: sphere edits per apply")
     ap.add_argument("--admit-steps", type=int, default=20,
                     help="uneven: applies run under predictor-driven admission (0: none)")
     ap.add_argument("--queries", type=int, default=1 << 24)
@@ -107,7 +108,7 @@ def parse():
     if a.load_factor is None:
         a.load_factor = float(a.pick["load_factor"]) if a.pick else 0.5
     if a.budget_mib is None:
-        big = a.config in ("uneven", "predictor") or (a.config == "tune" and a.tune_workload == "uneven")
+        big = a.config in ("uneven", "predictor", "edits") or (a.config == "tune" and a.tune_workload == "uneven")
         a.budget_mib = 16384 if big else 2048
     return a
 
@@ -698,7 +699,7 @@ def main():
         return bench_headline(a, a.config)
     return {"plain": bench_plain, "gets": bench_gets, "set": bench_set, "vtm": bench_vtm,
             "serve": bench_serve, "calib": bench_calib, "tune": bench_tune,
-            "predictor": bench_predictor}[a.config](a)
+            "predictor": bench_predictor, "edits": bench_edits}[a.config](a)
 
 
 # ---------------------------------------------------------------- profiler summaries
@@ -1377,6 +1378,71 @@ def bench_calib(a):
     print(json.dumps({"metric": "CPU-baseline calibration: oracle vs published reference rates",
                       "cpu": _cpu_model(), "nproc": os.cpu_count(), "threads": 1,
                       "geomean_ratio": float(np.exp(np.mean(np.log(ratios)))), "rows": rows}))
+
+
+def bench_edits(a):
+    """Edit lists through vx_trees_apply_edits: the served stream's sphere edits (radius
+    uniform 2-40 voxels, random centres, u16 ids, a quarter carving) on the 16384-tree
+    depth-5 u16 world, --spheres per apply.  The device radix sort of the edits by (tree,
+    Morton path), the scatter into Batch buffers and the apply are timed per kernel (HIP
+    events on the library's stream)."""
+    import torch
+    from voxelis_amd import api, serve
+    from voxelis_amd import workloads as W
+    world = _serve_world(api, serve, W, (32, 16, 32), a.budget_mib << 20)
+    f, n = world.forest, world.n
+    rng = np.random.default_rng(a.seed + 11)
+    size = np.array(world.shape) * world.side
+    batches = []
+    for k in range(a.warmup + a.steps):
+        per_tree = {}
+        for _ in range(a.spheres):
+            c = rng.integers(0, size)
+            r = float(rng.uniform(2, 40))
+            v = 0 if rng.random() < 0.25 else int(rng.integers(1, 65536))
+            for t, (loc, vals) in serve.sphere_edits(world, c, r, v).items():
+                per_tree.setdefault(t, []).append((loc, vals))
+        counts = np.zeros(n, dtype=np.int64)
+        xyz, val = [], []
+        for t in range(n):
+            for loc, vals in per_tree.get(t, ()):
+                xyz.append(loc)
+                val.append(vals)
+                counts[t] += len(vals)
+        offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+        batches.append((offs, api.to_device(np.concatenate(xyz).astype(np.int32)),
+                        api.to_device(np.concatenate(val).astype(np.uint16)), int(offs[-1]), len(per_tree)))
+    for k in range(a.warmup):
+        f.apply_edits(batches[k][0], batches[k][1], batches[k][2], sync=False)
+    world.it.sync()
+    world.it.profile(True)
+    world.it.profile_read()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.warmup, a.warmup + a.steps):
+        f.apply_edits(batches[k][0], batches[k][1], batches[k][2], sync=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = world.it.profile_read()
+    world.it.profile(False)
+    world.it.sync()
+    E = sum(b[3] for b in batches[a.warmup:])
+    kern = {k: round(v[0] / a.steps, 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    sort_ms = sum(prof[k][0] for k in ("k_sort_hist", "k_sort_scan", "k_sort_scatter") if k in prof) / a.steps
+    passes = prof.get("k_sort_scatter", (0, 0))[1] / a.steps
+    res = {"edits_per_apply": E / a.steps, "trees_per_apply": float(np.mean([b[4] for b in batches[a.warmup:]])),
+           "edits_per_s": E / wall, "ms_per_apply": 1e3 * wall / a.steps,
+           "sort_ms_per_apply": sort_ms, "sort_passes": passes,
+           "sort_keys_per_s": (E / a.steps) / (sort_ms / 1e3) if sort_ms else None,
+           "sort_bytes_per_key_per_pass": 2 * (8 + 4) + 8,
+           "kernels_ms_per_apply": kern,
+           "note": "each apply runs with profiling events around every kernel (launched one by one)"}
+    print(f"[edits] {E / a.steps / 1e6:.2f} M edits per apply over {res['trees_per_apply']:.0f} trees: "
+          f"{res['edits_per_s'] / 1e6:.1f} M edits/s, {res['ms_per_apply']:.3f} ms per apply; sort {sort_ms:.3f} ms "
+          f"({passes:.0f} passes, {res['sort_keys_per_s'] / 1e9:.2f} G keys/s)", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "edit lists: sorted on the device and applied", "config": {
+        "workload": f"{a.spheres} sphere edits per apply (radius 2-40, u16 ids, a quarter carving) on the "
+                    f"16384-tree depth-5 u16 world"}, "results": res}))
 
 
 def _serve_world(api, serve, W, shape, budget):

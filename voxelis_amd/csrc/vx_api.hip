@@ -95,13 +95,15 @@ enum KernelId : uint32_t {
     KID_BEGIN = 0, KID_PREPARE, KID_BRICK_SCAN, KID_LEAF_ALLOC, KID_BRICK_BUILD, KID_PROBE_BRICKS,
     KID_UNUSED, KID_LEVEL_BUILD, KID_PROBE_UPPER, KID_COMMIT, KID_FINALIZE,
     KID_END_ALLOC, KID_LEAF_CLEANUP, KID_GC, KID_GET, KID_CANON, KID_FILL, KID_ROLLBACK, KID_TO_VEC,
-    KID_SET_DESCEND, KID_SET_LEVEL, KID_GET_GROUPED, KID_COUNT
+    KID_SET_DESCEND, KID_SET_LEVEL, KID_GET_GROUPED, KID_EDIT_KEYS, KID_SORT_HIST, KID_SORT_SCAN,
+    KID_SORT_SCATTER, KID_EDIT_SCATTER, KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_begin", "k_prepare", "k_brick_retry", "k_leaf_alloc", "k_brick_build", "k_probe[bricks]",
     "", "k_level_build", "k_probe[upper]", "k_commit", "k_finish",
     "k_end_alloc", "k_leaf_cleanup", "k_gc_round", "k_get", "k_canon", "k_fill", "k_rollback", "k_to_vec",
-    "k_set_descend", "k_set_level", "k_get_grouped"};
+    "k_set_descend", "k_set_level", "k_get_grouped", "k_edit_keys", "k_sort_hist", "k_sort_scan",
+    "k_sort_scatter", "k_edit_scatter"};
 
 struct Prof {
     bool on = false;
@@ -748,23 +750,28 @@ int apply_edits_impl(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* d
     while ((1ull << tree_bits) <= n) tree_bits++;
     unsigned end_bit = kPathBits + tree_bits;
     if (E > 0) {
-        k_edit_keys<<<grid_for(E), kBlock, 0, s>>>(E, dfirst, (uint32_t)n, dtrees, xyz, k0, o0);
+        VX_LAUNCH(h, KID_EDIT_KEYS, k_edit_keys, grid_for(E), kBlock, E, dfirst, (uint32_t)n, dtrees, xyz, k0, o0);
         // stable LSD radix sort, 8 bits per pass, ping-ponging between the two buffers
         const unsigned G = (unsigned)std::min<uint64_t>(2048, (E + kSortBlock - 1) / kSortBlock);
         const uint64_t per = (E + G - 1) / G;
-        if (!h->edit_tmp.ensure((size_t)256 * G * 4)) return VX_E_HIP;
+        const uint32_t M = 256u * G;  // digit counts of a pass, digit-major
+        const uint32_t ntile = (M + kScanTileN - 1) / kScanTileN;
+        if (!h->edit_tmp.ensure((size_t)(M + ntile + 1) * 4)) return VX_E_HIP;
         uint32_t* hist = (uint32_t*)h->edit_tmp.ptr;
+        uint32_t* tile_off = hist + M;
         for (unsigned shift = 0; shift < end_bit; shift += 8) {
-            k_sort_hist<<<G, kSortBlock, 0, s>>>(k0, E, per, shift, hist);
-            k_sort_scan<<<1, kSortScanBlock, 0, s>>>(hist, 256u * G);
-            k_sort_scatter<<<G, kSortBlock, 0, s>>>(k0, o0, k1, o1, E, per, shift, hist);
+            VX_LAUNCH(h, KID_SORT_HIST, k_sort_hist, G, kSortBlock, k0, E, per, shift, hist);
+            VX_LAUNCH(h, KID_SORT_SCAN, k_sort_scan_tiles, ntile, kSortScanBlock, hist, M, tile_off);
+            VX_LAUNCH(h, KID_SORT_SCAN, k_sort_scan, 1, kSortScanBlock, tile_off, ntile);
+            VX_LAUNCH(h, KID_SORT_SCATTER, k_sort_scatter, G, kSortBlock, k0, o0, k1, o1, E, per, shift, hist,
+                      tile_off);
             std::swap(k0, k1);
             std::swap(o0, o1);
         }
         std::swap(k0, k1);  // the sorted pairs are in (k1, o1)
         std::swap(o0, o1);
-        k_edit_scatter<T><<<grid_for(E), kBlock, 0, s>>>(E, (uint32_t)n, dtrees, k1, o1, (const T*)values, mws,
-                                                         vws, false);
+        VX_LAUNCH(h, KID_EDIT_SCATTER, k_edit_scatter<T>, grid_for(E), kBlock, E, (uint32_t)n, dtrees, k1, o1,
+                  (const T*)values, mws, vws, false);
     }
     // per-tree failures recorded by k_edit_keys go in as pre-statuses
     k_edit_status<<<grid_for(n), kBlock, 0, s>>>((uint32_t)n, dtrees, dpre);
@@ -782,8 +789,8 @@ int apply_edits_impl(vx_interner* h, size_t n, uint64_t* roots, const uint8_t* d
     // the touched bricks are cleared again on every path, a failed apply included, so
     // the workspace stays zero for the next call
     if (E > 0)
-        k_edit_scatter<T><<<grid_for(E), kBlock, 0, s>>>(E, (uint32_t)n, dtrees, k1, o1, (const T*)values, mws,
-                                                         vws, true);
+        VX_LAUNCH(h, KID_EDIT_SCATTER, k_edit_scatter<T>, grid_for(E), kBlock, E, (uint32_t)n, dtrees, k1, o1,
+                  (const T*)values, mws, vws, true);
     if (rc) return rc;
     return hipGetLastError() == hipSuccess ? VX_OK : VX_E_HIP;
 }

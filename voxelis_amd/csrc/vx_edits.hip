@@ -123,6 +123,42 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const uint64_t* keys, 
     hist[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = c[threadIdx.x];
 }
 
+// Exclusive scan of the digit counts in tiles of kScanTileN: each tile in place (its own
+// prefix only), the tile's total in tot[tile].  The tile totals are scanned next by
+// k_sort_scan (one block: at most 128 tiles for 2048 sort blocks), and the scatter adds a
+// count's tile offset when it reads it.
+constexpr unsigned kSortScanPer = 4;
+constexpr unsigned kScanTileN = kSortScanBlock * kSortScanPer;
+__global__ __launch_bounds__(kSortScanBlock) void k_sort_scan_tiles(uint32_t* hist, uint32_t m, uint32_t* tot) {
+    __shared__ uint32_t s_w[kSortScanBlock / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t base = blockIdx.x * kScanTileN + threadIdx.x * kSortScanPer;
+    uint32_t v[kSortScanPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kSortScanPer; i++) {
+        v[i] = base + i < m ? hist[base + i] : 0u;
+        sum += v[i];
+    }
+    uint32_t x = sum;  // inclusive scan of the lanes' sums
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t before = 0;  // the earlier wavefronts of the block
+    for (uint32_t w = 0; w < wave; w++) before += s_w[w];
+    uint32_t run = before + x - sum;
+#pragma unroll
+    for (uint32_t i = 0; i < kSortScanPer; i++) {
+        if (base + i < m) hist[base + i] = run;
+        run += v[i];
+    }
+    if (threadIdx.x == kSortScanBlock - 1) tot[blockIdx.x] = run;
+}
+
 // Exclusive scan of m counts in place, one block: each thread scans a contiguous segment.
 __global__ __launch_bounds__(kSortScanBlock) void k_sort_scan(uint32_t* hist, uint32_t m) {
     __shared__ uint32_t part[kSortScanBlock];
@@ -148,11 +184,12 @@ __global__ __launch_bounds__(kSortScanBlock) void k_sort_scan(uint32_t* hist, ui
 
 __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
                                                              uint32_t* vout, uint64_t n, uint64_t per, unsigned shift,
-                                                             const uint32_t* offs) {
+                                                             const uint32_t* offs, const uint32_t* tile_off) {
     __shared__ uint32_t base[256];             // next output position of each digit (this block)
     __shared__ uint32_t wcnt[kSortBlock / 64][256];  // per wavefront: its digit counts, then offsets
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
-    base[tid] = offs[(uint64_t)tid * gridDim.x + blockIdx.x];
+    const uint64_t at = (uint64_t)tid * gridDim.x + blockIdx.x;  // digit-major count of this block
+    base[tid] = offs[at] + tile_off[at / kScanTileN];
     const uint64_t b0 = (uint64_t)blockIdx.x * per;
     const uint64_t b1 = min(n, b0 + per);
     const uint64_t lt = (1ull << lane) - 1ull;
