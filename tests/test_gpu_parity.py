@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from gpu_helpers import assert_same, full_reads_gpu, oracle_apply_dense
+from gpu_helpers import assert_same, full_reads_gpu, full_reads_oracle, oracle_apply_dense
 
 pytestmark = pytest.mark.gpu
 
@@ -225,6 +225,41 @@ def test_terrain_small_grid_and_alternation(vx):
         assert_same(it, f, it_o, trees_o, full_reads=(step < 2), what=f"step {step}")
     s = it.stats()
     assert s["h4_undercount"] == 0 and s["h5_collapse_cloned"] == 0
+
+
+def test_gc_follows_small_fronts_in_one_launch(vx):
+    """Replaced sparse paths are freed by blocks that follow their share of the roots
+    down the tree.  Alone in its apply, a depth-6 tree with 210 random voxels frees
+    1, 8, 59, 178, 206 and 209 nodes per height, so its block stages more than 512
+    freed nodes before one free-stack update; three trees of 419 to 3277 voxels in
+    another apply share one block, which passes more than 256 nodes of one height on
+    to the next launch.  Alternating values replace every path; counts, hashes and
+    reads must stay the oracle's."""
+    api, W = vx
+    spec = [(6, "sparse", 0.0008, 21), (6, "sparse", 0.0016, 22), (5, "sparse", 0.1, 23), (6, "sparse", 0.01, 24)]
+    batches = W.uneven_batches(spec)
+    it = api.VoxInterner(2, 128 * MB)
+    it_o = O.Interner(2, 128 * MB)
+    groups = [[0], [1, 2, 3]]
+    forests = [api.Forest(it, [spec[i][0] for i in g]) for g in groups]
+    trees_o = [O.Tree(d) for d, _, _, _ in spec]
+    for step in range(4):
+        vs = [v if step % 2 == 0 else np.where(v == 65535, 1, v + 1).astype(v.dtype) for _, v, _ in batches]
+        for f, g in zip(forests, groups):
+            st = f.apply([api.to_device(batches[i][0]) for i in g], [api.to_device(vs[i]) for i in g])
+            assert (st == 1).all(), st
+        for t_o, (m, _, _), v in zip(trees_o, batches, vs):
+            b = O.Batch(t_o.depth)
+            b.load_dense(m, v)
+            assert t_o.apply_batch(it_o, b)
+        for f, g in zip(forests, groups):
+            sub = [trees_o[i] for i in g]
+            g_hash = f.canonical_hashes()
+            for k, t_o in enumerate(sub):
+                assert g_hash[k] == t_o.canonical_hash(it_o), f"step {step} tree {g[k]}"
+                if step >= 2:
+                    assert np.array_equal(full_reads_gpu(f, k, t_o.depth), full_reads_oracle(it_o, t_o)), g[k]
+        assert it.counts() == it_o.counts(), f"step {step}: pattern counts differ"
 
 
 @pytest.mark.parametrize("brick_rounds", [0, 1])

@@ -101,25 +101,32 @@ __device__ inline void wave_ref_inc(uint32_t* refs, uint32_t idx) {
 }
 
 // refs[idx] -= 1 for every lane with idx != kNoIdx (aggregated like wave_ref_inc).
-// Returns true on exactly one lane per index whose count reached zero.
+// Returns true on exactly one lane per index whose count reached zero.  Every round's
+// atomic is issued before any result is read, so they are in flight together.
 __device__ inline bool wave_ref_dec_zero(uint32_t* refs, uint32_t idx) {
     bool pending = idx != kNoIdx;
-    bool owner = false;
+    uint32_t old[kMatchRounds], want[kMatchRounds];
+#pragma unroll
     for (int r = 0; r < kMatchRounds; r++) {
-        uint64_t act = __ballot(pending);
-        if (!act) return owner;
-        int leader = __ffsll((unsigned long long)act) - 1;
-        uint32_t li = __shfl(idx, leader);
-        bool same = pending && idx == li;
-        uint64_t sm = __ballot(same);
-        if ((int)__lane_id() == leader) {
-            uint32_t k = (uint32_t)__popcll(sm);
-            uint32_t old = atomicSub(&refs[li], k);
-            owner = old == k;
+        want[r] = 0;  // 0: this lane did not lead round r
+        old[r] = 0;
+        const uint64_t act = __ballot(pending);
+        if (act) {  // uniform over the wavefront
+            const int leader = __ffsll((unsigned long long)act) - 1;
+            const uint32_t li = __shfl(idx, leader);
+            const bool same = pending && idx == li;
+            const uint64_t sm = __ballot(same);
+            if ((int)__lane_id() == leader) {
+                want[r] = (uint32_t)__popcll(sm);
+                old[r] = atomicSub(&refs[li], want[r]);
+            }
+            if (same) pending = false;
         }
-        if (same) pending = false;
     }
-    if (pending) owner = atomicSub(&refs[idx], 1u) == 1u;
+    const uint32_t rest = pending ? atomicSub(&refs[idx], 1u) : 0u;
+    bool owner = pending && rest == 1u;
+#pragma unroll
+    for (int r = 0; r < kMatchRounds; r++) owner = owner || (want[r] != 0 && old[r] == want[r]);
     return owner;
 }
 
@@ -1597,29 +1604,36 @@ template <typename T, int TILE>
 __global__ __launch_bounds__(TILE) void k_commit(Pool<T> P) {
     __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
     __shared__ uint32_t s_owned;
+    // the levels' candidate arrays, from LDS (read once per owner, ahead of its loads)
+    __shared__ Cands s_lv[kLevels + 1];
     agg_init(akey, acnt);
     if (threadIdx.x == 0) s_owned = 0;
+    if (threadIdx.x < kLevels + 1) s_lv[threadIdx.x] = P.levels[threadIdx.x];
     __syncthreads();
     uint32_t owned = 0;
     const uint32_t n = P.ctl->alloc_count - P.ctl->own_base;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint64_t e = P.owners[i];
-        const Cands& K = P.levels[e >> 32];
+        const Cands& K = s_lv[e >> 32];
         const uint64_t k = (uint32_t)e;
+        // the owner's record in one round of loads: the slot and id decide whether it is
+        // written, the key and hash are what it writes
         const uint64_t ts = K.tslot[k];
         const uint64_t id = K.id[k];
-        if (ts == ~0ull || id == kInvalid) continue;  // its allocation failed
-        owned++;
-        uint32_t idx = bid_index(id);
         uint64_t key[9];
 #pragma unroll
         for (int j = 0; j < 9; j++) key[j] = key_word(K, k, j);
+        const uint64_t hk = K.hash[k];
+        asm volatile("" ::"v"(hk));  // (loaded with the rest, not after the slot test)
+        if (ts == ~0ull || id == kInvalid) continue;  // its allocation failed
+        owned++;
+        uint32_t idx = bid_index(id);
         uint64_t* dst = P.children + 8ull * idx;
 #pragma unroll
         for (int j = 0; j < 8; j++) dst[j] = key[j];
         P.tmg[idx] = tmg_make((uint32_t)key[8], bid_gen(id));
         if (ts & kTslotSingle) P.single[tslot_index(ts)] = id;
-        else P.tab[tslot_index(ts)] = slot_make(slot_tag(K.hash[k]), false, 0, idx);
+        else P.tab[tslot_index(ts)] = slot_make(slot_tag(hk), false, 0, idx);
         for (int j = 0; j < 8; j++) {
             if (key[j] == kEmpty) continue;
             uint32_t ci = bid_index(key[j]);
@@ -1995,20 +2009,141 @@ __device__ inline void leaf_cleanup_body(const Pool<T>& P) {
 // indices onto the free stack, children that reached zero onto the next queue) in
 // LDS and reserves room for them with one update of each shared counter: those
 // counters are serial points that every block of the launch updates.
-constexpr uint32_t kGcStage = 2048;  // children staged per pass (8 per lane of a 256-lane block)
+//
+// A small front (at most kGcLocalShare nodes per block of the launch) is split into
+// contiguous shares of at least kGcShareMin nodes, and each block follows its share's
+// subtrees itself: the children that reach zero in one pass are the block's next pass,
+// in the same launch, while they fit one pass (kGcLocal); more go to the next launch.
+// Only the block whose decrement took a count to zero holds that node, so no other
+// block has to see it, and a chain of small rounds (a few thousand replaced paths) costs
+// one launch instead of one per height.  A launch frees nodes of any depth at or below
+// its round's, so the D + 1 launches of a tree of depth D still reach every node.
+constexpr uint32_t kGcStage = 2048;      // children staged per pass (8 per lane of a 256-lane block)
+constexpr uint32_t kGcLocal = 256;       // the most nodes of one pass a block follows itself
+constexpr uint32_t kGcLocalShare = 64;   // a front up to this many nodes per block is followed
+constexpr uint32_t kGcShareMin = 16;     // ... in shares of at least this many nodes
+constexpr uint32_t kGcFreeStage = 768;   // freed nodes staged before one free-stack update
+
+// Frees node idx, whose count is zero: drops its pattern, marks the slot free and
+// releases its children.  Children decremented here that reach zero are staged in
+// s_next; children combined in the block's table (akey/acnt) are decremented when the
+// table is flushed.
+template <typename T>
+__device__ inline void gc_free(const Pool<T>& P, uint32_t idx, uint32_t* akey, uint32_t* acnt, uint32_t* s_next,
+                               uint32_t* s_nnext, bool* is_leaf, bool* tomb) {
+    // the node's columns in one round of loads (a leaf's children are empty).  A free
+    // writes only what a later reader needs: the generation, the types/mask word (0
+    // marks a free or leaf slot for the table rebuild), the pattern's table slot and the
+    // free stack.  The count is already 0, the value and the children are rewritten
+    // when the slot is allocated again.
+    const uint32_t w = P.tmg[idx];
+    const uint32_t tm = tmg_tm(w);
+    const uint32_t g = tmg_gen(w) + 1u;
+    uint64_t ch[8];
+    const uint64_t* src = P.children + 8ull * idx;
+#pragma unroll
+    for (int j = 0; j < 8; j++) ch[j] = src[j];
+    // Both loads complete before the leaf test: without this use of the children the
+    // compiler moves their load into the branch path, after the wait for the types/mask
+    // word (two round trips instead of one).
+    asm volatile("" ::"v"(w), "v"(ch[0]), "v"(ch[1]), "v"(ch[2]), "v"(ch[3]), "v"(ch[4]), "v"(ch[5]), "v"(ch[6]),
+                 "v"(ch[7]));
+    *is_leaf = tm == 0;
+    if (*is_leaf) {
+        leaf_remove(P, (uint32_t)P.values[idx]);
+    } else {
+        // children first: combined in LDS where the block's table has room, the rest
+        // decremented with independent atomics (issued under the lane mask, so they are
+        // in flight together while the pattern is removed below)
+        uint32_t gmask = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (ch[j] != kEmpty && !agg_add(akey, acnt, bid_index(ch[j]), 1u)) gmask |= 1u << j;
+        uint32_t old[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) old[j] = ((gmask >> j) & 1u) ? atomicSub(&P.refs[bid_index(ch[j])], 1u) : 0u;
+        // remove the pattern (mod.rs:518): a one-leaf key from its index (the leaf is
+        // still live: this node's reference is still being dropped), any other from the
+        // table, leaving a tombstone
+        if (P.single && tm_is_single(tm)) {
+            P.single[single_index(P, tm, ch)] = 0;
+        } else {
+            uint64_t h = branch_hash(ch, tm);
+            uint64_t want = slot_make(slot_tag(h), false, 0, idx);
+            uint64_t s = h & P.tmask;
+            for (uint64_t guard = 0; guard <= P.tmask; guard++) {
+                uint64_t e = P.tab[s];
+                if (e == want) {
+                    P.tab[s] = kSlotTomb;
+                    *tomb = true;
+                    break;
+                }
+                if (e == kSlotEmpty) break;
+                s = (s + 1) & P.tmask;
+            }
+        }
+        // A child reached zero when its old count was 1.  The 1 is hidden from the
+        // compiler here: a compare it can place next to each atomic makes it wait for
+        // each result before issuing the next (eight round trips in a row); compared
+        // after the pattern's removal, the eight atomics are in flight together.
+        uint32_t one = 1u;
+        asm volatile("" : "+v"(one));
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (((gmask >> j) & 1u) && old[j] == one) s_next[atomicAdd(s_nnext, 1u)] = bid_index(ch[j]);
+    }
+    P.tmg[idx] = tmg_make(0u, g >= kMaxGeneration ? 0u : g);  // no types/mask: free
+}
+
+// Applies the block's combined decrements (a 256-lane block: every lane takes
+// kAggSlots / 256 slots, whose atomics are in flight together); slots that reach zero
+// are staged in s_next.  reset: clear the slots for further use.
+template <typename T>
+__device__ inline void gc_agg_flush(const Pool<T>& P, uint32_t* akey, uint32_t* acnt, uint32_t* s_next,
+                                    uint32_t* s_nnext, bool reset) {
+    constexpr int kPer = kAggSlots / 256;
+    uint32_t key[kPer], cnt[kPer], old[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const uint32_t k = threadIdx.x + 256u * u;
+        key[u] = akey[k];
+        cnt[u] = acnt[k];
+        old[u] = key[u] != kNoIdx && cnt[u] ? atomicSub(&P.refs[key[u]], cnt[u]) : 0u;
+        if (reset && key[u] != kNoIdx) {
+            akey[k] = kNoIdx;
+            acnt[k] = 0;
+        }
+    }
+    // (the counts are hidden from the compiler so that it compares after every atomic
+    // was issued, not next to each one)
+    asm volatile("" : "+v"(cnt[0]));
+#pragma unroll
+    for (int u = 0; u < kPer; u++)
+        if (key[u] != kNoIdx && cnt[u] && old[u] == cnt[u]) s_next[atomicAdd(s_nnext, 1u)] = key[u];
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, const uint32_t* front, uint32_t* next,
                                                   bool decrement_self, uint64_t front_cap, bool leaf_cleanup) {
     __shared__ uint32_t akey[kAggSlots], acnt[kAggSlots];
     __shared__ uint32_t s_next[kGcStage];
+    __shared__ uint32_t s_cur[kGcLocal];
+    __shared__ uint32_t s_free[kGcFreeStage];
     __shared__ uint32_t s_nnext, s_nfree, s_bnext, s_bfree;
     __shared__ unsigned long long s_leaves, s_branches, s_tombs, s_recycled;
     uint64_t n = P.ctl->gc_count[round];
     if (n > front_cap) n = front_cap;
+    // a small front: the block's share [lo, hi), followed locally
+    const bool local = n <= (uint64_t)gridDim.x * kGcLocalShare;
+    uint64_t lo = 0, hi = 0;
+    if (local) {
+        const uint64_t share = max((uint64_t)kGcShareMin, (n + gridDim.x - 1) / gridDim.x);
+        lo = min(n, (uint64_t)blockIdx.x * share);
+        hi = min(n, lo + share);
+    }
     // a block past the frontier has nothing to do (the rounds near the leaves hold few
     // nodes); the first round also releases new leaves and orphans with every block
-    if (!leaf_cleanup && (uint64_t)blockIdx.x * blockDim.x >= n) return;
+    if (!leaf_cleanup && (local ? lo >= hi : (uint64_t)blockIdx.x * blockDim.x >= n)) return;
     agg_init(akey, acnt);
     if (threadIdx.x == 0) {
         s_nnext = s_nfree = 0;
@@ -2042,106 +2177,120 @@ __global__ __launch_bounds__(256) void k_gc_round(Pool<T> P, uint32_t round, con
         }
     }
     const int lane = (int)__lane_id();
-    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < n; b0 += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t i = b0 + threadIdx.x;  // the loop is uniform over the block
-        uint32_t idx = kNoIdx;
-        if (i < n) {
-            uint32_t v = front[i];
-            if (v != 0 && v < P.capacity) idx = v;
-        }
-        bool freed = decrement_self ? wave_ref_dec_zero(P.refs, idx) : idx != kNoIdx;
-        bool is_leaf = false, tomb = false;
-        if (freed) {
-            // the node's columns in one round of loads (a leaf's children are empty).  A
-            // free writes only what a later reader needs: the generation, the types/mask
-            // word (0 marks a free or leaf slot for the table rebuild), the pattern's
-            // table slot and the free stack.  The count is already 0, the value and the
-            // children are rewritten when the slot is allocated again.
-            const uint32_t w = P.tmg[idx];
-            const uint32_t tm = tmg_tm(w);
-            const uint32_t g = tmg_gen(w) + 1u;
-            uint64_t ch[8];
-            const uint64_t* src = P.children + 8ull * idx;
-#pragma unroll
-            for (int j = 0; j < 8; j++) ch[j] = src[j];
-            is_leaf = tm == 0;
-            if (is_leaf) {
-                leaf_remove(P, (uint32_t)P.values[idx]);
-            } else {
-                // children first: combined in LDS where the block's table has room, the
-                // rest decremented with independent atomics (issued under the lane mask,
-                // so they are in flight together while the pattern is removed below)
-                uint32_t gmask = 0;
-#pragma unroll
-                for (int j = 0; j < 8; j++)
-                    if (ch[j] != kEmpty && !agg_add(akey, acnt, bid_index(ch[j]), 1u)) gmask |= 1u << j;
-                uint32_t old[8];
-#pragma unroll
-                for (int j = 0; j < 8; j++)
-                    old[j] = ((gmask >> j) & 1u) ? atomicSub(&P.refs[bid_index(ch[j])], 1u) : 0u;
-                // remove the pattern (mod.rs:518): a one-leaf key from its index (the
-                // leaf is still live: this node's reference is still being dropped), any
-                // other from the table, leaving a tombstone
-                if (P.single && tm_is_single(tm)) {
-                    P.single[single_index(P, tm, ch)] = 0;
-                } else {
-                    uint64_t h = branch_hash(ch, tm);
-                    uint64_t want = slot_make(slot_tag(h), false, 0, idx);
-                    uint64_t s = h & P.tmask;
-                    for (uint64_t guard = 0; guard <= P.tmask; guard++) {
-                        uint64_t e = P.tab[s];
-                        if (e == want) {
-                            P.tab[s] = kSlotTomb;
-                            tomb = true;
-                            break;
-                        }
-                        if (e == kSlotEmpty) break;
-                        s = (s + 1) & P.tmask;
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (local) {
+        // passes over the block's share, then over the children that reached zero in the
+        // block, while they fit one pass (the loop is uniform over the block)
+        uint32_t cnt = (uint32_t)(hi - lo);
+        bool first = true;
+        for (;;) {
+            uint32_t idx = kNoIdx;
+            if (threadIdx.x < cnt) {
+                const uint32_t v = first ? front[lo + threadIdx.x] : s_cur[threadIdx.x];
+                if (v != 0 && v < P.capacity) idx = v;
+            }
+            const bool freed = first && decrement_self ? wave_ref_dec_zero(P.refs, idx) : idx != kNoIdx;
+            bool is_leaf = false, tomb = false;
+            if (freed) gc_free(P, idx, akey, acnt, s_next, &s_nnext, &is_leaf, &tomb);
+            const uint64_t fb = __ballot(freed);
+            const uint64_t bl = __ballot(freed && is_leaf), bb = __ballot(freed && !is_leaf), bt = __ballot(tomb);
+            uint32_t woff = 0;
+            if (lane == 0) {
+                if (fb) woff = atomicAdd(&s_nfree, (uint32_t)__popcll(fb));
+                if (bl) atomicAdd(&s_leaves, (unsigned long long)__popcll(bl));
+                if (bb) atomicAdd(&s_branches, (unsigned long long)__popcll(bb));
+                if (bt) atomicAdd(&s_tombs, (unsigned long long)__popcll(bt));
+                if (fb) atomicAdd(&s_recycled, (unsigned long long)__popcll(fb));
+            }
+            woff = __shfl(woff, 0);
+            if (freed) s_free[woff + (uint32_t)__popcll(fb & below)] = idx;
+            __syncthreads();
+            // the pass's combined decrements (children shared by several of its nodes):
+            // those that reach zero join the children of the next pass
+            gc_agg_flush(P, akey, acnt, s_next, &s_nnext, true);
+            __syncthreads();
+            const uint32_t nn = s_nnext, nf = s_nfree;
+            const bool more = nn != 0 && nn <= kGcLocal;
+            // the staged frees go to the free stack when the next pass might not fit
+            if (nf && (!more || nf + kGcLocal > kGcFreeStage)) {
+                if (threadIdx.x == 0) s_bfree = atomicAdd(&P.ctl->free_top, nf);
+                __syncthreads();
+                for (uint32_t k = threadIdx.x; k < nf; k += blockDim.x) {
+                    const uint32_t pos = s_bfree + k;
+                    if (pos < P.capacity) P.free_stack[pos] = s_free[k];
+                    else atomicOr(&P.ctl->err, kErrPoison);
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) s_nfree = 0;
+            }
+            if (!more) {
+                // more children than one pass: the next launch takes them
+                if (nn) {
+                    if (threadIdx.x == 0) s_bnext = atomicAdd(next_count, nn);
+                    __syncthreads();
+                    for (uint32_t k = threadIdx.x; k < nn; k += blockDim.x) {
+                        const uint32_t pos = s_bnext + k;
+                        if (pos < front_cap) next[pos] = s_next[k];
+                        else atomicOr(&P.ctl->err, kErrPoison);
                     }
                 }
-#pragma unroll
-                for (int j = 0; j < 8; j++)
-                    if (((gmask >> j) & 1u) && old[j] == 1u) s_next[atomicAdd(&s_nnext, 1u)] = bid_index(ch[j]);
+                __syncthreads();
+                if (threadIdx.x == 0) s_nnext = 0;
+                break;
             }
-            P.tmg[idx] = tmg_make(0u, g >= kMaxGeneration ? 0u : g);  // no types/mask: free
+            for (uint32_t k = threadIdx.x; k < nn; k += blockDim.x) s_cur[k] = s_next[k];
+            __syncthreads();
+            if (threadIdx.x == 0) s_nnext = 0;
+            __syncthreads();
+            cnt = nn;
+            first = false;
         }
-        const uint64_t fb = __ballot(freed);
-        const uint64_t bl = __ballot(freed && is_leaf), bb = __ballot(freed && !is_leaf), bt = __ballot(tomb);
-        uint32_t woff = 0;
-        if (lane == 0) {
-            if (fb) woff = atomicAdd(&s_nfree, (uint32_t)__popcll(fb));
-            if (bl) atomicAdd(&s_leaves, (unsigned long long)__popcll(bl));
-            if (bb) atomicAdd(&s_branches, (unsigned long long)__popcll(bb));
-            if (bt) atomicAdd(&s_tombs, (unsigned long long)__popcll(bt));
-            if (fb) atomicAdd(&s_recycled, (unsigned long long)__popcll(fb));
+    } else {
+        for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < n; b0 += (uint64_t)gridDim.x * blockDim.x) {
+            const uint64_t i = b0 + threadIdx.x;  // the loop is uniform over the block
+            uint32_t idx = kNoIdx;
+            if (i < n) {
+                uint32_t v = front[i];
+                if (v != 0 && v < P.capacity) idx = v;
+            }
+            bool freed = decrement_self ? wave_ref_dec_zero(P.refs, idx) : idx != kNoIdx;
+            bool is_leaf = false, tomb = false;
+            if (freed) gc_free(P, idx, akey, acnt, s_next, &s_nnext, &is_leaf, &tomb);
+            const uint64_t fb = __ballot(freed);
+            const uint64_t bl = __ballot(freed && is_leaf), bb = __ballot(freed && !is_leaf), bt = __ballot(tomb);
+            uint32_t woff = 0;
+            if (lane == 0) {
+                if (fb) woff = atomicAdd(&s_nfree, (uint32_t)__popcll(fb));
+                if (bl) atomicAdd(&s_leaves, (unsigned long long)__popcll(bl));
+                if (bb) atomicAdd(&s_branches, (unsigned long long)__popcll(bb));
+                if (bt) atomicAdd(&s_tombs, (unsigned long long)__popcll(bt));
+                if (fb) atomicAdd(&s_recycled, (unsigned long long)__popcll(fb));
+            }
+            woff = __shfl(woff, 0);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (s_nfree) s_bfree = atomicAdd(&P.ctl->free_top, s_nfree);
+                if (s_nnext) s_bnext = atomicAdd(next_count, s_nnext);
+            }
+            __syncthreads();
+            if (freed) {
+                uint32_t pos = s_bfree + woff + (uint32_t)__popcll(fb & below);
+                if (pos < P.capacity) P.free_stack[pos] = idx;
+                else atomicOr(&P.ctl->err, kErrPoison);
+            }
+            for (uint32_t k = threadIdx.x; k < s_nnext; k += blockDim.x) {
+                uint32_t pos = s_bnext + k;
+                if (pos < front_cap) next[pos] = s_next[k];
+                else atomicOr(&P.ctl->err, kErrPoison);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) s_nfree = s_nnext = 0;
+            __syncthreads();
         }
-        woff = __shfl(woff, 0);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            if (s_nfree) s_bfree = atomicAdd(&P.ctl->free_top, s_nfree);
-            if (s_nnext) s_bnext = atomicAdd(next_count, s_nnext);
-        }
-        __syncthreads();
-        if (freed) {
-            uint32_t pos = s_bfree + woff + (uint32_t)__popcll(fb & ((1ull << lane) - 1ull));
-            if (pos < P.capacity) P.free_stack[pos] = idx;
-            else atomicOr(&P.ctl->err, kErrPoison);
-        }
-        for (uint32_t k = threadIdx.x; k < s_nnext; k += blockDim.x) {
-            uint32_t pos = s_bnext + k;
-            if (pos < front_cap) next[pos] = s_next[k];
-            else atomicOr(&P.ctl->err, kErrPoison);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) s_nfree = s_nnext = 0;
-        __syncthreads();
     }
     // the block's combined decrements; children reaching zero are queued the same way
-    for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
-        uint32_t key = akey[i];
-        if (key != kNoIdx && acnt[i] && atomicSub(&P.refs[key], acnt[i]) == acnt[i])
-            s_next[atomicAdd(&s_nnext, 1u)] = key;
-    }
+    __syncthreads();
+    gc_agg_flush(P, akey, acnt, s_next, &s_nnext, false);
     __syncthreads();
     if (threadIdx.x == 0) {
         if (s_nnext) s_bnext = atomicAdd(next_count, s_nnext);
