@@ -918,11 +918,12 @@ constexpr int brick_rounds() { return TILE >= 512 ? 2 : 4; }
 // rounds of TILE: the old-tree descent and brick_work.  Round r of phase 2 emits
 // into candidate tile blk*R + r; a deferred brick keeps that tile (its round has a
 // free place for it) for the retry.  The pass is bound by memory latency, so it runs
-// as many waves per SIMD as fit without spilling: 6 for one slot per lane; the
-// four-slot form's phase-1 loads and phase-2 keys take more registers and its queue more
-// LDS (5 waves for u8 and u16, 3 for 4-byte values).
+// as many waves per SIMD as fit without spilling: 6 for one slot per lane (5 for u16,
+// whose eight leaf ids are loaded from the value table together); the four-slot form's
+// phase-1 loads and phase-2 keys take more registers and its queue more LDS (5 waves for
+// u8 and u16, 3 for 4-byte values).
 template <typename T, int TILE, int R>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 6 : (sizeof(T) == 4 ? 3 : 5), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (sizeof(T) == 2 ? 5 : 6) : (sizeof(T) == 4 ? 3 : 5), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
                                                        uint64_t* deferred, Drift* drift,
                                                        const uint64_t* old2) {
@@ -953,28 +954,73 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 6
     if (sizeof(T) == 1)
         for (uint32_t i = 0, v = threadIdx.x; v < 256; i++, v += TILE) lm[i] = P.ctl->leaf_ids8[v];
     const uint64_t b0 = (uint64_t)blockIdx.x * (R * TILE);
-    // phase 1 loads, without branches (lanes past the end load the last slot)
+    // phase 1 loads, without branches (lanes past the end load the last slot), in two
+    // rounds for all R slots of a lane: the chunk records, then the bricks' masks and
+    // values and the height-2 node above each brick.  (Taken one slot after the other,
+    // every wait for a record also waits for the previous slot's data: 2R round trips.)
     uint32_t m[R];
     Vox8 vw[R];
     uint64_t up[R];
+    {
+        uint32_t cj[R], pj[R];
+        uint64_t uj[R];
+        bool lj[R];
+        bool uni = true;  // every slot's chunk is one per wavefront: scalar loads
 #pragma unroll
-    for (int j = 0; j < R; j++) {
-        const uint64_t slot = b0 + (uint64_t)j * TILE + threadIdx.x;
-        const bool live = slot < L.nslots;
-        const uint64_t sl = live ? slot : L.nslots - 1;
-        const GroupPos G = group_pos(L, sl);
-        const uint64_t local = sl - G.base;
-        const uint32_t D = G.depth;
-        const uint32_t sh = 3 * (D - 1);
-        const uint32_t c = G.chunk_base + (uint32_t)(local >> sh);
-        const uint32_t p = (uint32_t)(local & ((1ull << sh) - 1));
-        const ChunkHead H = chunk_head(chunks, c);
-        m[j] = H.masks[2ull * p];
-        vw[j] = *(const Vox8*)((const T*)H.values + 8ull * p);
-        // the height-2 slot above this brick follows from the slot (a depth-1 tree has
-        // none; a tree with a fill does not use it)
-        up[j] = old2[D >= 2 ? G.next + (local >> 3) : 0];
-        if (!live || H.status < 0) m[j] = 0;
+        for (int j = 0; j < R; j++) {
+            const uint64_t slot = b0 + (uint64_t)j * TILE + threadIdx.x;
+            lj[j] = slot < L.nslots;
+            const uint64_t sl = lj[j] ? slot : L.nslots - 1;
+            const GroupPos G = group_pos(L, sl);
+            const uint64_t local = sl - G.base;
+            const uint32_t D = G.depth;
+            const uint32_t sh = 3 * (D - 1);
+            cj[j] = G.chunk_base + (uint32_t)(local >> sh);
+            pj[j] = (uint32_t)(local & ((1ull << sh) - 1));
+            // the height-2 slot above this brick follows from the slot (a depth-1 tree
+            // has none; a tree with a fill does not use it)
+            uj[j] = D >= 2 ? G.next + (local >> 3) : 0;
+            uni = uni && __ballot(cj[j] != __builtin_amdgcn_readfirstlane(cj[j])) == 0;
+        }
+        const uint8_t* hm[R];
+        const void* hv[R];
+        int32_t hs[R];
+        if (uni) {
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                ConstChunk& r = *(ConstChunk*)(chunks + __builtin_amdgcn_readfirstlane(cj[j]));
+                hm[j] = r.masks;
+                hv[j] = r.values;
+                hs[j] = r.status;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                const ChunkDev& r = chunks[cj[j]];
+                hm[j] = r.masks;
+                hv[j] = r.values;
+                hs[j] = r.status;
+            }
+        }
+        // (global-memory loads: a generic load may return out of order with the others,
+        // and a wait for any earlier load then waits for all of them)
+        typedef const __attribute__((address_space(1))) uint8_t GByte;
+        typedef unsigned int VoxWords __attribute__((ext_vector_type(sizeof(Vox8) / 4)));
+        typedef const __attribute__((address_space(1))) VoxWords GVox;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            m[j] = ((GByte*)hm[j])[2ull * pj[j]];
+            const VoxWords w = *(GVox*)((const T*)hv[j] + 8ull * pj[j]);
+            __builtin_memcpy(&vw[j], &w, sizeof(Vox8));
+            up[j] = old2[uj[j]];
+        }
+        // (used here, the masks are loaded with the rest: the compiler would otherwise
+        // load a mask only for a live slot, and wait for it before the next slot's loads)
+#pragma unroll
+        for (int j = 0; j < R; j++) asm volatile("" ::"v"(m[j]));
+#pragma unroll
+        for (int j = 0; j < R; j++)
+            if (!lj[j] || hs[j] < 0) m[j] = 0;
     }
     if (sizeof(T) == 1)
         for (uint32_t i = 0, vv = threadIdx.x; vv < 256; i++, vv += TILE) s_leaf[vv] = lm[i];
@@ -984,15 +1030,27 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 6
     if (threadIdx.x == 0) s_nq = 0;
     __syncthreads();
     uint32_t ncollapse = 0;
+    // the leaf of every all-same brick's value, for all R slots before any is used (u8
+    // and u16: one load from the value's table entry, made for every slot)
+    uint64_t lidj[R];
 #pragma unroll
     for (int j = 0; j < R; j++) {
-        const uint64_t slot = b0 + (uint64_t)j * TILE + threadIdx.x;
         uint32_t v[8];
         load8<T>((const T*)&vw[j], v);
         bool all_same = m[j] == 0xFFu;
 #pragma unroll
         for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
-        const uint64_t lid = all_same ? leaf_lookup(P, leaf_ids, v[0]) : kEmpty;
+        if constexpr (sizeof(T) <= 2) {
+            const uint64_t l = leaf_ids[v[0]];
+            lidj[j] = all_same ? l : kEmpty;
+        } else {
+            lidj[j] = all_same ? leaf_lookup(P, leaf_ids, v[0]) : kEmpty;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+        const uint64_t slot = b0 + (uint64_t)j * TILE + threadIdx.x;
+        const uint64_t lid = lidj[j];
         const bool queue = m[j] != 0 && lid == kEmpty;
         const uint32_t q = wave_append(&s_nq, queue);
         if (queue) {
@@ -1038,12 +1096,23 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? 6
             const ChunkDev& C = chunks[c];
             const bool c_fill = C.has_fill != 0;
             const uint64_t c_root = C.root_in;
+            // (u8 and u16: a table load for every voxel, edited or not, so that the eight
+            // are issued together; brick_work reads the leaf ids of edited voxels only)
             uint64_t lids[8];
+            if constexpr (sizeof(T) <= 2) {
 #pragma unroll
-            for (int i = 0; i < 8; i++) lids[i] = (mm >> i) & 1u ? leaf_lookup(P, leaf_ids, v[i]) : kEmpty;
+                for (int i = 0; i < 8; i++) lids[i] = leaf_ids[v[i]];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; i++) lids[i] = (mm >> i) & 1u ? leaf_lookup(P, leaf_ids, v[i]) : kEmpty;
+            }
             const bool up_branch = D >= 2 && upq != kEmpty && !bid_is_leaf(upq);
             const bool up_ok = up_branch && bid_index(upq) < P.capacity;
             const uint64_t below = P.children[8ull * (up_ok ? bid_index(upq) : 0u) + (p & 7u)];
+            // (used here, the record's fields and the node below are loaded with the leaf
+            // ids: the compiler would otherwise load each in the branch that needs it,
+            // one round trip after the other)
+            asm volatile("" ::"v"(below), "v"(c_root), "v"((uint32_t)c_fill));
             bool all_same = mm == 0xFFu;
 #pragma unroll
             for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
