@@ -1491,6 +1491,13 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
     for (uint32_t i = threadIdx.x; i < kSlots; i += TILE) s_set[i] = 0;
     if (threadIdx.x < kProbeStats) s_stat[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_done = 0;
+    // the levels' candidate arrays from LDS: a key met as another level's pending entry
+    // is compared against (and resolved through) that level's candidates without first
+    // loading the level's record
+    __shared__ Cands s_lv[kLevels + 1];
+    if (threadIdx.x < kLevels + 1) s_lv[threadIdx.x] = P.levels[threadIdx.x];
+    Pool<T> Q = P;
+    Q.levels = s_lv;
     uint32_t n_probes = 0, n_claims = 0, n_lost = 0, n_tomb = 0, n_hit = 0, n_dup = 0, n_miss = 0;
     // the allocation window of the sequence (constant during this launch)
     const uint32_t ft0 = P.ctl->free_top0, next0 = P.ctl->next0, own_base = P.ctl->own_base;
@@ -1576,8 +1583,8 @@ __device__ inline void probe_tiles(const Pool<T>& P, const Cands& K, const uint3
         uint32_t probes = 0, claims = 0;
         if (lead)
             kind = P.single && tm_is_single((uint32_t)key[8])
-                       ? probe_single(P, L.h, k, key, &result, &tslot, &probes, &claims)
-                       : probe_table(P, L.h, k, key, h, &result, &tslot, &probes, &claims);
+                       ? probe_single(Q, L.h, k, key, &result, &tslot, &probes, &claims)
+                       : probe_table(Q, L.h, k, key, h, &result, &tslot, &probes, &claims);
         // An owner's LOD value (core/voxel.rs:96-141) from its children's values, which
         // earlier launches wrote (a node gets its value at its own level): its loads are
         // issued now and complete while the block allocates.
@@ -1798,13 +1805,25 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
         // touched.
         const bool same_old =
             L.h == 2 && eq != kEmpty && !bid_is_leaf(eq) && bid_index(eq) < P.capacity && ell == 0;
+        uint32_t given = 0;  // children with a result (kInvalid: untouched)
+#pragma unroll
+        for (int i = 0; i < 8; i++) given |= (ch[i] != kInvalid ? 1u : 0u) << i;
+        // pending results resolve through the previous level's candidate ids (two hops at
+        // most: a follower's leader may follow another), each hop one load per child for
+        // all eight together (entry 0 for a child already resolved); taken child by child,
+        // the compiler waited for every load before the next
+#pragma unroll
+        for (int hop = 0; hop < 2; hop++) {
+            uint64_t v[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) v[i] = cand_id_prev[res_is_pending(ch[i]) ? (uint32_t)ch[i] : 0u];
+#pragma unroll
+            for (int i = 0; i < 8; i++) ch[i] = res_is_pending(ch[i]) ? v[i] : ch[i];
+        }
         uint32_t touched = 0;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const bool t = ch[i] != kInvalid;
-            ch[i] = resolve(ch[i], cand_id_prev);
-            touched |= (t && !(same_old && !bid_is_leaf(ch[i]) && ch[i] == ec[i]) ? 1u : 0u) << i;
-        }
+        for (int i = 0; i < 8; i++)
+            touched |= (((given >> i) & 1u) && !(same_old && !bid_is_leaf(ch[i]) && ch[i] == ec[i]) ? 1u : 0u) << i;
         uint64_t out = kInvalid;
         bool cand = false, collapsed = false, h5 = false;
         uint32_t tm = 0;
