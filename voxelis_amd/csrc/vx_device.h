@@ -63,12 +63,27 @@ __host__ __device__ inline uint64_t mix64(uint64_t x) {
 
 // Hash of a branch key (children ids, types, mask).  Identity is always decided by
 // a full-key compare; the hash only places the key in the table.
+//
+// NH-style (the UMAC/VHASH inner hash): the sum, mod 2^64, of products of pairs of
+// 32-bit halves, each offset (mod 2^32) by its own odd constant, then the splitmix64
+// finalizer.  Pairs join halves of different children (low with low, high with high),
+// so a product of two data terms keeps structured keys (consecutive leaf ids, one new
+// child among empty ones) from cancelling as a linear combination could.  Eight 32x32
+// products where mixing every word took eighteen 64-bit ones: the brick and level
+// passes hash every key they emit, and their issue time is mostly vector ALU.
 __host__ __device__ inline uint64_t branch_hash(const uint64_t ch[8], uint32_t tm) {
-    uint64_t a = mix64(ch[0] ^ 0x9E3779B97F4A7C15ull) + mix64(ch[1] ^ 0xC2B2AE3D27D4EB4Full);
-    uint64_t b = mix64(ch[2] ^ 0x165667B19E3779F9ull) + mix64(ch[3] ^ 0xD6E8FEB86659FD93ull);
-    uint64_t c = mix64(ch[4] ^ 0xFF51AFD7ED558CCDull) + mix64(ch[5] ^ 0xC4CEB9FE1A85EC53ull);
-    uint64_t d = mix64(ch[6] ^ 0x9FB21C651E98DF25ull) + mix64(ch[7] ^ 0x2545F4914F6CDD1Dull);
-    return mix64((a ^ (b << 1)) + (c ^ (d >> 1)) + 0x5851F42D4C957F2Dull * (tm + 1));
+    const auto lo = [&](int i) { return (uint32_t)ch[i]; };
+    const auto hi = [&](int i) { return (uint32_t)(ch[i] >> 32); };
+    uint64_t s = (uint64_t)(tm + 0x50AB4B57u) * 0x80FE62C5u;
+    s += (uint64_t)(lo(0) + 0x4ABEA221u) * (lo(1) + 0x23148989u);
+    s += (uint64_t)(lo(2) + 0x609DFE03u) * (lo(3) + 0xDEB12801u);
+    s += (uint64_t)(lo(4) + 0x6C442CB7u) * (lo(5) + 0xC7E4F8C5u);
+    s += (uint64_t)(lo(6) + 0xF8FBA7E5u) * (lo(7) + 0x3CDB9EEBu);
+    s += (uint64_t)(hi(0) + 0xF9520069u) * (hi(2) + 0xE116868Bu);
+    s += (uint64_t)(hi(1) + 0xA2023CBDu) * (hi(3) + 0xA37B51B9u);
+    s += (uint64_t)(hi(4) + 0x524F3905u) * (hi(6) + 0x6CA3B277u);
+    s += (uint64_t)(hi(5) + 0x5104E85Bu) * (hi(7) + 0x9FD533B3u);
+    return mix64(s);
 }
 
 __host__ __device__ inline uint32_t slot_tag(uint64_t h) {
