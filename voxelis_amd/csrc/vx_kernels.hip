@@ -918,12 +918,12 @@ constexpr int brick_rounds() { return TILE >= 512 ? 2 : 4; }
 // rounds of TILE: the old-tree descent and brick_work.  Round r of phase 2 emits
 // into candidate tile blk*R + r; a deferred brick keeps that tile (its round has a
 // free place for it) for the retry.  The pass is bound by memory latency, so it runs
-// as many waves per SIMD as fit without spilling: 6 for one slot per lane (5 for u16,
-// whose eight leaf ids are loaded from the value table together); the four-slot form's
-// phase-1 loads and phase-2 keys take more registers and its queue more LDS (5 waves for
-// u8 and u16, 3 for 4-byte values).
+// as many waves per SIMD as fit without spilling: 5 for one slot per lane with u8 and
+// u16 values (their eight leaf ids are loaded together), 6 with 4-byte values; the
+// four-slot form's phase-1 loads and phase-2 keys take more registers and its queue more
+// LDS (5 waves for u8 and u16, 3 for 4-byte values).
 template <typename T, int TILE, int R>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (sizeof(T) == 2 ? 5 : 6) : (sizeof(T) == 4 ? 3 : 5), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (sizeof(T) == 4 ? 6 : 5) : (sizeof(T) == 4 ? 3 : 5), 8))) void k_brick_build(Pool<T> P, ChunkDev* chunks, LevelInfo L,
                                                        uint64_t* res1, Cands K, uint32_t* tile_cnt,
                                                        uint64_t* deferred, Drift* drift,
                                                        const uint64_t* old2) {
@@ -1144,9 +1144,28 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (
                 chunks[c].status = -5;
                 P.ctl->h1_fail = 1u;
             } else {
-                uint32_t didx;
-                int32_t dd;
-                int rc = brick_work(P, C, mm, v, base, &out, ch, &tm, &didx, &dd, &flags, lids);
+                uint32_t didx = kNoIdx;
+                int32_t dd = 0;
+                int rc;
+                bool lids_all = true;
+#pragma unroll
+                for (int i = 0; i < 8; i++) lids_all = lids_all && lids[i] != kEmpty;
+                if (mm == 0xFFu && !all_same && base != kFillMark && !bid_is_leaf(base) && lids_all) {
+                    // brick_work's result for a completely written brick of distinct values
+                    // over a branch or nothing, whose leaves all exist (the dense case): the
+                    // key is the eight leaves, types = mask = 0xFF
+                    bool uniform = true;
+#pragma unroll
+                    for (int i = 0; i < 8; i++) {
+                        ch[i] = lids[i];
+                        uniform = uniform && lids[i] == lids[0];
+                    }
+                    if (uniform) flags |= 4;  // H2 (never for distinct values: a live leaf is the one leaf of its value)
+                    tm = 0xFFFFu;
+                    rc = 2;
+                } else {
+                    rc = brick_work(P, C, mm, v, base, &out, ch, &tm, &didx, &dd, &flags, lids);
+                }
                 if (didx != kNoIdx && !P.dry) {
                     uint32_t pos = atomicAdd(&P.ctl->n_drift, 1u);
                     drift[pos] = Drift{c, didx, dd, (uint32_t)(flags & 8 ? ST_H3 : ST_H4)};
