@@ -393,6 +393,22 @@ __device__ inline GroupPos group_pos(const LevelInfo& L, uint64_t slot) {
     return r;
 }
 
+// group_pos for the slots of one block: when the block's first and last slots lie in
+// one group (every block but those that straddle two), that group, found once per block
+// from block-uniform values (scalar instructions); else each slot's own.  first and last
+// must be the same for every lane.
+struct BlockGroup {
+    GroupPos G;
+    bool one;
+};
+__device__ inline BlockGroup block_group(const LevelInfo& L, uint64_t first, uint64_t last) {
+    const GroupPos a = group_pos(L, first), b = group_pos(L, last);
+    return BlockGroup{a, a.base == b.base};
+}
+__device__ inline GroupPos group_of(const LevelInfo& L, const BlockGroup& B, uint64_t slot) {
+    return B.one ? B.G : group_pos(L, slot);
+}
+
 // The fields of a chunk record that the build passes read first.  A wavefront whose
 // lanes all lie in one chunk (every wavefront of a tree of depth >= 3 at the brick
 // level) reads them with scalar loads: vector-memory address processing is what
@@ -954,6 +970,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (
     if (sizeof(T) == 1)
         for (uint32_t i = 0, v = threadIdx.x; v < 256; i++, v += TILE) lm[i] = P.ctl->leaf_ids8[v];
     const uint64_t b0 = (uint64_t)blockIdx.x * (R * TILE);
+    const BlockGroup BG = block_group(L, min(b0, L.nslots - 1), min(b0 + R * TILE, L.nslots) - 1);
     // phase 1 loads, without branches (lanes past the end load the last slot), in two
     // rounds for all R slots of a lane: the chunk records, then the bricks' masks and
     // values and the height-2 node above each brick.  (Taken one slot after the other,
@@ -971,7 +988,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (
             const uint64_t slot = b0 + (uint64_t)j * TILE + threadIdx.x;
             lj[j] = slot < L.nslots;
             const uint64_t sl = lj[j] ? slot : L.nslots - 1;
-            const GroupPos G = group_pos(L, sl);
+            const GroupPos G = group_of(L, BG, sl);
             const uint64_t local = sl - G.base;
             const uint32_t D = G.depth;
             const uint32_t sh = 3 * (D - 1);
@@ -1084,7 +1101,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (
             const uint64_t upq = s_up[q];
             uint32_t v[8];
             load8<T>((const T*)&w, v);
-            const GroupPos G = group_pos(L, slot);
+            const GroupPos G = group_of(L, BG, slot);
             const uint64_t local = slot - G.base;
             const uint32_t D = G.depth;
             const uint32_t sh = 3 * (D - 1);
@@ -1785,7 +1802,8 @@ __device__ inline void level_build_body(const Pool<T>& P, const ChunkDev* chunks
         uint64_t slot = (uint64_t)blk * TILE + threadIdx.x;
         const bool valid = slot < L.nslots;
         const uint64_t sl = valid ? slot : L.nslots - 1;
-        const GroupPos G = group_pos(L, sl);
+        const uint64_t s0 = min((uint64_t)blk * TILE, L.nslots - 1);
+        const GroupPos G = group_of(L, block_group(L, s0, min((uint64_t)blk * TILE + TILE, L.nslots) - 1), sl);
         const uint64_t local = sl - G.base;
         const uint32_t D = G.depth;
         const uint32_t sh = 3 * (D - L.h);
