@@ -613,14 +613,18 @@ int apply_tiled(vx_interner* h, size_t n, uint64_t* roots, const vx_batch_view* 
     VX_LAUNCH(h, KID_END_ALLOC, k_end_alloc<T>, 1, 256, P, true, opt.counts);
     if (!opt.dry) {
         // dec_ref_recursive of the replaced roots: a tree of depth D has nodes at
-        // heights D..0, one round each; the first round also releases unused new leaves
-        // (every round takes the GC grid: a small front is split over its blocks and
-        // followed locally, which takes one block per share)
+        // heights D..0, one round each; the first round also releases unused new leaves.
+        // A small front is split over the blocks in shares of kGcShareMin nodes and
+        // followed locally: the first round takes one block per share of the replaced
+        // roots (every block of that round also runs the leaf release), the later ones
+        // the GC grid
         static_assert(kBlock == 256, "k_gc_round follows up to 256 nodes per pass");
+        const unsigned g0 = std::max(grid_for(n), std::min<unsigned>(h->tune.gc_grid,
+                                                                    (unsigned)((n + kGcShareMin - 1) / kGcShareMin)));
         for (uint32_t r = 0; r <= dall; r++) {
             uint32_t* fr = (r & 1) ? gf1 : gf0;
             uint32_t* nx = (r & 1) ? gf0 : gf1;
-            unsigned grid = std::max(grid_for(n), (unsigned)h->tune.gc_grid);
+            unsigned grid = r == 0 ? g0 : h->tune.gc_grid;
             VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid, kBlock, P, r, fr, nx, r == 0, (uint64_t)gcap, r == 0);
         }
     }
@@ -813,7 +817,8 @@ int fill_impl(vx_interner* h, size_t n, uint64_t* roots, const uint64_t* vals) {
     for (uint32_t r = 0; r < (uint32_t)kLevels + 1; r++) {
         uint32_t* fr = (r & 1) ? gf1 : gf0;
         uint32_t* nx = (r & 1) ? gf0 : gf1;
-        unsigned grid = std::max(grid_for(n), 1024u);
+        unsigned grid = r == 0 ? std::max(grid_for(n), std::min<unsigned>(1024u, (unsigned)((n + kGcShareMin - 1) / kGcShareMin)))
+                               : 1024u;
         VX_LAUNCH(h, KID_GC, k_gc_round<T>, grid, kBlock, P, r, fr, nx, r == 0, (uint64_t)gcap, false);
     }
     VX_LAUNCH(h, KID_BEGIN, k_begin, 1, 1, h->ctl);
