@@ -1170,14 +1170,11 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (
                 if (mm == 0xFFu && !all_same && base != kFillMark && !bid_is_leaf(base) && lids_all) {
                     // brick_work's result for a completely written brick of distinct values
                     // over a branch or nothing, whose leaves all exist (the dense case): the
-                    // key is the eight leaves, types = mask = 0xFF
-                    bool uniform = true;
+                    // key is the eight leaves, types = mask = 0xFF.  (Never 8 equal leaves,
+                    // brick_work's H2 case: a live leaf is the one leaf of its value, and
+                    // the values are not all the same.)
 #pragma unroll
-                    for (int i = 0; i < 8; i++) {
-                        ch[i] = lids[i];
-                        uniform = uniform && lids[i] == lids[0];
-                    }
-                    if (uniform) flags |= 4;  // H2 (never for distinct values: a live leaf is the one leaf of its value)
+                    for (int i = 0; i < 8; i++) ch[i] = lids[i];
                     tm = 0xFFFFu;
                     rc = 2;
                 } else {
