@@ -947,7 +947,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (
     typedef typename std::conditional<sizeof(T) == 1, uint2, typename std::conditional<sizeof(T) == 2, uint4,
                                                                                           Vox8x4>::type>::type Vox8;
     static_assert(sizeof(Vox8) == 8 * sizeof(T), "8 voxels");
-    static_assert(R * TILE <= 65536, "queued brick index is 16 bits");
+    static_assert(R * TILE <= 32768, "queued brick index is 15 bits (bit 15: all eight values equal)");
     __shared__ uint32_t s_nq;
     __shared__ uint32_t s_cnt[R];
     __shared__ uint32_t s_st[8];
@@ -1050,6 +1050,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (
     // the leaf of every all-same brick's value, for all R slots before any is used (u8
     // and u16: one load from the value's table entry, made for every slot)
     uint64_t lidj[R];
+    bool asj[R];
 #pragma unroll
     for (int j = 0; j < R; j++) {
         uint32_t v[8];
@@ -1057,6 +1058,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (
         bool all_same = m[j] == 0xFFu;
 #pragma unroll
         for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
+        asj[j] = all_same;
         if constexpr (sizeof(T) <= 2) {
             const uint64_t l = leaf_ids[v[0]];
             lidj[j] = all_same ? l : kEmpty;
@@ -1071,7 +1073,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (
         const bool queue = m[j] != 0 && lid == kEmpty;
         const uint32_t q = wave_append(&s_nq, queue);
         if (queue) {
-            s_idx[q] = (uint16_t)(j * TILE + threadIdx.x);
+            s_idx[q] = (uint16_t)((j * TILE + threadIdx.x) | (asj[j] ? 0x8000u : 0u));
             s_m[q] = (uint8_t)m[j];
             s_v[q] = vw[j];
             s_up[q] = up[j];
@@ -1095,7 +1097,8 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (
         uint64_t ch[8];
         uint32_t tm = 0, c = 0;
         if (q < nq) {
-            slot = b0 + s_idx[q];
+            const uint32_t qi = s_idx[q];
+            slot = b0 + (qi & 0x7FFFu);
             const uint32_t mm = s_m[q];
             const Vox8 w = s_v[q];
             const uint64_t upq = s_up[q];
@@ -1130,9 +1133,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(R == 1 ? (
             // ids: the compiler would otherwise load each in the branch that needs it,
             // one round trip after the other)
             asm volatile("" ::"v"(below), "v"(c_root), "v"((uint32_t)c_fill));
-            bool all_same = mm == 0xFFu;
-#pragma unroll
-            for (int i = 1; i < 8; i++) all_same = all_same && v[i] == v[0];
+            const bool all_same = (qi >> 15) != 0;  // taken in phase 1
             uint64_t base = kEmpty;
             bool h1 = false;
             if (!all_same) {
