@@ -86,7 +86,8 @@ int vx_interner_set_stream(vx_interner* h, void* hip_stream);
  * passes (128, 256 or 512; a block owns one tile of level slots and its candidate
  * region); dedup_rounds: 0 probes every key, 1-16 merges the equal keys of each
  * probe pass (block-wide) first, so one leader per key reads the table;
- * commit_grid / gc_grid: blocks of the commit pass and of each GC round;
+ * commit_grid / gc_grid: blocks of the commit pass and of each GC round after
+ * the first (the first takes one block per 16 replaced roots, up to gc_grid);
  * probe_blocks: a probe launch over more tiles than this packs the candidates of
  * up to 16 tiles into one block, keeping at least this many blocks (0: one tile
  * per block), separately for the brick level (probe_blocks_bricks) and the levels
